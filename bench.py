@@ -1,0 +1,192 @@
+"""Benchmark of the HIT3DP time step on one MI355X.
+
+Workload: 512^3 forced turbulence, RE=800, 2/3-rule dealiasing, band forcing
+KFMAX=2 FAMP=0.5, variable DT, from the seeded k^4 exp(-2(k/3)^2) field
+(RN1=67748937).  A step is one pass of the driver loop body (main.f90:124-176):
+RHS, forcing, AB2 update, projection and the dt controller, with all state
+resident in HBM.  Both precision forms run from the same initial field: FP64
+throughout, and mixed (FP32 transforms and nonlinear term, FP64 state,
+update and reductions); the E(k) difference between them after the timed
+steps is reported.
+
+Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+PEAK_HBM_GBS = 8000.0
+METRIC = "s/step at 512^3 FP64 vs mixed (E(k) err stated) + % of HBM roofline"
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def params(n):
+    from hit3d_amd.driver import Params
+    return Params(nx=n, ny=n, nz_all=n, flow_type=1, re=800.0, dt=0.001, variable_dt=True,
+                  isp_type=2, ir_exp=4, peak_wavenum=3.0, kfmax=2, famp=0.5, dealias=0)
+
+
+def slot_bytes(n, word=8):
+    return (n + 2) * n * n * word
+
+
+def run_form(n, precision, steps, warmup):
+    from hit3d_amd.driver import Run
+    p = params(n)
+    r = Run(p, precision=precision)
+    t0 = time.time()
+    r.begin_new()
+    r.h.synchronize()
+    log(f"[bench] n={n} precision={precision} init {time.time() - t0:.1f}s")
+    for _ in range(warmup):
+        r.step()
+    r.h.synchronize()
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:  # torch is plumbing only; the library's own sync is authoritative
+        pass
+    r.h.timing_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r.step()
+    r.h.synchronize()
+    t1 = time.perf_counter()
+    timers = r.h.timing_read()
+    r.h.timing_enable(False)
+    st = r.stats()
+    log(f"[bench] n={n} precision={precision} {1e3 * (t1 - t0) / steps:.3f} ms/step "
+        f"courant={r.courant:.4f} dt={r.dt:.3e} energy={st.energy:.6e}")
+    r.h.close()
+    return (t1 - t0) / steps, timers, st, r
+
+
+# algorithmic HBM bytes per launch of each timed kernel class (FP64 slot S):
+# every axis pass reads and writes one slot's valid entries; the fused step
+# kernels are charged the slots they must read and write.
+def kernel_bytes(n, cls, precision):
+    s64 = slot_bytes(n)
+    sw = s64 if precision == 0 else s64 // 2
+    table = {
+        "fft_x": 2 * s64, "fft_y": 2 * s64, "fft_z": 2 * s64,
+        "step_inv_z": 3 * s64 + 3 * sw,      # fields(1:3) in, work out
+        "step_inv_y": 6 * sw,
+        "step_x": 9 * sw,                    # 3 in, 6 products out
+        "step_fwd_y": 12 * sw,
+        "step_fwd_z": 6 * sw + 12 * s64,     # products in; fields, rhs_old in/out
+    }
+    return table.get(cls)
+
+
+def cpu_baseline(n_target):
+    """The oracle (numpy/C port of the reference step) on the host, on a
+    bounded sample: one step at 128^3 or 256^3, scaled by N^3 log2 N."""
+    from oracle import hit3d_oracle as o
+    n = 256 if n_target >= 256 else n_target
+    import hit3d_amd
+    h = hit3d_amd.Hit3D(n)
+    h.init_velocity(67748937.0, 2, 4, 3.0)
+    h.dealias_all()
+    f0 = np.stack([h.get(hit3d_amd.FIELDS, c) for c in (1, 2, 3)])
+    h.close()
+    cfg = o.Config(n=n, nu=1 / 800.0, dt=0.001, dealias=0, flow_type=1, kfmax=2, famp=0.5,
+                   variable_dt=True)
+    st = o.State(fields=f0, rhs_old=np.zeros_like(f0), dt=cfg.dt)
+    t0 = time.perf_counter()
+    st = o.step(cfg, st)
+    dt_n = time.perf_counter() - t0
+    scale = (n_target / n) ** 3 * (math.log2(n_target) / math.log2(n))
+    return {"value": dt_n * scale, "unit": "s/step", "cores": 1, "kind": "port",
+            "sample": f"one oracle step (numpy pocketfft + C) at {n}^3, {dt_n:.2f} s, "
+                      f"scaled by N^3 log2 N to {n_target}^3"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs the whole box on one GPU; --gpus {a.gpus} is not "
+              "supported (multi-GPU sharding is design only)", file=sys.stderr)
+        sys.exit(2)
+    n = a.n
+    t64, tim64, st64, _ = run_form(n, 0, a.steps, a.warmup)
+    mixed = None
+    if not a.no_mixed:
+        try:
+            t32, tim32, st32, _ = run_form(n, 1, a.steps, a.warmup)
+            e64, e32 = st64.e_spec, st32.e_spec
+            sig = e64 > 1e-12 * e64.max()
+            mixed = {"s_per_step": t32, "speedup_vs_fp64": t64 / t32,
+                     "ek_max_rel_err_vs_fp64": float(np.max(np.abs(e32[sig] - e64[sig]) / e64[sig])),
+                     "energy_rel_err_vs_fp64": abs(st32.energy - st64.energy) / st64.energy,
+                     "eps_rel_err_vs_fp64": abs(st32.eps_v - st64.eps_v) / st64.eps_v,
+                     "after_steps": a.steps + a.warmup}
+        except Exception as e:  # reported, never silently replaced
+            mixed = {"error": str(e)}
+    # dominant kernel class of the FP64 step
+    best, best_ms = None, -1.0
+    for cls, (ms, cnt) in tim64.items():
+        if cnt > 0 and ms > best_ms and kernel_bytes(n, cls, 0) is not None:
+            best, best_ms = cls, ms
+    roof = None
+    if best is not None:
+        ms, cnt = tim64[best]
+        avg_s = ms / cnt / 1e3
+        byt = kernel_bytes(n, best, 0)
+        ach = byt / avg_s / 1e9
+        roof = {"bound": "hbm", "kernel_class": best, "achieved": ach, "peak": PEAK_HBM_GBS,
+                "unit": "GB/s", "frac": ach / PEAK_HBM_GBS, "traffic": None,
+                "alg_bytes_per_launch": byt, "avg_launch_ms": avg_s * 1e3,
+                "launches_per_step": cnt / a.steps}
+    s = slot_bytes(n)
+    b_alg = 6 * 9 * s + 4 * 3 * s  # B_alg of the measurement plan, n_fft = 9, no scalars
+    out = {
+        "metric": METRIC,
+        "value": t64,
+        "unit": "s/step",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": t64 * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded RANDu field, spectrum k^4 exp(-2(k/3)^2), RN1=67748937",
+        "config": {"workload": f"{n}^3 forced DNS, RE=800, 2/3-rule, KFMAX=2, FAMP=0.5, "
+                               "variable DT, no scalars", "grid": n, "precision": "fp64",
+                   "parallelism": "single GPU"},
+        "ns_per_point_step": t64 / n ** 3 * 1e9,
+        "step_hbm_roofline_frac": b_alg / t64 / (PEAK_HBM_GBS * 1e9),
+        "step_alg_bytes": b_alg,
+        "mixed": mixed,
+        "kernel_ms_per_step": {k: v[0] / a.steps for k, v in tim64.items() if v[1]},
+        "roofline": roof,
+        "cpu_baseline": None if a.no_cpu else cpu_baseline(n),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,786 @@
+// C ABI of the library (include/hit3d.h): device-resident state and the
+// entry points that replace the reference's x_fftw / rhs_* / pressure /
+// m_force / dealias_all subroutines.
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "hit3d.h"
+#include "init_host.hpp"
+#include "passes.hpp"
+#include "state.hpp"
+
+namespace hit3d {
+
+static Lib g_lib;
+Lib& lib() { return g_lib; }
+
+void check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw HipError{std::string(what) + ": " + hipGetErrorString(e)};
+}
+
+static const char* kTimerNames =
+    "fft_x,fft_y,fft_z,pointwise,reduce,step_inv_z,step_inv_y,step_x,step_fwd_y,step_fwd_z";
+
+Timed::Timed(int c) : cls(c) {
+    Lib& L = lib();
+    if (!L.timing) return;
+    check(hipEventCreate(&a), "hipEventCreate");
+    check(hipEventCreate(&b), "hipEventCreate");
+    check(hipEventRecord(a, L.st), "hipEventRecord");
+}
+Timed::~Timed() {
+    Lib& L = lib();
+    if (!a) return;
+    if (hipEventRecord(b, L.st) == hipSuccess) L.evs.push_back({cls, a, b});
+}
+
+static void collect_timers() {
+    Lib& L = lib();
+    if (L.evs.empty()) return;
+    check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
+    for (auto& e : L.evs) {
+        float ms = 0.f;
+        check(hipEventElapsedTime(&ms, e.a, e.b), "hipEventElapsedTime");
+        L.tms[e.cls] += ms;
+        L.tcnt[e.cls] += 1;
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    L.evs.clear();
+}
+
+// ---------------------------------------------------------------------------
+// transforms
+// ---------------------------------------------------------------------------
+static void pass_y(double* slot, int dir) {
+    Lib& L = lib();
+    auto* c = reinterpret_cast<cpx<double>*>(slot);
+    const int64_t n = L.n, P = L.g.P;
+    Timed t(TC_FFT_Y);
+    launch_cols<double>(L.n, dir, c, c, n * P, n * P, P, P, L.tw64, L.st);
+}
+
+void fft_forward(double*& slot) {
+    Lib& L = lib();
+    const int64_t n = L.n, P = L.g.P;
+    {
+        Timed t(TC_FFT_X);
+        launch_x_r2c<double>(L.n, slot, P, L.tw64, L.st);
+    }
+    pass_y(slot, -1);
+    {
+        Timed t(TC_FFT_Z);
+        launch_cols<double>(L.n, -1, reinterpret_cast<cpx<double>*>(slot),
+                            reinterpret_cast<cpx<double>*>(L.spare), P, n * P, n * P, P, L.tw64,
+                            L.st);
+    }
+    std::swap(slot, L.spare);
+}
+
+void fft_inverse(double*& slot) {
+    Lib& L = lib();
+    const int64_t n = L.n, P = L.g.P;
+    {
+        Timed t(TC_FFT_Z);
+        launch_cols<double>(L.n, 1, reinterpret_cast<cpx<double>*>(slot),
+                            reinterpret_cast<cpx<double>*>(L.spare), n * P, P, P, n * P, L.tw64,
+                            L.st);
+    }
+    std::swap(slot, L.spare);
+    pass_y(slot, 1);
+    {
+        Timed t(TC_FFT_X);
+        const double norm = 1.0 / ((double)n * (double)n * (double)n);
+        launch_x_c2r<double>(L.n, slot, P, L.tw64, norm, L.st);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+static void require_ready() {
+    if (!lib().ready) throw std::runtime_error("hit3d_init has not been called");
+}
+
+static size_t slot_bytes() { return (size_t)lib().g.nc * sizeof(cpx<double>); }
+
+static double*& slot_ref(int64_t array_id, int64_t s) {
+    Lib& L = lib();
+    if (array_id == 1) {
+        if (s < 1 || s > L.nf) throw std::runtime_error("fields slot out of range");
+        return L.F[s];
+    }
+    if (array_id == 2) {
+        if (s < 0 || s > L.nw) throw std::runtime_error("wrk slot out of range");
+        return L.W[s];
+    }
+    if (array_id == 3) {
+        if (s < 1 || s > L.nf) throw std::runtime_error("rhs_old slot out of range");
+        return L.R[s];
+    }
+    throw std::runtime_error("unknown array id");
+}
+
+static void copy_slot(double* dst, const double* src) {
+    Lib& L = lib();
+    Timed t(TC_POINTWISE);
+    check(hipMemcpyAsync(dst, src, slot_bytes(), hipMemcpyDeviceToDevice, L.st), "hipMemcpyAsync");
+}
+
+static void shift(const double* src, double* dst, int sign) {
+    Lib& L = lib();
+    Timed t(TC_POINTWISE);
+    launch_shift(L.g, src, dst, sign, L.ps_tab, L.st);
+}
+
+static void pmul(const double* a, const double* b, double* out) {
+    Lib& L = lib();
+    Timed t(TC_POINTWISE);
+    launch_phys_mul(L.g, a, b, out, L.st);
+}
+
+static void ps_acc(const double* Ps, double* rhs, int axis) {
+    Lib& L = lib();
+    Timed t(TC_POINTWISE);
+    launch_ps_accum(L.g, Ps, rhs, axis, L.st);
+}
+
+static void free_all() {
+    Lib& L = lib();
+    (void)hipDeviceSynchronize();
+    for (auto* p : L.F) if (p) (void)hipFree(p);
+    for (auto* p : L.W) if (p) (void)hipFree(p);
+    for (auto* p : L.R) if (p) (void)hipFree(p);
+    for (auto* p : L.W32) if (p) (void)hipFree(p);
+    if (L.spare) (void)hipFree(L.spare);
+    if (L.tw64) (void)hipFree(L.tw64);
+    if (L.tw32) (void)hipFree(L.tw32);
+    if (L.ps_tab) (void)hipFree(L.ps_tab);
+    if (L.dshell) (void)hipFree(L.dshell);
+    if (L.dpart) (void)hipFree(L.dpart);
+    if (L.dscal) (void)hipFree(L.dscal);
+    if (L.hpin) (void)hipHostFree(L.hpin);
+    if (L.st) (void)hipStreamDestroy(L.st);
+    for (auto& e : L.evs) {
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    std::string keep = L.err;
+    L = Lib();
+    L.err = keep;
+}
+
+static double* dalloc(size_t bytes) {
+    void* p = nullptr;
+    check(hipMalloc(&p, bytes), "hipMalloc");
+    check(hipMemset(p, 0, bytes), "hipMemset");
+    return static_cast<double*>(p);
+}
+
+// ---------------------------------------------------------------------------
+// reference subroutines
+// ---------------------------------------------------------------------------
+static void do_rhs_scalars(bool int_scalars) {
+    Lib& L = lib();
+    auto& F = L.F;
+    auto& W = L.W;
+    const Geo& g = L.g;
+    if (L.nles > 0) throw std::runtime_error("LES scalars are not supported");
+    if (!int_scalars || L.ns == 0) {  // rhs_scalars.f90:32-39
+        for (int c = 1; c <= 3; ++c) {
+            copy_slot(W[c], F[c]);
+            fft_inverse(W[c]);
+        }
+        return;
+    }
+    if (L.dealias == 0) {  // rhs_scalars.f90:48-134
+        for (int c = 1; c <= 3; ++c) {
+            copy_slot(W[c], F[c]);
+            fft_inverse(W[c]);
+        }
+        for (int s = 1; s <= L.ns; ++s) {
+            copy_slot(W[0], F[3 + s]);
+            fft_inverse(W[0]);
+            for (int i = 1; i <= 3; ++i) {
+                pmul(W[0], W[i], W[s + 2 + i]);
+                fft_forward(W[s + 2 + i]);
+            }
+            const double* Q[3] = {W[s + 3], W[s + 4], W[s + 5]};
+            Timed t(TC_POINTWISE);
+            launch_scalar_rhs(g, Q, F[3 + s], W[3 + s], L.pe[s - 1], L.st);
+        }
+    } else {  // rhs_scalars.f90:144-271
+        for (int c = 1; c <= 3 + L.ns; ++c) {
+            shift(F[c], W[c], +1);
+            fft_inverse(W[c]);
+        }
+        const int n1 = L.ns + 4, n2 = L.ns + 5;
+        for (int c = 4; c <= 3 + L.ns; ++c) {
+            pmul(W[c], W[1], W[n1]);
+            pmul(W[c], W[2], W[n2]);
+            pmul(W[c], W[3], W[c]);
+            fft_forward(W[n1]);
+            fft_forward(W[n2]);
+            fft_forward(W[c]);
+            shift(W[n1], W[n1], -1);
+            shift(W[n2], W[n2], -1);
+            shift(W[c], W[c], -1);
+            const double* Q[3] = {W[n1], W[n2], W[c]};
+            Timed t(TC_POINTWISE);
+            launch_scalar_rhs_ps(g, Q, F[c], W[c], 0.0, false, L.st);
+        }
+        for (int c = 1; c <= 3; ++c) {
+            copy_slot(W[c], F[c]);
+            fft_inverse(W[c]);
+        }
+        for (int c = 4; c <= 3 + L.ns; ++c) {
+            copy_slot(W[0], F[c]);
+            fft_inverse(W[0]);
+            pmul(W[0], W[1], W[n1]);
+            pmul(W[0], W[2], W[n2]);
+            pmul(W[0], W[3], W[0]);
+            fft_forward(W[n1]);
+            fft_forward(W[n2]);
+            fft_forward(W[0]);
+            const double* Q[3] = {W[n1], W[n2], W[0]};
+            Timed t(TC_POINTWISE);
+            launch_scalar_rhs_ps(g, Q, F[c], W[c], L.pe[c - 4], true, L.st);
+        }
+    }
+    for (int s = 1; s <= L.ns; ++s) {  // rhs_scalars.f90:387-389
+        if (L.stype[s - 1] == 0) {
+            Timed t(TC_POINTWISE);
+            launch_sub(g, W[3 + s], F[1], L.st);
+        }
+    }
+}
+
+static double read_courant_max() {
+    Lib& L = lib();
+    check(hipMemcpyAsync(L.hpin, L.dscal, sizeof(double), hipMemcpyDeviceToHost, L.st),
+          "hipMemcpyAsync");
+    check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
+    uint64_t bits;
+    std::memcpy(&bits, L.hpin, sizeof(bits));
+    double m;
+    std::memcpy(&m, &bits, sizeof(m));
+    return m;
+}
+
+static void do_rhs_velocity() {
+    Lib& L = lib();
+    auto& F = L.F;
+    auto& W = L.W;
+    const Geo& g = L.g;
+    check(hipMemsetAsync(L.dscal, 0, sizeof(double), L.st), "hipMemsetAsync");
+    {
+        double* out[6] = {W[1], W[2], W[3], W[4], W[5], W[6]};
+        Timed t(TC_POINTWISE);
+        launch_products(g, W[1], W[2], W[3], out, reinterpret_cast<unsigned long long*>(L.dscal),
+                        L.st);
+    }
+    for (int c = 1; c <= 6; ++c) fft_forward(W[c]);
+    {
+        const double* P[6] = {W[1], W[2], W[3], W[4], W[5], W[6]};
+        const double* U[3] = {F[1], F[2], F[3]};
+        double* out[3] = {W[1], W[2], W[3]};
+        Timed t(TC_POINTWISE);
+        launch_rhs_assemble(g, P, U, out, L.nu, L.dealias == 1, L.st);
+    }
+    if (L.dealias == 1) {  // rhs_velocity.f90:242-408
+        shift(F[1], W[4], +1);
+        shift(F[2], W[5], +1);
+        fft_inverse(W[4]);
+        fft_inverse(W[5]);
+        pmul(W[4], W[4], W[6]);
+        fft_forward(W[6]);
+        shift(W[6], W[6], -1);
+        ps_acc(W[6], W[1], 1);
+        pmul(W[4], W[5], W[6]);
+        fft_forward(W[6]);
+        shift(W[6], W[6], -1);
+        ps_acc(W[6], W[1], 2);
+        ps_acc(W[6], W[2], 1);
+        pmul(W[5], W[5], W[6]);
+        fft_forward(W[6]);
+        shift(W[6], W[6], -1);
+        ps_acc(W[6], W[2], 2);
+        shift(F[3], W[6], +1);
+        fft_inverse(W[6]);
+        pmul(W[4], W[6], W[4]);
+        pmul(W[5], W[6], W[5]);
+        pmul(W[6], W[6], W[6]);
+        for (int c = 4; c <= 6; ++c) {
+            fft_forward(W[c]);
+            shift(W[c], W[c], -1);
+        }
+        ps_acc(W[4], W[1], 3);
+        ps_acc(W[5], W[2], 3);
+        ps_acc(W[4], W[3], 1);
+        ps_acc(W[5], W[3], 2);
+        ps_acc(W[6], W[3], 3);
+        Timed t(TC_POINTWISE);
+        for (int c = 1; c <= 3; ++c) launch_mask_ps(g, W[c], L.st);
+    }
+}
+
+static void do_force(double* energy) {
+    Lib& L = lib();
+    if (L.kfmax < 1) throw std::runtime_error("force_velocity needs kfmax >= 1");
+    Timed t(TC_REDUCE);
+    launch_shell_sums(L.g, L.F[1], L.F[2], L.F[3], 1, L.kfmax, L.dshell, L.st);
+    const double* f[3] = {L.F[1], L.F[2], L.F[3]};
+    double* r[3] = {L.W[1], L.W[2], L.W[3]};
+    launch_force_apply(L.g, f, r, L.dshell, L.kfmax, L.famp, L.dscal + 1, L.st);
+    if (energy) {
+        check(hipMemcpyAsync(L.hpin + 1, L.dscal + 1, sizeof(double), hipMemcpyDeviceToHost, L.st),
+              "hipMemcpyAsync");
+        check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
+        *energy = L.hpin[1];
+    }
+}
+
+static void do_advance(int which, bool first, double dt) {
+    Lib& L = lib();
+    int lo = 1, hi = 3;
+    if (which == 2) {
+        lo = 4;
+        hi = 3 + L.ns + L.nles;
+    } else if (which != 1) {
+        throw std::runtime_error("advance: which must be 1 (velocity) or 2 (scalars)");
+    }
+    Timed t(TC_POINTWISE);
+    for (int c = lo; c <= hi; ++c) launch_advance(L.g, L.F[c], L.W[c], L.R[c], dt, first, L.st);
+}
+
+static void do_pressure() {
+    Lib& L = lib();
+    Timed t(TC_POINTWISE);
+    launch_pressure(L.g, L.F[1], L.F[2], L.F[3], L.st);
+}
+
+}  // namespace hit3d
+
+using namespace hit3d;
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+template <typename Fn>
+static int guarded(Fn&& fn, bool device = true) {
+    try {
+        fn();
+        const hipError_t e = device ? hipGetLastError() : hipSuccess;
+        if (e != hipSuccess) throw HipError{std::string("kernel launch: ") + hipGetErrorString(e)};
+        return 0;
+    } catch (const HipError& e) {
+        lib().err = e.msg;
+        return 2;
+    } catch (const std::exception& e) {
+        lib().err = e.what();
+        return 1;
+    } catch (...) {
+        lib().err = "unknown error";
+        return 3;
+    }
+}
+
+extern "C" {
+
+int hit3d_init(const int64_t* nx, const int64_t* ny, const int64_t* nz_all,
+               const int64_t* n_scalars, const int64_t* n_les, const int64_t* dealias) {
+    return guarded([&] {
+        Lib& L = lib();
+        if (L.ready) free_all();
+        if (*nx != *ny || *nx != *nz_all)
+            throw std::runtime_error("only cubic grids nx = ny = nz_all are supported");
+        if (!supported_n((int)*nx))
+            throw std::runtime_error("grid size must be a power of two in [8, 1024]");
+        if (*n_scalars < 0 || *n_les < 0) throw std::runtime_error("negative field count");
+        if (*dealias != 0 && *dealias != 1) throw std::runtime_error("dealias must be 0 or 1");
+        L.n = (int)*nx;
+        L.g.n = L.n;
+        L.g.P = row_pitch(L.n);
+        L.g.nc = (int64_t)L.n * L.n * L.g.P;
+        L.ns = (int)*n_scalars;
+        L.nles = (int)*n_les;
+        L.nf = 3 + L.ns + L.nles;
+        L.nw = std::max(6, 3 + L.ns + L.nles + 2);
+        L.dealias = (int)*dealias;
+        L.kmax = L.dealias == 0 ? L.n / 3 : (int)std::floor((double)L.n / 3.0 * std::sqrt(2.0));
+        const double pi = 4.0 * std::atan(1.0);
+        L.dx = 2.0 * pi / (double)L.n;
+        check(hipStreamCreateWithFlags(&L.st, hipStreamNonBlocking), "hipStreamCreate");
+        const size_t sb = slot_bytes();
+        L.F.assign(L.nf + 1, nullptr);
+        L.R.assign(L.nf + 1, nullptr);
+        L.W.assign(L.nw + 1, nullptr);
+        for (int i = 1; i <= L.nf; ++i) {
+            L.F[i] = dalloc(sb);
+            L.R[i] = dalloc(sb);
+        }
+        for (int i = 0; i <= L.nw; ++i) L.W[i] = dalloc(sb);
+        L.spare = dalloc(sb);
+        // twiddles exp(-2 pi i j / len) for every power of two len <= kMaxLen
+        std::vector<cpx<double>> t64(2 * kMaxLen);
+        std::vector<cpx<float>> t32(2 * kMaxLen);
+        for (int len = 1; len <= kMaxLen; len *= 2)
+            for (int j = 0; j < len; ++j) {
+                const long double a = -2.0L * 3.14159265358979323846264338327950288L * j / len;
+                t64[len + j] = {(double)cosl(a), (double)sinl(a)};
+                t32[len + j] = {(float)cosl(a), (float)sinl(a)};
+            }
+        L.tw64 = reinterpret_cast<cpx<double>*>(dalloc(t64.size() * sizeof(cpx<double>)));
+        L.tw32 = reinterpret_cast<cpx<float>*>(dalloc(t32.size() * sizeof(cpx<float>)));
+        check(hipMemcpy(L.tw64, t64.data(), t64.size() * sizeof(cpx<double>), hipMemcpyHostToDevice),
+              "hipMemcpy");
+        check(hipMemcpy(L.tw32, t32.data(), t32.size() * sizeof(cpx<float>), hipMemcpyHostToDevice),
+              "hipMemcpy");
+        // phase-shift factors cos/sin(half*(kx+ky+kz)*dx), rhs_velocity.f90:248-249
+        const int ntab = 3 * L.n;
+        std::vector<double> tab(2 * ntab);
+        for (int t = 0; t < ntab; ++t) {
+            const double ks = (double)(t - L.n);
+            const double a = 0.5 * ks * L.dx;
+            tab[2 * t] = std::cos(a);
+            tab[2 * t + 1] = std::sin(a);
+        }
+        L.ps_tab = dalloc(tab.size() * sizeof(double));
+        check(hipMemcpy(L.ps_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice),
+              "hipMemcpy");
+        L.dshell = dalloc((size_t)(L.n + 2) * sizeof(double) * 2);
+        L.dpart = dalloc((size_t)L.n * sizeof(double));
+        L.dscal = dalloc(16 * sizeof(double));
+        check(hipHostMalloc((void**)&L.hpin, 16 * sizeof(double), hipHostMallocDefault),
+              "hipHostMalloc");
+        check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        L.nu = 0.0;
+        L.ready = true;
+    });
+}
+
+int hit3d_set_physics(const double* nu, const int64_t* flow_type, const int64_t* kfmax,
+                      const double* famp, const double* pe, const int64_t* scalar_type) {
+    return guarded([&] {
+        require_ready();
+        Lib& L = lib();
+        L.nu = *nu;
+        L.flow_type = (int)*flow_type;
+        L.kfmax = (int)*kfmax;
+        L.famp = *famp;
+        if (L.kfmax > L.kmax) throw std::runtime_error("kfmax exceeds kmax");
+        L.pe.assign(L.ns, 0.0);
+        L.stype.assign(L.ns, 0);
+        for (int s = 0; s < L.ns; ++s) {
+            if (!pe || !scalar_type) throw std::runtime_error("scalar parameters missing");
+            L.pe[s] = pe[s];
+            L.stype[s] = scalar_type[s];
+            if (L.stype[s] >= 100) throw std::runtime_error("reacting scalars are not supported");
+        }
+    });
+}
+
+int hit3d_finalize(void) {
+    return guarded([&] { free_all(); });
+}
+
+int hit3d_put(const int64_t* array_id, const int64_t* slot, const double* host) {
+    return guarded([&] {
+        require_ready();
+        Lib& L = lib();
+        double* d = slot_ref(*array_id, *slot);
+        const size_t w = (size_t)(L.n + 2) * sizeof(double);
+        check(hipMemcpy2DAsync(d, (size_t)L.g.P * 2 * sizeof(double), host, w, w,
+                               (size_t)L.n * L.n, hipMemcpyHostToDevice, L.st),
+              "hipMemcpy2DAsync");
+        check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
+    });
+}
+
+int hit3d_get(const int64_t* array_id, const int64_t* slot, double* host) {
+    return guarded([&] {
+        require_ready();
+        Lib& L = lib();
+        double* d = slot_ref(*array_id, *slot);
+        const size_t w = (size_t)(L.n + 2) * sizeof(double);
+        check(hipMemcpy2DAsync(host, w, d, (size_t)L.g.P * 2 * sizeof(double), w,
+                               (size_t)L.n * L.n, hipMemcpyDeviceToHost, L.st),
+              "hipMemcpy2DAsync");
+        check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
+    });
+}
+
+int hit3d_copy(const int64_t* dst_array, const int64_t* dst_slot, const int64_t* src_array,
+               const int64_t* src_slot) {
+    return guarded([&] {
+        require_ready();
+        double* d = slot_ref(*dst_array, *dst_slot);
+        double* s = slot_ref(*src_array, *src_slot);
+        if (d != s) copy_slot(d, s);
+    });
+}
+
+int hit3d_xfft3d(const int64_t* flag, const int64_t* n) {
+    return guarded([&] {
+        require_ready();
+        double*& s = slot_ref(2, *n);
+        if (*flag == 1) fft_forward(s);
+        else if (*flag == -1) fft_inverse(s);
+        else throw std::runtime_error("xFFT3d: flag must be +1 or -1");
+    });
+}
+
+int hit3d_xfft3d_fields(const int64_t* flag, const int64_t* n) {
+    return guarded([&] {
+        require_ready();
+        double*& s = slot_ref(1, *n);
+        if (*flag == 1) fft_forward(s);
+        else if (*flag == -1) fft_inverse(s);
+        else throw std::runtime_error("xFFT3d_fields: flag must be +1 or -1");
+    });
+}
+
+int hit3d_x_derivative(const int64_t* n, const int64_t* axis, const int64_t* nto) {
+    return guarded([&] {
+        require_ready();
+        if (*axis < 1 || *axis > 3) throw std::runtime_error("x_derivative: wrong value of axis");
+        Lib& L = lib();
+        double* a = slot_ref(2, *n);
+        double* b = slot_ref(2, *nto);
+        Timed t(TC_POINTWISE);
+        launch_derivative(L.g, a, b, (int)*axis, L.st);
+    });
+}
+
+int hit3d_dealias_all(void) {
+    return guarded([&] {
+        require_ready();
+        Lib& L = lib();
+        Timed t(TC_POINTWISE);
+        for (int c = 1; c <= 3 + L.ns; ++c) launch_truncate(L.g, L.F[c], L.kmax, L.st);
+    });
+}
+
+int hit3d_rhs_scalars(const int64_t* int_scalars) {
+    return guarded([&] {
+        require_ready();
+        do_rhs_scalars(*int_scalars != 0);
+    });
+}
+
+int hit3d_rhs_velocity(const double* dt, double* courant) {
+    return guarded([&] {
+        require_ready();
+        do_rhs_velocity();
+        const double m = read_courant_max();
+        if (courant) *courant = m * (*dt) / lib().dx;
+    });
+}
+
+int hit3d_force_velocity(double* energy) {
+    return guarded([&] {
+        require_ready();
+        do_force(energy);
+    });
+}
+
+int hit3d_advance(const int64_t* which, int64_t* first_order, const double* dt) {
+    return guarded([&] {
+        require_ready();
+        do_advance((int)*which, *first_order != 0, *dt);
+        *first_order = 0;
+    });
+}
+
+int hit3d_pressure(void) {
+    return guarded([&] {
+        require_ready();
+        do_pressure();
+    });
+}
+
+int hit3d_step(const double* dt, int64_t* fov, int64_t* fos, const int64_t* int_scalars,
+               const int64_t* precision, double* courant, double* energy) {
+    return guarded([&] {
+        require_ready();
+        Lib& L = lib();
+        const bool mixed = *precision == 1;
+        if (*precision != 0 && *precision != 1)
+            throw std::runtime_error("precision must be 0 (FP64) or 1 (mixed)");
+        const bool scal = (*int_scalars != 0) && L.ns > 0;
+        double cmax = 0.0;
+        if (!scal && fused_step_supported(mixed)) {
+            // rhs_scalars + rhs_velocity + force + update + pressure in one pipeline
+            fused_step(*dt, *fov != 0, mixed, &cmax);
+            *fov = 0;
+            if (energy && L.flow_type == 1) {
+                check(hipMemcpyAsync(L.hpin + 1, L.dscal + 1, sizeof(double),
+                                     hipMemcpyDeviceToHost, L.st),
+                      "hipMemcpyAsync");
+                check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
+                *energy = L.hpin[1];
+            }
+        } else {
+            if (mixed) throw std::runtime_error("mixed precision needs the fused path");
+            do_rhs_scalars(scal);
+            if (scal) {
+                do_advance(2, *fos != 0, *dt);
+                *fos = 0;
+            }
+            do_rhs_velocity();
+            cmax = read_courant_max();
+            if (L.flow_type == 1) do_force(energy);
+            do_advance(1, *fov != 0, *dt);
+            *fov = 0;
+            do_pressure();
+        }
+        if (courant) *courant = cmax * (*dt) / L.dx;
+    });
+}
+
+int hit3d_stats(double* e_spec, double* enstrophy) {
+    return guarded([&] {
+        require_ready();
+        Lib& L = lib();
+        {
+            Timed t(TC_REDUCE);
+            launch_shell_sums(L.g, L.F[1], L.F[2], L.F[3], 1, L.kmax, L.dshell, L.st);
+            launch_enstrophy(L.g, L.F[1], L.F[2], L.F[3], L.dpart, L.dscal + 2, L.st);
+        }
+        if (e_spec)
+            check(hipMemcpyAsync(e_spec, L.dshell, (size_t)L.kmax * sizeof(double),
+                                 hipMemcpyDeviceToHost, L.st),
+                  "hipMemcpyAsync");
+        check(hipMemcpyAsync(L.hpin + 2, L.dscal + 2, sizeof(double), hipMemcpyDeviceToHost, L.st),
+              "hipMemcpyAsync");
+        check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
+        if (enstrophy) *enstrophy = L.hpin[2];
+    });
+}
+
+int hit3d_init_velocity(const double* rn1, const int64_t* isp_type, const int64_t* ir_exp,
+                        const double* peak) {
+    return guarded([&] {
+        require_ready();
+        Lib& L = lib();
+        const int n = L.n;
+        const size_t hsz = (size_t)(n + 2) * n * n;
+        std::vector<double> host(hsz);
+        if (*isp_type == -1) {  // init_velocity.f90:252-293
+            for (int c = 0; c < 3; ++c) {
+                taylor_green_slot(n, c, host.data());
+                const int64_t a = 2, s = c + 1;
+                if (hit3d_put(&a, &s, host.data()) != 0) throw std::runtime_error(L.err);
+                fft_forward(L.W[c + 1]);
+                copy_slot(L.F[c + 1], L.W[c + 1]);
+            }
+            check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
+            return;
+        }
+        // init_velocity.f90:67-120: uniforms, Box-Muller, forward transforms
+        for (int c = 0; c < 3; ++c) {
+            gaussian_slot(n, *rn1, c, host.data());
+            const int64_t a = 2, s = c + 1;
+            if (hit3d_put(&a, &s, host.data()) != 0) throw std::runtime_error(L.err);
+            fft_forward(L.W[c + 1]);
+        }
+        // :123-149 curl on shells 1..kmax
+        launch_init_curl(L.g, L.W[1], L.W[2], L.W[3], L.F[1], L.F[2], L.F[3], L.kmax, L.st);
+        // :156-186 shell spectrum
+        launch_shell_sums(L.g, L.F[1], L.F[2], L.F[3], 1, L.kmax, L.dshell, L.st);
+        std::vector<double> e(L.kmax);
+        check(hipMemcpyAsync(e.data(), L.dshell, L.kmax * sizeof(double), hipMemcpyDeviceToHost,
+                             L.st),
+              "hipMemcpyAsync");
+        check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
+        // :192-237 target spectrum and per-shell ratio
+        std::vector<double> tgt = target_spectrum(L.kmax, (int)*isp_type, *peak, (double)*ir_exp);
+        std::vector<double> ratio(L.kmax + 1, -1.0);
+        for (int k = 1; k <= L.kmax; ++k)
+            if (e[k - 1] > 0.0) ratio[k] = std::sqrt(tgt[k - 1] / e[k - 1]);
+        check(hipMemcpyAsync(L.dshell, ratio.data(), ratio.size() * sizeof(double),
+                             hipMemcpyHostToDevice, L.st),
+              "hipMemcpyAsync");
+        for (int c = 1; c <= 3; ++c) launch_init_rescale(L.g, L.F[c], L.dshell, L.kmax, L.st);
+        check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
+    });
+}
+
+int hit3d_random_draws(const double* seed, const int64_t* skip, const int64_t* count,
+                       double* out) {
+    return guarded([&] {
+        if (*skip < 0 || *count < 0) throw std::runtime_error("negative draw count");
+        random_draws(*seed, *skip, *count, out);
+    }, false);
+}
+
+int hit3d_init_gaussian(const int64_t* nx, const double* rn1, const int64_t* c, double* out) {
+    return guarded([&] {
+        if (*c < 1 || *c > 3) throw std::runtime_error("component must be 1..3");
+        if (*nx < 2) throw std::runtime_error("bad grid size");
+        gaussian_slot((int)*nx, *rn1, (int)*c - 1, out);
+    }, false);
+}
+
+int hit3d_get_ialias(int8_t* out) {
+    return guarded([&] {
+        require_ready();
+        Lib& L = lib();
+        const size_t bytes = (size_t)L.n * L.n * (L.n + 2);
+        void* d = nullptr;
+        check(hipMalloc(&d, bytes), "hipMalloc");
+        launch_ialias(L.g, static_cast<int8_t*>(d), L.st);
+        check(hipMemcpyAsync(out, d, bytes, hipMemcpyDeviceToHost, L.st), "hipMemcpyAsync");
+        check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
+        check(hipFree(d), "hipFree");
+    });
+}
+
+int hit3d_timing_enable(const int64_t* on) {
+    return guarded([&] {
+        Lib& L = lib();
+        collect_timers();
+        L.timing = *on != 0;
+        for (int i = 0; i < TC_COUNT; ++i) {
+            L.tms[i] = 0.0;
+            L.tcnt[i] = 0;
+        }
+    });
+}
+
+int hit3d_timing_read(const int64_t* cap, double* ms, int64_t* count) {
+    return guarded([&] {
+        Lib& L = lib();
+        collect_timers();
+        for (int i = 0; i < TC_COUNT && i < *cap; ++i) {
+            ms[i] = L.tms[i];
+            count[i] = L.tcnt[i];
+        }
+    });
+}
+
+int hit3d_timing_names(char* buf, const int64_t* len) {
+    return guarded([&] {
+        std::strncpy(buf, kTimerNames, (size_t)*len);
+        if (*len > 0) buf[*len - 1] = '\0';
+    }, false);
+}
+
+int hit3d_synchronize(void) {
+    return guarded([&] {
+        check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        check(hipGetLastError(), "kernel launch");
+    });
+}
+
+int hit3d_last_error(char* buf, const int64_t* len) {
+    if (*len <= 0) return 0;
+    std::strncpy(buf, lib().err.c_str(), (size_t)*len);
+    buf[*len - 1] = '\0';
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,198 @@
+// Register/LDS FFT engine for CDNA4 (64-wide wavefronts).
+//
+// A length-N complex DFT is computed cooperatively by TPF = N/E threads.
+// Thread t owns E values in "stride order": slot s holds element t + s*TPF.
+// The transform is a radix-r Stockham sequence (r = min(E, N/p) at span p),
+// whose butterflies read a thread's own slots; outputs of every stage but
+// the last go through LDS and are read back in stride order.  The output of
+// the last stage is again in stride order, in the same slots, so a caller
+// loads and stores global memory with consecutive lanes on consecutive
+// columns (coalesced) and never needs a separate reordering pass.
+//
+// DIR = -1 is the forward transform (exp(-2 pi i jk/N), FFTW_FORWARD),
+// DIR = +1 the unnormalised inverse.  Twiddles come from a table
+// tw[j] = exp(-2 pi i j / N), j < N, built on the host in long double.
+#pragma once
+#include "common.hpp"
+
+namespace hit3d {
+
+template <typename T>
+struct Consts;
+template <>
+struct Consts<double> {
+    static constexpr double rsqrt2 = 0.70710678118654752440084436210484903928;
+};
+template <>
+struct Consts<float> {
+    static constexpr float rsqrt2 = 0.70710678118654752440084436210484903928f;
+};
+
+// multiply by -i (DIR=-1) or +i (DIR=+1)
+template <int DIR, typename T>
+__device__ __forceinline__ cpx<T> rot90(cpx<T> a) {
+    if constexpr (DIR < 0) return mul_mi(a);
+    else return mul_i(a);
+}
+
+// multiply by exp(DIR * i pi/4)
+template <int DIR, typename T>
+__device__ __forceinline__ cpx<T> rot45(cpx<T> a) {
+    const T h = Consts<T>::rsqrt2;
+    if constexpr (DIR < 0) return {(a.x + a.y) * h, (a.y - a.x) * h};
+    else return {(a.x - a.y) * h, (a.y + a.x) * h};
+}
+
+// multiply by exp(DIR * i 3pi/4)
+template <int DIR, typename T>
+__device__ __forceinline__ cpx<T> rot135(cpx<T> a) {
+    const T h = Consts<T>::rsqrt2;
+    if constexpr (DIR < 0) return {(a.y - a.x) * h, -(a.x + a.y) * h};
+    else return {-(a.x + a.y) * h, (a.x - a.y) * h};
+}
+
+// In-register DFT of R values v[o + m*S], m = 0..R-1 (natural order in/out).
+template <int R, int DIR, typename T, int S>
+struct DFT;
+
+template <int DIR, typename T, int S>
+struct DFT<1, DIR, T, S> {
+    __device__ static __forceinline__ void run(cpx<T>*) {}
+};
+
+template <int DIR, typename T, int S>
+struct DFT<2, DIR, T, S> {
+    __device__ static __forceinline__ void run(cpx<T>* v) {
+        cpx<T> a = v[0], b = v[S];
+        v[0] = a + b;
+        v[S] = a - b;
+    }
+};
+
+template <int DIR, typename T, int S>
+struct DFT<4, DIR, T, S> {
+    __device__ static __forceinline__ void run(cpx<T>* v) {
+        cpx<T> a0 = v[0], a1 = v[S], a2 = v[2 * S], a3 = v[3 * S];
+        cpx<T> t0 = a0 + a2, t1 = a0 - a2, t2 = a1 + a3, t3 = rot90<DIR>(a1 - a3);
+        v[0] = t0 + t2;
+        v[2 * S] = t0 - t2;
+        v[S] = t1 + t3;
+        v[3 * S] = t1 - t3;
+    }
+};
+
+template <int DIR, typename T, int S>
+struct DFT<8, DIR, T, S> {
+    __device__ static __forceinline__ void run(cpx<T>* v) {
+        // even/odd split into two DFT4 then combine with W8^k
+        cpx<T> e[4] = {v[0], v[2 * S], v[4 * S], v[6 * S]};
+        cpx<T> o[4] = {v[S], v[3 * S], v[5 * S], v[7 * S]};
+        DFT<4, DIR, T, 1>::run(e);
+        DFT<4, DIR, T, 1>::run(o);
+        cpx<T> w1 = rot45<DIR>(o[1]);
+        cpx<T> w2 = rot90<DIR>(o[2]);
+        cpx<T> w3 = rot135<DIR>(o[3]);
+        v[0] = e[0] + o[0];
+        v[4 * S] = e[0] - o[0];
+        v[S] = e[1] + w1;
+        v[5 * S] = e[1] - w1;
+        v[2 * S] = e[2] + w2;
+        v[6 * S] = e[2] - w2;
+        v[3 * S] = e[3] + w3;
+        v[7 * S] = e[3] - w3;
+    }
+};
+
+template <int DIR, typename T, int S>
+struct DFT<16, DIR, T, S> {
+    __device__ static __forceinline__ void run(cpx<T>* v) {
+        // 16 = 4 x 4: column DFT4s, twiddle W16^(a*b), row DFT4s
+        cpx<T> a[16];
+#pragma unroll
+        for (int m = 0; m < 16; ++m) a[m] = v[m * S];
+        // DFT4 over index m = a + 4b along b for each a (stride 4)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) DFT<4, DIR, T, 4>::run(a + i);
+        // twiddles exp(DIR 2 pi i a*k1/16), a = 0..3, k1 = 0..3
+        constexpr T c1 = (T)0.92387953251128675612818318939678828682;
+        constexpr T s1 = (T)0.38268343236508977172845998403039886676;
+        const T sg = (T)DIR;
+        // k1 = 1: a=1 -> pi/8, a=2 -> pi/4, a=3 -> 3pi/8
+        a[1 + 4] = cmul(a[1 + 4], mk(c1, sg * s1));
+        a[2 + 4] = rot45<DIR>(a[2 + 4]);
+        a[3 + 4] = cmul(a[3 + 4], mk(s1, sg * c1));
+        // k1 = 2: a=1 -> pi/4, a=2 -> pi/2, a=3 -> 3pi/4
+        a[1 + 8] = rot45<DIR>(a[1 + 8]);
+        a[2 + 8] = rot90<DIR>(a[2 + 8]);
+        a[3 + 8] = rot135<DIR>(a[3 + 8]);
+        // k1 = 3: a=1 -> 3pi/8, a=2 -> 3pi/4, a=3 -> 9pi/8
+        a[1 + 12] = cmul(a[1 + 12], mk(s1, sg * c1));
+        a[2 + 12] = rot135<DIR>(a[2 + 12]);
+        a[3 + 12] = cmul(a[3 + 12], mk(-c1, -sg * s1));
+        // DFT4 along a for each k1 (contiguous groups of 4)
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) DFT<4, DIR, T, 1>::run(a + 4 * k1);
+        // output index k = k1 + 4*k2 sits at a[4*k1 + k2]
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1)
+#pragma unroll
+            for (int k2 = 0; k2 < 4; ++k2) v[(k1 + 4 * k2) * S] = a[4 * k1 + k2];
+    }
+};
+
+template <typename T>
+__device__ __forceinline__ cpx<T> ldtw(const cpx<T>* tw, int j) {
+    return tw[j];
+}
+
+// One Stockham stage at span Pspan, then recursion to the next.
+// v: the thread's E slots.  lds: this FFT's N-entry exchange buffer with
+// element j at lds[j*LS].  t: the thread's index within the FFT.
+template <int N, int E, int DIR, typename T, int LS, int Pspan>
+struct Stages {
+    static constexpr int R = (E < N / Pspan) ? E : N / Pspan;
+    static constexpr int TPF = N / E;
+    static constexpr int NB = E / R;  // butterflies per thread
+    __device__ static __forceinline__ void run(cpx<T> (&v)[E], int t, cpx<T>* lds,
+                                               const cpx<T>* __restrict__ tw) {
+#pragma unroll
+        for (int q = 0; q < NB; ++q) {
+            const int b = t + q * TPF;
+            const int k = b & (Pspan - 1);
+            if constexpr (Pspan > 1) {
+#pragma unroll
+                for (int m = 1; m < R; ++m) {
+                    cpx<T> w = ldtw(tw, m * k * (N / (Pspan * R)));
+                    if constexpr (DIR > 0) w.y = -w.y;
+                    v[q + m * NB] = cmul(v[q + m * NB], w);
+                }
+            }
+            DFT<R, DIR, T, NB>::run(v + q);
+        }
+        if constexpr (Pspan * R < N) {
+            __syncthreads();  // previous readers of lds are done
+#pragma unroll
+            for (int q = 0; q < NB; ++q) {
+                const int b = t + q * TPF;
+                const int k = b & (Pspan - 1);
+                const int base = (b - k) * R + k;
+#pragma unroll
+                for (int m = 0; m < R; ++m) lds[(base + m * Pspan) * LS] = v[q + m * NB];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int s = 0; s < E; ++s) v[s] = lds[(t + s * TPF) * LS];
+            Stages<N, E, DIR, T, LS, Pspan * R>::run(v, t, lds, tw);
+        }
+    }
+};
+
+// Full transform.  Every thread of the block must call this (it contains
+// block-wide barriers), also threads whose data is masked out.
+template <int N, int E, int DIR, typename T, int LS>
+__device__ __forceinline__ void block_fft(cpx<T> (&v)[E], int t, cpx<T>* lds,
+                                          const cpx<T>* __restrict__ tw) {
+    Stages<N, E, DIR, T, LS, 1>::run(v, t, lds, tw);
+}
+
+}  // namespace hit3d

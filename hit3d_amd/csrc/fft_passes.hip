@@ -1,0 +1,212 @@
+// Axis passes of the 3-D real FFT (forward: x R2C, y C2C, z C2C with the
+// (y,z) -> (kz,ky) transpose; inverse in reverse order), replacing the FFTW
+// plane/stick loops and the slab transpose of xFFT3d (x_fftw.f90:852-1085).
+#include <stdexcept>
+#include <string>
+
+#include "fft.hpp"
+#include "passes.hpp"
+
+namespace hit3d {
+
+bool supported_n(int n) {
+    for (int p = 8; p <= 1024; p *= 2)
+        if (n == p) return true;
+    return false;
+}
+
+// ---------------------------------------------------------------------------
+// column pass (y, and z with the transpose)
+// ---------------------------------------------------------------------------
+template <typename T>
+struct ColCfg {
+    // columns per block: 128-byte row segments in global memory
+    static constexpr int B = 128 / (int)sizeof(cpx<T>);
+};
+
+template <int N, typename T>
+struct ColE {
+    static constexpr int B = ColCfg<T>::B;
+    static constexpr int E = (N < 8) ? N : ((B * N / 8 <= 1024) ? 8 : 16);
+};
+
+template <int N, int E, int B, int DIR, typename T>
+__global__ __launch_bounds__(B* N / E) void k_cols(const cpx<T>* __restrict__ in, cpx<T>* out,
+                                                   int64_t a_in, int64_t a_out, int64_t s_in,
+                                                   int64_t s_out, int ncols,
+                                                   const cpx<T>* __restrict__ tw) {
+    constexpr int TPF = N / E;
+    __shared__ cpx<T> lds[N * B];
+    const int c = threadIdx.x % B;
+    const int t = threadIdx.x / B;
+    const int col = blockIdx.x * B + c;
+    const bool valid = col < ncols;
+    const int64_t a = blockIdx.y;
+    const cpx<T>* src = in + a * a_in + col;
+    cpx<T>* dst = out + a * a_out + col;
+    cpx<T> v[E];
+#pragma unroll
+    for (int s = 0; s < E; ++s)
+        v[s] = valid ? src[(int64_t)(t + s * TPF) * s_in] : cpx<T>{(T)0, (T)0};
+    block_fft<N, E, DIR, T, B>(v, t, lds + c, tw + N);
+    if (valid) {
+#pragma unroll
+        for (int s = 0; s < E; ++s) dst[(int64_t)(t + s * TPF) * s_out] = v[s];
+    }
+}
+
+template <int N, typename T>
+static void cols_n(int dir, const cpx<T>* in, cpx<T>* out, int64_t a_in, int64_t a_out,
+                   int64_t s_in, int64_t s_out, const cpx<T>* tw, hipStream_t s) {
+    constexpr int B = ColCfg<T>::B;
+    constexpr int E = ColE<N, T>::E;
+    const int ncols = N / 2 + 1;
+    dim3 grid((ncols + B - 1) / B, N);
+    dim3 block(B * N / E);
+    if (dir < 0)
+        hipLaunchKernelGGL((k_cols<N, E, B, -1, T>), grid, block, 0, s, in, out, a_in, a_out, s_in,
+                           s_out, ncols, tw);
+    else
+        hipLaunchKernelGGL((k_cols<N, E, B, 1, T>), grid, block, 0, s, in, out, a_in, a_out, s_in,
+                           s_out, ncols, tw);
+}
+
+// ---------------------------------------------------------------------------
+// row pass (x): real <-> half-spectrum through an N/2-point complex FFT
+// ---------------------------------------------------------------------------
+template <int N, typename T>
+struct RowCfg {
+    static constexpr int M = N / 2;
+    static constexpr int E = (M < 8) ? M : 8;
+    static constexpr int TPF = M / E;
+    static constexpr int ROWS = (256 / TPF < N * N) ? 256 / TPF : N * N;
+};
+
+template <int N, typename T>
+__global__ __launch_bounds__((RowCfg<N, T>::ROWS * RowCfg<N, T>::TPF)) void k_r2c(
+    T* data, int64_t P, const cpx<T>* __restrict__ tw) {
+    using C = RowCfg<N, T>;
+    constexpr int M = C::M, E = C::E, TPF = C::TPF;
+    __shared__ cpx<T> lds[C::ROWS][M + 1];
+    const int t = threadIdx.x % TPF;
+    const int r = threadIdx.x / TPF;
+    const int64_t row = (int64_t)blockIdx.x * C::ROWS + r;
+    cpx<T>* rowp = reinterpret_cast<cpx<T>*>(data) + row * P;
+    cpx<T> v[E];
+#pragma unroll
+    for (int s = 0; s < E; ++s) v[s] = rowp[t + s * TPF];
+    block_fft<M, E, -1, T, 1>(v, t, lds[r], tw + M);
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < E; ++s) lds[r][t + s * TPF] = v[s];
+    __syncthreads();
+    const cpx<T>* twN = tw + N;
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+        const int k = t + s * TPF;
+        cpx<T> zk = lds[r][k];
+        cpx<T> zm = conj(lds[r][(M - k) & (M - 1)]);
+        cpx<T> e = scale(zk + zm, (T)0.5);
+        cpx<T> o = scale(mul_mi(zk - zm), (T)0.5);
+        rowp[k] = e + cmul(twN[k], o);
+    }
+    if (t == 0) {
+        cpx<T> z0 = lds[r][0];
+        rowp[M] = cpx<T>{z0.x - z0.y, (T)0};
+    }
+}
+
+template <int N, typename T>
+__global__ __launch_bounds__((RowCfg<N, T>::ROWS * RowCfg<N, T>::TPF)) void k_c2r(
+    T* data, int64_t P, const cpx<T>* __restrict__ tw, T norm) {
+    using C = RowCfg<N, T>;
+    constexpr int M = C::M, E = C::E, TPF = C::TPF;
+    __shared__ cpx<T> lds[C::ROWS][M + 1];
+    const int t = threadIdx.x % TPF;
+    const int r = threadIdx.x / TPF;
+    const int64_t row = (int64_t)blockIdx.x * C::ROWS + r;
+    cpx<T>* rowp = reinterpret_cast<cpx<T>*>(data) + row * P;
+#pragma unroll
+    for (int s = 0; s < E; ++s) lds[r][t + s * TPF] = rowp[t + s * TPF];
+    if (t == 0) lds[r][M] = rowp[M];
+    __syncthreads();
+    const cpx<T>* twN = tw + N;
+    cpx<T> v[E];
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+        const int k = t + s * TPF;
+        cpx<T> xk = lds[r][k];
+        cpx<T> xm = lds[r][M - k];
+        if (k == 0) {  // c2r ignores Im of the kx=0 and kx=N/2 terms
+            xk.y = (T)0;
+            xm.y = (T)0;
+        }
+        xm = conj(xm);
+        // Z_k = (X_k + conj X_{M-k}) + i W_N^{-k} (X_k - conj X_{M-k})
+        v[s] = (xk + xm) + mul_i(cmulc(xk - xm, twN[k]));
+    }
+    block_fft<M, E, 1, T, 1>(v, t, lds[r], tw + M);
+#pragma unroll
+    for (int s = 0; s < E; ++s) rowp[t + s * TPF] = scale(v[s], norm);
+}
+
+template <int N, typename T>
+static void r2c_n(T* slot, int64_t P, const cpx<T>* tw, hipStream_t s) {
+    using C = RowCfg<N, T>;
+    const int64_t rows = (int64_t)N * N;
+    hipLaunchKernelGGL((k_r2c<N, T>), dim3(rows / C::ROWS), dim3(C::ROWS * C::TPF), 0, s, slot, P,
+                       tw);
+}
+
+template <int N, typename T>
+static void c2r_n(T* slot, int64_t P, const cpx<T>* tw, T norm, hipStream_t s) {
+    using C = RowCfg<N, T>;
+    const int64_t rows = (int64_t)N * N;
+    hipLaunchKernelGGL((k_c2r<N, T>), dim3(rows / C::ROWS), dim3(C::ROWS * C::TPF), 0, s, slot, P,
+                       tw, norm);
+}
+
+// ---------------------------------------------------------------------------
+// dispatch over the supported sizes
+// ---------------------------------------------------------------------------
+template <typename F>
+static void dispatch(int n, F&& f) {
+    switch (n) {
+        case 8: f(std::integral_constant<int, 8>{}); break;
+        case 16: f(std::integral_constant<int, 16>{}); break;
+        case 32: f(std::integral_constant<int, 32>{}); break;
+        case 64: f(std::integral_constant<int, 64>{}); break;
+        case 128: f(std::integral_constant<int, 128>{}); break;
+        case 256: f(std::integral_constant<int, 256>{}); break;
+        case 512: f(std::integral_constant<int, 512>{}); break;
+        case 1024: f(std::integral_constant<int, 1024>{}); break;
+        default: throw std::runtime_error("unsupported grid size " + std::to_string(n));
+    }
+}
+
+template <typename T>
+void launch_x_r2c(int n, T* slot, int64_t P, const cpx<T>* tw, hipStream_t s) {
+    dispatch(n, [&](auto NC) { r2c_n<decltype(NC)::value, T>(slot, P, tw, s); });
+}
+template <typename T>
+void launch_x_c2r(int n, T* slot, int64_t P, const cpx<T>* tw, T norm, hipStream_t s) {
+    dispatch(n, [&](auto NC) { c2r_n<decltype(NC)::value, T>(slot, P, tw, norm, s); });
+}
+template <typename T>
+void launch_cols(int n, int dir, const cpx<T>* in, cpx<T>* out, int64_t a_in, int64_t a_out,
+                 int64_t s_in, int64_t s_out, const cpx<T>* tw, hipStream_t s) {
+    dispatch(n, [&](auto NC) {
+        cols_n<decltype(NC)::value, T>(dir, in, out, a_in, a_out, s_in, s_out, tw, s);
+    });
+}
+
+template void launch_x_r2c<double>(int, double*, int64_t, const cpx<double>*, hipStream_t);
+template void launch_x_r2c<float>(int, float*, int64_t, const cpx<float>*, hipStream_t);
+template void launch_x_c2r<double>(int, double*, int64_t, const cpx<double>*, double, hipStream_t);
+template void launch_x_c2r<float>(int, float*, int64_t, const cpx<float>*, float, hipStream_t);
+template void launch_cols<double>(int, int, const cpx<double>*, cpx<double>*, int64_t, int64_t,
+                                  int64_t, int64_t, const cpx<double>*, hipStream_t);
+template void launch_cols<float>(int, int, const cpx<float>*, cpx<float>*, int64_t, int64_t,
+                                 int64_t, int64_t, const cpx<float>*, hipStream_t);
+
+}  // namespace hit3d

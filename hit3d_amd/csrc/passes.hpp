@@ -1,0 +1,30 @@
+// Host-side launchers of the FFT axis passes (fft_passes.hip).
+#pragma once
+#include "common.hpp"
+
+namespace hit3d {
+
+// Twiddle tables: for every power of two L <= kMaxLen the entries
+// exp(-2 pi i j / L), j < L, stored at offset L of one buffer.
+constexpr int kMaxLen = 4096;
+
+bool supported_n(int n);
+
+// x pass, forward: each of the n*n rows (n reals, pitch 2P) -> n/2+1 complex
+// values in place (real-to-complex via a half-length complex transform).
+template <typename T>
+void launch_x_r2c(int n, T* slot, int64_t P, const cpx<T>* tw, hipStream_t s);
+
+// x pass, inverse: n/2+1 complex values per row -> n reals times `norm`.
+template <typename T>
+void launch_x_c2r(int n, T* slot, int64_t P, const cpx<T>* tw, T norm, hipStream_t s);
+
+// Column pass: n complex values per column at stride s_in (s_out), for the
+// n/2+1 columns kx of each of the n groups a; element e of column kx of
+// group a is read at in[a*a_in + kx + e*s_in] and written at
+// out[a*a_out + kx + e*s_out].  dir = -1 forward, +1 inverse.
+template <typename T>
+void launch_cols(int n, int dir, const cpx<T>* in, cpx<T>* out, int64_t a_in, int64_t a_out,
+                 int64_t s_in, int64_t s_out, const cpx<T>* tw, hipStream_t s);
+
+}  // namespace hit3d

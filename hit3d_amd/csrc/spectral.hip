@@ -1,0 +1,528 @@
+// Pointwise kernels and deterministic FP64 reductions of the time step
+// (unfused forms, one kernel per reference loop nest).
+#include "spectral.hpp"
+
+namespace hit3d {
+
+namespace {
+
+constexpr int TB = 256;
+
+__host__ int grid_for(int64_t total) {
+    int64_t b = (total + TB - 1) / TB;
+    if (b > 65536) b = 65536;
+    return (int)(b < 1 ? 1 : b);
+}
+
+struct SIdx {
+    int kx, kyv, kzv, j, k;
+    int64_t off;
+};
+
+// Decode a linear index over the valid spectral entries (kx <= n/2).
+__device__ __forceinline__ SIdx sdecode(const Geo& g, int64_t idx) {
+    const int H = g.n / 2 + 1;
+    SIdx s;
+    s.kx = (int)(idx % H);
+    const int64_t r = idx / H;
+    s.j = (int)(r % g.n);
+    s.k = (int)(r / g.n);
+    s.kzv = wavenum(s.j, g.n);
+    s.kyv = wavenum(s.k, g.n);
+    s.off = r * g.P + s.kx;
+    return s;
+}
+
+__device__ __forceinline__ int64_t nspec(const Geo& g) { return (int64_t)g.n * g.n * (g.n / 2 + 1); }
+
+__device__ __forceinline__ const cpx<double>* C(const double* p) {
+    return reinterpret_cast<const cpx<double>*>(p);
+}
+__device__ __forceinline__ cpx<double>* C(double* p) { return reinterpret_cast<cpx<double>*>(p); }
+
+#define GRID_STRIDE(i, total) \
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (total); i += (int64_t)gridDim.x * blockDim.x)
+
+// fixed-order block sum (wave shuffles, then wave partials in order)
+__device__ double block_sum(double v, double* sh) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[wv] = v;
+    __syncthreads();
+    double tot = 0.0;
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int i = 0; i < nw; ++i) tot += sh[i];
+    }
+    return tot;  // valid in thread 0
+}
+
+__device__ __forceinline__ int isqrt_floor(int64_t x) {
+    int64_t s = (int64_t)sqrt((double)x);
+    while (s * s > x) --s;
+    while ((s + 1) * (s + 1) <= x) ++s;
+    return (int)s;
+}
+
+// ---------------------------------------------------------------------------
+
+__global__ void k_derivative(Geo g, const double* in, double* out, int axis) {
+    GRID_STRIDE(i, nspec(g)) {
+        SIdx s = sdecode(g, i);
+        const double k = axis == 1 ? (double)s.kx : (axis == 2 ? (double)s.kyv : (double)s.kzv);
+        cpx<double> a = C(in)[s.off];
+        C(out)[s.off] = cpx<double>{-(k * a.y), k * a.x};
+    }
+}
+
+__global__ void k_truncate(Geo g, double* slot, int kmax) {
+    GRID_STRIDE(i, nspec(g)) {
+        SIdx s = sdecode(g, i);
+        if (s.kx > kmax || abs(s.kyv) > kmax || abs(s.kzv) > kmax) C(slot)[s.off] = {0.0, 0.0};
+    }
+}
+
+__global__ void k_pressure(Geo g, double* u, double* v, double* w) {
+    GRID_STRIDE(i, nspec(g)) {
+        SIdx s = sdecode(g, i);
+        const double kx = s.kx, ky = s.kyv, kz = s.kzv;
+        cpx<double> a = C(u)[s.off], b = C(v)[s.off], c = C(w)[s.off];
+        const double dre = kx * a.x + ky * b.x + kz * c.x;
+        const double dim = kx * a.y + ky * b.y + kz * c.y;
+        double lap = kx * kx + ky * ky + kz * kz;
+        if (lap == 0.0) lap = 9e20;
+        const double pr = dre / lap, pi = dim / lap;
+        C(u)[s.off] = {a.x - pr * kx, a.y - pi * kx};
+        C(v)[s.off] = {b.x - pr * ky, b.y - pi * ky};
+        C(w)[s.off] = {c.x - pr * kz, c.y - pi * kz};
+    }
+}
+
+__global__ void k_advance(Geo g, double* f, const double* r, double* ro, double dt, int first) {
+    GRID_STRIDE(i, nspec(g)) {
+        SIdx s = sdecode(g, i);
+        cpx<double> a = C(f)[s.off], b = C(r)[s.off];
+        if (first) {
+            C(ro)[s.off] = b;
+            C(f)[s.off] = {a.x + dt * b.x, a.y + dt * b.y};
+        } else {
+            cpx<double> o = C(ro)[s.off];
+            C(f)[s.off] = {a.x + dt * (1.5 * b.x - 0.5 * o.x), a.y + dt * (1.5 * b.y - 0.5 * o.y)};
+            C(ro)[s.off] = b;
+        }
+    }
+}
+
+struct Ptr6 {
+    double* p[6];
+};
+struct CPtr6 {
+    const double* p[6];
+};
+struct CPtr3 {
+    const double* p[3];
+};
+struct Ptr3 {
+    double* p[3];
+};
+
+__global__ void k_products(Geo g, const double* u, const double* v, const double* w, Ptr6 out,
+                           unsigned long long* maxbits) {
+    __shared__ double sh[TB / 64];
+    const int64_t total = (int64_t)g.n * g.n * g.n;
+    double m = 0.0;
+    GRID_STRIDE(i, total) {
+        const int64_t x = i % g.n, row = i / g.n;
+        const int64_t o = row * 2 * g.P + x;
+        const double a = u[o], b = v[o], c = w[o];
+        const double t = (fabs(a) + fabs(b)) + fabs(c);
+        m = t > m ? t : m;
+        const double p0 = a * a, p1 = a * b, p2 = a * c, p3 = b * b, p4 = b * c, p5 = c * c;
+        out.p[0][o] = p0;
+        out.p[1][o] = p1;
+        out.p[2][o] = p2;
+        out.p[3][o] = p3;
+        out.p[4][o] = p4;
+        out.p[5][o] = p5;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        double q = __shfl_down(m, o, 64);
+        m = q > m ? q : m;
+    }
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double mm = 0.0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) mm = sh[i] > mm ? sh[i] : mm;
+        atomicMax(maxbits, (unsigned long long)__double_as_longlong(mm));
+    }
+}
+
+__global__ void k_rhs_assemble(Geo g, CPtr6 P, CPtr3 U, Ptr3 out, double nu, int phase_shift) {
+    GRID_STRIDE(i, nspec(g)) {
+        SIdx s = sdecode(g, i);
+        const double kx = s.kx, ky = s.kyv, kz = s.kzv;
+        const int ia = alias_count(s.kx, s.kyv, s.kzv, g.n);
+        if (ia > (phase_shift ? 1 : 0)) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) C(out.p[c])[s.off] = {0.0, 0.0};
+            continue;
+        }
+        cpx<double> p[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) p[q] = C(P.p[q])[s.off];
+        cpx<double> u[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) u[c] = C(U.p[c])[s.off];
+        const double lap = -nu * (kx * kx + ky * ky + kz * kz);
+        const int idx[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+        const double h = phase_shift ? 0.5 : 1.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const cpx<double> a = p[idx[c][0]], b = p[idx[c][1]], d = p[idx[c][2]];
+            const double tre = -(kx * a.x + ky * b.x + kz * d.x);
+            const double tim = -(kx * a.y + ky * b.y + kz * d.y);
+            C(out.p[c])[s.off] = {-(h * tim) + lap * u[c].x, h * tre + lap * u[c].y};
+        }
+    }
+}
+
+__global__ void k_shift(Geo g, const double* src, double* dst, int sign, const double* tab) {
+    GRID_STRIDE(i, nspec(g)) {
+        SIdx s = sdecode(g, i);
+        const int t = s.kx + s.kyv + s.kzv + g.n;
+        const double c = tab[2 * t], sn = tab[2 * t + 1];
+        cpx<double> a = C(src)[s.off];
+        if (sign > 0)
+            C(dst)[s.off] = {a.x * c - a.y * sn, a.y * c + a.x * sn};
+        else
+            C(dst)[s.off] = {a.x * c + a.y * sn, a.y * c - a.x * sn};
+    }
+}
+
+__global__ void k_ps_accum(Geo g, const double* Ps, double* rhs, int axis) {
+    GRID_STRIDE(i, nspec(g)) {
+        SIdx s = sdecode(g, i);
+        const double k = axis == 1 ? (double)s.kx : (axis == 2 ? (double)s.kyv : (double)s.kzv);
+        cpx<double> a = C(Ps)[s.off], r = C(rhs)[s.off];
+        C(rhs)[s.off] = {r.x + 0.5 * k * a.y, r.y - 0.5 * k * a.x};
+    }
+}
+
+__global__ void k_mask_ps(Geo g, double* slot) {
+    GRID_STRIDE(i, nspec(g)) {
+        SIdx s = sdecode(g, i);
+        if (alias_count(s.kx, s.kyv, s.kzv, g.n) >= 2) C(slot)[s.off] = {0.0, 0.0};
+    }
+}
+
+__global__ void k_phys_mul(Geo g, const double* a, const double* b, double* out) {
+    const int64_t total = (int64_t)g.n * g.n * g.n;
+    GRID_STRIDE(i, total) {
+        const int64_t x = i % g.n, row = i / g.n;
+        const int64_t o = row * 2 * g.P + x;
+        out[o] = a[o] * b[o];
+    }
+}
+
+__global__ void k_scalar_rhs(Geo g, CPtr3 Q, const double* F, double* out, double pe) {
+    GRID_STRIDE(i, nspec(g)) {
+        SIdx s = sdecode(g, i);
+        if (alias_count(s.kx, s.kyv, s.kzv, g.n) > 0) {
+            C(out)[s.off] = {0.0, 0.0};
+            continue;
+        }
+        const double kx = s.kx, ky = s.kyv, kz = s.kzv;
+        const cpx<double> a = C(Q.p[0])[s.off], b = C(Q.p[1])[s.off], c = C(Q.p[2])[s.off];
+        const cpx<double> f = C(F)[s.off];
+        const double r1 = kx * a.y + ky * b.y + kz * c.y;
+        const double r2 = kx * a.x + ky * b.x + kz * c.x;
+        const double k2 = kx * kx + ky * ky + kz * kz;
+        C(out)[s.off] = {r1 - pe * k2 * f.x, -r2 - pe * k2 * f.y};
+    }
+}
+
+__global__ void k_scalar_rhs_ps(Geo g, CPtr3 Q, const double* F, double* out, double pe, int acc) {
+    GRID_STRIDE(i, nspec(g)) {
+        SIdx s = sdecode(g, i);
+        const bool keep = alias_count(s.kx, s.kyv, s.kzv, g.n) <= 1;
+        const double kx = s.kx, ky = s.kyv, kz = s.kzv;
+        const cpx<double> a = C(Q.p[0])[s.off], b = C(Q.p[1])[s.off], c = C(Q.p[2])[s.off];
+        const double r1 = kx * a.y + ky * b.y + kz * c.y;
+        const double r2 = kx * a.x + ky * b.x + kz * c.x;
+        if (!acc) {
+            C(out)[s.off] = keep ? cpx<double>{0.5 * r1, -0.5 * r2} : cpx<double>{0.0, 0.0};
+        } else if (keep) {
+            const cpx<double> f = C(F)[s.off], o = C(out)[s.off];
+            const double k2 = kx * kx + ky * ky + kz * kz;
+            C(out)[s.off] = {o.x + 0.5 * r1 - pe * k2 * f.x, o.y - 0.5 * r2 - pe * k2 * f.y};
+        }
+    }
+}
+
+__global__ void k_sub(Geo g, double* out, const double* a) {
+    GRID_STRIDE(i, nspec(g)) {
+        SIdx s = sdecode(g, i);
+        cpx<double> o = C(out)[s.off], b = C(a)[s.off];
+        C(out)[s.off] = {o.x - b.x, o.y - b.y};
+    }
+}
+
+__global__ void k_shell_sums(Geo g, const double* u, const double* v, const double* w, int s1,
+                             double fac, double* out) {
+    __shared__ double sh[TB / 64];
+    const int s = s1 + blockIdx.x;
+    const int side = 2 * s + 1;
+    const int64_t lo_k2 = (int64_t)s * s - s + 1, hi_k2 = (int64_t)s * s + s;
+    const int half = g.n / 2;
+    double acc = 0.0;
+    for (int p = threadIdx.x; p < side * side; p += blockDim.x) {
+        const int ky = p / side - s, kz = p % side - s;
+        if (ky <= -half || ky > half || kz <= -half || kz > half) continue;
+        const int64_t q = (int64_t)ky * ky + (int64_t)kz * kz;
+        if (q > hi_k2) continue;
+        const int64_t lo2 = lo_k2 - q;
+        int kx0 = 0;
+        if (lo2 > 0) {
+            kx0 = isqrt_floor(lo2);
+            if ((int64_t)kx0 * kx0 < lo2) ++kx0;
+        }
+        int kx1 = isqrt_floor(hi_k2 - q);
+        if (kx1 > half - 1) kx1 = half - 1;
+        const int jy = ky < 0 ? ky + g.n : ky, jz = kz < 0 ? kz + g.n : kz;
+        const int64_t rowoff = ((int64_t)jy * g.n + jz) * g.P;
+        for (int kx = kx0; kx <= kx1; ++kx) {
+            const cpx<double> a = C(u)[rowoff + kx], b = C(v)[rowoff + kx], c = C(w)[rowoff + kx];
+            double e1 = fac * (a.x * a.x + b.x * b.x + c.x * c.x);
+            double e2 = fac * (a.y * a.y + b.y * b.y + c.y * c.y);
+            if (kx == 0) {
+                e1 *= 0.5;
+                e2 *= 0.5;
+            }
+            acc += e1;
+            acc += e2;
+        }
+    }
+    const double tot = block_sum(acc, sh);
+    if (threadIdx.x == 0) out[blockIdx.x] = tot;
+}
+
+__global__ void k_force_apply(Geo g, CPtr3 f, Ptr3 r, const double* e, int kfmax, double famp,
+                              double* energy_out) {
+    __shared__ double fac_sh;
+    if (threadIdx.x == 0) {
+        double en = 0.0;
+        for (int i = 0; i < kfmax; ++i) en += e[i];
+        fac_sh = famp / en;
+        *energy_out = en;
+    }
+    __syncthreads();
+    const double fac = fac_sh;
+    const int side = 2 * kfmax + 1;
+    const int cnt = side * side * (kfmax + 1);
+    const int half = g.n / 2;
+    for (int p = threadIdx.x; p < cnt; p += blockDim.x) {
+        const int kx = p % (kfmax + 1);
+        const int q = p / (kfmax + 1);
+        const int ky = q / side - kfmax, kz = q % side - kfmax;
+        if (kx >= half || ky <= -half || ky > half || kz <= -half || kz > half) continue;
+        const int sh = shell_of(kx * kx + ky * ky + kz * kz);
+        if (sh <= 0 || sh > kfmax) continue;
+        const int jy = ky < 0 ? ky + g.n : ky, jz = kz < 0 ? kz + g.n : kz;
+        const int64_t off = ((int64_t)jy * g.n + jz) * g.P + kx;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const cpx<double> a = C(f.p[c])[off];
+            cpx<double> b = C(r.p[c])[off];
+            C(r.p[c])[off] = {b.x + fac * a.x, b.y + fac * a.y};
+        }
+    }
+}
+
+__global__ void k_enstrophy_plane(Geo g, const double* u, const double* v, const double* w,
+                                  double* partial) {
+    __shared__ double sh[TB / 64];
+    const int k = blockIdx.x;
+    const int H = g.n / 2 + 1;
+    const double ky = wavenum(k, g.n);
+    double acc = 0.0;
+    for (int p = threadIdx.x; p < g.n * H; p += blockDim.x) {
+        const int kx = p % H, j = p / H;
+        const double kz = wavenum(j, g.n), kxd = kx;
+        const int64_t off = ((int64_t)k * g.n + j) * g.P + kx;
+        const cpx<double> a = C(u)[off], b = C(v)[off], c = C(w)[off];
+        // |k x U|^2 (the factor i does not change the modulus)
+        const double x1r = ky * c.x - kz * b.x, x1i = ky * c.y - kz * b.y;
+        const double x2r = kz * a.x - kxd * c.x, x2i = kz * a.y - kxd * c.y;
+        const double x3r = kxd * b.x - ky * a.x, x3i = kxd * b.y - ky * a.y;
+        double t = x1r * x1r + x1i * x1i + x2r * x2r + x2i * x2i + x3r * x3r + x3i * x3i;
+        if (kx != 0 && kx != g.n / 2) t *= 2.0;
+        acc += t;
+    }
+    const double tot = block_sum(acc, sh);
+    if (threadIdx.x == 0) partial[k] = tot;
+}
+
+__global__ void k_sum_serial(const double* partial, int cnt, double scale, double* out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double s = 0.0;
+        for (int i = 0; i < cnt; ++i) s += partial[i];
+        *out = s * scale;
+    }
+}
+
+__global__ void k_ialias(Geo g, int8_t* out) {
+    const int64_t total = (int64_t)g.n * g.n * (g.n + 2);
+    GRID_STRIDE(i, total) {
+        const int ii = (int)(i % (g.n + 2));
+        const int64_t r = i / (g.n + 2);
+        const int j = (int)(r % g.n), k = (int)(r / g.n);
+        out[i] = (int8_t)alias_count(ii / 2, wavenum(k, g.n), wavenum(j, g.n), g.n);
+    }
+}
+
+__global__ void k_init_curl(Geo g, const double* w1, const double* w2, const double* w3,
+                            double* f1, double* f2, double* f3, int kmax) {
+    GRID_STRIDE(i, nspec(g)) {
+        SIdx s = sdecode(g, i);
+        const int sh = shell_of(s.kx * s.kx + s.kyv * s.kyv + s.kzv * s.kzv);
+        if (sh > 0 && sh <= kmax) {
+            const double kx = s.kx, ky = s.kyv, kz = s.kzv;
+            const cpx<double> a = C(w1)[s.off], b = C(w2)[s.off], c = C(w3)[s.off];
+            C(f1)[s.off] = {-(ky * c.y - kz * b.y), ky * c.x - kz * b.x};
+            C(f2)[s.off] = {-(kz * a.y - kx * c.y), kz * a.x - kx * c.x};
+            C(f3)[s.off] = {-(kx * b.y - ky * a.y), kx * b.x - ky * a.x};
+        } else {
+            C(f1)[s.off] = {0.0, 0.0};
+            C(f2)[s.off] = {0.0, 0.0};
+            C(f3)[s.off] = {0.0, 0.0};
+        }
+    }
+}
+
+__global__ void k_init_rescale(Geo g, double* f, const double* ratio, int kmax) {
+    GRID_STRIDE(i, nspec(g)) {
+        SIdx s = sdecode(g, i);
+        const int sh = shell_of(s.kx * s.kx + s.kyv * s.kyv + s.kzv * s.kzv);
+        cpx<double> a = C(f)[s.off];
+        if (sh > 0 && sh <= kmax && ratio[sh] >= 0.0)
+            C(f)[s.off] = {a.x * ratio[sh], a.y * ratio[sh]};
+        else
+            C(f)[s.off] = {0.0, 0.0};
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static int64_t nspec_h(const Geo& g) { return (int64_t)g.n * g.n * (g.n / 2 + 1); }
+static int64_t nphys_h(const Geo& g) { return (int64_t)g.n * g.n * g.n; }
+
+void launch_derivative(const Geo& g, const double* in, double* out, int axis, hipStream_t s) {
+    hipLaunchKernelGGL(k_derivative, dim3(grid_for(nspec_h(g))), dim3(TB), 0, s, g, in, out, axis);
+}
+void launch_truncate(const Geo& g, double* slot, int kmax, hipStream_t s) {
+    hipLaunchKernelGGL(k_truncate, dim3(grid_for(nspec_h(g))), dim3(TB), 0, s, g, slot, kmax);
+}
+void launch_pressure(const Geo& g, double* u, double* v, double* w, hipStream_t s) {
+    hipLaunchKernelGGL(k_pressure, dim3(grid_for(nspec_h(g))), dim3(TB), 0, s, g, u, v, w);
+}
+void launch_advance(const Geo& g, double* f, const double* r, double* ro, double dt, bool first,
+                    hipStream_t s) {
+    hipLaunchKernelGGL(k_advance, dim3(grid_for(nspec_h(g))), dim3(TB), 0, s, g, f, r, ro, dt,
+                       first ? 1 : 0);
+}
+void launch_products(const Geo& g, const double* u, const double* v, const double* w,
+                     double* const out[6], unsigned long long* maxbits, hipStream_t s) {
+    Ptr6 o;
+    for (int i = 0; i < 6; ++i) o.p[i] = out[i];
+    hipLaunchKernelGGL(k_products, dim3(grid_for(nphys_h(g))), dim3(TB), 0, s, g, u, v, w, o,
+                       maxbits);
+}
+void launch_rhs_assemble(const Geo& g, const double* const P[6], const double* const U[3],
+                         double* const out[3], double nu, bool phase_shift, hipStream_t s) {
+    CPtr6 p;
+    CPtr3 u;
+    Ptr3 o;
+    for (int i = 0; i < 6; ++i) p.p[i] = P[i];
+    for (int i = 0; i < 3; ++i) {
+        u.p[i] = U[i];
+        o.p[i] = out[i];
+    }
+    hipLaunchKernelGGL(k_rhs_assemble, dim3(grid_for(nspec_h(g))), dim3(TB), 0, s, g, p, u, o, nu,
+                       phase_shift ? 1 : 0);
+}
+void launch_shift(const Geo& g, const double* src, double* dst, int sign, const double* tab,
+                  hipStream_t s) {
+    hipLaunchKernelGGL(k_shift, dim3(grid_for(nspec_h(g))), dim3(TB), 0, s, g, src, dst, sign, tab);
+}
+void launch_ps_accum(const Geo& g, const double* Ps, double* rhs, int axis, hipStream_t s) {
+    hipLaunchKernelGGL(k_ps_accum, dim3(grid_for(nspec_h(g))), dim3(TB), 0, s, g, Ps, rhs, axis);
+}
+void launch_mask_ps(const Geo& g, double* slot, hipStream_t s) {
+    hipLaunchKernelGGL(k_mask_ps, dim3(grid_for(nspec_h(g))), dim3(TB), 0, s, g, slot);
+}
+void launch_phys_mul(const Geo& g, const double* a, const double* b, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_phys_mul, dim3(grid_for(nphys_h(g))), dim3(TB), 0, s, g, a, b, out);
+}
+void launch_scalar_rhs(const Geo& g, const double* const Q[3], const double* F, double* out,
+                       double pe, hipStream_t s) {
+    CPtr3 q;
+    for (int i = 0; i < 3; ++i) q.p[i] = Q[i];
+    hipLaunchKernelGGL(k_scalar_rhs, dim3(grid_for(nspec_h(g))), dim3(TB), 0, s, g, q, F, out, pe);
+}
+void launch_scalar_rhs_ps(const Geo& g, const double* const Q[3], const double* F, double* out,
+                          double pe, bool accumulate, hipStream_t s) {
+    CPtr3 q;
+    for (int i = 0; i < 3; ++i) q.p[i] = Q[i];
+    hipLaunchKernelGGL(k_scalar_rhs_ps, dim3(grid_for(nspec_h(g))), dim3(TB), 0, s, g, q, F, out,
+                       pe, accumulate ? 1 : 0);
+}
+void launch_sub(const Geo& g, double* out, const double* a, hipStream_t s) {
+    hipLaunchKernelGGL(k_sub, dim3(grid_for(nspec_h(g))), dim3(TB), 0, s, g, out, a);
+}
+void launch_shell_sums(const Geo& g, const double* u, const double* v, const double* w, int s1,
+                       int ns, double* out, hipStream_t s) {
+    if (ns <= 0) return;
+    const double n3 = (double)g.n * (double)g.n * (double)g.n;
+    const double fac = 1.0 / (n3 * n3);
+    hipLaunchKernelGGL(k_shell_sums, dim3(ns), dim3(TB), 0, s, g, u, v, w, s1, fac, out);
+}
+void launch_force_apply(const Geo& g, const double* const f[3], double* const r[3],
+                        const double* e, int kfmax, double famp, double* energy_out,
+                        hipStream_t s) {
+    CPtr3 ff;
+    Ptr3 rr;
+    for (int i = 0; i < 3; ++i) {
+        ff.p[i] = f[i];
+        rr.p[i] = r[i];
+    }
+    hipLaunchKernelGGL(k_force_apply, dim3(1), dim3(TB), 0, s, g, ff, rr, e, kfmax, famp,
+                       energy_out);
+}
+void launch_enstrophy(const Geo& g, const double* u, const double* v, const double* w,
+                      double* partial, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_enstrophy_plane, dim3(g.n), dim3(TB), 0, s, g, u, v, w, partial);
+    const double n3 = (double)g.n * (double)g.n * (double)g.n;
+    hipLaunchKernelGGL(k_sum_serial, dim3(1), dim3(64), 0, s, partial, g.n, 1.0 / (n3 * n3), out);
+}
+void launch_ialias(const Geo& g, int8_t* out, hipStream_t s) {
+    const int64_t total = (int64_t)g.n * g.n * (g.n + 2);
+    hipLaunchKernelGGL(k_ialias, dim3(grid_for(total)), dim3(TB), 0, s, g, out);
+}
+void launch_init_curl(const Geo& g, const double* w1, const double* w2, const double* w3,
+                      double* f1, double* f2, double* f3, int kmax, hipStream_t s) {
+    hipLaunchKernelGGL(k_init_curl, dim3(grid_for(nspec_h(g))), dim3(TB), 0, s, g, w1, w2, w3, f1,
+                       f2, f3, kmax);
+}
+void launch_init_rescale(const Geo& g, double* f, const double* ratio, int kmax, hipStream_t s) {
+    hipLaunchKernelGGL(k_init_rescale, dim3(grid_for(nspec_h(g))), dim3(TB), 0, s, g, f, ratio,
+                       kmax);
+}
+
+}  // namespace hit3d

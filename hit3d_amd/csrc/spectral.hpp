@@ -1,0 +1,76 @@
+// Pointwise spectral/physical kernels and FP64 reductions of the time step.
+#pragma once
+#include "common.hpp"
+
+namespace hit3d {
+
+struct Geo {
+    int n;        // grid points per direction
+    int64_t P;    // row pitch in complex entries
+    int64_t nc;   // complex entries per slot = n*n*P
+};
+
+// Physical-space per-row views use 2P reals per row.
+
+// x_fftw.f90:637-682: out = i * k_axis * in (axis 1=x, 2=y, 3=z).
+void launch_derivative(const Geo& g, const double* in, double* out, int axis, hipStream_t s);
+// dealias_all.f90:19-32: zero entries with any |k_i| > kmax.
+void launch_truncate(const Geo& g, double* slot, int kmax, hipStream_t s);
+// pressure.f90:13-47.
+void launch_pressure(const Geo& g, double* u, double* v, double* w, hipStream_t s);
+// main.f90:137-146,156-164: f += dt*r (first) or f += dt*(1.5 r - 0.5 ro); ro = r.
+void launch_advance(const Geo& g, double* f, const double* r, double* ro, double dt, bool first,
+                    hipStream_t s);
+// rhs_velocity.f90:31-32,56-72: max(|u|+|v|+|w|) into *maxbits (as ordered
+// bits of a non-negative double) and the six products u_i u_j (written to
+// out[0..5], which may alias u, v, w).
+void launch_products(const Geo& g, const double* u, const double* v, const double* w,
+                     double* const out[6], unsigned long long* maxbits, hipStream_t s);
+// rhs_velocity.f90:85-142 (half = false) or :183-232 (half = true):
+// r_i = mask * (c * i * t_i - nu k^2 U_i), t_i = -k_j P_ij; c = 1 or 1/2;
+// mask: alias_count == 0 (2/3 rule) or <= 1 (phase shift).
+void launch_rhs_assemble(const Geo& g, const double* const P[6], const double* const U[3],
+                         double* const out[3], double nu, bool phase_shift, hipStream_t s);
+// dst = src * exp(sign * i * theta), theta = (kx+ky+kz)*dx/2 (rhs_velocity.f90:248-257,276-278);
+// tab holds cos/sin per kx+ky+kz.
+void launch_shift(const Geo& g, const double* src, double* dst, int sign, const double* tab,
+                  hipStream_t s);
+// rhs += -1/2 i k_axis Ps (Ps already shifted back), rhs_velocity.f90:282-336,389-398.
+void launch_ps_accum(const Geo& g, const double* Ps, double* rhs, int axis, hipStream_t s);
+// zero entries with alias_count >= 2 (rhs_velocity.f90:400-404).
+void launch_mask_ps(const Geo& g, double* slot, hipStream_t s);
+// physical out = a * b (x < n only).
+void launch_phys_mul(const Geo& g, const double* a, const double* b, double* out, hipStream_t s);
+// rhs_scalars.f90:88-121: r = mask0 * (-i k.Q - pe k^2 F); Q[3] = (u phi, v phi, w phi)^.
+void launch_scalar_rhs(const Geo& g, const double* const Q[3], const double* F, double* out,
+                       double pe, hipStream_t s);
+// rhs_scalars.f90:196-219 (phase-shifted half, Q already shifted back):
+// out = mask1 * (-1/2 i k.Q)   and :253-269 (unshifted half):
+// out += mask1 * (-1/2 i k.Q - pe k^2 F).
+void launch_scalar_rhs_ps(const Geo& g, const double* const Q[3], const double* F, double* out,
+                          double pe, bool accumulate, hipStream_t s);
+// out -= a over all entries (rhs_scalars.f90:387-389).
+void launch_sub(const Geo& g, double* out, const double* a, hipStream_t s);
+// Deterministic FP64 shell sums of (|u|^2+|v|^2+|w|^2)/n^6 over entries
+// kx < n/2, kx = 0 halved, for shells s1..s1+ns-1 (m_stats.f90:207-228).
+void launch_shell_sums(const Geo& g, const double* u, const double* v, const double* w, int s1,
+                       int ns, double* out, hipStream_t s);
+// m_force.f90:153-173: energy = sum(e[0..kfmax)), r += famp/energy * f on
+// forced entries (0 < shell <= kfmax, kx < n/2).  *energy_out on device.
+void launch_force_apply(const Geo& g, const double* const f[3], double* const r[3],
+                        const double* e, int kfmax, double famp, double* energy_out,
+                        hipStream_t s);
+// Enstrophy <|omega|^2> by Parseval over the half spectrum, deterministic.
+// partial must hold n doubles.
+void launch_enstrophy(const Geo& g, const double* u, const double* v, const double* w,
+                      double* partial, double* out, hipStream_t s);
+// ialias in the reference layout (n, n, n+2) int8 [ky][kz][i].
+void launch_ialias(const Geo& g, int8_t* out, hipStream_t s);
+// Init curl with the FP64 shell mask (init_velocity.f90:123-147).
+void launch_init_curl(const Geo& g, const double* w1, const double* w2, const double* w3,
+                      double* f1, double* f2, double* f3, int kmax, hipStream_t s);
+// init_velocity.f90:224-237: multiply by ratio[shell] (shell from the float32
+// rule), zero elsewhere; ratio has kmax+1 entries (index 0 unused, <0 = zero).
+void launch_init_rescale(const Geo& g, double* f, const double* ratio, int kmax, hipStream_t s);
+
+}  // namespace hit3d

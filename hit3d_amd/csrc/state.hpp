@@ -1,0 +1,91 @@
+// Library state: the device-resident arrays of the reference's modules and
+// the tables the kernels share.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "spectral.hpp"
+
+namespace hit3d {
+
+enum TimerClass {
+    TC_FFT_X = 0,
+    TC_FFT_Y,
+    TC_FFT_Z,
+    TC_POINTWISE,
+    TC_REDUCE,
+    TC_STEP_INV_Z,
+    TC_STEP_INV_Y,
+    TC_STEP_X,
+    TC_STEP_FWD_Y,
+    TC_STEP_FWD_Z,
+    TC_COUNT
+};
+
+struct HipError {
+    std::string msg;
+};
+
+void check(hipError_t e, const char* what);
+
+struct Lib {
+    bool ready = false;
+    int n = 0;
+    Geo g{};
+    int ns = 0, nles = 0, nf = 0, nw = 0;
+    int dealias = 0, kmax = 0;
+    double nu = 0.0, famp = 0.0, dx = 0.0;
+    int flow_type = 0, kfmax = 0;
+    std::vector<double> pe;
+    std::vector<int64_t> stype;
+
+    std::vector<double*> F;  // fields, index 1..nf (0 unused)
+    std::vector<double*> W;  // wrk, index 0..nw
+    std::vector<double*> R;  // rhs_old, index 1..nf
+    double* spare = nullptr;
+
+    cpx<double>* tw64 = nullptr;
+    cpx<float>* tw32 = nullptr;
+    double* ps_tab = nullptr;        // cos/sin(0.5*(kx+ky+kz)*dx), index (kx+ky+kz+n)
+    double* dshell = nullptr;        // shell sums (kmax+2 doubles)
+    double* dpart = nullptr;         // per-plane partial sums (n doubles)
+    double* dscal = nullptr;         // [0] courant max bits, [1] band energy, [2] enstrophy
+    double* hpin = nullptr;          // pinned host mirror of dscal
+    hipStream_t st = nullptr;
+
+    // mixed-precision work buffers (allocated on first use): nine FP32 slots
+    std::vector<float*> W32;
+
+    bool timing = false;
+    struct Ev {
+        int cls;
+        hipEvent_t a, b;
+    };
+    std::vector<Ev> evs;
+    double tms[TC_COUNT] = {0};
+    int64_t tcnt[TC_COUNT] = {0};
+
+    std::string err;
+};
+
+Lib& lib();
+
+// RAII event pair around launches of one class when timing is enabled.
+struct Timed {
+    int cls;
+    hipEvent_t a = nullptr, b = nullptr;
+    explicit Timed(int c);
+    ~Timed();
+};
+
+// internal transforms on a slot pointer (the pointer may be swapped with
+// the spare buffer, which is why it is passed by reference)
+void fft_forward(double*& slot);
+void fft_inverse(double*& slot);
+
+// fused FP64 / mixed step (step.hip)
+void fused_step(double dt, bool first, bool mixed, double* courant_max);
+bool fused_step_supported(bool mixed);
+
+}  // namespace hit3d

@@ -1,0 +1,276 @@
+"""Host-side driver: the reference's input file, time-step controller and
+main loop (main.f90:22-303) over the GPU library.
+
+This mirrors the Fortran driver for Python callers (tests, benchmark); the
+Fortran drop-in is hit3d_amd/fortran/.  All field arithmetic runs on the GPU.
+"""
+from __future__ import annotations
+
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .lib import Hit3D
+
+
+@dataclass
+class Params:
+    """read_input_file (m_parameters.f90:172-511)."""
+    nx: int = 32
+    ny: int = 32
+    nz_all: int = 32
+    itmin: int = 0
+    itmax: int = 10
+    iprint1: int = 1
+    iprint2: int = 10 ** 9
+    iwrite4: int = 10 ** 9
+    tmax: float = 1e6
+    trescale: float = 1e6
+    nrescale: int = 0
+    tscalar: float = 0.0
+    flow_type: int = 0
+    re: float = 100.0
+    dt: float = 0.01
+    variable_dt: bool = False
+    isp_type: int = 2
+    ir_exp: int = 4
+    peak_wavenum: float = 3.0
+    force_type: int = 1
+    kfmax: int = 2
+    famp: float = 0.5
+    dealias: int = 0
+    det_rand: int = 0
+    rn1: float = 67748937.0
+    rn2: float = 93712139.0
+    rn3: float = 10765397.0
+    nptot: int = 0
+    particles_tracking_scheme: int = 0
+    starttime_particles: float = 0.0
+    particles_filter_size: float = 0.0
+    les_model: int = 0
+    n_scalars: int = 0
+    scalar_type: list = field(default_factory=list)
+    sc: list = field(default_factory=list)
+    ir_exp_sc: list = field(default_factory=list)
+    peak_wavenum_sc: list = field(default_factory=list)
+    reac_sc: list = field(default_factory=list)
+
+    @property
+    def nu(self) -> float:
+        return 1.0 / self.re
+
+    @property
+    def pe(self):
+        return [self.nu / s for s in self.sc]
+
+    @property
+    def kmax(self) -> int:
+        if self.dealias == 0:
+            return self.nx // 3
+        return int(math.floor(float(self.nx) / 3.0 * math.sqrt(2.0)))
+
+
+def _first(line: str) -> str:
+    return line.split()[0]
+
+
+def read_input_file(path: str) -> Params:
+    """Parses the positional input file of m_parameters.f90:196-451 (run name
+    checks aside).  A negative DT means a fixed step (:277-284)."""
+    with open(path) as f:
+        lines = f.read().splitlines()
+    it = iter(lines)
+    nxt = lambda: next(it)  # noqa: E731
+    p = Params()
+    for _ in range(3):
+        nxt()
+    p.nx, p.ny, p.nz_all = (int(v) for v in nxt().split()[:3])
+    nxt()
+    p.itmin = int(_first(nxt()))
+    p.itmax = int(_first(nxt()))
+    p.iprint1 = int(_first(nxt()))
+    p.iprint2 = int(_first(nxt()))
+    p.iwrite4 = int(_first(nxt()))
+    nxt()
+    p.tmax = float(_first(nxt()))
+    parts = nxt().split()
+    p.trescale = float(parts[0])
+    try:
+        p.nrescale = int(parts[1])
+    except (IndexError, ValueError):  # m_parameters.f90:8000
+        p.nrescale = 1 if p.trescale > 0 else 0
+    p.tscalar = float(_first(nxt()))
+    nxt()
+    p.flow_type = int(_first(nxt()))
+    nxt()
+    p.re = float(_first(nxt()))
+    dt = float(_first(nxt()))
+    p.variable_dt = dt >= 0.0
+    p.dt = abs(dt)
+    nxt()
+    p.isp_type = int(_first(nxt()))
+    p.ir_exp = int(float(_first(nxt())))
+    p.peak_wavenum = float(_first(nxt()))
+    nxt()
+    p.force_type = int(_first(nxt()))
+    p.kfmax = int(_first(nxt()))
+    p.famp = float(_first(nxt()))
+    nxt()
+    p.dealias = int(_first(nxt()))
+    nxt()
+    p.det_rand = int(_first(nxt()))
+    p.rn1 = float(_first(nxt()))
+    p.rn2 = float(_first(nxt()))
+    p.rn3 = float(_first(nxt()))
+    nxt()
+    p.nptot = int(_first(nxt()))
+    p.particles_tracking_scheme = int(_first(nxt()))
+    p.starttime_particles = float(_first(nxt()))
+    p.particles_filter_size = float(_first(nxt()))
+    nxt()
+    p.les_model = int(_first(nxt()))
+    nxt()
+    if p.les_model > 0 and p.dealias != 0:  # m_parameters.f90:415-420
+        p.dealias = 0
+    p.n_scalars = int(_first(nxt()))
+    nxt()
+    if p.n_scalars > 0:
+        nxt()
+        for _ in range(p.n_scalars):
+            v = nxt().split()
+            p.scalar_type.append(int(v[0]))
+            p.sc.append(float(v[1]))
+            p.ir_exp_sc.append(float(v[2]))
+            p.peak_wavenum_sc.append(float(v[3]))
+            p.reac_sc.append(float(v[4]) if len(v) > 4 else 0.0)
+    return p
+
+
+def my_dt(dt: float, courant: float, int_scalars: bool = False, sc=()) -> float:
+    """my_dt.f90:1-34 (flow_type <= 1)."""
+    courant_max = 0.1
+    if courant < courant_max:
+        dt = dt * 1.01
+        dt = min(dt, 0.01)
+    else:
+        dt = dt / 1.05
+    if int_scalars:
+        for s in sc:
+            dt = min(dt, dt * courant_max / courant / s)
+    return dt
+
+
+@dataclass
+class StatLine:
+    itime: int
+    time: float
+    energy: float
+    eps_v: float
+    eta: float
+    enstrophy: float
+    re_lambda: float
+    x_length: float
+    lambda_: float
+    re_lambda1: float
+    tau_e: float
+    etakmax: float
+    e_spec: np.ndarray
+    diss_spec: np.ndarray
+
+
+def derived_stats(e_spec: np.ndarray, enstrophy: float, nu: float, re: float, kmax: int,
+                  itime: int = 0, time: float = 0.0) -> StatLine:
+    """stat_velocity (m_stats.f90:115-148) from the device shell sums."""
+    energy = float(np.sum(e_spec))
+    k = np.arange(1, kmax + 1, dtype=np.float64)
+    eps_v = float(np.sum(e_spec * k ** 2 * 2.0 * nu))
+    eta = (nu ** 3 / eps_v) ** 0.25
+    uvar = 2.0 / 3.0 * energy
+    x_length = math.pi / 2.0 * float(np.sum(e_spec / k)) / uvar
+    lam = math.sqrt(15.0 * uvar * nu / eps_v)
+    return StatLine(itime=itime, time=time, energy=energy, eps_v=eps_v, eta=eta,
+                    enstrophy=enstrophy, re_lambda=uvar * math.sqrt(15.0 / eps_v * re),
+                    x_length=x_length, lambda_=lam, re_lambda1=math.sqrt(uvar) * lam / nu,
+                    tau_e=x_length / math.sqrt(uvar), etakmax=eta * kmax, e_spec=e_spec,
+                    diss_spec=e_spec * k ** 2 * 2.0 * nu)
+
+
+def _e15(v: float) -> str:
+    return f"{v:15.6E}"
+
+
+def write_stats(s: StatLine, directory: str):
+    """stat1.gp / stat2.gp / es.gp in the reference's formats (m_stats.f90:152-185)."""
+    p1 = os.path.join(directory, "stat1.gp")
+    new = not os.path.exists(p1)
+    with open(p1, "a") as f:
+        if new:
+            f.write(" # 1.itime 2.time         3.energy       4.diss         5.eta          "
+                    "6.enstrophy    7.R_lambda\n")
+        f.write(f"{s.itime:8d}" + "".join(_e15(v) for v in (s.time, s.energy, s.eps_v, s.eta,
+                                                               s.enstrophy, s.re_lambda)) + "\n")
+    p2 = os.path.join(directory, "stat2.gp")
+    new = not os.path.exists(p2)
+    with open(p2, "a") as f:
+        if new:
+            f.write("# 1.itime  2.time         3.int LS       4. lambda      5.R_lambda1    "
+                    "6.tau_e        7.etakmax\n")
+        f.write(f"{s.itime:8d}" + "".join(_e15(v) for v in (s.time, s.x_length, s.lambda_,
+                                                               s.re_lambda1, s.tau_e,
+                                                               s.etakmax)) + "\n")
+    with open(os.path.join(directory, "es.gp"), "a") as f:
+        f.write("\n\n")
+        f.write(f"# ITIME={s.itime:7d} TIME={s.time:17.8E}\n")
+        for k, (e, d) in enumerate(zip(s.e_spec, s.diss_spec), start=1):
+            f.write(f"{k:4d}{_e15(e)}{_e15(d)}\n")
+
+
+class Run:
+    """A run of the driver loop (main.f90:56-288) on the GPU library."""
+
+    def __init__(self, p: Params, precision: int = 0):
+        if p.nx != p.ny or p.nx != p.nz_all:
+            raise ValueError("only cubic grids are supported")
+        if p.les_model != 0 or p.nptot != 0:
+            raise ValueError("LES and particles are not part of this build")
+        self.p = p
+        self.precision = precision
+        self.h = Hit3D(p.nx, n_scalars=p.n_scalars, dealias=p.dealias)
+        self.h.set_physics(p.nu, flow_type=p.flow_type, kfmax=p.kfmax, famp=p.famp, pe=p.pe,
+                           scalar_type=p.scalar_type)
+        self.time = 0.0
+        self.dt = p.dt
+        self.itime = p.itmin
+        self.fov = True
+        self.fos = True
+        self.int_scalars = False
+        self.courant = 0.0
+
+    def begin_new(self):
+        """begin_new.f90:10-31 + main.f90:71-76 (velocity; type-0 scalars are zero)."""
+        p = self.p
+        self.h.init_velocity(p.rn1, p.isp_type, p.ir_exp, p.peak_wavenum)
+        for s in p.scalar_type:
+            if s % 100 != 0:
+                raise ValueError("only scalar type 0 (zero field, mean gradient) is supported")
+        self.fov = self.fos = True
+        self.h.dealias_all()
+
+    def step(self):
+        """main.f90:90-207 for one iteration (no rescaling, restart or output)."""
+        p = self.p
+        self.itime += 1
+        self.courant, self.energy, self.fov, self.fos = self.h.step(
+            self.dt, self.fov, self.fos, int_scalars=self.int_scalars, precision=self.precision)
+        self.time += self.dt
+        if p.variable_dt:
+            self.dt = my_dt(self.dt, self.courant, self.int_scalars, p.sc)
+        if (self.itime % p.iprint1 == 0 or self.itime % p.iwrite4 == 0):
+            if p.n_scalars > 0 and not self.int_scalars and self.time > p.tscalar:
+                self.int_scalars = True  # main.f90:199-204 (type-0 scalars start at zero)
+
+    def stats(self) -> StatLine:
+        e, ens = self.h.stats()
+        return derived_stats(e, ens, self.p.nu, self.p.re, self.p.kmax, self.itime, self.time)

@@ -1,0 +1,245 @@
+"""ctypes binding of libhit3d_hip.so (the C ABI declared in include/hit3d.h).
+
+The library is the product: every call here runs HIP kernels on the GPU.
+There is no CPU fallback; a missing library or a missing GPU raises.
+
+Method names follow the reference's Fortran subroutines (x_fftw.f90,
+rhs_velocity.f90, rhs_scalars.f90, m_force.f90, pressure.f90,
+dealias_all.f90, m_stats.f90, init_velocity.f90); slots and array ids have
+the reference's meaning (fields 1..nf, wrk 0..nw, rhs_old 1..nf).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhit3d_hip.so")
+
+FIELDS, WRK, RHS_OLD = 1, 2, 3
+_AXES = {"x": 1, "y": 2, "z": 3}
+
+_I64P = ctypes.POINTER(ctypes.c_int64)
+_F64P = ctypes.POINTER(ctypes.c_double)
+
+_SIGNATURES = {
+    "hit3d_init": [_I64P] * 6,
+    "hit3d_set_physics": [_F64P, _I64P, _I64P, _F64P, _F64P, _I64P],
+    "hit3d_finalize": [],
+    "hit3d_put": [_I64P, _I64P, _F64P],
+    "hit3d_get": [_I64P, _I64P, _F64P],
+    "hit3d_copy": [_I64P] * 4,
+    "hit3d_xfft3d": [_I64P, _I64P],
+    "hit3d_xfft3d_fields": [_I64P, _I64P],
+    "hit3d_x_derivative": [_I64P, _I64P, _I64P],
+    "hit3d_dealias_all": [],
+    "hit3d_rhs_scalars": [_I64P],
+    "hit3d_rhs_velocity": [_F64P, _F64P],
+    "hit3d_force_velocity": [_F64P],
+    "hit3d_advance": [_I64P, _I64P, _F64P],
+    "hit3d_pressure": [],
+    "hit3d_step": [_F64P, _I64P, _I64P, _I64P, _I64P, _F64P, _F64P],
+    "hit3d_stats": [_F64P, _F64P],
+    "hit3d_init_velocity": [_F64P, _I64P, _I64P, _F64P],
+    "hit3d_random_draws": [_F64P, _I64P, _I64P, _F64P],
+    "hit3d_init_gaussian": [_I64P, _F64P, _I64P, _F64P],
+    "hit3d_get_ialias": [ctypes.POINTER(ctypes.c_int8)],
+    "hit3d_timing_enable": [_I64P],
+    "hit3d_timing_read": [_I64P, _F64P, _I64P],
+    "hit3d_timing_names": [ctypes.c_char_p, _I64P],
+    "hit3d_synchronize": [],
+    "hit3d_last_error": [ctypes.c_char_p, _I64P],
+}
+
+
+class Hit3dError(RuntimeError):
+    """A non-zero status from the library (the reference would call my_exit(-1))."""
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load the shared library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise Hit3dError(
+                f"{path} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+        lib = ctypes.CDLL(path)
+        for name, args in _SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_int
+        _lib = lib
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGNATURES)
+
+
+def _i(v):
+    return ctypes.byref(ctypes.c_int64(int(v)))
+
+
+def _d(v):
+    return ctypes.byref(ctypes.c_double(float(v)))
+
+
+def _dp(a):
+    return a.ctypes.data_as(_F64P)
+
+
+def _check(rc):
+    if rc != 0:
+        buf = ctypes.create_string_buffer(1024)
+        _lib.hit3d_last_error(buf, _i(1024))
+        raise Hit3dError(buf.value.decode(errors="replace"))
+
+
+def last_error() -> str:
+    buf = ctypes.create_string_buffer(1024)
+    load().hit3d_last_error(buf, _i(1024))
+    return buf.value.decode(errors="replace")
+
+
+def random_draws(seed: float, skip: int, count: int) -> np.ndarray:
+    """RANDu.f:12-52: fresh generator, random(-seed), skip, then count draws."""
+    lib = load()
+    out = np.empty(count, np.float64)
+    _check(lib.hit3d_random_draws(_d(seed), _i(skip), _i(count), _dp(out)))
+    return out
+
+
+def init_gaussian(n: int, rn1: float, c: int) -> np.ndarray:
+    """init_velocity.f90:87-113 Gaussian field of component c (1..3), host code."""
+    lib = load()
+    out = np.empty((n, n, n + 2))
+    _check(lib.hit3d_init_gaussian(_i(n), _d(rn1), _i(c), _dp(out)))
+    return out
+
+
+class Hit3D:
+    """The device-resident state of one run (one per process)."""
+
+    def __init__(self, n: int, n_scalars: int = 0, n_les: int = 0, dealias: int = 0):
+        self.lib = load()
+        self.n = n
+        self.n_scalars = n_scalars
+        self.n_les = n_les
+        self.dealias = dealias
+        self.nf = 3 + n_scalars + n_les
+        self.nw = max(6, 3 + n_scalars + n_les + 2)
+        _check(self.lib.hit3d_init(_i(n), _i(n), _i(n), _i(n_scalars), _i(n_les), _i(dealias)))
+
+    @property
+    def kmax(self) -> int:
+        if self.dealias == 0:
+            return self.n // 3
+        return int(np.floor(float(self.n) / 3.0 * np.sqrt(2.0)))
+
+    def close(self):
+        _check(self.lib.hit3d_finalize())
+
+    # -- parameters and staging ---------------------------------------------
+    def set_physics(self, nu, flow_type=0, kfmax=0, famp=0.0, pe=(), scalar_type=()):
+        pe_a = np.ascontiguousarray(pe, np.float64) if len(pe) else np.zeros(1)
+        st_a = np.ascontiguousarray(scalar_type, np.int64) if len(scalar_type) else np.zeros(1, np.int64)
+        _check(self.lib.hit3d_set_physics(_d(nu), _i(flow_type), _i(kfmax), _d(famp), _dp(pe_a),
+                                          st_a.ctypes.data_as(_I64P)))
+
+    def put(self, array_id: int, slot: int, a: np.ndarray):
+        a = np.ascontiguousarray(a, np.float64)
+        if a.shape != (self.n, self.n, self.n + 2):
+            raise ValueError("slot shape must be (n, n, n+2)")
+        _check(self.lib.hit3d_put(_i(array_id), _i(slot), _dp(a)))
+
+    def get(self, array_id: int, slot: int) -> np.ndarray:
+        out = np.empty((self.n, self.n, self.n + 2))
+        _check(self.lib.hit3d_get(_i(array_id), _i(slot), _dp(out)))
+        return out
+
+    def copy(self, dst_array, dst_slot, src_array, src_slot):
+        _check(self.lib.hit3d_copy(_i(dst_array), _i(dst_slot), _i(src_array), _i(src_slot)))
+
+    # -- x_fftw ----------------------------------------------------------------
+    def xFFT3d(self, flag: int, n: int):
+        _check(self.lib.hit3d_xfft3d(_i(flag), _i(n)))
+
+    def xFFT3d_fields(self, flag: int, n: int):
+        _check(self.lib.hit3d_xfft3d_fields(_i(flag), _i(n)))
+
+    def x_derivative(self, n: int, axis: str, nto: int):
+        if axis not in _AXES:
+            raise Hit3dError(f"x_derivative: wrong value of axis: {axis}")
+        _check(self.lib.hit3d_x_derivative(_i(n), _i(_AXES[axis]), _i(nto)))
+
+    def ialias(self) -> np.ndarray:
+        out = np.empty((self.n, self.n, self.n + 2), np.int8)
+        _check(self.lib.hit3d_get_ialias(out.ctypes.data_as(ctypes.POINTER(ctypes.c_int8))))
+        return out
+
+    # -- physics -----------------------------------------------------------------
+    def dealias_all(self):
+        _check(self.lib.hit3d_dealias_all())
+
+    def rhs_scalars(self, int_scalars: bool = True):
+        _check(self.lib.hit3d_rhs_scalars(_i(1 if int_scalars else 0)))
+
+    def rhs_velocity(self, dt: float) -> float:
+        c = ctypes.c_double(0.0)
+        _check(self.lib.hit3d_rhs_velocity(_d(dt), ctypes.byref(c)))
+        return c.value
+
+    def force_velocity(self) -> float:
+        e = ctypes.c_double(0.0)
+        _check(self.lib.hit3d_force_velocity(ctypes.byref(e)))
+        return e.value
+
+    def advance(self, which: int, first_order: bool, dt: float) -> bool:
+        f = ctypes.c_int64(1 if first_order else 0)
+        _check(self.lib.hit3d_advance(_i(which), ctypes.byref(f), _d(dt)))
+        return bool(f.value)
+
+    def pressure(self):
+        _check(self.lib.hit3d_pressure())
+
+    def step(self, dt: float, fov: bool, fos: bool, int_scalars: bool = False, precision: int = 0):
+        """main.f90:124-167 fused.  Returns (courant, energy, fov, fos)."""
+        fv = ctypes.c_int64(1 if fov else 0)
+        fs = ctypes.c_int64(1 if fos else 0)
+        c = ctypes.c_double(0.0)
+        e = ctypes.c_double(0.0)
+        _check(self.lib.hit3d_step(_d(dt), ctypes.byref(fv), ctypes.byref(fs),
+                                   _i(1 if int_scalars else 0), _i(precision), ctypes.byref(c),
+                                   ctypes.byref(e)))
+        return c.value, e.value, bool(fv.value), bool(fs.value)
+
+    def stats(self):
+        """(e_spec[1..kmax], enstrophy) -- m_stats.f90:195-289."""
+        e = np.zeros(self.kmax)
+        ens = ctypes.c_double(0.0)
+        _check(self.lib.hit3d_stats(_dp(e), ctypes.byref(ens)))
+        return e, ens.value
+
+    def init_velocity(self, rn1: float, isp_type: int, ir_exp: int, peak: float):
+        _check(self.lib.hit3d_init_velocity(_d(rn1), _i(isp_type), _i(ir_exp), _d(peak)))
+
+    # -- timing -------------------------------------------------------------------
+    def timing_enable(self, on: bool):
+        _check(self.lib.hit3d_timing_enable(_i(1 if on else 0)))
+
+    def timing_read(self) -> dict:
+        buf = ctypes.create_string_buffer(512)
+        _check(self.lib.hit3d_timing_names(buf, _i(512)))
+        names = buf.value.decode().split(",")
+        ms = np.zeros(len(names))
+        cnt = np.zeros(len(names), np.int64)
+        _check(self.lib.hit3d_timing_read(_i(len(names)), _dp(ms), cnt.ctypes.data_as(_I64P)))
+        return {nm: (float(m), int(c)) for nm, m, c in zip(names, ms, cnt)}
+
+    def synchronize(self):
+        _check(self.lib.hit3d_synchronize())

@@ -1,0 +1,147 @@
+/*
+ * hit3d.h -- C ABI of the MI355X-native HIT3DP time step (libhit3d_hip.so).
+ *
+ * Drop-in boundary for the reference's Fortran subroutines.  Conventions:
+ *   - every integer argument is an int64_t passed by reference and every real
+ *     a double passed by reference, as a Fortran caller built with
+ *     -fdefault-integer-8 -fdefault-real-8 passes its default kinds
+ *     (reference Makefile:11,29,53); Fortran binds them with bind(C) and
+ *     VALUE-less dummies;
+ *   - every entry point returns 0 on success and a non-zero status on error;
+ *     hit3d_last_error() copies the message.  The reference has no error
+ *     codes: its fatal paths call my_exit(-1) (my_exit.f90:23-62), which the
+ *     Fortran wrapper does on a non-zero status;
+ *   - the state the reference keeps in modules (fields, wrk(0:nw), rhs_old,
+ *     m_fields.f90:5,21; m_work.f90:16-22,102-109) lives on the GPU.  Host
+ *     copies in the reference layout (nx+2, ny, nz) are exchanged only by
+ *     hit3d_put / hit3d_get, at the points where host code reads or writes
+ *     them (init, restart, output).  Array ids: 1 = fields (slots 1..nf),
+ *     2 = wrk (slots 0..nw), 3 = rhs_old (slots 1..nf).
+ *   - one process, one GPU, one HIP stream; calls are synchronous from the
+ *     caller's point of view where they return host values.
+ */
+#ifndef HIT3D_H
+#define HIT3D_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* m_fields_init + m_work_init + x_fftw_allocate(1) + x_fftw_init
+ * (m_fields.f90:11-33, m_work.f90:30-110, x_fftw.f90:81-317): allocates
+ * fields(3+ns+nles), wrk(0:max(6,3+ns+nles+2)), rhs_old(3+ns+nles) on the
+ * device (zeroed), builds twiddle and phase-shift tables.  nx = ny = nz_all,
+ * a power of two in [8, 1024].  dealias: 0 = 2/3 rule, 1 = phase shift. */
+int hit3d_init(const int64_t *nx, const int64_t *ny, const int64_t *nz_all,
+               const int64_t *n_scalars, const int64_t *n_les, const int64_t *dealias);
+
+/* Physical parameters read by read_input_file (m_parameters.f90:274-451):
+ * nu = 1/RE, flow_type, kfmax, famp; per scalar: pe(n) = nu/sc(n) and
+ * scalar_type(n) (arrays of n_scalars entries, may be NULL when 0). */
+int hit3d_set_physics(const double *nu, const int64_t *flow_type, const int64_t *kfmax,
+                      const double *famp, const double *pe, const int64_t *scalar_type);
+
+/* x_fftw_allocate(-1) (x_fftw.f90:134-142): frees everything. */
+int hit3d_finalize(void);
+
+/* Host <-> device staging of one slot in the reference layout, pitch nx+2
+ * reals (sync points of the drop-in; no reference counterpart). */
+int hit3d_put(const int64_t *array_id, const int64_t *slot, const double *host);
+int hit3d_get(const int64_t *array_id, const int64_t *slot, double *host);
+
+/* dst(:,:,:,dslot) = src(:,:,:,sslot), the whole-slot array assignments the
+ * reference writes inline, e.g. wrk(:,:,:,1:3) = fields(:,:,:,1:3)
+ * (rhs_scalars.f90:34, m_stats.f90:254-256). */
+int hit3d_copy(const int64_t *dst_array, const int64_t *dst_slot, const int64_t *src_array,
+               const int64_t *src_slot);
+
+/* xFFT3d(flag, n) (x_fftw.f90:852-1085): in-place 3-D real FFT of wrk slot n;
+ * flag=+1 forward (physical (x,y,z) -> spectral (kx,kz,ky)), -1 inverse with
+ * norm 1/(nx*ny*nz_all). */
+int hit3d_xfft3d(const int64_t *flag, const int64_t *n);
+
+/* xFFT3d_fields(flag, n) (x_fftw.f90:1098-1282): the same on fields slot n. */
+int hit3d_xfft3d_fields(const int64_t *flag, const int64_t *n);
+
+/* x_derivative(n, axis, nto) (x_fftw.f90:637-682): wrk(nto) = i k_axis wrk(n);
+ * axis 1 = 'x', 2 = 'y', 3 = 'z' (the Fortran wrapper maps the character). */
+int hit3d_x_derivative(const int64_t *n, const int64_t *axis, const int64_t *nto);
+
+/* dealias_all (dealias_all.f90:1-35). */
+int hit3d_dealias_all(void);
+
+/* rhs_scalars (rhs_scalars.f90:1-402, reaction and LES hooks excluded):
+ * leaves the physical velocity in wrk(1:3) and, when scalars are active
+ * (*int_scalars != 0), the scalar RHS in wrk(4:3+ns). */
+int hit3d_rhs_scalars(const int64_t *int_scalars);
+
+/* rhs_velocity (rhs_velocity.f90:1-419, LES hook excluded): reads the
+ * physical velocity in wrk(1:3), writes the spectral RHS to wrk(1:3) and
+ * returns courant = max(|u|+|v|+|w|)*dt/dx (:31-38). */
+int hit3d_rhs_velocity(const double *dt, double *courant);
+
+/* force_velocity (m_force.f90:101-182): band energy of shells 1..kfmax and
+ * wrk(1:3) += famp/energy * fields(1:3) on the forced entries. */
+int hit3d_force_velocity(double *energy);
+
+/* The Euler/AB2 updates the driver writes inline (main.f90:137-146 for
+ * which = 2, scalars 4..3+ns+nles; main.f90:156-164 for which = 1, velocity):
+ * *first_order != 0: rhs_old = wrk; fields += dt*rhs_old; then *first_order = 0.
+ * else fields += dt*(1.5 wrk - 0.5 rhs_old); rhs_old = wrk. */
+int hit3d_advance(const int64_t *which, int64_t *first_order, const double *dt);
+
+/* pressure (pressure.f90:1-51): projection of fields(1:3). */
+int hit3d_pressure(void);
+
+/* One hydro step of the driver loop (main.f90:124-167) fused on the GPU:
+ * rhs_scalars, scalar update, rhs_velocity, force_velocity (flow_type 1),
+ * velocity update, pressure.  fov/fos as in main.f90:71-72 (updated).
+ * precision: 0 = FP64 throughout; 1 = FP32 transforms and nonlinear term
+ * with FP64 state, update and reductions.  Returns courant and, when
+ * forced, the band energy.  Leaves wrk undefined. */
+int hit3d_step(const double *dt, int64_t *fov, int64_t *fos, const int64_t *int_scalars,
+               const int64_t *precision, double *courant, double *energy);
+
+/* Statistics of m_stats.f90:195-289: e_spec(1:kmax) (FP64 shell sums) and
+ * the mean enstrophy <|omega|^2>. */
+int hit3d_stats(double *e_spec, double *enstrophy);
+
+/* init_velocity (init_velocity.f90:1-246): seeded random solenoidal field
+ * from RANDu (RANDu.f:12-52) with seed rn1, shell-rescaled to the spectrum
+ * isp_type (0, 1, 3 as the reference; 2 = k^ir_exp exp(-(ir_exp/2)(k/peak)^2),
+ * an addition) into fields(1:3).  isp_type -1 = Taylor-Green (:252-293). */
+int hit3d_init_velocity(const double *rn1, const int64_t *isp_type, const int64_t *ir_exp,
+                        const double *peak);
+
+/* Draws of the reference generator (RANDu.f:12-52): a fresh generator,
+ * random(-seed), `skip` draws discarded, then `count` draws into out. */
+int hit3d_random_draws(const double *seed, const int64_t *skip, const int64_t *count,
+                       double *out);
+
+/* The Gaussian field sqrt(-2 log a) sin(2 pi b) of velocity component c
+ * (1..3) before its transform (init_velocity.f90:87-113), in the reference
+ * layout, computed by the same host code hit3d_init_velocity uses. */
+int hit3d_init_gaussian(const int64_t *nx, const double *rn1, const int64_t *c, double *out);
+
+/* ialias (x_fftw.f90:298-310) as int8 in the reference layout (nx+2,ny,nz). */
+int hit3d_get_ialias(int8_t *out);
+
+/* Per-kernel-class GPU time, measured with HIP events on the library's
+ * stream while enabled (no reference counterpart).  hit3d_timing_read fills
+ * up to *cap entries of total milliseconds and launch counts; the order of
+ * classes is given by hit3d_timing_names (comma-separated). */
+int hit3d_timing_enable(const int64_t *on);
+int hit3d_timing_read(const int64_t *cap, double *ms, int64_t *count);
+int hit3d_timing_names(char *buf, const int64_t *len);
+
+/* Waits for all GPU work of the library. */
+int hit3d_synchronize(void);
+
+/* Copies the last error message (NUL-terminated, truncated to len). */
+int hit3d_last_error(char *buf, const int64_t *len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
