@@ -612,7 +612,7 @@ int hit3d_step(const double* dt, int64_t* fov, int64_t* fos, const int64_t* int_
             throw std::runtime_error("precision must be 0 (FP64) or 1 (mixed)");
         const bool scal = (*int_scalars != 0) && L.ns > 0;
         double cmax = 0.0;
-        if (!scal && fused_step_supported(mixed)) {
+        if (!scal && L.fuse && fused_step_supported(mixed)) {
             // rhs_scalars + rhs_velocity + force + update + pressure in one pipeline
             fused_step(*dt, *fov != 0, mixed, &cmax);
             *fov = 0;
@@ -736,6 +736,13 @@ int hit3d_get_ialias(int8_t* out) {
         check(hipMemcpyAsync(out, d, bytes, hipMemcpyDeviceToHost, L.st), "hipMemcpyAsync");
         check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
         check(hipFree(d), "hipFree");
+    });
+}
+
+int hit3d_set_fusion(const int64_t* on) {
+    return guarded([&] {
+        require_ready();
+        lib().fuse = *on != 0;
     });
 }
 

@@ -57,6 +57,7 @@ struct Lib {
     // mixed-precision work buffers (allocated on first use): nine FP32 slots
     std::vector<float*> W32;
 
+    bool fuse = true;  // hit3d_step uses the fused pipeline where it applies
     bool timing = false;
     struct Ev {
         int cls;

@@ -1,11 +1,428 @@
-// Fused time step (filled in below); the unfused sequence in capi.hip is the
-// fallback for configurations the fused pipeline does not cover.
+// Fused time step for the 2/3-rule velocity equation (main.f90:124-167 with
+// rhs_scalars.f90:32-39, rhs_velocity.f90:31-144, m_force.f90:101-182,
+// main.f90:156-164, pressure.f90:13-47).
+//
+// The prognostic fields are zero outside the cube |k_i| <= kmax (dealias_all
+// at start-up, a masked RHS every step), so every transform is pruned to
+// that cube: the inverse transforms read only the kmax+1 kx columns and the
+// 2kmax+1 ky/kz rows that can be non-zero, and the forward transforms of the
+// six products keep only those outputs.  Five kernels, each one pass over
+// HBM:
+//   A  inverse z transform of u,v,w, reading fields        (kx, kz, ky) -> (kx, ky, z)
+//   B  inverse y transform, in place                        (kx, ky, z)  -> (kx, y, z)
+//   C  x: c2r of u,v,w, Courant max, six products u_i u_j, r2c of each product
+//   D  forward y transform of the products, in place, keeping kept ky rows
+//   E  forward z transform of the products + RHS assembly -i k_j (u_i u_j)^
+//      - nu k^2 u + band forcing + Euler/AB2 update + projection, writing
+//      fields and rhs_old.
+// The work buffers (wrk slots) use a pruned row pitch PA >= kmax+1 and T =
+// double (FP64 form) or float (mixed form: FP32 transforms and nonlinear
+// term, FP64 state, update, forcing energy and Courant reduction).
+#include "fft.hpp"
+#include "passes.hpp"
 #include "state.hpp"
 
 namespace hit3d {
 
-bool fused_step_supported(bool) { return false; }
+namespace {
 
-void fused_step(double, bool, bool, double*) {}
+struct SG {
+    int K;        // kmax
+    int NK;       // kept kx columns = K + 1
+    int64_t P;    // field row pitch (complex)
+    int64_t PA;   // work row pitch (complex)
+};
+
+template <typename T>
+struct BCols {
+    static constexpr int B = 128 / (int)sizeof(cpx<T>);
+};
+
+template <int N, typename T>
+struct CE {
+    static constexpr int B = BCols<T>::B;
+    static constexpr int E = (B * N / 8 <= 1024) ? 8 : 16;
+    static constexpr int TPF = N / E;
+    static constexpr int THREADS = B * TPF;
+};
+
+template <int N, typename T>
+struct XC {
+    static constexpr int M = N / 2;
+    static constexpr int E = (M < 8) ? M : 8;
+    static constexpr int TPF = M / E;
+    static constexpr int ROWS = (256 / TPF < N * N) ? 256 / TPF : N * N;
+    static constexpr int THREADS = ROWS * TPF;
+};
+
+__device__ __forceinline__ bool kept(int j, int K, int n) { return j <= K || j >= n - K; }
+__device__ __forceinline__ int kept_row(int kp, int K, int n) { return kp <= K ? kp : n - 2 * K - 1 + kp; }
+
+template <typename T>
+__device__ __forceinline__ cpx<T> to(cpx<double> a) {
+    return cpx<T>{(T)a.x, (T)a.y};
+}
+
+struct P3 {
+    const double* p[3];
+};
+template <typename T>
+struct W6 {
+    T* p[6];
+};
+
+// ---------------------------------------------------------------- A
+template <int N, typename T>
+__global__ __launch_bounds__((CE<N, T>::THREADS)) void k_inv_z(SG g, P3 F, W6<T> A,
+                                                             const cpx<T>* __restrict__ tw) {
+    constexpr int E = CE<N, T>::E, B = CE<N, T>::B, TPF = CE<N, T>::TPF;
+    __shared__ cpx<T> lds[N * B];
+    const int c = threadIdx.x % B, t = threadIdx.x / B;
+    const int kx = blockIdx.x * B + c;
+    const bool valid = kx < g.NK;
+    const int jy = kept_row(blockIdx.y, g.K, N);
+    const int comp = blockIdx.z;
+    const cpx<double>* src = reinterpret_cast<const cpx<double>*>(F.p[comp]) + (int64_t)jy * N * g.P + kx;
+    cpx<T>* dst = reinterpret_cast<cpx<T>*>(A.p[comp]) + (int64_t)jy * g.PA + kx;
+    cpx<T> v[E];
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+        const int e = t + s * TPF;
+        v[s] = (valid && kept(e, g.K, N)) ? to<T>(src[(int64_t)e * g.P]) : cpx<T>{(T)0, (T)0};
+    }
+    block_fft<N, E, 1, T, B>(v, t, lds + c, tw + N);
+    if (valid) {
+#pragma unroll
+        for (int s = 0; s < E; ++s) dst[(int64_t)(t + s * TPF) * N * g.PA] = v[s];
+    }
+}
+
+// ---------------------------------------------------------------- B / D
+// DIR=+1: inverse y, loads kept rows only, stores all rows.
+// DIR=-1: forward y, loads all rows, stores kept rows only.
+template <int N, int DIR, typename T>
+__global__ __launch_bounds__((CE<N, T>::THREADS)) void k_y(SG g, W6<T> A,
+                                                         const cpx<T>* __restrict__ tw) {
+    constexpr int E = CE<N, T>::E, B = CE<N, T>::B, TPF = CE<N, T>::TPF;
+    __shared__ cpx<T> lds[N * B];
+    const int c = threadIdx.x % B, t = threadIdx.x / B;
+    const int kx = blockIdx.x * B + c;
+    const bool valid = kx < g.NK;
+    const int z = blockIdx.y;
+    cpx<T>* base = reinterpret_cast<cpx<T>*>(A.p[blockIdx.z]) + (int64_t)z * N * g.PA + kx;
+    cpx<T> v[E];
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+        const int e = t + s * TPF;
+        const bool ld = valid && (DIR < 0 || kept(e, g.K, N));
+        v[s] = ld ? base[(int64_t)e * g.PA] : cpx<T>{(T)0, (T)0};
+    }
+    block_fft<N, E, DIR, T, B>(v, t, lds + c, tw + N);
+    if (valid) {
+#pragma unroll
+        for (int s = 0; s < E; ++s) {
+            const int e = t + s * TPF;
+            if (DIR > 0 || kept(e, g.K, N)) base[(int64_t)e * g.PA] = v[s];
+        }
+    }
+}
+
+// ---------------------------------------------------------------- C
+template <int N, typename T>
+__global__ __launch_bounds__((XC<N, T>::THREADS)) void k_x(SG g, W6<T> Wb,
+                                                         const cpx<T>* __restrict__ tw, T norm,
+                                                         unsigned long long* maxbits) {
+    using C = XC<N, T>;
+    constexpr int M = C::M, E = C::E, TPF = C::TPF;
+    __shared__ cpx<T> lds[C::ROWS][M + 1];
+    __shared__ double redm[C::THREADS / 64 + 1];
+    const int t = threadIdx.x % TPF;
+    const int rr = threadIdx.x / TPF;
+    const int64_t row = (int64_t)blockIdx.x * C::ROWS + rr;
+    const cpx<T>* twN = tw + N;
+    cpx<T> ph[3][E];
+    // c2r of u, v, w (x_fftw.f90:1066-1073 semantics: Im of kx=0 ignored)
+#pragma unroll
+    for (int comp = 0; comp < 3; ++comp) {
+        const cpx<T>* rowp = reinterpret_cast<const cpx<T>*>(Wb.p[comp]) + row * g.PA;
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < E; ++s) {
+            const int k = t + s * TPF;
+            lds[rr][k] = (k < g.NK) ? rowp[k] : cpx<T>{(T)0, (T)0};
+        }
+        if (t == 0) lds[rr][M] = cpx<T>{(T)0, (T)0};
+        __syncthreads();
+        cpx<T> v[E];
+#pragma unroll
+        for (int s = 0; s < E; ++s) {
+            const int k = t + s * TPF;
+            cpx<T> xk = lds[rr][k];
+            cpx<T> xm = lds[rr][M - k];
+            if (k == 0) {
+                xk.y = (T)0;
+                xm.y = (T)0;
+            }
+            xm = conj(xm);
+            v[s] = (xk + xm) + mul_i(cmulc(xk - xm, twN[k]));
+        }
+        block_fft<M, E, 1, T, 1>(v, t, lds[rr], tw + M);
+#pragma unroll
+        for (int s = 0; s < E; ++s) ph[comp][s] = scale(v[s], norm);
+    }
+    // Courant number, rhs_velocity.f90:31-32 (FP64 max)
+    double m = 0.0;
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+        const double a = ((double)fabs(ph[0][s].x) + (double)fabs(ph[1][s].x)) + (double)fabs(ph[2][s].x);
+        const double b = ((double)fabs(ph[0][s].y) + (double)fabs(ph[1][s].y)) + (double)fabs(ph[2][s].y);
+        m = a > m ? a : m;
+        m = b > m ? b : m;
+    }
+    // six products, rhs_velocity.f90:56-72, then r2c keeping kx <= kmax
+    constexpr int pa[6] = {0, 0, 0, 1, 1, 2};
+    constexpr int pb[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+    for (int p = 0; p < 6; ++p) {
+        cpx<T> v[E];
+#pragma unroll
+        for (int s = 0; s < E; ++s)
+            v[s] = cpx<T>{ph[pa[p]][s].x * ph[pb[p]][s].x, ph[pa[p]][s].y * ph[pb[p]][s].y};
+        block_fft<M, E, -1, T, 1>(v, t, lds[rr], tw + M);
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < E; ++s) lds[rr][t + s * TPF] = v[s];
+        __syncthreads();
+        cpx<T>* outp = reinterpret_cast<cpx<T>*>(Wb.p[p]) + row * g.PA;
+#pragma unroll
+        for (int s = 0; s < E; ++s) {
+            const int k = t + s * TPF;
+            if (k < g.NK) {
+                cpx<T> zk = lds[rr][k];
+                cpx<T> zm = conj(lds[rr][(M - k) & (M - 1)]);
+                cpx<T> e = scale(zk + zm, (T)0.5);
+                cpx<T> o = scale(mul_mi(zk - zm), (T)0.5);
+                outp[k] = e + cmul(twN[k], o);
+            }
+        }
+    }
+    // block max -> one atomic
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double q = __shfl_down(m, o, 64);
+        m = q > m ? q : m;
+    }
+    if ((threadIdx.x & 63) == 0) redm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double mm = 0.0;
+        for (int i = 0; i < (C::THREADS + 63) / 64; ++i) mm = redm[i] > mm ? redm[i] : mm;
+        atomicMax(maxbits, (unsigned long long)__double_as_longlong(mm));
+    }
+}
+
+// ---------------------------------------------------------------- E
+struct Upd {
+    double* F[3];
+    double* R[3];
+    const double* eshell;  // band energy per shell 1..kfmax (device)
+    double* energy_out;
+    double dt, nu, famp;
+    int kfmax, forced, first;
+};
+
+template <int N, typename T>
+__global__ __launch_bounds__((CE<N, T>::THREADS)) void k_fwd_z_update(SG g, W6<T> Q, Upd u,
+                                                                    const cpx<T>* __restrict__ tw) {
+    constexpr int E = CE<N, T>::E, B = CE<N, T>::B, TPF = CE<N, T>::TPF;
+    __shared__ cpx<T> lds[N * B];
+    const int c = threadIdx.x % B, t = threadIdx.x / B;
+    const int kx = blockIdx.x * B + c;
+    const bool valid = kx < g.NK;
+    const int jy = kept_row(blockIdx.y, g.K, N);
+    const int ky = wavenum(jy, N);
+    const T kxT = (T)kx, kyT = (T)ky;
+    // t_i = k_j P_ij accumulated in the reference's order (rhs_velocity.f90:89-91)
+    cpx<T> acc[3][E];
+#pragma unroll
+    for (int p = 0; p < 6; ++p) {
+        const cpx<T>* src = reinterpret_cast<const cpx<T>*>(Q.p[p]) + (int64_t)jy * g.PA + kx;
+        cpx<T> v[E];
+#pragma unroll
+        for (int s = 0; s < E; ++s)
+            v[s] = valid ? src[(int64_t)(t + s * TPF) * N * g.PA] : cpx<T>{(T)0, (T)0};
+        block_fft<N, E, -1, T, B>(v, t, lds + c, tw + N);
+#pragma unroll
+        for (int s = 0; s < E; ++s) {
+            const T kzT = (T)wavenum(t + s * TPF, N);
+            // p: 0 uu, 1 uv, 2 uw, 3 vv, 4 vw, 5 ww
+            if (p == 0) acc[0][s] = scale(v[s], kxT);
+            if (p == 1) {
+                acc[0][s] = acc[0][s] + scale(v[s], kyT);
+                acc[1][s] = scale(v[s], kxT);
+            }
+            if (p == 2) {
+                acc[0][s] = acc[0][s] + scale(v[s], kzT);
+                acc[2][s] = scale(v[s], kxT);
+            }
+            if (p == 3) acc[1][s] = acc[1][s] + scale(v[s], kyT);
+            if (p == 4) {
+                acc[1][s] = acc[1][s] + scale(v[s], kzT);
+                acc[2][s] = acc[2][s] + scale(v[s], kyT);
+            }
+            if (p == 5) acc[2][s] = acc[2][s] + scale(v[s], kzT);
+        }
+    }
+    // band forcing factor, m_force.f90:153-161
+    double fac = 0.0;
+    if (u.forced) {
+        double en = 0.0;
+        for (int i = 0; i < u.kfmax; ++i) en += u.eshell[i];
+        fac = u.famp / en;
+        if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *u.energy_out = en;
+    }
+    if (!valid) return;
+    const double kxd = kx, kyd = ky;
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+        const int e = t + s * TPF;
+        if (!kept(e, g.K, N)) continue;
+        const int kz = wavenum(e, N);
+        const double kzd = kz;
+        const int64_t off = ((int64_t)jy * N + e) * g.P + kx;
+        const double k2 = kxd * kxd + kyd * kyd + kzd * kzd;
+        const double lap = -u.nu * k2;
+        const bool fnode = u.forced && kx < N / 2 && [&] {
+            const int sh = shell_of(kx * kx + ky * ky + kz * kz);
+            return sh > 0 && sh <= u.kfmax;
+        }();
+        cpx<double> fn[3];
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) {
+            const cpx<double> f = reinterpret_cast<const cpx<double>*>(u.F[cc])[off];
+            // r = i * (-t) + lap * f, t = k.P  (rhs_velocity.f90:89-136)
+            const double tre = -(double)acc[cc][s].x, tim = -(double)acc[cc][s].y;
+            cpx<double> r = {-tim + lap * f.x, tre + lap * f.y};
+            if (fnode) r = {r.x + fac * f.x, r.y + fac * f.y};
+            cpx<double>* ro = reinterpret_cast<cpx<double>*>(u.R[cc]) + off;
+            if (u.first) {
+                fn[cc] = {f.x + u.dt * r.x, f.y + u.dt * r.y};
+            } else {
+                const cpx<double> o = *ro;
+                fn[cc] = {f.x + u.dt * (1.5 * r.x - 0.5 * o.x), f.y + u.dt * (1.5 * r.y - 0.5 * o.y)};
+            }
+            *ro = r;
+        }
+        // projection, pressure.f90:21-43
+        const double dre = kxd * fn[0].x + kyd * fn[1].x + kzd * fn[2].x;
+        const double dim = kxd * fn[0].y + kyd * fn[1].y + kzd * fn[2].y;
+        const double lapl = k2 == 0.0 ? 9e20 : k2;
+        const double pr = dre / lapl, pi = dim / lapl;
+        reinterpret_cast<cpx<double>*>(u.F[0])[off] = {fn[0].x - pr * kxd, fn[0].y - pi * kxd};
+        reinterpret_cast<cpx<double>*>(u.F[1])[off] = {fn[1].x - pr * kyd, fn[1].y - pi * kyd};
+        reinterpret_cast<cpx<double>*>(u.F[2])[off] = {fn[2].x - pr * kzd, fn[2].y - pi * kzd};
+    }
+}
+
+__global__ void k_zero_bits(unsigned long long* p) { *p = 0ull; }
+
+// ---------------------------------------------------------------- host
+template <typename F>
+static void dispatch_step(int n, F&& f) {
+    switch (n) {
+        case 16: f(std::integral_constant<int, 16>{}); break;
+        case 32: f(std::integral_constant<int, 32>{}); break;
+        case 64: f(std::integral_constant<int, 64>{}); break;
+        case 128: f(std::integral_constant<int, 128>{}); break;
+        case 256: f(std::integral_constant<int, 256>{}); break;
+        case 512: f(std::integral_constant<int, 512>{}); break;
+        case 1024: f(std::integral_constant<int, 1024>{}); break;
+        default: throw HipError{"fused step: unsupported grid size"};
+    }
+}
+
+template <int N, typename T>
+static void run(Lib& L, double dt, bool first, const cpx<T>* tw) {
+    SG g;
+    g.K = L.kmax;
+    g.NK = L.kmax + 1;
+    g.P = L.g.P;
+    g.PA = ((g.NK + 15) / 16) * 16;
+    P3 F{{L.F[1], L.F[2], L.F[3]}};
+    W6<T> Wb;
+    for (int i = 0; i < 6; ++i) Wb.p[i] = reinterpret_cast<T*>(L.W[i + 1]);
+    constexpr int B = CE<N, T>::B;
+    const int ncb = (g.NK + B - 1) / B;
+    const int nky = 2 * g.K + 1;
+    const bool forced = L.flow_type == 1;
+    auto* maxbits = reinterpret_cast<unsigned long long*>(L.dscal);
+    if (forced) {
+        Timed tm(TC_REDUCE);
+        launch_shell_sums(L.g, L.F[1], L.F[2], L.F[3], 1, L.kfmax, L.dshell, L.st);
+    }
+    hipLaunchKernelGGL(k_zero_bits, dim3(1), dim3(1), 0, L.st, maxbits);
+    {
+        Timed tm(TC_STEP_INV_Z);
+        hipLaunchKernelGGL((k_inv_z<N, T>), dim3(ncb, nky, 3), dim3(CE<N, T>::THREADS), 0, L.st,
+                           g, F, Wb, tw);
+    }
+    {
+        Timed tm(TC_STEP_INV_Y);
+        hipLaunchKernelGGL((k_y<N, 1, T>), dim3(ncb, N, 3), dim3(CE<N, T>::THREADS), 0, L.st, g, Wb,
+                           tw);
+    }
+    {
+        Timed tm(TC_STEP_X);
+        const T norm = (T)(1.0 / ((double)N * (double)N * (double)N));
+        hipLaunchKernelGGL((k_x<N, T>), dim3((int64_t)N * N / XC<N, T>::ROWS),
+                           dim3(XC<N, T>::THREADS), 0, L.st, g, Wb, tw, norm, maxbits);
+    }
+    {
+        Timed tm(TC_STEP_FWD_Y);
+        hipLaunchKernelGGL((k_y<N, -1, T>), dim3(ncb, N, 6), dim3(CE<N, T>::THREADS), 0, L.st, g,
+                           Wb, tw);
+    }
+    {
+        Timed tm(TC_STEP_FWD_Z);
+        Upd u;
+        for (int i = 0; i < 3; ++i) {
+            u.F[i] = L.F[i + 1];
+            u.R[i] = L.R[i + 1];
+        }
+        u.eshell = L.dshell;
+        u.energy_out = L.dscal + 1;
+        u.dt = dt;
+        u.nu = L.nu;
+        u.famp = L.famp;
+        u.kfmax = L.kfmax;
+        u.forced = forced ? 1 : 0;
+        u.first = first ? 1 : 0;
+        hipLaunchKernelGGL((k_fwd_z_update<N, T>), dim3(ncb, nky), dim3(CE<N, T>::THREADS), 0,
+                           L.st, g, Wb, u, tw);
+    }
+}
+
+}  // namespace
+
+bool fused_step_supported(bool) {
+    Lib& L = lib();
+    return L.dealias == 0 && L.nles == 0 && L.n >= 16 && L.kmax >= 1;
+}
+
+void fused_step(double dt, bool first, bool mixed, double* courant_max) {
+    Lib& L = lib();
+    if (mixed)
+        dispatch_step(L.n, [&](auto NC) { run<decltype(NC)::value, float>(L, dt, first, L.tw32); });
+    else
+        dispatch_step(L.n, [&](auto NC) { run<decltype(NC)::value, double>(L, dt, first, L.tw64); });
+    check(hipMemcpyAsync(L.hpin, L.dscal, 2 * sizeof(double), hipMemcpyDeviceToHost, L.st),
+          "hipMemcpyAsync");
+    check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
+    uint64_t bits;
+    __builtin_memcpy(&bits, L.hpin, sizeof(bits));
+    double m;
+    __builtin_memcpy(&m, &bits, sizeof(m));
+    *courant_max = m;
+}
 
 }  // namespace hit3d

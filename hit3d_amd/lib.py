@@ -41,6 +41,7 @@ _SIGNATURES = {
     "hit3d_advance": [_I64P, _I64P, _F64P],
     "hit3d_pressure": [],
     "hit3d_step": [_F64P, _I64P, _I64P, _I64P, _I64P, _F64P, _F64P],
+    "hit3d_set_fusion": [_I64P],
     "hit3d_stats": [_F64P, _F64P],
     "hit3d_init_velocity": [_F64P, _I64P, _I64P, _F64P],
     "hit3d_random_draws": [_F64P, _I64P, _I64P, _F64P],
@@ -217,6 +218,9 @@ class Hit3D:
                                    _i(1 if int_scalars else 0), _i(precision), ctypes.byref(c),
                                    ctypes.byref(e)))
         return c.value, e.value, bool(fv.value), bool(fs.value)
+
+    def set_fusion(self, on: bool):
+        _check(self.lib.hit3d_set_fusion(_i(1 if on else 0)))
 
     def stats(self):
         """(e_spec[1..kmax], enstrophy) -- m_stats.f90:195-289."""

@@ -103,6 +103,12 @@ int hit3d_pressure(void);
 int hit3d_step(const double *dt, int64_t *fov, int64_t *fos, const int64_t *int_scalars,
                const int64_t *precision, double *courant, double *energy);
 
+/* Selects how hit3d_step runs: *on != 0 (default) the fused pipeline where
+ * it applies (2/3 rule without active scalars), 0 the unfused sequence of
+ * the reference subroutines above (no reference counterpart; for A/B
+ * checks).  The mixed form needs the fused pipeline. */
+int hit3d_set_fusion(const int64_t *on);
+
 /* Statistics of m_stats.f90:195-289: e_spec(1:kmax) (FP64 shell sums) and
  * the mean enstrophy <|omega|^2>. */
 int hit3d_stats(double *e_spec, double *enstrophy);

@@ -193,8 +193,8 @@ def test_advance(oracle):
     first = h.advance(1, True, 0.01)
     assert first is False
     fr, ro = oracle.advance(f[:3], r[:3], None, 0.01, True)
-    for c in range(3):
-        assert np.array_equal(h.get(FIELDS, c + 1), fr[c])
+    for c in range(3):  # the device may contract f + dt*r into one fma
+        assert rel(h.get(FIELDS, c + 1), fr[c]) < 1e-15
         assert np.array_equal(h.get(RHS_OLD, c + 1), ro[c])
     r2 = rng.standard_normal((4, n, n, n + 2))
     for c in range(4):
