@@ -1,0 +1,229 @@
+!==============================================================================!
+! hit3d_gpu -- Fortran driver of the GPU time step.
+!
+! Reads <run_name>.in (the reference's positional input file,
+! m_parameters.f90:172-451), creates the seeded initial field
+! (begin_new.f90 / init_velocity.f90), and runs the driver loop of
+! main.f90:87-297 with the hydro step on the GPU (hit3d_step: rhs_scalars,
+! rhs_velocity, force_velocity, AB2 update, pressure).  Statistics every
+! IPRINT1 steps go to stat1.gp, stat2.gp and es.gp in the reference's formats
+! (m_stats.f90:152-185); the final restart file <run_name>.64.NNNNNN has the
+! reference's layout (restart_io.f90:287-314).
+!
+! Usage:  hit3d_gpu <run_name> [precision]     (precision 0 = FP64, 1 = mixed)
+! Build with -fdefault-integer-8 -fdefault-real-8, like the reference.
+!==============================================================================!
+program hit3d_gpu
+  use iso_c_binding
+  use hit3d_hip
+  implicit none
+
+  character(len=10) :: run_name
+  character(len=80) :: str
+  integer :: nx, ny, nz_all, itmin, itmax, iprint1, iprint2, iwrite4, nrescale
+  integer :: flow_type, isp_type, ir_exp, force_type, kfmax, dealias, det_rand
+  integer :: nptot, ptrack, les_model, n_scalars, n, kmax, itime, precision, ios
+  real*8 :: tmax, trescale, tscalar, re, dt, peak, famp, rn1, rn2, rn3, tp, pfs
+  real*8 :: nu, time, courant, energy, enstrophy
+  logical :: variable_dt, int_scalars
+  integer, allocatable :: scalar_type(:)
+  real*8, allocatable :: sc(:), pe(:), e_spec(:)
+  real*8 :: dummy(3)
+  integer(c_int64_t) :: fov, fos, isc, prec
+
+  if (command_argument_count() < 1) stop 'usage: hit3d_gpu <run_name> [precision]'
+  call get_command_argument(1, run_name)
+  if (len_trim(run_name) /= 10) stop 'Length of run name is not 10'
+  precision = 0
+  if (command_argument_count() >= 2) then
+     call get_command_argument(2, str)
+     read(str, *) precision
+  end if
+
+  ! ---------------------------------------------------------------- input
+  open(10, file=run_name//'.in', form='formatted', status='old')
+  read(10,*); read(10,*); read(10,*)
+  read(10,*) nx, ny, nz_all
+  read(10,*)
+  read(10,*) itmin
+  read(10,*) itmax
+  read(10,*) iprint1
+  read(10,*) iprint2
+  read(10,*) iwrite4
+  read(10,*)
+  read(10,*) tmax
+  read(10,*, iostat=ios) trescale, nrescale
+  if (ios /= 0) then
+     nrescale = 0
+     if (trescale > 0.d0) nrescale = 1
+  end if
+  read(10,*) tscalar
+  read(10,*)
+  read(10,*) flow_type
+  read(10,*)
+  read(10,*) re
+  read(10,*) dt
+  variable_dt = dt >= 0.d0
+  dt = abs(dt)
+  read(10,*)
+  read(10,*) isp_type
+  read(10,*) ir_exp
+  read(10,*) peak
+  read(10,*)
+  read(10,*) force_type
+  read(10,*) kfmax
+  read(10,*) famp
+  read(10,*)
+  read(10,*) dealias
+  read(10,*)
+  read(10,*) det_rand
+  read(10,*) rn1
+  read(10,*) rn2
+  read(10,*) rn3
+  read(10,*)
+  read(10,*) nptot
+  read(10,*) ptrack
+  read(10,*) tp
+  read(10,*) pfs
+  read(10,*)
+  read(10,*) les_model
+  read(10,*)
+  read(10,*) n_scalars
+  read(10,*)
+  allocate(scalar_type(max(n_scalars,1)), sc(max(n_scalars,1)), pe(max(n_scalars,1)))
+  scalar_type = 0; sc = 1.d0
+  nu = 1.d0 / re
+  if (n_scalars > 0) then
+     read(10,*)
+     do n = 1, n_scalars
+        read(10,*) scalar_type(n), sc(n)
+        pe(n) = nu / sc(n)
+     end do
+  end if
+  close(10)
+  if (les_model > 0 .or. nptot > 0) stop 'LES and particles are not part of this driver'
+  if (itmin /= 0) stop 'restart input is not part of this driver'
+
+  if (dealias == 0) then
+     kmax = nx / 3
+  else
+     kmax = floor(real(nx, 8) / 3.d0 * sqrt(2.d0))
+  end if
+  allocate(e_spec(kmax))
+
+  ! ----------------------------------------------------------- start-up
+  call h3d_check(c_init(int(nx, c_int64_t), int(ny, c_int64_t), int(nz_all, c_int64_t), &
+       int(n_scalars, c_int64_t), 0_c_int64_t, int(dealias, c_int64_t)), 'init')
+  call h3d_check(c_set_physics(nu, int(flow_type, c_int64_t), int(kfmax, c_int64_t), famp, &
+       pe, int(scalar_type, c_int64_t)), 'set_physics')
+  call h3d_check(c_init_velocity(rn1, int(isp_type, c_int64_t), int(ir_exp, c_int64_t), peak), &
+       'init_velocity')
+  call dealias_all
+  time = 0.d0
+  fov = 1; fos = 1
+  int_scalars = .false.
+  prec = precision
+
+  ! ---------------------------------------------------------- main loop
+  do itime = itmin + 1, itmax
+     isc = 0
+     if (int_scalars) isc = 1
+     call h3d_check(c_step(dt, fov, fos, isc, prec, courant, energy), 'step')
+     time = time + dt
+     if (variable_dt) call my_dt_host
+     if (mod(itime, iprint1) == 0) then
+        write(*, "('ITIME=',i6,3x,'TIME=',f8.4,4x,'DT=',f8.5,3x,'Courn= ',f6.4)") &
+             itime, time, dt, courant
+        call stats_out
+     end if
+     if (mod(itime, iprint1) == 0 .or. mod(itime, iwrite4) == 0) then
+        if (time > tmax) exit
+        if (n_scalars > 0 .and. .not. int_scalars .and. time > tscalar) int_scalars = .true.
+     end if
+  end do
+
+  if (itime > itmax) itime = itmax
+  call restart_out
+  call h3d_check(c_finalize(), 'finalize')
+
+contains
+
+  subroutine my_dt_host                      ! my_dt.f90:8-30
+    real*8, parameter :: courant_max = 0.1d0
+    integer :: k
+    if (courant < courant_max) then
+       dt = dt * 1.01d0
+       dt = min(dt, 0.01d0)
+    else
+       dt = dt / 1.05d0
+    end if
+    if (int_scalars) then
+       do k = 1, n_scalars
+          dt = min(dt, dt * courant_max / courant / sc(k))
+       end do
+    end if
+  end subroutine my_dt_host
+
+  subroutine stats_out                       ! m_stats.f90:115-185
+    real*8 :: en, eps_v, eta, uvar, xl, lam, rl, rl1, tau, s
+    integer :: k
+    logical :: there
+    call h3d_check(c_stats(e_spec, enstrophy), 'stats')
+    en = sum(e_spec(1:kmax))
+    eps_v = 0.d0; s = 0.d0
+    do k = 1, kmax
+       eps_v = eps_v + e_spec(k) * real(k**2, 8) * 2.d0 * nu
+       s = s + e_spec(k) / real(k, 8)
+    end do
+    eta = (nu**3 / eps_v)**0.25d0
+    uvar = 2.d0 / 3.d0 * en
+    xl = 4.d0 * atan(1.d0) / 2.d0 * s / uvar
+    lam = sqrt(15.d0 * uvar * nu / eps_v)
+    rl = uvar * sqrt(15.d0 / eps_v * re)
+    rl1 = sqrt(uvar) * lam / nu
+    tau = xl / sqrt(uvar)
+    inquire(file='stat1.gp', exist=there)
+    open(69, file='stat1.gp', position='append')
+    if (.not. there) write(69,*) '# 1.itime 2.time         3.energy       4.diss         5.eta'// &
+         '          6.enstrophy    7.R_lambda'
+    write(69, "(i8,20e15.6)") itime, time, en, eps_v, eta, enstrophy, rl
+    close(69)
+    inquire(file='stat2.gp', exist=there)
+    open(70, file='stat2.gp', position='append')
+    if (.not. there) write(70,'(A)') '# 1.itime  2.time         3.int LS       4. lambda      '// &
+         '5.R_lambda1    6.tau_e        7.etakmax'
+    write(70, "(i8,20e15.6)") itime, time, xl, lam, rl1, tau, eta * real(kmax, 8)
+    close(70)
+    open(900, file='es.gp', position='append')
+    write(900, "()")
+    write(900, "()")
+    write(900, "('# ITIME=',i7,' TIME=',e17.8)") itime, time
+    do k = 1, kmax
+       write(900, "(i4,2e15.6)") k, e_spec(k), e_spec(k) * real(k**2, 8) * 2.d0 * nu
+    end do
+    close(900)
+  end subroutine stats_out
+
+  subroutine restart_out                     ! restart_io.f90:287-314 layout
+    real*8, allocatable :: slot(:,:,:)
+    integer(c_int64_t) :: c
+    integer :: nout, u
+    integer*4 :: hdr(5)
+    character(len=6) :: ext
+    allocate(slot(nx+2, ny, nz_all))
+    nout = 0
+    if (int_scalars) nout = n_scalars
+    write(ext, '(i6.6)') itime
+    open(newunit=u, file=run_name//'.64.'//ext, access='stream', form='unformatted', &
+         status='replace')
+    hdr = (/ int(nx,4), int(ny,4), int(nz_all,4), int(nout,4), 0_4 /)
+    write(u) hdr, time, dt
+    do c = 1, 3 + nout
+       call h3d_check(c_get(H3D_FIELDS, c, slot), 'get')
+       write(u) slot(1:nx, 1:ny, 1:nz_all)
+    end do
+    close(u)
+    deallocate(slot)
+  end subroutine restart_out
+
+end program hit3d_gpu
