@@ -161,6 +161,7 @@ static void free_all() {
     if (L.ps_tab) (void)hipFree(L.ps_tab);
     if (L.dshell) (void)hipFree(L.dshell);
     if (L.dpart) (void)hipFree(L.dpart);
+    if (L.tv) (void)hipFree(L.tv);
     if (L.dscal) (void)hipFree(L.dscal);
     if (L.hpin) (void)hipHostFree(L.hpin);
     if (L.st) (void)hipStreamDestroy(L.st);
@@ -183,12 +184,86 @@ static double* dalloc(size_t bytes) {
 // ---------------------------------------------------------------------------
 // reference subroutines
 // ---------------------------------------------------------------------------
+// les_get_turb_visc_smag (m_les.f90:1142-1215): turb_visc from fields(1:3),
+// using wrk(4:6) as work space like the reference.
+static void les_turb_visc() {
+    Lib& L = lib();
+    auto& W = L.W;
+    const double* U[3] = {L.F[1], L.F[2], L.F[3]};
+    const double coef = (L.c_smag * L.dx) * (L.c_smag * L.dx);
+    for (int group = 1; group <= 2; ++group) {
+        {
+            double* out[3] = {W[4], W[5], W[6]};
+            Timed t(TC_POINTWISE);
+            launch_strain(L.g, U, out, group, L.st);
+        }
+        for (int c = 4; c <= 6; ++c) fft_inverse(W[c]);
+        Timed t(TC_POINTWISE);
+        launch_tv_accum(L.g, W[4], W[5], W[6], L.tv, group, coef, L.st);
+    }
+}
+
+// les_rhs_velocity for model 1 (m_les.f90:636-690 with les_rhsv_turb_visc
+// :902-1018): rhs += -d_j tau_ij, tau_ij = -2 nu_t S_ij, then the ialias mask.
+static void les_velocity() {
+    Lib& L = lib();
+    auto& W = L.W;
+    const double* U[3] = {L.F[1], L.F[2], L.F[3]};
+    for (int group = 1; group <= 2; ++group) {
+        {
+            double* out[3] = {W[4], W[5], W[6]};
+            Timed t(TC_POINTWISE);
+            launch_strain(L.g, U, out, group, L.st);
+        }
+        for (int c = 4; c <= 6; ++c) {
+            fft_inverse(W[c]);
+            {
+                Timed t(TC_POINTWISE);
+                launch_tau(L.g, W[c], L.tv, L.st);
+            }
+            fft_forward(W[c]);
+        }
+        const double* tau[3] = {W[4], W[5], W[6]};
+        double* rhs[3] = {W[1], W[2], W[3]};
+        Timed t(TC_POINTWISE);
+        launch_les_vel_accum(L.g, tau, rhs, group, L.st);
+    }
+    Timed t(TC_POINTWISE);
+    for (int c = 1; c <= 3; ++c) launch_mask_alias(L.g, W[c], 0, L.st);
+}
+
+// m_les_rhss_turb_visc1 (m_les.f90:1041-1091) for field slot n, then the
+// ialias mask of les_rhs_scalars (:863-871).
+static void les_scalar(int n) {
+    Lib& L = lib();
+    auto& W = L.W;
+    const int t1 = 3 + L.ns + L.nles + 1, t2 = t1 + 1;
+    copy_slot(W[t1], L.F[n]);
+    check(hipMemsetAsync(W[0], 0, slot_bytes(), L.st), "hipMemsetAsync");
+    for (int axis = 1; axis <= 3; ++axis) {
+        {
+            Timed t(TC_POINTWISE);
+            launch_derivative(L.g, W[t1], W[t2], axis, L.st);
+        }
+        fft_inverse(W[t2]);
+        pmul(W[t2], L.tv, W[t2]);
+        fft_forward(W[t2]);
+        Timed t(TC_POINTWISE);
+        launch_derivative(L.g, W[t2], W[t2], axis, L.st);
+        launch_add(L.g, W[0], W[t2], false, L.st);
+    }
+    Timed t(TC_POINTWISE);
+    launch_add(L.g, W[n], W[0], true, L.st);
+    launch_mask_alias(L.g, W[n], 0, L.st);
+}
+
 static void do_rhs_scalars(bool int_scalars) {
     Lib& L = lib();
     auto& F = L.F;
     auto& W = L.W;
     const Geo& g = L.g;
     if (L.nles > 0) throw std::runtime_error("LES scalars are not supported");
+    if (L.les_model == 1) les_turb_visc();  // rhs_scalars.f90:18
     if (!int_scalars || L.ns == 0) {  // rhs_scalars.f90:32-39
         for (int c = 1; c <= 3; ++c) {
             copy_slot(W[c], F[c]);
@@ -256,6 +331,8 @@ static void do_rhs_scalars(bool int_scalars) {
             launch_sub(g, W[3 + s], F[1], L.st);
         }
     }
+    if (L.les_model == 1)  // rhs_scalars.f90:396-398
+        for (int s = 1; s <= L.ns; ++s) les_scalar(3 + s);
 }
 
 static double read_courant_max() {
@@ -325,6 +402,7 @@ static void do_rhs_velocity() {
         Timed t(TC_POINTWISE);
         for (int c = 1; c <= 3; ++c) launch_mask_ps(g, W[c], L.st);
     }
+    if (L.les_model == 1) les_velocity();  // rhs_velocity.f90:414-416
 }
 
 static void do_force(double* energy) {
@@ -612,7 +690,7 @@ int hit3d_step(const double* dt, int64_t* fov, int64_t* fos, const int64_t* int_
             throw std::runtime_error("precision must be 0 (FP64) or 1 (mixed)");
         const bool scal = (*int_scalars != 0) && L.ns > 0;
         double cmax = 0.0;
-        if (!scal && L.fuse && fused_step_supported(mixed)) {
+        if (!scal && L.fuse && L.les_model == 0 && fused_step_supported(mixed)) {
             // rhs_scalars + rhs_velocity + force + update + pressure in one pipeline
             fused_step(*dt, *fov != 0, mixed, &cmax);
             *fov = 0;
@@ -736,6 +814,20 @@ int hit3d_get_ialias(int8_t* out) {
         check(hipMemcpyAsync(out, d, bytes, hipMemcpyDeviceToHost, L.st), "hipMemcpyAsync");
         check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
         check(hipFree(d), "hipFree");
+    });
+}
+
+int hit3d_set_les(const int64_t* les_model, const double* c_smag) {
+    return guarded([&] {
+        require_ready();
+        Lib& L = lib();
+        if (*les_model != 0 && *les_model != 1)
+            throw std::runtime_error("only les_model 0 (none) and 1 (Smagorinsky) are supported");
+        if (*les_model == 1 && L.dealias != 0)
+            throw std::runtime_error("LES needs the 2/3 rule (m_parameters.f90:415-420)");
+        L.les_model = (int)*les_model;
+        L.c_smag = *c_smag;
+        if (L.les_model == 1 && !L.tv) L.tv = dalloc(slot_bytes());
     });
 }
 

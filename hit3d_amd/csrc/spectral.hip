@@ -525,4 +525,133 @@ void launch_init_rescale(const Geo& g, double* f, const double* ratio, int kmax,
                        kmax);
 }
 
+// ---------------------------------------------------------------------------
+// Smagorinsky model
+// ---------------------------------------------------------------------------
+namespace {
+
+__global__ void k_strain(Geo g, CPtr3 U, Ptr3 out, int group) {
+    GRID_STRIDE(i, nspec(g)) {
+        SIdx s = sdecode(g, i);
+        const double kx = s.kx, ky = s.kyv, kz = s.kzv;
+        const cpx<double> u = C(U.p[0])[s.off], v = C(U.p[1])[s.off], w = C(U.p[2])[s.off];
+        if (group == 1) {
+            C(out.p[0])[s.off] = {-(kx * u.y), kx * u.x};
+            C(out.p[1])[s.off] = {-0.5 * (ky * u.y + kx * v.y), 0.5 * (ky * u.x + kx * v.x)};
+            C(out.p[2])[s.off] = {-0.5 * (kz * u.y + kx * w.y), 0.5 * (kz * u.x + kx * w.x)};
+        } else {
+            C(out.p[0])[s.off] = {-(ky * v.y), ky * v.x};
+            C(out.p[1])[s.off] = {-0.5 * (kz * v.y + ky * w.y), 0.5 * (kz * v.x + ky * w.x)};
+            C(out.p[2])[s.off] = {-(kz * w.y), kz * w.x};
+        }
+    }
+}
+
+__global__ void k_tv_accum(Geo g, const double* a, const double* b, const double* c, double* tv,
+                           int group, double coef) {
+    const int64_t total = (int64_t)g.n * g.n * g.n;
+    GRID_STRIDE(i, total) {
+        const int64_t x = i % g.n, row = i / g.n;
+        const int64_t o = row * 2 * g.P + x;
+        const double A = a[o], Bv = b[o], Cv = c[o];
+        if (group == 1) {
+            double t = A * A;
+            t = t + 2.0 * (Bv * Bv);
+            t = t + 2.0 * (Cv * Cv);
+            tv[o] = t;
+        } else {
+            double t = tv[o];
+            t = t + A * A;
+            t = t + 2.0 * (Bv * Bv);
+            t = t + Cv * Cv;
+            tv[o] = sqrt(2.0 * t) * coef;
+        }
+    }
+}
+
+__global__ void k_tau(Geo g, double* a, const double* tv) {
+    const int64_t total = (int64_t)g.n * g.n * g.n;
+    GRID_STRIDE(i, total) {
+        const int64_t x = i % g.n, row = i / g.n;
+        const int64_t o = row * 2 * g.P + x;
+        a[o] = -2.0 * tv[o] * a[o];
+    }
+}
+
+__global__ void k_les_vel_accum(Geo g, CPtr3 T, Ptr3 R, int group) {
+    GRID_STRIDE(i, nspec(g)) {
+        SIdx s = sdecode(g, i);
+        if (alias_count(s.kx, s.kyv, s.kzv, g.n) != 0) continue;
+        const double kx = s.kx, ky = s.kyv, kz = s.kzv;
+        const cpx<double> a = C(T.p[0])[s.off], b = C(T.p[1])[s.off], c = C(T.p[2])[s.off];
+        if (group == 1) {  // a = tau11, b = tau12, c = tau13
+            cpx<double> r = C(R.p[0])[s.off];
+            C(R.p[0])[s.off] = {r.x + kx * a.y + ky * b.y + kz * c.y, r.y - kx * a.x - ky * b.x - kz * c.x};
+            r = C(R.p[1])[s.off];
+            C(R.p[1])[s.off] = {r.x + kx * b.y, r.y - kx * b.x};
+            r = C(R.p[2])[s.off];
+            C(R.p[2])[s.off] = {r.x + kx * c.y, r.y - kx * c.x};
+        } else {  // a = tau22, b = tau23, c = tau33
+            cpx<double> r = C(R.p[1])[s.off];
+            C(R.p[1])[s.off] = {r.x + ky * a.y + kz * b.y, r.y - ky * a.x - kz * b.x};
+            r = C(R.p[2])[s.off];
+            C(R.p[2])[s.off] = {r.x + ky * b.y + kz * c.y, r.y - ky * b.x - kz * c.x};
+        }
+    }
+}
+
+__global__ void k_mask_alias(Geo g, double* slot, int maxcount) {
+    GRID_STRIDE(i, nspec(g)) {
+        SIdx s = sdecode(g, i);
+        if (alias_count(s.kx, s.kyv, s.kzv, g.n) > maxcount) C(slot)[s.off] = {0.0, 0.0};
+    }
+}
+
+__global__ void k_add(Geo g, double* out, const double* a, int only_unaliased) {
+    GRID_STRIDE(i, nspec(g)) {
+        SIdx s = sdecode(g, i);
+        if (only_unaliased && alias_count(s.kx, s.kyv, s.kzv, g.n) != 0) continue;
+        const cpx<double> o = C(out)[s.off], b = C(a)[s.off];
+        C(out)[s.off] = {o.x + b.x, o.y + b.y};
+    }
+}
+
+}  // namespace
+
+void launch_strain(const Geo& g, const double* const U[3], double* const out[3], int group,
+                   hipStream_t s) {
+    CPtr3 u;
+    Ptr3 o;
+    for (int i = 0; i < 3; ++i) {
+        u.p[i] = U[i];
+        o.p[i] = out[i];
+    }
+    hipLaunchKernelGGL(k_strain, dim3(grid_for(nspec_h(g))), dim3(TB), 0, s, g, u, o, group);
+}
+void launch_tv_accum(const Geo& g, const double* a, const double* b, const double* c, double* tv,
+                     int group, double coef, hipStream_t s) {
+    hipLaunchKernelGGL(k_tv_accum, dim3(grid_for(nphys_h(g))), dim3(TB), 0, s, g, a, b, c, tv,
+                       group, coef);
+}
+void launch_tau(const Geo& g, double* a, const double* tv, hipStream_t s) {
+    hipLaunchKernelGGL(k_tau, dim3(grid_for(nphys_h(g))), dim3(TB), 0, s, g, a, tv);
+}
+void launch_les_vel_accum(const Geo& g, const double* const tau[3], double* const rhs[3],
+                          int group, hipStream_t s) {
+    CPtr3 t;
+    Ptr3 r;
+    for (int i = 0; i < 3; ++i) {
+        t.p[i] = tau[i];
+        r.p[i] = rhs[i];
+    }
+    hipLaunchKernelGGL(k_les_vel_accum, dim3(grid_for(nspec_h(g))), dim3(TB), 0, s, g, t, r, group);
+}
+void launch_mask_alias(const Geo& g, double* slot, int maxcount, hipStream_t s) {
+    hipLaunchKernelGGL(k_mask_alias, dim3(grid_for(nspec_h(g))), dim3(TB), 0, s, g, slot, maxcount);
+}
+void launch_add(const Geo& g, double* out, const double* a, bool only_unaliased, hipStream_t s) {
+    hipLaunchKernelGGL(k_add, dim3(grid_for(nspec_h(g))), dim3(TB), 0, s, g, out, a,
+                       only_unaliased ? 1 : 0);
+}
+
 }  // namespace hit3d

@@ -73,4 +73,25 @@ void launch_init_curl(const Geo& g, const double* w1, const double* w2, const do
 // rule), zero elsewhere; ratio has kmax+1 entries (index 0 unused, <0 = zero).
 void launch_init_rescale(const Geo& g, double* f, const double* ratio, int kmax, hipStream_t s);
 
+// Smagorinsky model (m_les.f90:902-1215).
+// Strain components of group 1 (S11, S12, S13) or group 2 (S22, S23, S33)
+// from the spectral velocity (m_les.f90:1154-1196).
+void launch_strain(const Geo& g, const double* const U[3], double* const out[3], int group,
+                   hipStream_t s);
+// nu_t accumulation over physical strain components: group 1 sets
+// tv = a^2 + 2b^2 + 2c^2, group 2 adds a^2 + 2b^2 + c^2 and finishes
+// tv = sqrt(2 tv) * coef (m_les.f90:1170-1212).
+void launch_tv_accum(const Geo& g, const double* a, const double* b, const double* c, double* tv,
+                     int group, double coef, hipStream_t s);
+// physical a = -2 tv a (m_les.f90:930-934).
+void launch_tau(const Geo& g, double* a, const double* tv, hipStream_t s);
+// rhs += -i k_j tau_ij on entries with alias_count == 0 for one group
+// (m_les.f90:937-960 group 1, :987-1015 group 2).
+void launch_les_vel_accum(const Geo& g, const double* const tau[3], double* const rhs[3],
+                          int group, hipStream_t s);
+// zero entries with alias_count > maxcount.
+void launch_mask_alias(const Geo& g, double* slot, int maxcount, hipStream_t s);
+// out += a on all entries (only_unaliased = false) or on alias_count == 0.
+void launch_add(const Geo& g, double* out, const double* a, bool only_unaliased, hipStream_t s);
+
 }  // namespace hit3d

@@ -58,6 +58,9 @@ struct Lib {
     std::vector<float*> W32;
 
     bool fuse = true;  // hit3d_step uses the fused pipeline where it applies
+    int les_model = 0;       // 0 none, 1 Smagorinsky (m_les.f90:94-104)
+    double c_smag = 0.18;    // m_les.f90:1148
+    double* tv = nullptr;    // turb_visc, physical slot layout
     bool timing = false;
     struct Ev {
         int cls;

@@ -42,6 +42,7 @@ _SIGNATURES = {
     "hit3d_pressure": [],
     "hit3d_step": [_F64P, _I64P, _I64P, _I64P, _I64P, _F64P, _F64P],
     "hit3d_set_fusion": [_I64P],
+    "hit3d_set_les": [_I64P, _F64P],
     "hit3d_stats": [_F64P, _F64P],
     "hit3d_init_velocity": [_F64P, _I64P, _I64P, _F64P],
     "hit3d_random_draws": [_F64P, _I64P, _I64P, _F64P],
@@ -218,6 +219,9 @@ class Hit3D:
                                    _i(1 if int_scalars else 0), _i(precision), ctypes.byref(c),
                                    ctypes.byref(e)))
         return c.value, e.value, bool(fv.value), bool(fs.value)
+
+    def set_les(self, les_model: int, c_smag: float = 0.18):
+        _check(self.lib.hit3d_set_les(_i(les_model), _d(c_smag)))
 
     def set_fusion(self, on: bool):
         _check(self.lib.hit3d_set_fusion(_i(1 if on else 0)))

@@ -103,6 +103,14 @@ int hit3d_pressure(void);
 int hit3d_step(const double *dt, int64_t *fov, int64_t *fos, const int64_t *int_scalars,
                const int64_t *precision, double *courant, double *energy);
 
+/* LES model selection (m_les.f90:69-104, dispatch :636-690, :700-895):
+ * les_model 0 = none, 1 = Smagorinsky, nu_t = (c_smag dx)^2 sqrt(2 S_ij S_ij)
+ * (les_get_turb_visc_smag, m_les.f90:1142-1215) applied to velocities
+ * (les_rhsv_turb_visc :902-1018) and scalars (m_les_rhss_turb_visc1
+ * :1041-1091).  The reference hard-codes c_smag = 0.18 (:1148).  Needs the
+ * 2/3 rule, as the reference forces (m_parameters.f90:415-420). */
+int hit3d_set_les(const int64_t *les_model, const double *c_smag);
+
 /* Selects how hit3d_step runs: *on != 0 (default) the fused pipeline where
  * it applies (2/3 rule without active scalars), 0 the unfused sequence of
  * the reference subroutines above (no reference counterpart; for A/B

@@ -334,6 +334,62 @@ def rhs_scalars(fields: np.ndarray, n_scalars: int, pe, scalar_type, dealias: in
 # forcing, update, projection, masks, dt
 # ----------------------------------------------------------------------------
 
+def _strain(fields, n):
+    """S_ij (complex) of m_les.f90:1154-1196: S11, S12, S13, S22, S23, S33."""
+    kx, ky, kz = _kgrid(n)
+    U = [to_complex(fields[c]) for c in range(3)]
+    return [1j * kx * U[0],
+            0.5j * (ky * U[0] + kx * U[1]),
+            0.5j * (kz * U[0] + kx * U[2]),
+            1j * ky * U[1],
+            0.5j * (kz * U[1] + ky * U[2]),
+            1j * kz * U[2]]
+
+
+def turb_visc_smag(fields: np.ndarray, n: int, c_smag: float = 0.18) -> np.ndarray:
+    """les_get_turb_visc_smag (m_les.f90:1142-1215): (c_s dx)^2 sqrt(2 S_ij S_ij),
+    in physical space (n, n, n), les_delta = dx (m_les.f90:86)."""
+    S = [fft_inverse(to_interleaved(s))[:, :, :n] for s in _strain(fields, n)]
+    tv = S[0] ** 2
+    tv = tv + 2.0 * S[1] ** 2
+    tv = tv + 2.0 * S[2] ** 2
+    tv = tv + S[3] ** 2
+    tv = tv + 2.0 * S[4] ** 2
+    tv = tv + S[5] ** 2
+    dx = 2.0 * np.pi / n
+    return np.sqrt(2.0 * tv) * (c_smag * dx) ** 2
+
+
+def les_rhs_velocity(fields: np.ndarray, rhs: np.ndarray, tv: np.ndarray, n: int) -> np.ndarray:
+    """les_rhsv_turb_visc (m_les.f90:902-1018) then the ialias != 0 mask of
+    les_rhs_velocity (:682-690): rhs_i += -i k_j tau_ij, tau_ij = -2 nu_t S_ij."""
+    kx, ky, kz = _kgrid(n)
+    ia = ialias(n)[:, :, 0::2]
+    S = _strain(fields, n)
+    tau = []
+    for s in S:
+        ph = fft_inverse(to_interleaved(s))[:, :, :n]
+        tau.append(to_complex(fft_forward(_pad(-2.0 * tv * ph, n))))
+    r = [to_complex(rhs[c]) for c in range(3)]
+    keep = ia == 0
+    r[0] = np.where(keep, r[0] - 1j * ((kx * tau[0] + ky * tau[1]) + kz * tau[2]), r[0])
+    r[1] = np.where(keep, r[1] - 1j * ((kx * tau[1] + ky * tau[3]) + kz * tau[4]), r[1])
+    r[2] = np.where(keep, r[2] - 1j * ((kx * tau[2] + ky * tau[4]) + kz * tau[5]), r[2])
+    r = [np.where(keep, x, 0.0) for x in r]
+    return np.stack([to_interleaved(x) for x in r])
+
+
+def les_rhs_scalar(field_s: np.ndarray, rhs_s: np.ndarray, tv: np.ndarray, n: int) -> np.ndarray:
+    """m_les_rhss_turb_visc1 (m_les.f90:1041-1091): rhs += sum_dir
+    d_dir(nu_t d_dir phi) on entries with ialias == 0."""
+    ia = ialias(n)
+    acc = np.zeros_like(field_s)
+    for ax in ("x", "y", "z"):
+        d = fft_inverse(x_derivative(field_s, ax))[:, :, :n]
+        acc = acc + x_derivative(fft_forward(_pad(d * tv, n)), ax)
+    return np.where(ia == 0, rhs_s + acc, rhs_s)
+
+
 def forced_mask(n: int, kfmax: int) -> np.ndarray:
     """m_force.f90:37-78: entries i=1..n with 0 < shell <= kfmax (interleaved)."""
     k2 = _k2_interleaved(n)
@@ -521,6 +577,8 @@ class Config:
     sc: tuple = ()
     scalar_type: tuple = ()
     re: float = 0.0
+    les_model: int = 0
+    c_smag: float = 0.18
 
     @property
     def kmax(self):
@@ -551,12 +609,17 @@ def step(cfg: Config, st: State) -> State:
     fields = st.fields.copy()
     rhs_old = st.rhs_old.copy()
     ns = cfg.n_scalars
+    tv = turb_visc_smag(fields, n, cfg.c_smag) if cfg.les_model == 1 else None
     if ns > 0:
         rs = rhs_scalars(fields, ns, cfg.pe, cfg.scalar_type, cfg.dealias, n)
+        if tv is not None:
+            rs = np.stack([les_rhs_scalar(fields[3 + s], rs[s], tv, n) for s in range(ns)])
         new, old = advance(fields[3:3 + ns], rs, rhs_old[3:3 + ns], st.dt, st.fos)
         fields[3:3 + ns] = new
         rhs_old[3:3 + ns] = old
     rhs, phys = rhs_velocity(fields, cfg.nu, cfg.dealias, n)
+    if tv is not None:
+        rhs = les_rhs_velocity(fields, rhs, tv, n)
     courant = courant_number(phys, n, st.dt)
     energy = st.energy
     if cfg.flow_type == 1:

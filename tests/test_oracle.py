@@ -351,3 +351,41 @@ def test_init_velocity_properties(oracle):
     # the Gaussian field is what Box-Muller makes of the first six slots
     uni, _ = oracle.init_velocity_uniforms(n, RN1)
     assert np.array_equal(g, oracle.box_muller(uni[0:3], uni[3:6]))
+
+
+# ------------------------------------------------------------------- Smagorinsky
+
+def _shear_field(oracle, n):
+    """u = (sin y, 0, cos y): divergence-free with |S| = 1 everywhere, so the
+    Smagorinsky viscosity is the constant (c_s dx)^2."""
+    z = np.zeros((n, n, n + 2))
+    return np.stack([oracle.fft_forward(_phys(n, lambda x, y, zz: np.sin(y))), z,
+                     oracle.fft_forward(_phys(n, lambda x, y, zz: np.cos(y)))])
+
+
+def test_smagorinsky_viscosity_closed_form(oracle):
+    n = 16
+    tv = oracle.turb_visc_smag(_shear_field(oracle, n), n)
+    dx = 2 * np.pi / n
+    assert np.abs(tv - (0.18 * dx) ** 2).max() < 1e-15
+
+
+def test_smagorinsky_velocity_source_closed_form(oracle):
+    """With constant nu_t the stress divergence is nu_t lap(u) = -nu_t u for
+    this field: catches a sign error or a missing factor 2 in tau = -2 nu_t S."""
+    n = 16
+    f = _shear_field(oracle, n)
+    tv = oracle.turb_visc_smag(f, n)
+    r = oracle.les_rhs_velocity(f, np.zeros_like(f), tv, n)
+    nut = (0.18 * 2 * np.pi / n) ** 2
+    for c in range(3):
+        assert np.abs(r[c] + nut * f[c]).max() < 1e-12 * n ** 3
+
+
+def test_smagorinsky_scalar_source_closed_form(oracle):
+    n = 16
+    tv = np.full((n, n, n), 0.3)
+    phi = oracle.fft_forward(_phys(n, lambda x, y, z: np.cos(x) + np.sin(2 * z)))
+    r = oracle.les_rhs_scalar(phi, np.zeros_like(phi), tv, n)
+    lap = oracle.fft_forward(_phys(n, lambda x, y, z: -np.cos(x) - 4 * np.sin(2 * z)))
+    assert np.abs(r - 0.3 * lap).max() < 1e-12 * n ** 3
