@@ -80,15 +80,23 @@ def run_form(n, precision, steps, warmup):
 # every axis pass reads and writes one slot's valid entries; the fused step
 # kernels are charged the slots they must read and write.
 def kernel_bytes(n, cls, precision):
+    """Bytes each kernel must move: the unfused axis passes read and write one
+    slot's valid entries; the fused step kernels move only the 2/3-rule cube
+    (kmax+1 kx columns, 2kmax+1 ky/kz rows) they read or produce."""
     s64 = slot_bytes(n)
-    sw = s64 if precision == 0 else s64 // 2
+    k = n // 3
+    nk, nky = k + 1, 2 * k + 1
+    w = 16 if precision == 0 else 8          # work complex word
+    cube = nk * nky * nky * 16               # one FP64 field inside the cube
+    zy = n * nky * nk * w                     # one work buffer with kept ky rows, all z
+    full = n * n * nk * w                     # one work buffer, all y and z
     table = {
         "fft_x": 2 * s64, "fft_y": 2 * s64, "fft_z": 2 * s64,
-        "step_inv_z": 3 * s64 + 3 * sw,      # fields(1:3) in, work out
-        "step_inv_y": 6 * sw,
-        "step_x": 9 * sw,                    # 3 in, 6 products out
-        "step_fwd_y": 12 * sw,
-        "step_fwd_z": 6 * sw + 12 * s64,     # products in; fields, rhs_old in/out
+        "step_inv_z": 3 * cube + 3 * zy,
+        "step_inv_y": 3 * zy + 3 * full,
+        "step_x": 3 * full + 6 * full,
+        "step_fwd_y": 6 * full + 6 * zy,
+        "step_fwd_z": 6 * zy + 6 * cube + 6 * cube,
     }
     return table.get(cls)
 

@@ -49,7 +49,9 @@ struct CE {
 template <int N, typename T>
 struct XC {
     static constexpr int M = N / 2;
-    static constexpr int E = (M < 8) ? M : 8;
+    // FP64 keeps three physical rows in registers: 4 values per thread keeps
+    // the kernel under 128 VGPRs (4 waves per SIMD); FP32 affords 8.
+    static constexpr int E = (M < 8) ? M : (sizeof(T) == 8 ? 4 : 8);
     static constexpr int TPF = M / E;
     static constexpr int ROWS = (256 / TPF < N * N) ? 256 / TPF : N * N;
     static constexpr int THREADS = ROWS * TPF;
@@ -180,20 +182,31 @@ __global__ __launch_bounds__((XC<N, T>::THREADS)) void k_x(SG g, W6<T> Wb,
         m = b > m ? b : m;
     }
     // six products, rhs_velocity.f90:56-72, then r2c keeping kx <= kmax
-    constexpr int pa[6] = {0, 0, 0, 1, 1, 2};
-    constexpr int pb[6] = {0, 1, 2, 1, 2, 2};
+    // factor indices of products p = uu, uv, uw, vv, vw, ww packed 4 bits
+    // per product (a table indexed at run time would live in scratch)
+    constexpr unsigned pa = 0x211000u, pb = 0x221210u;
 #pragma unroll
     for (int p = 0; p < 6; ++p) {
+        const int ia = (pa >> (4 * p)) & 15, ib = (pb >> (4 * p)) & 15;
         cpx<T> v[E];
 #pragma unroll
-        for (int s = 0; s < E; ++s)
-            v[s] = cpx<T>{ph[pa[p]][s].x * ph[pb[p]][s].x, ph[pa[p]][s].y * ph[pb[p]][s].y};
+        for (int s = 0; s < E; ++s) {
+            const cpx<T> a = ia == 0 ? ph[0][s] : (ia == 1 ? ph[1][s] : ph[2][s]);
+            const cpx<T> b = ib == 0 ? ph[0][s] : (ib == 1 ? ph[1][s] : ph[2][s]);
+            v[s] = cpx<T>{a.x * b.x, a.y * b.y};
+        }
         block_fft<M, E, -1, T, 1>(v, t, lds[rr], tw + M);
         __syncthreads();
 #pragma unroll
         for (int s = 0; s < E; ++s) lds[rr][t + s * TPF] = v[s];
         __syncthreads();
-        cpx<T>* outp = reinterpret_cast<cpx<T>*>(Wb.p[p]) + row * g.PA;
+        T* op = Wb.p[0];
+        if (p == 1) op = Wb.p[1];
+        if (p == 2) op = Wb.p[2];
+        if (p == 3) op = Wb.p[3];
+        if (p == 4) op = Wb.p[4];
+        if (p == 5) op = Wb.p[5];
+        cpx<T>* outp = reinterpret_cast<cpx<T>*>(op) + row * g.PA;
 #pragma unroll
         for (int s = 0; s < E; ++s) {
             const int k = t + s * TPF;
@@ -245,8 +258,17 @@ __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_fwd_z_update(SG g, W6<T
     // t_i = k_j P_ij accumulated in the reference's order (rhs_velocity.f90:89-91)
     cpx<T> acc[3][E];
 #pragma unroll
+    for (int s = 0; s < E; ++s) acc[0][s] = acc[1][s] = acc[2][s] = cpx<T>{(T)0, (T)0};
+#pragma unroll 1
     for (int p = 0; p < 6; ++p) {
-        const cpx<T>* src = reinterpret_cast<const cpx<T>*>(Q.p[p]) + (int64_t)jy * g.PA + kx;
+        // select the buffer without indexing the argument array at run time
+        const T* qp = Q.p[0];
+        if (p == 1) qp = Q.p[1];
+        if (p == 2) qp = Q.p[2];
+        if (p == 3) qp = Q.p[3];
+        if (p == 4) qp = Q.p[4];
+        if (p == 5) qp = Q.p[5];
+        const cpx<T>* src = reinterpret_cast<const cpx<T>*>(qp) + (int64_t)jy * g.PA + kx;
         cpx<T> v[E];
 #pragma unroll
         for (int s = 0; s < E; ++s)

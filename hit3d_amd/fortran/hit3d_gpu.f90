@@ -184,7 +184,7 @@ contains
     tau = xl / sqrt(uvar)
     inquire(file='stat1.gp', exist=there)
     open(69, file='stat1.gp', position='append')
-    if (.not. there) write(69,*) '# 1.itime 2.time         3.energy       4.diss         5.eta'// &
+    if (.not. there) write(69,'(A)') ' # 1.itime 2.time         3.energy       4.diss         5.eta'// &
          '          6.enstrophy    7.R_lambda'
     write(69, "(i8,20e15.6)") itime, time, en, eps_v, eta, enstrophy, rl
     close(69)
