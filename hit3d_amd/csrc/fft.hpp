@@ -145,15 +145,49 @@ __device__ __forceinline__ cpx<T> ldtw(const cpx<T>* tw, int j) {
     return tw[j];
 }
 
-// One Stockham stage at span Pspan, then recursion to the next.
-// v: the thread's E slots.  lds: this FFT's N-entry exchange buffer with
-// element j at lds[j*LS].  t: the thread's index within the FFT.
-template <int N, int E, int DIR, typename T, int LS, int Pspan>
+// LDS entry padding: one spare entry every 2^PADSH entries spreads the
+// strided Stockham writes over all banks (PADSH = 0: no padding).
+template <int PADSH>
+__device__ __forceinline__ int padi(int i) {
+    if constexpr (PADSH > 0) return i + (i >> PADSH);
+    else return i;
+}
+template <int PADSH>
+__host__ __device__ constexpr int padded_size(int n) {
+    return PADSH > 0 ? n + (n >> PADSH) + 1 : n;
+}
+// PADSH that gives one pad per 256 bytes of entries
+template <typename T>
+struct PadFor {
+    static constexpr int value = sizeof(cpx<T>) == 16 ? 4 : 5;
+};
+
+// Synchronisation between LDS exchange phases: the whole block, or only the
+// wavefront when every transform lives inside one wavefront (LDS operations
+// of one wavefront complete in issue order; the fences and the wave barrier
+// keep the compiler from moving accesses across the phase boundary).
+template <bool WAVE>
+__device__ __forceinline__ void lds_sync() {
+    if constexpr (WAVE) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
+// One Stockham stage at span Pspan for NF independent transforms that share
+// the thread mapping (and so the twiddles), then recursion to the next.
+// v[f][s]: transform f, slot s.  lds: exchange buffer; element j of
+// transform f sits at lds[f*FS + padi(j*LS)].  t: thread index in the transform.
+template <int N, int E, int DIR, typename T, int LS, int NF, int FS, int PADSH, bool WAVE,
+          int Pspan>
 struct Stages {
     static constexpr int R = (E < N / Pspan) ? E : N / Pspan;
     static constexpr int TPF = N / E;
     static constexpr int NB = E / R;  // butterflies per thread
-    __device__ static __forceinline__ void run(cpx<T> (&v)[E], int t, cpx<T>* lds,
+    __device__ static __forceinline__ void run(cpx<T> (&v)[NF][E], int t, cpx<T>* lds,
                                                const cpx<T>* __restrict__ tw) {
 #pragma unroll
         for (int q = 0; q < NB; ++q) {
@@ -164,35 +198,50 @@ struct Stages {
                 for (int m = 1; m < R; ++m) {
                     cpx<T> w = ldtw(tw, m * k * (N / (Pspan * R)));
                     if constexpr (DIR > 0) w.y = -w.y;
-                    v[q + m * NB] = cmul(v[q + m * NB], w);
+#pragma unroll
+                    for (int f = 0; f < NF; ++f) v[f][q + m * NB] = cmul(v[f][q + m * NB], w);
                 }
             }
-            DFT<R, DIR, T, NB>::run(v + q);
+#pragma unroll
+            for (int f = 0; f < NF; ++f) DFT<R, DIR, T, NB>::run(v[f] + q);
         }
         if constexpr (Pspan * R < N) {
-            __syncthreads();  // previous readers of lds are done
+            lds_sync<WAVE>();  // previous readers of lds are done
 #pragma unroll
             for (int q = 0; q < NB; ++q) {
                 const int b = t + q * TPF;
                 const int k = b & (Pspan - 1);
                 const int base = (b - k) * R + k;
 #pragma unroll
-                for (int m = 0; m < R; ++m) lds[(base + m * Pspan) * LS] = v[q + m * NB];
-            }
-            __syncthreads();
+                for (int m = 0; m < R; ++m)
 #pragma unroll
-            for (int s = 0; s < E; ++s) v[s] = lds[(t + s * TPF) * LS];
-            Stages<N, E, DIR, T, LS, Pspan * R>::run(v, t, lds, tw);
+                    for (int f = 0; f < NF; ++f)
+                        lds[f * FS + padi<PADSH>((base + m * Pspan) * LS)] = v[f][q + m * NB];
+            }
+            lds_sync<WAVE>();
+#pragma unroll
+            for (int s = 0; s < E; ++s)
+#pragma unroll
+                for (int f = 0; f < NF; ++f) v[f][s] = lds[f * FS + padi<PADSH>((t + s * TPF) * LS)];
+            Stages<N, E, DIR, T, LS, NF, FS, PADSH, WAVE, Pspan * R>::run(v, t, lds, tw);
         }
     }
 };
 
-// Full transform.  Every thread of the block must call this (it contains
-// block-wide barriers), also threads whose data is masked out.
-template <int N, int E, int DIR, typename T, int LS>
+// NF transforms at once.  Every thread taking part in the exchange must call
+// this (it synchronises), also threads whose data is masked out.
+template <int N, int E, int DIR, typename T, int LS, int NF, int FS, int PADSH, bool WAVE>
+__device__ __forceinline__ void multi_fft(cpx<T> (&v)[NF][E], int t, cpx<T>* lds,
+                                          const cpx<T>* __restrict__ tw) {
+    Stages<N, E, DIR, T, LS, NF, FS, PADSH, WAVE, 1>::run(v, t, lds, tw);
+}
+
+// One transform with block-wide synchronisation; element j at lds[padi(j*LS)].
+template <int N, int E, int DIR, typename T, int LS, int PADSH = 0>
 __device__ __forceinline__ void block_fft(cpx<T> (&v)[E], int t, cpx<T>* lds,
                                           const cpx<T>* __restrict__ tw) {
-    Stages<N, E, DIR, T, LS, 1>::run(v, t, lds, tw);
+    cpx<T>(&vv)[1][E] = *reinterpret_cast<cpx<T>(*)[1][E]>(&v);
+    Stages<N, E, DIR, T, LS, 1, 0, PADSH, false, 1>::run(vv, t, lds, tw);
 }
 
 }  // namespace hit3d

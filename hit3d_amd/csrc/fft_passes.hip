@@ -36,7 +36,8 @@ __global__ __launch_bounds__(B* N / E) void k_cols(const cpx<T>* __restrict__ in
                                                    int64_t s_out, int ncols,
                                                    const cpx<T>* __restrict__ tw) {
     constexpr int TPF = N / E;
-    __shared__ cpx<T> lds[N * B];
+    constexpr int PS = PadFor<T>::value;
+    __shared__ cpx<T> lds[padded_size<PS>(N * B)];
     const int c = threadIdx.x % B;
     const int t = threadIdx.x / B;
     const int col = blockIdx.x * B + c;
@@ -48,7 +49,7 @@ __global__ __launch_bounds__(B* N / E) void k_cols(const cpx<T>* __restrict__ in
 #pragma unroll
     for (int s = 0; s < E; ++s)
         v[s] = valid ? src[(int64_t)(t + s * TPF) * s_in] : cpx<T>{(T)0, (T)0};
-    block_fft<N, E, DIR, T, B>(v, t, lds + c, tw + N);
+    block_fft<N, E, DIR, T, B, PS>(v, t, lds + c, tw + N);
     if (valid) {
 #pragma unroll
         for (int s = 0; s < E; ++s) dst[(int64_t)(t + s * TPF) * s_out] = v[s];

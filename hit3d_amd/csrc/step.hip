@@ -49,9 +49,9 @@ struct CE {
 template <int N, typename T>
 struct XC {
     static constexpr int M = N / 2;
-    // FP64 keeps three physical rows in registers: 4 values per thread keeps
-    // the kernel under 128 VGPRs (4 waves per SIMD); FP32 affords 8.
-    static constexpr int E = (M < 8) ? M : (sizeof(T) == 8 ? 4 : 8);
+    // three physical rows and three products stay in registers: 4 values per
+    // thread and transform keeps the kernel at 3 waves per SIMD
+    static constexpr int E = (M < 4) ? M : 4;
     static constexpr int TPF = M / E;
     static constexpr int ROWS = (256 / TPF < N * N) ? 256 / TPF : N * N;
     static constexpr int THREADS = ROWS * TPF;
@@ -78,7 +78,8 @@ template <int N, typename T>
 __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_inv_z(SG g, P3 F, W6<T> A,
                                                              const cpx<T>* __restrict__ tw) {
     constexpr int E = CE<N, T>::E, B = CE<N, T>::B, TPF = CE<N, T>::TPF;
-    __shared__ cpx<T> lds[N * B];
+    constexpr int PS = PadFor<T>::value;
+    __shared__ cpx<T> lds[padded_size<PS>(N * B)];
     const int c = threadIdx.x % B, t = threadIdx.x / B;
     const int kx = blockIdx.x * B + c;
     const bool valid = kx < g.NK;
@@ -92,7 +93,7 @@ __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_inv_z(SG g, P3 F, W6<T>
         const int e = t + s * TPF;
         v[s] = (valid && kept(e, g.K, N)) ? to<T>(src[(int64_t)e * g.P]) : cpx<T>{(T)0, (T)0};
     }
-    block_fft<N, E, 1, T, B>(v, t, lds + c, tw + N);
+    block_fft<N, E, 1, T, B, PS>(v, t, lds + c, tw + N);
     if (valid) {
 #pragma unroll
         for (int s = 0; s < E; ++s) dst[(int64_t)(t + s * TPF) * N * g.PA] = v[s];
@@ -106,7 +107,8 @@ template <int N, int DIR, typename T>
 __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_y(SG g, W6<T> A,
                                                          const cpx<T>* __restrict__ tw) {
     constexpr int E = CE<N, T>::E, B = CE<N, T>::B, TPF = CE<N, T>::TPF;
-    __shared__ cpx<T> lds[N * B];
+    constexpr int PS = PadFor<T>::value;
+    __shared__ cpx<T> lds[padded_size<PS>(N * B)];
     const int c = threadIdx.x % B, t = threadIdx.x / B;
     const int kx = blockIdx.x * B + c;
     const bool valid = kx < g.NK;
@@ -119,7 +121,7 @@ __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_y(SG g, W6<T> A,
         const bool ld = valid && (DIR < 0 || kept(e, g.K, N));
         v[s] = ld ? base[(int64_t)e * g.PA] : cpx<T>{(T)0, (T)0};
     }
-    block_fft<N, E, DIR, T, B>(v, t, lds + c, tw + N);
+    block_fft<N, E, DIR, T, B, PS>(v, t, lds + c, tw + N);
     if (valid) {
 #pragma unroll
         for (int s = 0; s < E; ++s) {
@@ -130,48 +132,60 @@ __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_y(SG g, W6<T> A,
 }
 
 // ---------------------------------------------------------------- C
+// Rows live inside one wavefront (TPF divides 64), so the LDS exchanges of
+// the x transforms synchronise per wavefront, and u, v, w (then three
+// products at a time) go through each stage together, sharing twiddles.
 template <int N, typename T>
 __global__ __launch_bounds__((XC<N, T>::THREADS)) void k_x(SG g, W6<T> Wb,
                                                          const cpx<T>* __restrict__ tw, T norm,
                                                          unsigned long long* maxbits) {
     using C = XC<N, T>;
     constexpr int M = C::M, E = C::E, TPF = C::TPF;
-    __shared__ cpx<T> lds[C::ROWS][M + 1];
+    constexpr int PS = PadFor<T>::value;
+    constexpr int FS = padded_size<PS>(M + 1);  // one transform's padded LDS row
+    static_assert(64 % TPF == 0 || TPF % 64 == 0, "row transforms must not straddle waves");
+    constexpr bool WAVE = TPF <= 64;
+    __shared__ cpx<T> lds[C::ROWS][3 * FS];
     __shared__ double redm[C::THREADS / 64 + 1];
     const int t = threadIdx.x % TPF;
     const int rr = threadIdx.x / TPF;
     const int64_t row = (int64_t)blockIdx.x * C::ROWS + rr;
     const cpx<T>* twN = tw + N;
-    cpx<T> ph[3][E];
+    cpx<T>* L = lds[rr];
     // c2r of u, v, w (x_fftw.f90:1066-1073 semantics: Im of kx=0 ignored)
+    cpx<T> ph[3][E];
 #pragma unroll
     for (int comp = 0; comp < 3; ++comp) {
         const cpx<T>* rowp = reinterpret_cast<const cpx<T>*>(Wb.p[comp]) + row * g.PA;
-        __syncthreads();
 #pragma unroll
         for (int s = 0; s < E; ++s) {
             const int k = t + s * TPF;
-            lds[rr][k] = (k < g.NK) ? rowp[k] : cpx<T>{(T)0, (T)0};
+            L[comp * FS + padi<PS>(k)] = (k < g.NK) ? rowp[k] : cpx<T>{(T)0, (T)0};
         }
-        if (t == 0) lds[rr][M] = cpx<T>{(T)0, (T)0};
-        __syncthreads();
-        cpx<T> v[E];
+        if (t == 0) L[comp * FS + padi<PS>(M)] = cpx<T>{(T)0, (T)0};
+    }
+    lds_sync<WAVE>();
 #pragma unroll
-        for (int s = 0; s < E; ++s) {
-            const int k = t + s * TPF;
-            cpx<T> xk = lds[rr][k];
-            cpx<T> xm = lds[rr][M - k];
+    for (int s = 0; s < E; ++s) {
+        const int k = t + s * TPF;
+        const cpx<T> w = twN[k];
+#pragma unroll
+        for (int comp = 0; comp < 3; ++comp) {
+            cpx<T> xk = L[comp * FS + padi<PS>(k)];
+            cpx<T> xm = L[comp * FS + padi<PS>(M - k)];
             if (k == 0) {
                 xk.y = (T)0;
                 xm.y = (T)0;
             }
             xm = conj(xm);
-            v[s] = (xk + xm) + mul_i(cmulc(xk - xm, twN[k]));
+            ph[comp][s] = (xk + xm) + mul_i(cmulc(xk - xm, w));
         }
-        block_fft<M, E, 1, T, 1>(v, t, lds[rr], tw + M);
-#pragma unroll
-        for (int s = 0; s < E; ++s) ph[comp][s] = scale(v[s], norm);
     }
+    multi_fft<M, E, 1, T, 1, 3, FS, PS, WAVE>(ph, t, L, tw + M);
+#pragma unroll
+    for (int s = 0; s < E; ++s)
+#pragma unroll
+        for (int comp = 0; comp < 3; ++comp) ph[comp][s] = scale(ph[comp][s], norm);
     // Courant number, rhs_velocity.f90:31-32 (FP64 max)
     double m = 0.0;
 #pragma unroll
@@ -181,43 +195,46 @@ __global__ __launch_bounds__((XC<N, T>::THREADS)) void k_x(SG g, W6<T> Wb,
         m = a > m ? a : m;
         m = b > m ? b : m;
     }
-    // six products, rhs_velocity.f90:56-72, then r2c keeping kx <= kmax
-    // factor indices of products p = uu, uv, uw, vv, vw, ww packed 4 bits
-    // per product (a table indexed at run time would live in scratch)
-    constexpr unsigned pa = 0x211000u, pb = 0x221210u;
+    // six products (rhs_velocity.f90:56-72) in two groups of three, r2c
+    // keeping kx <= kmax: group 0 = uu, uv, uw; group 1 = vv, vw, ww
 #pragma unroll
-    for (int p = 0; p < 6; ++p) {
-        const int ia = (pa >> (4 * p)) & 15, ib = (pb >> (4 * p)) & 15;
-        cpx<T> v[E];
+    for (int grp = 0; grp < 2; ++grp) {
+        cpx<T> v[3][E];
 #pragma unroll
         for (int s = 0; s < E; ++s) {
-            const cpx<T> a = ia == 0 ? ph[0][s] : (ia == 1 ? ph[1][s] : ph[2][s]);
-            const cpx<T> b = ib == 0 ? ph[0][s] : (ib == 1 ? ph[1][s] : ph[2][s]);
-            v[s] = cpx<T>{a.x * b.x, a.y * b.y};
-        }
-        block_fft<M, E, -1, T, 1>(v, t, lds[rr], tw + M);
-        __syncthreads();
-#pragma unroll
-        for (int s = 0; s < E; ++s) lds[rr][t + s * TPF] = v[s];
-        __syncthreads();
-        T* op = Wb.p[0];
-        if (p == 1) op = Wb.p[1];
-        if (p == 2) op = Wb.p[2];
-        if (p == 3) op = Wb.p[3];
-        if (p == 4) op = Wb.p[4];
-        if (p == 5) op = Wb.p[5];
-        cpx<T>* outp = reinterpret_cast<cpx<T>*>(op) + row * g.PA;
-#pragma unroll
-        for (int s = 0; s < E; ++s) {
-            const int k = t + s * TPF;
-            if (k < g.NK) {
-                cpx<T> zk = lds[rr][k];
-                cpx<T> zm = conj(lds[rr][(M - k) & (M - 1)]);
-                cpx<T> e = scale(zk + zm, (T)0.5);
-                cpx<T> o = scale(mul_mi(zk - zm), (T)0.5);
-                outp[k] = e + cmul(twN[k], o);
+            if (grp == 0) {
+                v[0][s] = cpx<T>{ph[0][s].x * ph[0][s].x, ph[0][s].y * ph[0][s].y};
+                v[1][s] = cpx<T>{ph[0][s].x * ph[1][s].x, ph[0][s].y * ph[1][s].y};
+                v[2][s] = cpx<T>{ph[0][s].x * ph[2][s].x, ph[0][s].y * ph[2][s].y};
+            } else {
+                v[0][s] = cpx<T>{ph[1][s].x * ph[1][s].x, ph[1][s].y * ph[1][s].y};
+                v[1][s] = cpx<T>{ph[1][s].x * ph[2][s].x, ph[1][s].y * ph[2][s].y};
+                v[2][s] = cpx<T>{ph[2][s].x * ph[2][s].x, ph[2][s].y * ph[2][s].y};
             }
         }
+        multi_fft<M, E, -1, T, 1, 3, FS, PS, WAVE>(v, t, L, tw + M);
+        lds_sync<WAVE>();
+#pragma unroll
+        for (int s = 0; s < E; ++s)
+#pragma unroll
+            for (int f = 0; f < 3; ++f) L[f * FS + padi<PS>(t + s * TPF)] = v[f][s];
+        lds_sync<WAVE>();
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            cpx<T>* outp = reinterpret_cast<cpx<T>*>(Wb.p[3 * grp + f]) + row * g.PA;
+#pragma unroll
+            for (int s = 0; s < E; ++s) {
+                const int k = t + s * TPF;
+                if (k < g.NK) {
+                    cpx<T> zk = L[f * FS + padi<PS>(k)];
+                    cpx<T> zm = conj(L[f * FS + padi<PS>((M - k) & (M - 1))]);
+                    cpx<T> e = scale(zk + zm, (T)0.5);
+                    cpx<T> o = scale(mul_mi(zk - zm), (T)0.5);
+                    outp[k] = e + cmul(twN[k], o);
+                }
+            }
+        }
+        lds_sync<WAVE>();
     }
     // block max -> one atomic
 #pragma unroll
@@ -248,7 +265,8 @@ template <int N, typename T>
 __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_fwd_z_update(SG g, W6<T> Q, Upd u,
                                                                     const cpx<T>* __restrict__ tw) {
     constexpr int E = CE<N, T>::E, B = CE<N, T>::B, TPF = CE<N, T>::TPF;
-    __shared__ cpx<T> lds[N * B];
+    constexpr int PS = PadFor<T>::value;
+    __shared__ cpx<T> lds[padded_size<PS>(N * B)];
     const int c = threadIdx.x % B, t = threadIdx.x / B;
     const int kx = blockIdx.x * B + c;
     const bool valid = kx < g.NK;
@@ -273,7 +291,7 @@ __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_fwd_z_update(SG g, W6<T
 #pragma unroll
         for (int s = 0; s < E; ++s)
             v[s] = valid ? src[(int64_t)(t + s * TPF) * N * g.PA] : cpx<T>{(T)0, (T)0};
-        block_fft<N, E, -1, T, B>(v, t, lds + c, tw + N);
+        block_fft<N, E, -1, T, B, PS>(v, t, lds + c, tw + N);
 #pragma unroll
         for (int s = 0; s < E; ++s) {
             const T kzT = (T)wavenum(t + s * TPF, N);
