@@ -22,7 +22,8 @@ void check(hipError_t e, const char* what) {
 }
 
 static const char* kTimerNames =
-    "fft_x,fft_y,fft_z,pointwise,reduce,step_inv_z,step_inv_y,step_x,step_fwd_y,step_fwd_z";
+    "fft_x,fft_y,fft_z,pointwise,reduce,step_inv_z,step_inv_y,step_x,step_fwd_y,step_fwd_z,"
+    "step_update";
 
 Timed::Timed(int c) : cls(c) {
     Lib& L = lib();

@@ -20,6 +20,7 @@ enum TimerClass {
     TC_STEP_X,
     TC_STEP_FWD_Y,
     TC_STEP_FWD_Z,
+    TC_STEP_UPDATE,
     TC_COUNT
 };
 
