@@ -96,8 +96,7 @@ def kernel_bytes(n, cls, precision):
         "step_inv_y": 3 * zy + 3 * full,
         "step_x": 3 * full + 6 * full,
         "step_fwd_y": 6 * full + 6 * zy,
-        "step_fwd_z": 6 * zy + 3 * (cube // 16 * w),
-        "step_update": 3 * (cube // 16 * w) + 6 * cube + 6 * cube,
+        "step_fwd_z": 6 * zy + 6 * cube + 6 * cube,   # products in; fields, rhs_old in/out
     }
     return table.get(cls)
 

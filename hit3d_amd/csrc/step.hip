@@ -46,16 +46,6 @@ struct CE {
     static constexpr int THREADS = B * TPF;
 };
 
-// kernel E keeps three RHS accumulators per value: 4 values per thread when
-// a 1024-thread block still covers the 128-byte column tile
-template <int N, typename T>
-struct CEz {
-    static constexpr int B = BCols<T>::B;
-    static constexpr int E = (N >= 64 && B * N / 4 <= 1024) ? 4 : CE<N, T>::E;
-    static constexpr int TPF = N / E;
-    static constexpr int THREADS = B * TPF;
-};
-
 template <int N, typename T>
 struct XC {
     static constexpr int M = N / 2;
@@ -88,7 +78,7 @@ template <int N, typename T>
 __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_inv_z(SG g, P3 F, W6<T> A,
                                                              const cpx<T>* __restrict__ tw) {
     constexpr int E = CE<N, T>::E, B = CE<N, T>::B, TPF = CE<N, T>::TPF;
-    constexpr int PS = PadFor<T>::value;
+    constexpr int PS = 0;  // padding does not pay in the column passes
     __shared__ cpx<T> lds[padded_size<PS>(N * B)];
     const int c = threadIdx.x % B, t = threadIdx.x / B;
     const int kx = blockIdx.x * B + c;
@@ -117,7 +107,7 @@ template <int N, int DIR, typename T>
 __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_y(SG g, W6<T> A,
                                                          const cpx<T>* __restrict__ tw) {
     constexpr int E = CE<N, T>::E, B = CE<N, T>::B, TPF = CE<N, T>::TPF;
-    constexpr int PS = PadFor<T>::value;
+    constexpr int PS = 0;
     __shared__ cpx<T> lds[padded_size<PS>(N * B)];
     const int c = threadIdx.x % B, t = threadIdx.x / B;
     const int kx = blockIdx.x * B + c;
@@ -272,10 +262,10 @@ struct Upd {
 };
 
 template <int N, typename T>
-__global__ __launch_bounds__((CEz<N, T>::THREADS)) void k_fwd_z_rhs(SG g, W6<T> Q, W6<T> Tb,
-                                                                 const cpx<T>* __restrict__ tw) {
-    constexpr int E = CEz<N, T>::E, B = CEz<N, T>::B, TPF = CEz<N, T>::TPF;
-    constexpr int PS = PadFor<T>::value;
+__global__ __launch_bounds__((CE<N, T>::THREADS)) void k_fwd_z_update(SG g, W6<T> Q, Upd u,
+                                                                    const cpx<T>* __restrict__ tw) {
+    constexpr int E = CE<N, T>::E, B = CE<N, T>::B, TPF = CE<N, T>::TPF;
+    constexpr int PS = 0;
     __shared__ cpx<T> lds[padded_size<PS>(N * B)];
     const int c = threadIdx.x % B, t = threadIdx.x / B;
     const int kx = blockIdx.x * B + c;
@@ -302,70 +292,44 @@ __global__ __launch_bounds__((CEz<N, T>::THREADS)) void k_fwd_z_rhs(SG g, W6<T> 
         for (int s = 0; s < E; ++s)
             v[s] = valid ? src[(int64_t)(t + s * TPF) * N * g.PA] : cpx<T>{(T)0, (T)0};
         block_fft<N, E, -1, T, B, PS>(v, t, lds + c, tw + N);
-        // p: 0 uu, 1 uv, 2 uw, 3 vv, 4 vw, 5 ww.  Product p enters t_c with
-        // the wavenumber of the other index (0: none, 1: kx, 2: ky, 3: kz);
-        // adding 0*P leaves a sum unchanged, so the order of the reference's
-        // three-term sums is kept.
-        const int w0 = (0x000321 >> (4 * p)) & 15;
-        const int w1 = (0x032010 >> (4 * p)) & 15;
-        const int w2 = (0x320100 >> (4 * p)) & 15;
 #pragma unroll
         for (int s = 0; s < E; ++s) {
             const T kzT = (T)wavenum(t + s * TPF, N);
-            const T c0 = w0 == 1 ? kxT : (w0 == 2 ? kyT : (w0 == 3 ? kzT : (T)0));
-            const T c1 = w1 == 1 ? kxT : (w1 == 2 ? kyT : (w1 == 3 ? kzT : (T)0));
-            const T c2 = w2 == 1 ? kxT : (w2 == 2 ? kyT : (w2 == 3 ? kzT : (T)0));
-            acc[0][s] = acc[0][s] + scale(v[s], c0);
-            acc[1][s] = acc[1][s] + scale(v[s], c1);
-            acc[2][s] = acc[2][s] + scale(v[s], c2);
+            // p: 0 uu, 1 uv, 2 uw, 3 vv, 4 vw, 5 ww
+            if (p == 0) acc[0][s] = scale(v[s], kxT);
+            if (p == 1) {
+                acc[0][s] = acc[0][s] + scale(v[s], kyT);
+                acc[1][s] = scale(v[s], kxT);
+            }
+            if (p == 2) {
+                acc[0][s] = acc[0][s] + scale(v[s], kzT);
+                acc[2][s] = scale(v[s], kxT);
+            }
+            if (p == 3) acc[1][s] = acc[1][s] + scale(v[s], kyT);
+            if (p == 4) {
+                acc[1][s] = acc[1][s] + scale(v[s], kzT);
+                acc[2][s] = acc[2][s] + scale(v[s], kyT);
+            }
+            if (p == 5) acc[2][s] = acc[2][s] + scale(v[s], kzT);
         }
     }
-    if (!valid) return;
-    // t_i into the compact cube buffers [ky'][kz'][kx] (kept rows only)
-    const int nky = 2 * g.K + 1;
-#pragma unroll
-    for (int s = 0; s < E; ++s) {
-        const int e = t + s * TPF;
-        if (!kept(e, g.K, N)) continue;
-        const int kzp = e <= g.K ? e : e - (N - nky);
-        const int64_t o = ((int64_t)blockIdx.y * nky + kzp) * g.PA + kx;
-        reinterpret_cast<cpx<T>*>(Tb.p[0])[o] = acc[0][s];
-        reinterpret_cast<cpx<T>*>(Tb.p[1])[o] = acc[1][s];
-        reinterpret_cast<cpx<T>*>(Tb.p[2])[o] = acc[2][s];
-    }
-}
-
-// RHS assembly from t_i (rhs_velocity.f90:85-136), band forcing
-// (m_force.f90:153-173), Euler/AB2 update (main.f90:156-164) and projection
-// (pressure.f90:13-47) for every entry of the 2/3-rule cube.
-template <typename T>
-__global__ void k_update(SG g, int n, W6<T> Tb, Upd u) {
-    const int nky = 2 * g.K + 1;
     // band forcing factor, m_force.f90:153-161
     double fac = 0.0;
     if (u.forced) {
         double en = 0.0;
         for (int i = 0; i < u.kfmax; ++i) en += u.eshell[i];
         fac = u.famp / en;
-        if (blockIdx.x == 0 && threadIdx.x == 0) *u.energy_out = en;
+        if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *u.energy_out = en;
     }
-    const int64_t total = (int64_t)nky * nky * g.NK;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int kx = (int)(i % g.NK);
-        const int64_t r = i / g.NK;
-        const int kzp = (int)(r % nky), jyp = (int)(r / nky);
-        const int jy = kept_row(jyp, g.K, n), e = kept_row(kzp, g.K, n);
-        const int ky = wavenum(jy, n), kz = wavenum(e, n);
-        const double kxd = kx, kyd = ky, kzd = kz;
-        const int64_t toff = ((int64_t)jyp * nky + kzp) * g.PA + kx;
-        const int64_t off = ((int64_t)jy * n + e) * g.P + kx;
-        cpx<T> acc[3][1];
-        acc[0][0] = reinterpret_cast<const cpx<T>*>(Tb.p[0])[toff];
-        acc[1][0] = reinterpret_cast<const cpx<T>*>(Tb.p[1])[toff];
-        acc[2][0] = reinterpret_cast<const cpx<T>*>(Tb.p[2])[toff];
-        const int s = 0;
-        const int N = n;
+    if (!valid) return;
+    const double kxd = kx, kyd = ky;
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+        const int e = t + s * TPF;
+        if (!kept(e, g.K, N)) continue;
+        const int kz = wavenum(e, N);
+        const double kzd = kz;
+        const int64_t off = ((int64_t)jy * N + e) * g.P + kx;
         const double k2 = kxd * kxd + kyd * kyd + kzd * kzd;
         const double lap = -u.nu * k2;
         const bool fnode = u.forced && kx < N / 2 && [&] {
@@ -458,22 +422,8 @@ static void run(Lib& L, double dt, bool first, const cpx<T>* tw) {
         hipLaunchKernelGGL((k_y<N, -1, T>), dim3(ncb, N, 6), dim3(CE<N, T>::THREADS), 0, L.st, g,
                            Wb, tw);
     }
-    // t_i cube buffers: two in wrk(0), one in the spare slot (both free here)
-    W6<T> Tb;
-    const int64_t cube = (int64_t)nky * nky * g.PA * 2;  // T elements per component
-    Tb.p[0] = reinterpret_cast<T*>(L.W[0]);
-    Tb.p[1] = reinterpret_cast<T*>(L.W[0]) + cube;
-    Tb.p[2] = reinterpret_cast<T*>(L.spare);
-    Tb.p[3] = Tb.p[4] = Tb.p[5] = nullptr;
-    if (2 * cube * (int64_t)sizeof(T) > L.g.nc * (int64_t)sizeof(cpx<double>))
-        throw HipError{"fused step: work slot too small for the RHS cube"};
     {
         Timed tm(TC_STEP_FWD_Z);
-        hipLaunchKernelGGL((k_fwd_z_rhs<N, T>), dim3(ncb, nky), dim3(CEz<N, T>::THREADS), 0, L.st,
-                           g, Wb, Tb, tw);
-    }
-    {
-        Timed tm(TC_STEP_UPDATE);
         Upd u;
         for (int i = 0; i < 3; ++i) {
             u.F[i] = L.F[i + 1];
@@ -487,10 +437,8 @@ static void run(Lib& L, double dt, bool first, const cpx<T>* tw) {
         u.kfmax = L.kfmax;
         u.forced = forced ? 1 : 0;
         u.first = first ? 1 : 0;
-        const int64_t total = (int64_t)nky * nky * g.NK;
-        int64_t nb = (total + 255) / 256;
-        if (nb > 65536) nb = 65536;
-        hipLaunchKernelGGL((k_update<T>), dim3((unsigned)nb), dim3(256), 0, L.st, g, N, Tb, u);
+        hipLaunchKernelGGL((k_fwd_z_update<N, T>), dim3(ncb, nky), dim3(CE<N, T>::THREADS), 0,
+                           L.st, g, Wb, u, tw);
     }
 }
 
