@@ -46,13 +46,12 @@ struct CE {
     static constexpr int THREADS = B * TPF;
 };
 
+// x pass as full-length complex transforms of two real rows at once:
+// z = a + i b, TPF threads per row, all inside one wavefront
 template <int N, typename T>
-struct XC {
-    static constexpr int M = N / 2;
-    // three physical rows and three products stay in registers: 4 values per
-    // thread and transform keeps the kernel at 3 waves per SIMD
-    static constexpr int E = (M < 4) ? M : 4;
-    static constexpr int TPF = M / E;
+struct XP {
+    static constexpr int E = (N >= 1024) ? 16 : (N >= 8 ? 8 : N);
+    static constexpr int TPF = N / E;
     static constexpr int ROWS = (256 / TPF < N * N) ? 256 / TPF : N * N;
     static constexpr int THREADS = ROWS * TPF;
 };
@@ -131,110 +130,105 @@ __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_y(SG g, W6<T> A,
     }
 }
 
-// ---------------------------------------------------------------- C
-// Rows live inside one wavefront (TPF divides 64), so the LDS exchanges of
-// the x transforms synchronise per wavefront, and u, v, w (then three
-// products at a time) go through each stage together, sharing twiddles.
+// ---------------------------------------------------------------- C (pairs)
+// The x pass with two real rows per complex transform of length N.
+// Inverse: Z_k = U_k + i V_k with U_{N-k} = conj U_k (Im of kx=0 ignored,
+// as the reference's c2r does, x_fftw.f90:1066-1073); z_n = u_n + i v_n.
+// w rides alone as Z_k = W_k.  Forward: z = p + i q for the product pairs
+// (uu,uv), (uw,vv), (vw,ww); P_k = (Z_k + conj Z_{N-k})/2,
+// Q_k = -i (Z_k - conj Z_{N-k})/2, kept for kx <= kmax.  One LDS row of N
+// entries per row, exchanges synchronised per wavefront.
 template <int N, typename T>
-__global__ __launch_bounds__((XC<N, T>::THREADS)) void k_x(SG g, W6<T> Wb,
-                                                         const cpx<T>* __restrict__ tw, T norm,
-                                                         unsigned long long* maxbits) {
-    using C = XC<N, T>;
-    constexpr int M = C::M, E = C::E, TPF = C::TPF;
+__device__ __forceinline__ cpx<T> herm_load(const cpx<T>* rowp, int k, int nk) {
+    // entry k of the Hermitian-extended half spectrum (zero outside kx < nk)
+    if (2 * k <= N) {
+        cpx<T> a = (k < nk) ? rowp[k] : cpx<T>{(T)0, (T)0};
+        if (k == 0) a.y = (T)0;
+        return a;
+    }
+    const int kk = N - k;
+    return (kk < nk) ? conj(rowp[kk]) : cpx<T>{(T)0, (T)0};
+}
+
+template <int N, typename T>
+__global__ __launch_bounds__((XP<N, T>::THREADS)) void k_xp(SG g, W6<T> Wb,
+                                                          const cpx<T>* __restrict__ tw, T norm,
+                                                          unsigned long long* maxbits) {
+    using C = XP<N, T>;
+    constexpr int E = C::E, TPF = C::TPF;
     constexpr int PS = PadFor<T>::value;
-    constexpr int FS = padded_size<PS>(M + 1);  // one transform's padded LDS row
+    constexpr int FS = padded_size<PS>(N);
     static_assert(64 % TPF == 0 || TPF % 64 == 0, "row transforms must not straddle waves");
     constexpr bool WAVE = TPF <= 64;
-    __shared__ cpx<T> lds[C::ROWS][3 * FS];
+    __shared__ cpx<T> lds[C::ROWS][FS];
     __shared__ double redm[C::THREADS / 64 + 1];
     const int t = threadIdx.x % TPF;
     const int rr = threadIdx.x / TPF;
     const int64_t row = (int64_t)blockIdx.x * C::ROWS + rr;
-    const cpx<T>* twN = tw + N;
     cpx<T>* L = lds[rr];
-    // c2r of u, v, w (x_fftw.f90:1066-1073 semantics: Im of kx=0 ignored)
-    cpx<T> ph[3][E];
-#pragma unroll
-    for (int comp = 0; comp < 3; ++comp) {
-        const cpx<T>* rowp = reinterpret_cast<const cpx<T>*>(Wb.p[comp]) + row * g.PA;
+    T u[E], v[E], w[E];
+    {
+        const cpx<T>* U = reinterpret_cast<const cpx<T>*>(Wb.p[0]) + row * g.PA;
+        const cpx<T>* V = reinterpret_cast<const cpx<T>*>(Wb.p[1]) + row * g.PA;
+        cpx<T> z[1][E];
 #pragma unroll
         for (int s = 0; s < E; ++s) {
             const int k = t + s * TPF;
-            L[comp * FS + padi<PS>(k)] = (k < g.NK) ? rowp[k] : cpx<T>{(T)0, (T)0};
+            const cpx<T> a = herm_load<N>(U, k, g.NK), b = herm_load<N>(V, k, g.NK);
+            z[0][s] = cpx<T>{a.x - b.y, a.y + b.x};
         }
-        if (t == 0) L[comp * FS + padi<PS>(M)] = cpx<T>{(T)0, (T)0};
-    }
-    lds_sync<WAVE>();
+        multi_fft<N, E, 1, T, 1, 1, FS, PS, WAVE>(z, t, L, tw + N);
 #pragma unroll
-    for (int s = 0; s < E; ++s) {
-        const int k = t + s * TPF;
-        const cpx<T> w = twN[k];
-#pragma unroll
-        for (int comp = 0; comp < 3; ++comp) {
-            cpx<T> xk = L[comp * FS + padi<PS>(k)];
-            cpx<T> xm = L[comp * FS + padi<PS>(M - k)];
-            if (k == 0) {
-                xk.y = (T)0;
-                xm.y = (T)0;
-            }
-            xm = conj(xm);
-            ph[comp][s] = (xk + xm) + mul_i(cmulc(xk - xm, w));
+        for (int s = 0; s < E; ++s) {
+            u[s] = z[0][s].x * norm;
+            v[s] = z[0][s].y * norm;
         }
     }
-    multi_fft<M, E, 1, T, 1, 3, FS, PS, WAVE>(ph, t, L, tw + M);
+    {
+        const cpx<T>* Wr = reinterpret_cast<const cpx<T>*>(Wb.p[2]) + row * g.PA;
+        cpx<T> z[1][E];
 #pragma unroll
-    for (int s = 0; s < E; ++s)
+        for (int s = 0; s < E; ++s) z[0][s] = herm_load<N>(Wr, t + s * TPF, g.NK);
+        lds_sync<WAVE>();  // the previous transform's readers are done
+        multi_fft<N, E, 1, T, 1, 1, FS, PS, WAVE>(z, t, L, tw + N);
 #pragma unroll
-        for (int comp = 0; comp < 3; ++comp) ph[comp][s] = scale(ph[comp][s], norm);
+        for (int s = 0; s < E; ++s) w[s] = z[0][s].x * norm;
+    }
     // Courant number, rhs_velocity.f90:31-32 (FP64 max)
     double m = 0.0;
 #pragma unroll
     for (int s = 0; s < E; ++s) {
-        const double a = ((double)fabs(ph[0][s].x) + (double)fabs(ph[1][s].x)) + (double)fabs(ph[2][s].x);
-        const double b = ((double)fabs(ph[0][s].y) + (double)fabs(ph[1][s].y)) + (double)fabs(ph[2][s].y);
+        const double a = ((double)fabs(u[s]) + (double)fabs(v[s])) + (double)fabs(w[s]);
         m = a > m ? a : m;
-        m = b > m ? b : m;
     }
-    // six products (rhs_velocity.f90:56-72) in two groups of three, r2c
-    // keeping kx <= kmax: group 0 = uu, uv, uw; group 1 = vv, vw, ww
+    // product pairs (uu,uv), (uw,vv), (vw,ww): rhs_velocity.f90:56-72
 #pragma unroll
-    for (int grp = 0; grp < 2; ++grp) {
-        cpx<T> v[3][E];
+    for (int pr = 0; pr < 3; ++pr) {
+        cpx<T> z[1][E];
 #pragma unroll
         for (int s = 0; s < E; ++s) {
-            if (grp == 0) {
-                v[0][s] = cpx<T>{ph[0][s].x * ph[0][s].x, ph[0][s].y * ph[0][s].y};
-                v[1][s] = cpx<T>{ph[0][s].x * ph[1][s].x, ph[0][s].y * ph[1][s].y};
-                v[2][s] = cpx<T>{ph[0][s].x * ph[2][s].x, ph[0][s].y * ph[2][s].y};
-            } else {
-                v[0][s] = cpx<T>{ph[1][s].x * ph[1][s].x, ph[1][s].y * ph[1][s].y};
-                v[1][s] = cpx<T>{ph[1][s].x * ph[2][s].x, ph[1][s].y * ph[2][s].y};
-                v[2][s] = cpx<T>{ph[2][s].x * ph[2][s].x, ph[2][s].y * ph[2][s].y};
-            }
-        }
-        multi_fft<M, E, -1, T, 1, 3, FS, PS, WAVE>(v, t, L, tw + M);
-        lds_sync<WAVE>();
-#pragma unroll
-        for (int s = 0; s < E; ++s)
-#pragma unroll
-            for (int f = 0; f < 3; ++f) L[f * FS + padi<PS>(t + s * TPF)] = v[f][s];
-        lds_sync<WAVE>();
-#pragma unroll
-        for (int f = 0; f < 3; ++f) {
-            cpx<T>* outp = reinterpret_cast<cpx<T>*>(Wb.p[3 * grp + f]) + row * g.PA;
-#pragma unroll
-            for (int s = 0; s < E; ++s) {
-                const int k = t + s * TPF;
-                if (k < g.NK) {
-                    cpx<T> zk = L[f * FS + padi<PS>(k)];
-                    cpx<T> zm = conj(L[f * FS + padi<PS>((M - k) & (M - 1))]);
-                    cpx<T> e = scale(zk + zm, (T)0.5);
-                    cpx<T> o = scale(mul_mi(zk - zm), (T)0.5);
-                    outp[k] = e + cmul(twN[k], o);
-                }
-            }
+            if (pr == 0) z[0][s] = cpx<T>{u[s] * u[s], u[s] * v[s]};
+            if (pr == 1) z[0][s] = cpx<T>{u[s] * w[s], v[s] * v[s]};
+            if (pr == 2) z[0][s] = cpx<T>{v[s] * w[s], w[s] * w[s]};
         }
         lds_sync<WAVE>();
+        multi_fft<N, E, -1, T, 1, 1, FS, PS, WAVE>(z, t, L, tw + N);
+        lds_sync<WAVE>();
+#pragma unroll
+        for (int s = 0; s < E; ++s) L[padi<PS>(t + s * TPF)] = z[0][s];
+        lds_sync<WAVE>();
+        cpx<T>* outp = reinterpret_cast<cpx<T>*>(Wb.p[2 * pr]) + row * g.PA;
+        cpx<T>* outq = reinterpret_cast<cpx<T>*>(Wb.p[2 * pr + 1]) + row * g.PA;
+#pragma unroll
+        for (int s = 0; s < E; ++s) {
+            const int k = t + s * TPF;
+            if (k < g.NK) {
+                const cpx<T> zk = L[padi<PS>(k)];
+                const cpx<T> zm = conj(L[padi<PS>((N - k) & (N - 1))]);
+                outp[k] = scale(zk + zm, (T)0.5);
+                outq[k] = scale(mul_mi(zk - zm), (T)0.5);
+            }
+        }
     }
     // block max -> one atomic
 #pragma unroll
@@ -414,8 +408,8 @@ static void run(Lib& L, double dt, bool first, const cpx<T>* tw) {
     {
         Timed tm(TC_STEP_X);
         const T norm = (T)(1.0 / ((double)N * (double)N * (double)N));
-        hipLaunchKernelGGL((k_x<N, T>), dim3((int64_t)N * N / XC<N, T>::ROWS),
-                           dim3(XC<N, T>::THREADS), 0, L.st, g, Wb, tw, norm, maxbits);
+        hipLaunchKernelGGL((k_xp<N, T>), dim3((int64_t)N * N / XP<N, T>::ROWS),
+                           dim3(XP<N, T>::THREADS), 0, L.st, g, Wb, tw, norm, maxbits);
     }
     {
         Timed tm(TC_STEP_FWD_Y);
