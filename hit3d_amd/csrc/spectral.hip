@@ -526,6 +526,156 @@ void launch_init_rescale(const Geo& g, double* f, const double* ratio, int kmax,
 }
 
 // ---------------------------------------------------------------------------
+// scalar statistics, divergence
+// ---------------------------------------------------------------------------
+namespace {
+
+__global__ void k_shell_sums1(Geo g, const double* f, int s1, double fac, double* out) {
+    __shared__ double sh[TB / 64];
+    const int s = s1 + blockIdx.x;
+    const int side = 2 * s + 1;
+    const int64_t lo_k2 = (int64_t)s * s - s + 1, hi_k2 = (int64_t)s * s + s;
+    const int half = g.n / 2;
+    double acc = 0.0;
+    for (int p = threadIdx.x; p < side * side; p += blockDim.x) {
+        const int ky = p / side - s, kz = p % side - s;
+        if (ky <= -half || ky > half || kz <= -half || kz > half) continue;
+        const int64_t q = (int64_t)ky * ky + (int64_t)kz * kz;
+        if (q > hi_k2) continue;
+        const int64_t lo2 = lo_k2 - q;
+        int kx0 = 0;
+        if (lo2 > 0) {
+            kx0 = isqrt_floor(lo2);
+            if ((int64_t)kx0 * kx0 < lo2) ++kx0;
+        }
+        int kx1 = isqrt_floor(hi_k2 - q);
+        if (kx1 > half - 1) kx1 = half - 1;
+        const int jy = ky < 0 ? ky + g.n : ky, jz = kz < 0 ? kz + g.n : kz;
+        const int64_t rowoff = ((int64_t)jy * g.n + jz) * g.P;
+        for (int kx = kx0; kx <= kx1; ++kx) {
+            const cpx<double> a = C(f)[rowoff + kx];
+            double e1 = fac * (a.x * a.x), e2 = fac * (a.y * a.y);
+            if (kx == 0) {
+                e1 *= 0.5;
+                e2 *= 0.5;
+            }
+            acc += e1;
+            acc += e2;
+        }
+    }
+    const double tot = block_sum(acc, sh);
+    if (threadIdx.x == 0) out[blockIdx.x] = tot;
+}
+
+// per-z-plane sums of a physical slot: with b and c given, sum((a^2+b^2)+c^2);
+// otherwise sum(a) and sum(a^2).  x < n only.
+__global__ void k_phys_plane_sums(Geo g, const double* a, const double* b, const double* c,
+                                  double* partial) {
+    __shared__ double sh[TB / 64];
+    const int z = blockIdx.x;
+    double s0 = 0.0, s1 = 0.0;
+    for (int p = threadIdx.x; p < g.n * g.n; p += blockDim.x) {
+        const int x = p % g.n, y = p / g.n;
+        const int64_t o = ((int64_t)z * g.n + y) * 2 * g.P + x;
+        const double va = a[o];
+        if (b) {
+            const double vb = b[o], vc = c[o];
+            s0 += (va * va + vb * vb) + vc * vc;
+        } else {
+            s0 += va;
+            s1 += va * va;
+        }
+    }
+    const double t0 = block_sum(s0, sh);
+    const double t1 = block_sum(s1, sh);
+    if (threadIdx.x == 0) {
+        partial[2 * z] = t0;
+        partial[2 * z + 1] = t1;
+    }
+}
+
+__global__ void k_sum2_serial(const double* partial, int cnt, double* out) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double s0 = 0.0, s1 = 0.0;
+        for (int i = 0; i < cnt; ++i) {
+            s0 += partial[2 * i];
+            s1 += partial[2 * i + 1];
+        }
+        out[0] = s0;
+        out[1] = s1;
+    }
+}
+
+// order-preserving map of doubles onto unsigned keys
+__device__ __forceinline__ unsigned long long dkey(double x) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+__global__ void k_minmax_init(unsigned long long* keys) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        keys[0] = ~0ull;
+        keys[1] = 0ull;
+    }
+}
+
+// min and max of (a + b) + c over x < n (b, c optional); when sum is given,
+// the combined value is stored there (physical layout, pad entries untouched).
+__global__ void k_minmax(Geo g, const double* a, const double* b, const double* c, double* sum,
+                         unsigned long long* keys) {
+    __shared__ double shl[TB / 64], shh[TB / 64];
+    const int64_t total = (int64_t)g.n * g.n * g.n;
+    double lo = 1.0e308, hi = -1.0e308;
+    GRID_STRIDE(i, total) {
+        const int64_t x = i % g.n, row = i / g.n;
+        const int64_t o = row * 2 * g.P + x;
+        double v = a[o];
+        if (b) v = (v + b[o]) + c[o];
+        if (sum) sum[o] = v;
+        lo = v < lo ? v : lo;
+        hi = v > hi ? v : hi;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double l = __shfl_down(lo, o, 64), h = __shfl_down(hi, o, 64);
+        lo = l < lo ? l : lo;
+        hi = h > hi ? h : hi;
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) {
+        shl[wv] = lo;
+        shh[wv] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < (int)(blockDim.x >> 6); ++i) {
+            lo = shl[i] < lo ? shl[i] : lo;
+            hi = shh[i] > hi ? shh[i] : hi;
+        }
+        atomicMin(keys, dkey(lo));
+        atomicMax(keys + 1, dkey(hi));
+    }
+}
+
+}  // namespace
+
+void launch_shell_sums1(const Geo& g, const double* f, int s1, int ns, double* out, hipStream_t s) {
+    if (ns <= 0) return;
+    const double n3 = (double)g.n * (double)g.n * (double)g.n;
+    hipLaunchKernelGGL(k_shell_sums1, dim3(ns), dim3(TB), 0, s, g, f, s1, 1.0 / (n3 * n3), out);
+}
+void launch_phys_sums(const Geo& g, const double* a, const double* b, const double* c,
+                      double* partial, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_phys_plane_sums, dim3(g.n), dim3(TB), 0, s, g, a, b, c, partial);
+    hipLaunchKernelGGL(k_sum2_serial, dim3(1), dim3(64), 0, s, partial, g.n, out);
+}
+void launch_minmax(const Geo& g, const double* a, const double* b, const double* c, double* sum,
+                   unsigned long long* keys, hipStream_t s) {
+    hipLaunchKernelGGL(k_minmax_init, dim3(1), dim3(64), 0, s, keys);
+    hipLaunchKernelGGL(k_minmax, dim3(grid_for(nphys_h(g))), dim3(TB), 0, s, g, a, b, c, sum, keys);
+}
+
+// ---------------------------------------------------------------------------
 // Smagorinsky model
 // ---------------------------------------------------------------------------
 namespace {

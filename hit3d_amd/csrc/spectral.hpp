@@ -73,6 +73,21 @@ void launch_init_curl(const Geo& g, const double* w1, const double* w2, const do
 // rule), zero elsewhere; ratio has kmax+1 entries (index 0 unused, <0 = zero).
 void launch_init_rescale(const Geo& g, double* f, const double* ratio, int kmax, hipStream_t s);
 
+// Scalar statistics (m_stats.f90:295-440) and the divergence check
+// (pressure.f90:55-93).
+// Shell sums of |phi|^2/n^6 (one field), kx = 0 halved, kx < n/2.
+void launch_shell_sums1(const Geo& g, const double* f, int s1, int ns, double* out, hipStream_t s);
+// Deterministic physical-space sums over x < n: with b and c non-null,
+// out[0] = sum((a^2+b^2)+c^2); otherwise out[0] = sum(a), out[1] = sum(a^2).
+// partial must hold 2n doubles.
+void launch_phys_sums(const Geo& g, const double* a, const double* b, const double* c,
+                      double* partial, double* out, hipStream_t s);
+// min and max over x < n of a (b, c null) or (a + b) + c, into keys[0..1] as
+// order-preserving unsigned keys; the combined field is stored into sum when
+// sum is non-null.
+void launch_minmax(const Geo& g, const double* a, const double* b, const double* c, double* sum,
+                   unsigned long long* keys, hipStream_t s);
+
 // Smagorinsky model (m_les.f90:902-1215).
 // Strain components of group 1 (S11, S12, S13) or group 2 (S22, S23, S33)
 // from the spectral velocity (m_les.f90:1154-1196).

@@ -227,6 +227,26 @@ def write_stats(s: StatLine, directory: str):
             f.write(f"{k:4d}{_e15(e)}{_e15(d)}\n")
 
 
+def write_scalar_stats(st: dict, itime: int, time: float, directory: str):
+    """scNN.gp (m_stats.f90:66-80) and es_NN.gp (:421-430) per scalar from
+    Hit3D.stat_scalars()."""
+    for i in range(len(st["sc_diss"])):
+        p = os.path.join(directory, f"sc{i + 1:02d}.gp")
+        new = not os.path.exists(p)
+        with open(p, "a") as f:
+            if new:
+                f.write("# 1.itime 2.time          3.sc.diss       4. mean      5.variance"
+                        "    6.min     7.max\n")
+            vals = (time, st["sc_diss"][i], st["mean"][i], st["variance"][i], st["sc_min"][i],
+                    st["sc_max"][i])
+            f.write(f"{itime:7d}" + "".join(_e15(v) for v in vals) + "\n")
+        with open(os.path.join(directory, f"es_{i + 1:02d}.gp"), "a") as f:
+            f.write("\n\n")
+            f.write(f"# ITIME={itime:7d} TIME={time:17.8E}\n")
+            for k, e in enumerate(st["es_spec"][i], start=1):
+                f.write(f"{k:4d}{_e15(e)}\n")
+
+
 class Run:
     """A run of the driver loop (main.f90:56-288) on the GPU library."""
 
@@ -270,6 +290,15 @@ class Run:
         if (self.itime % p.iprint1 == 0 or self.itime % p.iwrite4 == 0):
             if p.n_scalars > 0 and not self.int_scalars and self.time > p.tscalar:
                 self.int_scalars = True  # main.f90:199-204 (type-0 scalars start at zero)
+
+    def scalar_stats(self) -> dict:
+        """stat_scalars (m_stats.f90:295-366); the reference runs it only once
+        scalars are integrated (:62)."""
+        return self.h.stat_scalars()
+
+    def divergence(self):
+        """pressure.f90:53-93, run by main.f90:66 before the loop."""
+        return self.h.divergence()
 
     def stats(self) -> StatLine:
         e, ens = self.h.stats()

@@ -119,6 +119,7 @@ program hit3d_gpu
   call h3d_check(c_init_velocity(rn1, int(isp_type, c_int64_t), int(ir_exp, c_int64_t), peak), &
        'init_velocity')
   call dealias_all
+  call divergence                            ! main.f90:66
   time = 0.d0
   fov = 1; fos = 1
   int_scalars = .false.
@@ -202,7 +203,37 @@ contains
        write(900, "(i4,2e15.6)") k, e_spec(k), e_spec(k) * real(k**2, 8) * 2.d0 * nu
     end do
     close(900)
+    if (int_scalars) call scalar_stats_out
   end subroutine stats_out
+
+  subroutine scalar_stats_out                ! m_stats.f90:62-80, :371-437
+    real*8, allocatable :: d(:), m1(:), m2(:), mn(:), mx(:), es(:,:)
+    character(len=16) :: fname
+    logical :: there
+    integer :: k, n
+    allocate(d(n_scalars), m1(n_scalars), m2(n_scalars), mn(n_scalars), mx(n_scalars), &
+         es(kmax, n_scalars))
+    call h3d_check(c_stat_scalars(d, m1, m2, mn, mx, es), 'stat_scalars')
+    do n = 1, n_scalars
+       write(fname, "('es_',i2.2,'.gp')") n
+       open(900, file=fname, position='append')
+       write(900, "()")
+       write(900, "()")
+       write(900, "('# ITIME=',i7,' TIME=',e17.8)") itime, time
+       do k = 1, kmax
+          write(900, "(i4,4e15.6)") k, es(k, n)
+       end do
+       close(900)
+       write(fname, "('sc',i2.2,'.gp')") n
+       inquire(file=fname, exist=there)
+       open(901, file=fname, position='append')
+       if (.not. there) write(901, '(A)') '# 1.itime 2.time          3.sc.diss       '// &
+            '4. mean      5.variance    6.min     7.max'
+       write(901, "(i7,10e15.6)") itime, time, d(n), m1(n), m2(n), mn(n), mx(n)
+       close(901)
+    end do
+    deallocate(d, m1, m2, mn, mx, es)
+  end subroutine scalar_stats_out
 
   subroutine restart_out                     ! restart_io.f90:287-314 layout
     real*8, allocatable :: slot(:,:,:)

@@ -91,6 +91,16 @@ module hit3d_hip
        import :: c_int, c_double
        real(c_double), intent(out) :: e_spec(*), enstrophy
      end function c_stats
+     integer(c_int) function c_stat_scalars(sc_diss, mean, variance, sc_min, sc_max, es_spec) &
+          bind(C, name="hit3d_stat_scalars")
+       import :: c_int, c_double
+       real(c_double), intent(out) :: sc_diss(*), mean(*), variance(*), sc_min(*), sc_max(*)
+       real(c_double), intent(out) :: es_spec(*)
+     end function c_stat_scalars
+     integer(c_int) function c_divergence(dmin, dmax) bind(C, name="hit3d_divergence")
+       import :: c_int, c_double
+       real(c_double), intent(out) :: dmin, dmax
+     end function c_divergence
      integer(c_int) function c_init_velocity(rn1, isp_type, ir_exp, peak) &
           bind(C, name="hit3d_init_velocity")
        import :: c_int, c_int64_t, c_double
@@ -173,5 +183,11 @@ contains
   subroutine pressure_gpu                    ! pressure.f90:1
     call h3d_check(c_pressure(), 'pressure')
   end subroutine pressure_gpu
+
+  subroutine divergence                      ! pressure.f90:53
+    real(c_double) :: dmin, dmax
+    call h3d_check(c_divergence(dmin, dmax), 'divergence')
+    write(*,*) 'divergence:', dmin, dmax
+  end subroutine divergence
 
 end module hit3d_hip

@@ -44,6 +44,8 @@ _SIGNATURES = {
     "hit3d_set_fusion": [_I64P],
     "hit3d_set_les": [_I64P, _F64P],
     "hit3d_stats": [_F64P, _F64P],
+    "hit3d_stat_scalars": [_F64P] * 6,
+    "hit3d_divergence": [_F64P, _F64P],
     "hit3d_init_velocity": [_F64P, _I64P, _I64P, _F64P],
     "hit3d_random_draws": [_F64P, _I64P, _I64P, _F64P],
     "hit3d_init_gaussian": [_I64P, _F64P, _I64P, _F64P],
@@ -232,6 +234,26 @@ class Hit3D:
         ens = ctypes.c_double(0.0)
         _check(self.lib.hit3d_stats(_dp(e), ctypes.byref(ens)))
         return e, ens.value
+
+    def stat_scalars(self) -> dict:
+        """Per-scalar statistics of m_stats.f90:295-437 (stat_scalars,
+        get_scalar_spectra): arrays over n = 1..n_scalars of sc_diss, mean,
+        variance, min, max, and es_spec[n-1, 0:kmax] (already doubled)."""
+        ns = self.n_scalars
+        out = {k: np.zeros(ns) for k in ("sc_diss", "mean", "variance", "sc_min", "sc_max")}
+        es = np.zeros((ns, self.kmax))
+        if ns:
+            _check(self.lib.hit3d_stat_scalars(_dp(out["sc_diss"]), _dp(out["mean"]),
+                                               _dp(out["variance"]), _dp(out["sc_min"]),
+                                               _dp(out["sc_max"]), _dp(es)))
+        out["es_spec"] = es
+        return out
+
+    def divergence(self):
+        """(dmin, dmax) of div u in physical space -- pressure.f90:53-93."""
+        lo, hi = ctypes.c_double(0.0), ctypes.c_double(0.0)
+        _check(self.lib.hit3d_divergence(ctypes.byref(lo), ctypes.byref(hi)))
+        return lo.value, hi.value
 
     def init_velocity(self, rn1: float, isp_type: int, ir_exp: int, peak: float):
         _check(self.lib.hit3d_init_velocity(_d(rn1), _i(isp_type), _i(ir_exp), _d(peak)))

@@ -121,6 +121,20 @@ int hit3d_set_fusion(const int64_t *on);
  * the mean enstrophy <|omega|^2>. */
 int hit3d_stats(double *e_spec, double *enstrophy);
 
+/* stat_scalars with get_scalar_spectra (m_stats.f90:295-437), for every
+ * scalar n = 1..n_scalars into element n-1 of each array (any may be NULL):
+ * sc_diss = 2 pe <|grad phi|^2>, mean, variance, sc_min, sc_max (physical
+ * space, the reference's moments(3+n,1:2)), and es_spec(kmax, n_scalars) =
+ * 2 x the FP64 shell sums of |phi|^2/n^6 (written to es_NN.gp by the
+ * reference).  Uses wrk(0:3) as work space like the reference. */
+int hit3d_stat_scalars(double *sc_diss, double *mean, double *variance, double *sc_min,
+                       double *sc_max, double *es_spec);
+
+/* divergence (pressure.f90:53-93): min and max over the grid of
+ * du/dx + dv/dy + dw/dz in physical space; leaves wrk(1:3) = fields(1:3),
+ * wrk(4:6) = the three derivatives in physical space and wrk(0) = their sum. */
+int hit3d_divergence(double *dmin, double *dmax);
+
 /* init_velocity (init_velocity.f90:1-246): seeded random solenoidal field
  * from RANDu (RANDu.f:12-52) with seed rn1, shell-rescaled to the spectrum
  * isp_type (0, 1, 3 as the reference; 2 = k^ir_exp exp(-(ir_exp/2)(k/peak)^2),

@@ -490,6 +490,34 @@ def stats(fields: np.ndarray, n: int, kmax: int, nu: float, re: float):
                 re_lambda1=np.sqrt(uvar) * lam / nu, tau_e=x_length / np.sqrt(uvar))
 
 
+def scalar_stats(phi: np.ndarray, n: int, kmax: int, pe: float) -> dict:
+    """stat_scalars (m_stats.f90:295-366) with get_scalar_spectra (:371-437)
+    for one spectral scalar field phi.  The variance spectrum is the FP64
+    shell sum of |phi|^2/n^6 over i=1..n with kx=0 halved (same loop as
+    get_e_spec), doubled (:418); dissipation 2 pe <|grad phi|^2> from the
+    three derivatives brought to physical space (:320-338); mean, variance,
+    min and max from phi in physical space (:341-360)."""
+    z = np.zeros_like(phi)
+    es = 2.0 * shell_spectrum(np.stack([phi, z, z]), n, kmax)[1:kmax + 1]
+    fac = 1.0 / float(n * n * n)
+    g = [fft_inverse(x_derivative(phi, ax))[:, :, :n] for ax in ("x", "y", "z")]
+    w4 = (g[0] ** 2 + g[1] ** 2) + g[2] ** 2
+    sc_diss = 2.0 * pe * float(np.sum(w4)) * fac
+    p = fft_inverse(phi)[:, :, :n]
+    mean = float(np.sum(p)) * fac
+    var = float(np.sum(p ** 2)) * fac - mean ** 2
+    return dict(sc_diss=sc_diss, mean=mean, variance=var, sc_min=float(p.min()),
+                sc_max=float(p.max()), es_spec=es)
+
+
+def divergence(fields: np.ndarray, n: int):
+    """pressure.f90:53-93: min and max of du/dx + dv/dy + dw/dz, each
+    derivative taken spectrally and brought to physical space separately."""
+    d = [fft_inverse(x_derivative(fields[c], ax))[:, :, :n] for c, ax in ((0, "x"), (1, "y"), (2, "z"))]
+    s = (d[0] + d[1]) + d[2]
+    return float(s.min()), float(s.max())
+
+
 # ----------------------------------------------------------------------------
 # seeded initial velocity
 # ----------------------------------------------------------------------------

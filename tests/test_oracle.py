@@ -308,6 +308,49 @@ def test_stats_parseval(oracle):
     assert abs(e[1:].sum() + e[0] - half) < 1e-12 * half
 
 
+def test_scalar_stats_closed_form(oracle):
+    """phi = c + a cos x + b sin 2y: mean c, variance (a^2+b^2)/2, E(1) = a^2/2,
+    E(2) = b^2/2 (catches a missing doubling or kx=0 halving), dissipation
+    2 pe <|grad phi|^2> = 2 pe (a^2/2 + 2 b^2), min/max c -+ (a+b) on grid
+    points x = pi, y = 3pi/4."""
+    n, a, b, c, pe = 16, 0.7, 0.3, 1.25, 0.02
+    phi = oracle.fft_forward(_phys(n, lambda x, y, z: c + a * np.cos(x) + b * np.sin(2 * y)))
+    st = oracle.scalar_stats(phi, n, 5, pe)
+    assert abs(st["mean"] - c) < 1e-14
+    assert abs(st["variance"] - (a * a + b * b) / 2) < 1e-13
+    es = st["es_spec"]
+    assert es.shape == (5,)
+    assert abs(es[0] - a * a / 2) < 1e-14 and abs(es[1] - b * b / 2) < 1e-14
+    assert np.abs(es[2:]).max() < 1e-28
+    assert abs(st["sc_diss"] - 2 * pe * (a * a / 2 + 2 * b * b)) < 1e-14
+    assert abs(st["sc_min"] - (c - a - b)) < 1e-14 and abs(st["sc_max"] - (c + a + b)) < 1e-14
+
+
+def test_scalar_spectrum_sums_to_variance(oracle):
+    """For a field truncated inside the counted shells, sum E(k) = variance
+    (Parseval with the half-spectrum weights)."""
+    n = 16
+    phi = _random_spec(oracle, n, 7, count=1)[0]
+    sh = oracle.shell_index_f32(oracle._k2_interleaved(n))
+    phi = np.where((sh <= 5) & (np.arange(n + 2) < n)[None, None, :], phi, 0.0)
+    st = oracle.scalar_stats(phi, n, 5, 0.1)
+    assert abs(st["es_spec"].sum() - st["variance"]) < 1e-12 * st["variance"]
+
+
+def test_divergence_closed_form(oracle):
+    """u = (a sin x, b sin 2y, c sin 4z): div u = a cos x + 2b cos 2y + 4c cos 4z,
+    extremes -+(a + 2b + 4c) at grid points; zero after projection."""
+    n, a, b, c = 16, 0.5, -0.25, 0.125
+    f = np.stack([oracle.fft_forward(_phys(n, g)) for g in (
+        lambda x, y, z: a * np.sin(x), lambda x, y, z: b * np.sin(2 * y),
+        lambda x, y, z: c * np.sin(4 * z))])
+    lo, hi = oracle.divergence(f, n)
+    m = abs(a) + 2 * abs(b) + 4 * abs(c)
+    assert abs(lo + m) < 1e-14 and abs(hi - m) < 1e-14
+    lo, hi = oracle.divergence(oracle.pressure(_random_spec(oracle, n, 3), n), n)
+    assert max(abs(lo), abs(hi)) < 1e-12
+
+
 # ------------------------------------------------------------------ update, dt
 
 def test_advance_euler_then_ab2(oracle):
