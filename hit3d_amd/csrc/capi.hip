@@ -23,7 +23,7 @@ void check(hipError_t e, const char* what) {
 
 static const char* kTimerNames =
     "fft_x,fft_y,fft_z,pointwise,reduce,step_inv_z,step_inv_y,step_x,step_fwd_y,step_fwd_z,"
-    "step_update";
+    "step_update,particles";
 
 Timed::Timed(int c) : cls(c) {
     Lib& L = lib();
@@ -163,6 +163,8 @@ static void free_all() {
     if (L.dshell) (void)hipFree(L.dshell);
     if (L.dpart) (void)hipFree(L.dpart);
     if (L.tv) (void)hipFree(L.tv);
+    if (L.xyzp) (void)hipFree(L.xyzp);
+    if (L.uvwp) (void)hipFree(L.uvwp);
     if (L.dscal) (void)hipFree(L.dscal);
     if (L.hpin) (void)hipHostFree(L.hpin);
     if (L.st) (void)hipStreamDestroy(L.st);
@@ -833,6 +835,77 @@ int hit3d_stats(double* e_spec, double* enstrophy) {
               "hipMemcpyAsync");
         check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
         if (enstrophy) *enstrophy = L.hpin[2];
+    });
+}
+
+int hit3d_particles_set(const int64_t* np, const double* xyzp, const double* uvwp) {
+    return guarded([&] {
+        require_ready();
+        Lib& L = lib();
+        const int64_t cnt = *np;
+        if (cnt < 0) throw std::runtime_error("particles_set: negative particle count");
+        const double hi = (double)(L.n + 1);
+        for (int64_t i = 0; i < 3 * cnt; ++i)
+            if (!(xyzp[i] >= 1.0 && xyzp[i] < hi))
+                throw std::runtime_error("particles_set: position outside [1, n+1) grid units");
+        if (cnt > L.np_cap) {
+            if (L.xyzp) check(hipFree(L.xyzp), "hipFree");
+            if (L.uvwp) check(hipFree(L.uvwp), "hipFree");
+            L.xyzp = L.uvwp = nullptr;
+            L.np_cap = 0;
+            L.xyzp = static_cast<double*>(dalloc((size_t)cnt * 3 * sizeof(double)));
+            L.uvwp = static_cast<double*>(dalloc((size_t)cnt * 3 * sizeof(double)));
+            L.np_cap = cnt;
+        }
+        L.np = cnt;
+        if (cnt == 0) return;
+        const size_t b = (size_t)cnt * 3 * sizeof(double);
+        check(hipMemcpyAsync(L.xyzp, xyzp, b, hipMemcpyHostToDevice, L.st), "hipMemcpyAsync");
+        if (uvwp) {
+            check(hipMemcpyAsync(L.uvwp, uvwp, b, hipMemcpyHostToDevice, L.st), "hipMemcpyAsync");
+        } else {  // particles_init: uvwp = -999 (m_particles.f90:327)
+            std::vector<double> fill((size_t)cnt * 3, -999.0);
+            check(hipMemcpyAsync(L.uvwp, fill.data(), b, hipMemcpyHostToDevice, L.st),
+                  "hipMemcpyAsync");
+            check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
+        }
+        check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
+    });
+}
+
+int hit3d_particles_get(double* xyzp, double* uvwp) {
+    return guarded([&] {
+        require_ready();
+        Lib& L = lib();
+        const size_t b = (size_t)L.np * 3 * sizeof(double);
+        if (b == 0) return;
+        if (xyzp) check(hipMemcpyAsync(xyzp, L.xyzp, b, hipMemcpyDeviceToHost, L.st), "hipMemcpyAsync");
+        if (uvwp) check(hipMemcpyAsync(uvwp, L.uvwp, b, hipMemcpyDeviceToHost, L.st), "hipMemcpyAsync");
+        check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
+    });
+}
+
+int hit3d_fields_to_parts(void) {
+    return guarded([&] {
+        require_ready();
+        Lib& L = lib();
+        for (int c = 1; c <= 3; ++c) {
+            copy_slot(L.W[c], L.F[c]);
+            fft_inverse(L.W[c]);
+        }
+    });
+}
+
+int hit3d_particles_move(const double* dt, const int64_t* scheme) {
+    return guarded([&] {
+        require_ready();
+        Lib& L = lib();
+        if (*scheme != 0 && *scheme != 1)
+            throw std::runtime_error("particles_tracking_scheme must be 0 (trilinear) or 1 (CINT)");
+        if (*scheme == 1 && L.n < 4) throw std::runtime_error("CINT needs n >= 4");
+        Timed t(TC_PARTICLES);
+        launch_particles_move(L.g, L.W[1], L.W[2], L.W[3], L.xyzp, L.uvwp, L.np, *dt, L.dx,
+                              (int)*scheme, L.st);
     });
 }
 

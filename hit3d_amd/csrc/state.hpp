@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "particles.hpp"
 #include "spectral.hpp"
 
 namespace hit3d {
@@ -21,6 +22,7 @@ enum TimerClass {
     TC_STEP_FWD_Y,
     TC_STEP_FWD_Z,
     TC_STEP_UPDATE,
+    TC_PARTICLES,
     TC_COUNT
 };
 
@@ -63,6 +65,12 @@ struct Lib {
     int les_model = 0;       // 0 none, 1 Smagorinsky (m_les.f90:94-104)
     double c_smag = 0.18;    // m_les.f90:1148
     double* tv = nullptr;    // turb_visc, physical slot layout
+    // Lagrangian particles (m_particles.f90:20-31): positions in grid units
+    // and the previous interpolated velocity, (3, np) order, device-resident
+    int64_t np = 0, np_cap = 0;
+    double* xyzp = nullptr;
+    double* uvwp = nullptr;
+
     bool timing = false;
     struct Ev {
         int cls;

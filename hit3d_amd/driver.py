@@ -197,8 +197,25 @@ def derived_stats(e_spec: np.ndarray, enstrophy: float, nu: float, re: float, km
                     diss_spec=e_spec * k ** 2 * 2.0 * nu)
 
 
+def fortran_e(v: float, w: int = 15, d: int = 6) -> str:
+    """Fortran Ew.d output editing: sign, 0., d digits of the correctly
+    rounded mantissa in [0.1, 1), E and a signed two-digit exponent (the E is
+    dropped for exponents beyond 99), right-justified in w columns."""
+    v = float(v)
+    if v != v or v in (float("inf"), float("-inf")):
+        return f"{'NaN' if v != v else ('-Infinity' if v < 0 else 'Infinity'):>{w}}"
+    if v == 0.0:
+        digits, e = "0" * d, 0
+    else:
+        m, ex = f"{abs(v):.{d - 1}e}".split("e")
+        digits, e = m.replace(".", ""), int(ex) + 1
+    sign = "-" if (v < 0 or (v == 0.0 and str(v).startswith("-"))) else ""
+    exp = f"E{'-' if e < 0 else '+'}{abs(e):02d}" if abs(e) <= 99 else f"{'-' if e < 0 else '+'}{abs(e):03d}"
+    return f"{sign}0.{digits}{exp}".rjust(w)
+
+
 def _e15(v: float) -> str:
-    return f"{v:15.6E}"
+    return fortran_e(v, 15, 6)
 
 
 def write_stats(s: StatLine, directory: str):
@@ -222,7 +239,7 @@ def write_stats(s: StatLine, directory: str):
                                                                s.etakmax)) + "\n")
     with open(os.path.join(directory, "es.gp"), "a") as f:
         f.write("\n\n")
-        f.write(f"# ITIME={s.itime:7d} TIME={s.time:17.8E}\n")
+        f.write(f"# ITIME={s.itime:7d} TIME={fortran_e(s.time, 17, 8)}\n")
         for k, (e, d) in enumerate(zip(s.e_spec, s.diss_spec), start=1):
             f.write(f"{k:4d}{_e15(e)}{_e15(d)}\n")
 
@@ -242,9 +259,76 @@ def write_scalar_stats(st: dict, itime: int, time: float, directory: str):
             f.write(f"{itime:7d}" + "".join(_e15(v) for v in vals) + "\n")
         with open(os.path.join(directory, f"es_{i + 1:02d}.gp"), "a") as f:
             f.write("\n\n")
-            f.write(f"# ITIME={itime:7d} TIME={time:17.8E}\n")
+            f.write(f"# ITIME={itime:7d} TIME={fortran_e(time, 17, 8)}\n")
             for k, e in enumerate(st["es_spec"][i], start=1):
                 f.write(f"{k:4d}{_e15(e)}\n")
+
+
+# ----------------------------------------------------------------------------
+# particles: initial positions and the binary particle file
+# ----------------------------------------------------------------------------
+
+_PT_REC = np.dtype([("ipart", "<i4"), ("xyz", "<f8", (3,))])  # packed, 28 bytes
+
+
+def _to_grid_units(xyz: np.ndarray, n: int) -> np.ndarray:
+    """m_particles.f90:320-324: x / (2 pi / n) + 1 (one rank)."""
+    return xyz / (2.0 * math.pi / float(n)) + 1.0
+
+
+def particles_init(nptot: int, n: int, pt_text: str | None = None):
+    """particles_init (m_particles.f90:157-337) on one rank: positions from
+    the text file <run_name>.pt (3 numbers per particle, physical units) or,
+    without it, the lattice of the nearest cube to nptot.  Positions are
+    wrapped into [0, 2 pi) and converted to grid units.  Returns (ipart,
+    xyzp[np, 3])."""
+    two_pi = 2.0 * math.pi
+    if pt_text is not None and os.path.exists(pt_text):
+        with open(pt_text) as f:
+            vals = np.array(f.read().replace(",", " ").split()[:3 * nptot], dtype=np.float64)
+        if vals.size < 3 * nptot:
+            raise ValueError(f"PARTICLES_INIT: {pt_text} holds fewer than {nptot} particles")
+        xyz = vals.reshape(nptot, 3)
+    else:
+        npthird = int(float(nptot) ** (1.0 / 3.0))
+        if nptot - npthird ** 3 >= (npthird + 1) ** 3 - nptot:
+            npthird += 1
+        nptot = min(npthird ** 3, 2 ** 30)
+        sctmp = two_pi / float(npthird)
+        i = np.arange(npthird, dtype=np.float64)
+        kk, jj, ii = np.meshgrid(i, i, i, indexing="ij")
+        xyz = np.stack([(ii.ravel() + 0.5) * sctmp, (jj.ravel() + 0.5) * sctmp,
+                        (kk.ravel() + 0.5) * sctmp], axis=1)
+    bad = (xyz >= two_pi) | (xyz < 0.0)
+    xyz = np.where(bad, xyz - two_pi * np.floor(xyz / two_pi), xyz)
+    return np.arange(1, nptot + 1, dtype=np.int32), _to_grid_units(xyz, n)
+
+
+def write_particles_binary(path: str, ipart: np.ndarray, xyzp: np.ndarray, n: int, time: float):
+    """particles_restart_write_binary (m_particles.f90:558-648): stream file
+    of int32 nptot, then per particle int32 id and the FP64 position
+    (x - 1) * dx, then the FP64 time."""
+    dx = 2.0 * math.pi / float(n)
+    rec = np.empty(len(ipart), _PT_REC)
+    rec["ipart"] = ipart
+    rec["xyz"] = (xyzp - 1.0) * dx
+    with open(path, "wb") as f:
+        f.write(np.array([len(ipart)], "<i4").tobytes())
+        f.write(rec.tobytes())
+        f.write(np.array([time], "<f8").tobytes())
+
+
+def read_particles_binary(path: str, n: int):
+    """particles_restart_read_binary (m_particles.f90:655-744) on one rank.
+    Returns (ipart, xyzp[np, 3] in grid units, time)."""
+    with open(path, "rb") as f:
+        nptot = int(np.frombuffer(f.read(4), "<i4")[0])
+        rec = np.frombuffer(f.read(_PT_REC.itemsize * nptot), _PT_REC)
+        if len(rec) != nptot:
+            raise ValueError(f"Error reading particle restart file {path}")
+        tail = f.read(8)
+    t = float(np.frombuffer(tail, "<f8")[0]) if len(tail) == 8 else float("nan")
+    return rec["ipart"].astype(np.int32), _to_grid_units(rec["xyz"].astype(np.float64), n), t
 
 
 class Run:
@@ -253,13 +337,19 @@ class Run:
     def __init__(self, p: Params, precision: int = 0):
         if p.nx != p.ny or p.nx != p.nz_all:
             raise ValueError("only cubic grids are supported")
-        if p.les_model != 0 or p.nptot != 0:
-            raise ValueError("LES and particles are not part of this build")
+        if p.les_model not in (0, 1):
+            raise ValueError("only les_model 0 (none) and 1 (Smagorinsky) are built")
+        if p.nptot > 0 and p.particles_filter_size > 0.0:
+            raise ValueError("particles on a locally averaged velocity are not built")
         self.p = p
         self.precision = precision
         self.h = Hit3D(p.nx, n_scalars=p.n_scalars, dealias=p.dealias)
         self.h.set_physics(p.nu, flow_type=p.flow_type, kfmax=p.kfmax, famp=p.famp, pe=p.pe,
                            scalar_type=p.scalar_type)
+        if p.les_model == 1:
+            self.h.set_les(1, 0.18)   # m_les.f90:1148
+        self.ipart = None
+        self.int_particles = False
         self.time = 0.0
         self.dt = p.dt
         self.itime = p.itmin
@@ -278,9 +368,42 @@ class Run:
         self.fov = self.fos = True
         self.h.dealias_all()
 
-    def step(self):
-        """main.f90:90-207 for one iteration (no rescaling, restart or output)."""
+    def init_particles(self, run_name: str | None = None, directory: str = "."):
+        """particles_init (m_particles.f90:157-337): pt.<itmin> binary restart
+        if present, else <run_name>.pt, else the lattice."""
         p = self.p
+        if p.nptot <= 0:
+            return
+        binary = os.path.join(directory, f"pt.{p.itmin:06d}")
+        if os.path.exists(binary):
+            self.ipart, xyz, _ = read_particles_binary(binary, p.nx)
+        else:
+            text = os.path.join(directory, f"{run_name}.pt") if run_name else None
+            self.ipart, xyz = particles_init(p.nptot, p.nx, text)
+        self.h.particles_set(xyz)
+
+    def write_particles(self, directory: str = "."):
+        """particles_restart_write_binary (m_particles.f90:558-648)."""
+        xyz, _ = self.h.particles_get()
+        write_particles_binary(os.path.join(directory, f"pt.{self.itime:06d}"), self.ipart, xyz,
+                               self.p.nx, self.time)
+
+    def move_particles(self):
+        """The parts task of main.f90:235-247: velocity of this time level
+        (fields_to_parts), then particles_move with this step's dt."""
+        p = self.p
+        if self.ipart is None or self.time < p.starttime_particles:
+            return
+        self.int_particles = True
+        self.h.fields_to_parts()
+        self.h.particles_move(self.dt, p.particles_tracking_scheme)
+
+    def step(self):
+        """main.f90:90-207 for one iteration (no rescaling, restart or output);
+        particles, when present, move first with the velocity and dt the
+        hydro step starts from."""
+        p = self.p
+        self.move_particles()
         self.itime += 1
         self.courant, self.energy, self.fov, self.fos = self.h.step(
             self.dt, self.fov, self.fos, int_scalars=self.int_scalars, precision=self.precision)

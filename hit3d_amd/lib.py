@@ -46,6 +46,10 @@ _SIGNATURES = {
     "hit3d_stats": [_F64P, _F64P],
     "hit3d_stat_scalars": [_F64P] * 6,
     "hit3d_divergence": [_F64P, _F64P],
+    "hit3d_particles_set": [_I64P, _F64P, _F64P],
+    "hit3d_particles_get": [_F64P, _F64P],
+    "hit3d_fields_to_parts": [],
+    "hit3d_particles_move": [_F64P, _I64P],
     "hit3d_init_velocity": [_F64P, _I64P, _I64P, _F64P],
     "hit3d_random_draws": [_F64P, _I64P, _I64P, _F64P],
     "hit3d_init_gaussian": [_I64P, _F64P, _I64P, _F64P],
@@ -254,6 +258,30 @@ class Hit3D:
         lo, hi = ctypes.c_double(0.0), ctypes.c_double(0.0)
         _check(self.lib.hit3d_divergence(ctypes.byref(lo), ctypes.byref(hi)))
         return lo.value, hi.value
+
+    # -- particles (m_particles.f90) ------------------------------------------
+    def particles_set(self, xyzp: np.ndarray, uvwp: np.ndarray | None = None):
+        """xyzp[np, 3] in grid units (1-based, as the reference's xyzp(3, np));
+        uvwp None means -999 (first move is Euler)."""
+        x = np.ascontiguousarray(xyzp, dtype=np.float64).reshape(-1, 3)
+        u = None if uvwp is None else np.ascontiguousarray(uvwp, dtype=np.float64).reshape(-1, 3)
+        if u is not None and u.shape != x.shape:
+            raise ValueError("uvwp must have the shape of xyzp")
+        self.np = x.shape[0]
+        _check(self.lib.hit3d_particles_set(_i(self.np), _dp(x), None if u is None else _dp(u)))
+
+    def particles_get(self):
+        n = getattr(self, "np", 0)
+        x = np.zeros((n, 3))
+        u = np.zeros((n, 3))
+        _check(self.lib.hit3d_particles_get(_dp(x), _dp(u)))
+        return x, u
+
+    def fields_to_parts(self):
+        _check(self.lib.hit3d_fields_to_parts())
+
+    def particles_move(self, dt: float, scheme: int):
+        _check(self.lib.hit3d_particles_move(_d(dt), _i(scheme)))
 
     def init_velocity(self, rn1: float, isp_type: int, ir_exp: int, peak: float):
         _check(self.lib.hit3d_init_velocity(_d(rn1), _i(isp_type), _i(ir_exp), _d(peak)))

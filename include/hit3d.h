@@ -130,6 +130,28 @@ int hit3d_stats(double *e_spec, double *enstrophy);
 int hit3d_stat_scalars(double *sc_diss, double *mean, double *variance, double *sc_min,
                        double *sc_max, double *es_spec);
 
+/* Lagrangian particles (m_particles.f90).  Positions are grid units, 1-based,
+ * in [1, n+1) per coordinate, in the reference's xyzp(3, np) order (x, y, z
+ * of particle 1 first); uvwp(3, np) holds the velocity interpolated at the
+ * previous move, or -999 before the first (Euler) move.
+ *
+ * hit3d_particles_set replaces the device copy (particles_init and
+ * particles_restart_read_binary end with these arrays, :300-327, :720-740);
+ * uvwp NULL means -999 everywhere.  Positions outside [1, n+1) are rejected. */
+int hit3d_particles_set(const int64_t *np, const double *xyzp, const double *uvwp);
+/* Copies the device particles back (either pointer may be NULL). */
+int hit3d_particles_get(double *xyzp, double *uvwp);
+/* fields_to_parts (m_fields.f90:209-290) on one GPU: wrk(1:3) = the physical
+ * velocity of fields(1:3), the array rhs_scalars leaves there
+ * (rhs_scalars.f90:32-39). */
+int hit3d_fields_to_parts(void);
+/* particles_move (m_particles.f90:749-791) without the locally averaged
+ * velocity option: interpolate wrk(1:3) at every particle (scheme 0
+ * trilinear :815-1009, 1 CINT tricubic :1017-1230 with cint_3d :1451-1508),
+ * Euler or AB2 in grid units with step dt, periodic wrap in x, y and z (the
+ * slab exchange of :1236-1300 on one rank).  Particles keep their order. */
+int hit3d_particles_move(const double *dt, const int64_t *scheme);
+
 /* divergence (pressure.f90:53-93): min and max over the grid of
  * du/dx + dv/dy + dw/dz in physical space; leaves wrk(1:3) = fields(1:3),
  * wrk(4:6) = the three derivatives in physical space and wrk(0) = their sum. */

@@ -519,6 +519,128 @@ def divergence(fields: np.ndarray, n: int):
 
 
 # ----------------------------------------------------------------------------
+# Lagrangian particles
+# ----------------------------------------------------------------------------
+
+def particles_lattice(nptot: int, n: int):
+    """particles_init (m_particles.f90:213-262,300-327) without a <run>.pt
+    file, one rank: nptot rounded to the nearest cube, a cell-centred
+    lattice over [0, 2 pi)^3, converted to grid units x/(2 pi/n) + 1.
+    Returns (nptot, xyzp[np, 3])."""
+    npthird = int(float(nptot) ** (1.0 / 3.0))
+    if nptot - npthird ** 3 >= (npthird + 1) ** 3 - nptot:
+        npthird += 1
+    nptot = min(npthird ** 3, 2 ** 30)
+    sctmp = 2.0 * np.pi / float(npthird)
+    i = np.arange(npthird, dtype=np.float64)
+    kk, jj, ii = np.meshgrid(i, i, i, indexing="ij")
+    xyz = np.stack([(ii.ravel() + 0.5) * sctmp, (jj.ravel() + 0.5) * sctmp,
+                    (kk.ravel() + 0.5) * sctmp], axis=1)
+    two_pi = 2.0 * np.pi
+    bad = (xyz >= two_pi) | (xyz < 0.0)
+    xyz = np.where(bad, xyz - two_pi * np.floor(xyz / two_pi), xyz)
+    return nptot, xyz / (two_pi / float(n)) + 1.0
+
+
+def _cint_r(x):
+    """cint's weight vector R(x), m_particles.f90:1496-1502."""
+    s = (x - 1.0) * (x - 2.0)
+    r1 = -x * s
+    r2 = 3.0 * (x + 1.0) * s
+    s = (x + 1.0) * x
+    r3 = -3.0 * s * (x - 2.0)
+    r4 = s * (x - 1.0)
+    return r1, r2, r3, r4
+
+
+def _cint(r, p1, p2, p3, p4):
+    """m_particles.f90:1505: (r1 p1 + r2 p2 + r3 p3 + r4 p4) / 6."""
+    return (r[0] * p1 + r[1] * p2 + r[2] * p3 + r[3] * p4) / 6.0
+
+
+def particles_move(vel, xyzp: np.ndarray, uvwp: np.ndarray, dt: float, n: int, scheme: int):
+    """particles_interpolate_trilinear (m_particles.f90:815-1009, scheme 0)
+    or particles_interpolate_cint (:1017-1230 with cint_3d :1451-1480,
+    scheme 1), then the one-rank slab update (:1236-1300: z >= n+1 -> z-n,
+    z < 1 -> z+n).  vel = three physical slots a[z, y, x]; xyzp/uvwp[np, 3]
+    in grid units.  Returns new (xyzp, uvwp); particles keep their order."""
+    dx = 2.0 * np.pi / float(n)
+    x, y, z = xyzp[:, 0], xyzp[:, 1], xyzp[:, 2]
+    i = np.floor(x).astype(np.int64)
+    j = np.floor(y).astype(np.int64)
+    k = np.floor(z).astype(np.int64)
+    xp, yp, zp = x - i, y - j, z - k
+    f = [np.asarray(v)[:, :, :n] for v in vel]
+
+    def at(c, ii, jj, kk):  # 1-based fields(ii, jj, kk, c), periodic
+        return f[c][(kk - 1) % n, (jj - 1) % n, (ii - 1) % n]
+
+    up = []
+    if scheme == 0:
+        c1 = (1.0 - xp) * (1.0 - yp) * (1.0 - zp)
+        c2 = (xp) * (1.0 - yp) * (1.0 - zp)
+        c3 = (1.0 - xp) * (yp) * (1.0 - zp)
+        c4 = (xp) * (yp) * (1.0 - zp)
+        c5 = (1.0 - xp) * (1.0 - yp) * (zp)
+        c6 = (xp) * (1.0 - yp) * (zp)
+        c7 = (1.0 - xp) * (yp) * (zp)
+        c8 = (xp) * (yp) * (zp)
+        i1 = np.where(i == n, 1, i + 1)
+        j1 = np.where(j == n, 1, j + 1)
+        k1 = np.where(k == n, 1, k + 1)   # one rank: the next slab is this one
+        for c in range(3):
+            s = c1 * at(c, i, j, k) + c2 * at(c, i1, j, k) + c3 * at(c, i, j1, k) + \
+                c4 * at(c, i1, j1, k)
+            s = s + c5 * at(c, i, j, k1) + c6 * at(c, i1, j, k1) + c7 * at(c, i, j1, k1) + \
+                c8 * at(c, i1, j1, k1)
+            up.append(s)
+    else:
+        rx, ry, rz = _cint_r(xp), _cint_r(yp), _cint_r(zp)
+        for c in range(3):
+            t1 = []
+            for kk in range(4):
+                t2 = []
+                for jj in range(4):
+                    pts = [at(c, i - 2 + ii + 1, j - 2 + jj + 1, k - 2 + kk + 1) for ii in range(4)]
+                    t2.append(_cint(rx, *pts))
+                t1.append(_cint(ry, *t2))
+            up.append(_cint(rz, *t1))
+    u, v, w = up
+    first = uvwp[:, 0] == -999.0
+    xe = xp + dt * u / dx
+    ye = yp + dt * v / dx
+    ze = zp + dt * w / dx
+    xa = xp + (1.50 * u - 0.50 * uvwp[:, 0]) * dt / dx
+    ya = yp + (1.50 * v - 0.50 * uvwp[:, 1]) * dt / dx
+    za = zp + (1.50 * w - 0.50 * uvwp[:, 2]) * dt / dx
+    xp = np.where(first, xe, xa)
+    yp = np.where(first, ye, ya)
+    zp = np.where(first, ze, za)
+
+    def wrap(fr, cell, periodic):
+        hi = fr > 1.0
+        fr = np.where(hi, fr - 1.0, fr)
+        cell = np.where(hi, cell + 1, cell)
+        if periodic:
+            cell = np.where(hi & (cell > n), cell - n, cell)
+        lo = fr < 0.0
+        fr = np.where(lo, fr + 1.0, fr)
+        cell = np.where(lo, cell - 1, cell)
+        if periodic:
+            cell = np.where(lo & (cell < 1), cell + n, cell)
+        return fr, cell
+
+    xp, i = wrap(xp, i, True)
+    yp, j = wrap(yp, j, True)
+    zp, k = wrap(zp, k, False)
+    zn = zp + k.astype(np.float64)
+    zn = np.where(zn >= float(n + 1), zn - float(n), zn)
+    zn = np.where(zn < 1.0, zn + float(n), zn)
+    new_xyz = np.stack([xp + i.astype(np.float64), yp + j.astype(np.float64), zn], axis=1)
+    return new_xyz, np.stack([u, v, w], axis=1)
+
+
+# ----------------------------------------------------------------------------
 # seeded initial velocity
 # ----------------------------------------------------------------------------
 

@@ -49,6 +49,22 @@ def test_random_draws_match_oracle(oracle):
     assert np.array_equal(hit3d_amd.random_draws(RN1, 3 * 10 ** 7, 1000), far)
 
 
+def test_random_draws_match_golden_fixture():
+    """The library's jump-ahead generator against committed draws of the
+    reference's own RANDu.f build (first run of tests/golden/randu_golden.npz:
+    index 0 is random(-RN1), index i the i-th draw after it)."""
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                             "randu_golden.npz"))
+    rn1 = float(g["seeds"][0])
+    idx, bits = g["idx1"], g["bits1"]
+    head = idx < 4096
+    got = hit3d_amd.random_draws(rn1, 0, 4095)
+    assert np.array_equal(got.view(np.int64), bits[head][1:])
+    for i, b in zip(idx[~head][::7], bits[~head][::7]):
+        v = hit3d_amd.random_draws(rn1, int(i) - 1, 1)
+        assert v.view(np.int64)[0] == b, i
+
+
 def test_gaussian_field_bit_exact(oracle):
     """init_velocity.f90:87-113 on one rank: the product's Gaussian field for
     each component equals the oracle's (same draws, glibc Box-Muller)."""

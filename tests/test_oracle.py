@@ -44,6 +44,23 @@ def test_rng_matches_reference_build(oracle):
     assert np.array_equal(ref.view(np.int64), mine.view(np.int64))
 
 
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "randu_golden.npz")
+
+
+def test_rng_matches_golden_fixture(oracle):
+    """The C restatement against committed draws of the reference's own
+    RANDu.f build (tests/golden/make_randu_golden.py): random(-RN1), 3e6
+    draws, then random(-RN2) on the same generator and 5e5 draws, sampled."""
+    g = np.load(GOLDEN)
+    rn1, rn2 = g["seeds"]
+    c1, c2 = (int(v) for v in g["counts"])
+    gen = oracle.RanduGenerator()
+    run1 = np.concatenate([[gen.random(-rn1)], gen.draw(rn1, c1)])
+    run2 = np.concatenate([[gen.random(-rn2)], gen.draw(rn2, c2)])
+    assert np.array_equal(run1[g["idx1"]].view(np.int64), g["bits1"])
+    assert np.array_equal(run2[g["idx2"]].view(np.int64), g["bits2"])
+
+
 def test_rng_known_values(oracle):
     """First values for the reference seed RN1 (as printed by the reference
     build: 0.65056227208012396 for random(-RN1))."""
@@ -349,6 +366,83 @@ def test_divergence_closed_form(oracle):
     assert abs(lo + m) < 1e-14 and abs(hi - m) < 1e-14
     lo, hi = oracle.divergence(oracle.pressure(_random_spec(oracle, n, 3), n), n)
     assert max(abs(lo), abs(hi)) < 1e-12
+
+
+# ------------------------------------------------------------------ particles
+
+def _index_field(n, fun):
+    """physical slot with a[z, y, x] = fun(x+1, y+1, z+1) (1-based grid units)."""
+    i = np.arange(1, n + 1, dtype=np.float64)
+    zz, yy, xx = np.meshgrid(i, i, i, indexing="ij")
+    out = np.zeros((n, n, n + 2))
+    out[:, :, :n] = fun(xx, yy, zz)
+    return out
+
+
+def test_particles_trilinear_reproduces_linear(oracle):
+    """Trilinear weights reproduce a + bx + cy + dz + e xyz-free bilinear terms
+    exactly inside one cell (stencil away from the periodic seam)."""
+    n = 16
+    fun = lambda x, y, z: 0.3 + 0.25 * x - 0.5 * y + 0.125 * z + 0.0625 * x * y
+    vel = [_index_field(n, fun), _index_field(n, lambda x, y, z: 2.0 * x),
+           _index_field(n, lambda x, y, z: -z)]
+    rng = np.random.default_rng(5)
+    xyz = rng.uniform(3.0, 12.0, (200, 3))
+    new, uvw = oracle.particles_move(vel, xyz, np.full((200, 3), -999.0), 0.0, n, 0)
+    x, y, z = xyz.T
+    assert np.abs(uvw[:, 0] - fun(x, y, z)).max() < 1e-13
+    assert np.abs(uvw[:, 1] - 2 * x).max() < 1e-13 and np.abs(uvw[:, 2] + z).max() < 1e-13
+    assert np.array_equal(new, xyz)
+
+
+def test_particles_cint_reproduces_cubic(oracle):
+    """The CINT stencil (nodes -1..2 around the cell) reproduces products of
+    cubics exactly; a shifted or mis-weighted node breaks this."""
+    n = 16
+    px = lambda x: 0.5 - 0.1 * x + 0.02 * x ** 2 - 0.003 * x ** 3
+    py = lambda y: 1.0 + 0.05 * y ** 3
+    pz = lambda z: 0.2 * z - 0.01 * z ** 2
+    fun = lambda x, y, z: px(x) * py(y) * pz(z)
+    vel = [_index_field(n, fun)] * 3
+    rng = np.random.default_rng(6)
+    xyz = rng.uniform(3.0, 13.0, (300, 3))
+    _, uvw = oracle.particles_move(vel, xyz, np.full((300, 3), -999.0), 0.0, n, 1)
+    want = fun(*xyz.T)
+    assert np.abs(uvw[:, 0] - want).max() < 1e-12 * np.abs(want).max()
+    # trilinear does not reproduce the cubic (the check has teeth)
+    _, uvw0 = oracle.particles_move(vel, xyz, np.full((300, 3), -999.0), 0.0, n, 0)
+    assert np.abs(uvw0[:, 0] - want).max() > 1e-4 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("scheme", [0, 1])
+def test_particles_uniform_flow_euler_ab2_and_wrap(oracle, scheme):
+    """Uniform velocity: each move advances by dt*U/dx (Euler first, then AB2
+    with 1.5U - 0.5U = U), wrapping periodically in x, y and z."""
+    n = 16
+    dx = 2 * np.pi / n
+    U = (0.7, -0.4, 0.9)
+    vel = [_index_field(n, lambda x, y, z, c=c: np.full_like(x, c)) for c in U]
+    xyz = np.array([[16.9, 1.05, 16.95], [1.2, 8.0, 1.01], [8.5, 8.5, 8.5]])
+    uvw = np.full((3, 3), -999.0)
+    dt = 0.6 * dx / 0.9 * 0.3
+    pos = xyz.copy()
+    for step in range(6):
+        pos, uvw = oracle.particles_move(vel, pos, uvw, dt, n, scheme)
+        assert np.allclose(uvw, np.array(U)[None, :], rtol=0, atol=1e-14)
+        want = (xyz - 1.0 + (step + 1) * dt * np.array(U) / dx) % n + 1.0
+        assert np.abs(pos - want).max() < 1e-12
+        assert (pos >= 1.0).all() and (pos < n + 1.0).all()
+
+
+def test_particles_lattice(oracle):
+    """nptot rounds to the nearest cube (m_particles.f90:222-228); lattice at
+    cell centres of a 2 pi / npthird spacing, in grid units."""
+    n = 32
+    nptot, xyz = oracle.particles_lattice(1100, n)
+    assert nptot == 1000 and xyz.shape == (1000, 3)
+    assert oracle.particles_lattice(1200, n)[0] == 1331
+    assert np.allclose(xyz[0], 0.5 * n / 10 + 1.0) and np.allclose(xyz[1, 0], 1.5 * n / 10 + 1.0)
+    assert np.allclose(xyz[10, 1], 1.5 * n / 10 + 1.0) and np.allclose(xyz[100, 2], 1.5 * n / 10 + 1.0)
 
 
 # ------------------------------------------------------------------ update, dt
