@@ -131,6 +131,12 @@ static void copy_slot(double* dst, const double* src) {
     check(hipMemcpyAsync(dst, src, slot_bytes(), hipMemcpyDeviceToDevice, L.st), "hipMemcpyAsync");
 }
 
+static void zero_slot(double* dst) {
+    Lib& L = lib();
+    Timed t(TC_POINTWISE);
+    check(hipMemsetAsync(dst, 0, slot_bytes(), L.st), "hipMemsetAsync");
+}
+
 static void shift(const double* src, double* dst, int sign) {
     Lib& L = lib();
     Timed t(TC_POINTWISE);
@@ -287,8 +293,21 @@ static void do_rhs_scalars(bool int_scalars) {
                 fft_forward(W[s + 2 + i]);
             }
             const double* Q[3] = {W[s + 3], W[s + 4], W[s + 5]};
-            Timed t(TC_POINTWISE);
-            launch_scalar_rhs(g, Q, F[3 + s], W[3 + s], L.pe[s - 1], L.st);
+            {
+                Timed t(TC_POINTWISE);
+                launch_scalar_rhs(g, Q, F[3 + s], W[3 + s], L.pe[s - 1], L.st);
+            }
+            if (L.stype[s - 1] >= 100) {  // add_reaction + dealias_rhs, :124-130
+                {
+                    Timed t(TC_POINTWISE);
+                    launch_reaction(g, W[0], W[0], (int)(L.stype[s - 1] / 100), L.reac[s - 1],
+                                    F[3 + s], L.st);
+                }
+                fft_forward(W[0]);
+                Timed t(TC_POINTWISE);
+                launch_add(g, W[3 + s], W[0], false, L.st);
+                launch_truncate(g, W[3 + s], L.kmax, L.st);
+            }
         }
     } else {  // rhs_scalars.f90:144-271
         for (int c = 1; c <= 3 + L.ns; ++c) {
@@ -326,6 +345,22 @@ static void do_rhs_scalars(bool int_scalars) {
             const double* Q[3] = {W[n1], W[n2], W[0]};
             Timed t(TC_POINTWISE);
             launch_scalar_rhs_ps(g, Q, F[c], W[c], L.pe[c - 4], true, L.st);
+        }
+        for (int c = 4; c <= 3 + L.ns; ++c) {  // reaction_rates, rhs_scalars.f90:280-380
+            if (L.stype[c - 4] <= 300) continue;
+            copy_slot(W[n1], F[c]);
+            {
+                Timed t(TC_POINTWISE);
+                launch_truncate(g, W[n1], L.n / 4, L.st);
+            }
+            fft_inverse(W[n1]);
+            {
+                Timed t(TC_POINTWISE);
+                launch_reaction(g, W[n1], W[n2], 3, L.reac[c - 4], F[c], L.st);
+            }
+            fft_forward(W[n2]);
+            Timed t(TC_POINTWISE);
+            launch_add(g, W[c], W[n2], false, L.st);
         }
     }
     for (int s = 1; s <= L.ns; ++s) {  // rhs_scalars.f90:387-389
@@ -651,13 +686,29 @@ int hit3d_set_physics(const double* nu, const int64_t* flow_type, const int64_t*
         if (L.kfmax > L.kmax) throw std::runtime_error("kfmax exceeds kmax");
         L.pe.assign(L.ns, 0.0);
         L.stype.assign(L.ns, 0);
+        L.reac.assign(L.ns, 0.0);
         for (int s = 0; s < L.ns; ++s) {
             if (!pe || !scalar_type) throw std::runtime_error("scalar parameters missing");
             L.pe[s] = pe[s];
             L.stype[s] = scalar_type[s];
-            if (L.stype[s] >= 100) throw std::runtime_error("reacting scalars are not supported");
+            const int64_t rtype = L.stype[s] / 100;
+            if (L.stype[s] < 0 || rtype > 3)
+                throw std::runtime_error("scalar type: reaction digit must be 0..3");
+            if (L.dealias == 1 && L.stype[s] > 300 && L.stype[s] != 311)
+                throw std::runtime_error(
+                    "The scalar type has a reaction rate that is not supported yet (phase "
+                    "shifting supports 311 only)");
         }
     });
+}
+
+int hit3d_set_reaction(const double* reac_sc) {
+    return guarded([&] {
+        require_ready();
+        Lib& L = lib();
+        if ((int)L.reac.size() != L.ns) throw std::runtime_error("set_reaction: set_physics first");
+        for (int s = 0; s < L.ns; ++s) L.reac[s] = reac_sc[s];
+    }, false);
 }
 
 int hit3d_finalize(void) {
@@ -950,6 +1001,7 @@ int hit3d_init_velocity(const double* rn1, const int64_t* isp_type, const int64_
             if (hit3d_put(&a, &s, host.data()) != 0) throw std::runtime_error(L.err);
             fft_forward(L.W[c + 1]);
         }
+        L.rng_iy = velocity_rng_iy(n, *rn1);
         // :123-149 curl on shells 1..kmax
         launch_init_curl(L.g, L.W[1], L.W[2], L.W[3], L.F[1], L.F[2], L.F[3], L.kmax, L.st);
         // :156-186 shell spectrum
@@ -970,6 +1022,131 @@ int hit3d_init_velocity(const double* rn1, const int64_t* isp_type, const int64_
         for (int c = 1; c <= 3; ++c) launch_init_rescale(L.g, L.F[c], L.dshell, L.kmax, L.st);
         check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
     });
+}
+
+int hit3d_init_scalar(const int64_t* n_scalar, const double* rn2, const double* peak) {
+    return guarded([&] {
+        require_ready();
+        Lib& L = lib();
+        const int s = (int)*n_scalar;
+        if ((int)L.stype.size() != L.ns) throw std::runtime_error("init_scalar: set_physics first");
+        if (s < 1 || s > L.ns) throw std::runtime_error("init_scalar: scalar number out of range");
+        const int n = L.n;
+        std::vector<double> host((size_t)(n + 2) * n * n);
+        const int64_t st = L.stype[s - 1];
+        const int ic = (int)(st - (st / 100) * 100);
+        double* F = L.F[3 + s];
+        if (ic == 0) {
+            zero_slot(F);
+        } else if (ic < 10) {  // init_scalar_spectrum, :84-270
+            if (st < 1 || st > 3)
+                throw std::runtime_error("INIT_SCALARS: WRONG INITIAL SPECTRUM TYPE");
+            L.rng_iy = scalar_gaussian_slot(n, *rn2, L.rng_iy, host.data());
+            const int64_t a = 2, w3 = 3;
+            if (hit3d_put(&a, &w3, host.data()) != 0) throw std::runtime_error(L.err);
+            fft_forward(L.W[3]);
+            {
+                Timed t(TC_REDUCE);
+                launch_shell_sums1(L.g, L.W[3], 1, L.kmax, L.dshell, L.st);
+            }
+            std::vector<double> e(L.kmax);
+            check(hipMemcpyAsync(e.data(), L.dshell, L.kmax * sizeof(double),
+                                 hipMemcpyDeviceToHost, L.st),
+                  "hipMemcpyAsync");
+            check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
+            const std::vector<double> tgt = scalar_target_spectrum(L.kmax, st, *peak);
+            std::vector<double> ratio(L.kmax + 1, -1.0);
+            for (int k = 1; k <= L.kmax; ++k)
+                if (e[k - 1] > 0.0) ratio[k] = std::sqrt(tgt[k - 1] / e[k - 1]);
+            check(hipMemcpyAsync(L.dshell, ratio.data(), ratio.size() * sizeof(double),
+                                 hipMemcpyHostToDevice, L.st),
+                  "hipMemcpyAsync");
+            copy_slot(F, L.W[3]);
+            {
+                Timed t(TC_POINTWISE);
+                launch_init_rescale(L.g, F, L.dshell, L.kmax, L.st);
+            }
+            check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
+            if (st == 3) {  // double-delta PDF, :239-262
+                copy_slot(L.W[0], F);
+                fft_inverse(L.W[0]);
+                {
+                    Timed t(TC_POINTWISE);
+                    launch_double_delta(L.g, L.W[0], L.st);
+                }
+                fft_forward(L.W[0]);
+                {
+                    Timed t(TC_POINTWISE);
+                    launch_shell_cut(L.g, L.W[0], L.kmax * 2 / 3, L.st);
+                }
+                copy_slot(F, L.W[0]);
+            }
+        } else {  // init_scalar_space, :278-383
+            scalar_space_slot(n, ic, *peak, host.data());
+            const int64_t a = 1, slot = 3 + s;
+            if (hit3d_put(&a, &slot, host.data()) != 0) throw std::runtime_error(L.err);
+            fft_forward(L.F[3 + s]);  // may exchange the slot with the spare buffer
+            if (ic == 11 || ic == 12)  // fields(1,1,1,3+n) = 0
+                check(hipMemsetAsync(L.F[3 + s], 0, sizeof(double), L.st), "hipMemsetAsync");
+        }
+        check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
+    });
+}
+
+int hit3d_init_scalars(const double* rn2, const double* peak_sc) {
+    return guarded([&] {
+        require_ready();
+        Lib& L = lib();
+        for (int64_t s = 1; s <= L.ns; ++s)  // init_scalar, init_scalars.f90:20-22
+            if (hit3d_init_scalar(&s, rn2, peak_sc + (s - 1)) != 0) throw std::runtime_error(L.err);
+        Timed t(TC_POINTWISE);  // :27-36 zero ialias > 0 on every scalar
+        for (int s = 1; s <= L.ns; ++s) launch_mask_alias(L.g, L.F[3 + s], 0, L.st);
+        check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
+    });
+}
+
+int hit3d_snapshot(const int64_t* n_field, float* out) {
+    return guarded([&] {
+        require_ready();
+        Lib& L = lib();
+        const int64_t f = *n_field;
+        if (f < 1 || f > L.nf) throw std::runtime_error("snapshot: field number out of range");
+        copy_slot(L.W[0], L.F[f]);
+        {
+            Timed t(TC_POINTWISE);
+            launch_sphere_cut(L.g, L.W[0], L.kmax, L.st);
+        }
+        fft_inverse(L.W[0]);
+        float* stage = reinterpret_cast<float*>(L.spare);  // free between transforms
+        {
+            Timed t(TC_POINTWISE);
+            launch_pack_f32(L.g, L.W[0], stage, L.st);
+        }
+        check(hipMemcpyAsync(out, stage, sizeof(float) * (size_t)L.n * L.n * L.n,
+                             hipMemcpyDeviceToHost, L.st),
+              "hipMemcpyAsync");
+        check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
+    });
+}
+
+int hit3d_rng_state(int64_t* iy) {
+    return guarded([&] { *iy = lib().rng_iy; }, false);
+}
+
+int hit3d_scalar_gaussian(const int64_t* nx, const double* rn2, const int64_t* iy_prev,
+                          double* out, int64_t* iy_next) {
+    return guarded([&] {
+        if (*nx < 2) throw std::runtime_error("bad grid size");
+        const int64_t iy = scalar_gaussian_slot((int)*nx, *rn2, *iy_prev, out);
+        if (iy_next) *iy_next = iy;
+    }, false);
+}
+
+int hit3d_velocity_rng_state(const int64_t* nx, const double* rn1, int64_t* iy) {
+    return guarded([&] {
+        if (*nx < 2) throw std::runtime_error("bad grid size");
+        *iy = velocity_rng_iy((int)*nx, *rn1);
+    }, false);
 }
 
 int hit3d_random_draws(const double* seed, const int64_t* skip, const int64_t* count,

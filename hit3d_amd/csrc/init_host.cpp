@@ -121,13 +121,19 @@ void closed_form(int64_t c, int64_t idum2_base, int64_t base, int64_t first, int
 }
 
 // Generates draws [first, first+count) (0-based, after random(-seed)) and
-// passes each to f(index, value).
+// passes each to f(index, value).  iy0 is the generator's saved iy before the
+// initialising call (0 in a fresh process; random(-seed) refills the table
+// and resets idum2 but keeps iy, RANDu.f:28-37).  Returns iy after draw
+// index total-1 (after the initialising call itself when total == 0).
 template <typename F>
-void draws(double seed, int64_t first, int64_t count, F&& f) {
+int64_t draws(double seed, int64_t first, int64_t count, F&& f, int64_t iy0 = 0,
+              int64_t total = -1) {
     Gen g;
+    g.iy = iy0;
     g.call(-seed);
     g.c = lcg1(std::llround(seed / 987.0));
     if (std::llround(seed / 987.0) <= 0) throw std::runtime_error("seed must be >= 494");
+    int64_t iy_total = total == 0 ? g.iy : -1;
     // serial warm-up until the shuffle table is uniform
     int64_t done = 0;  // draws made so far
     const int64_t cap = 1 << 20;
@@ -135,14 +141,22 @@ void draws(double seed, int64_t first, int64_t count, F&& f) {
         const double v = g.call(seed);
         if (done >= first && done < first + count) f(done, v);
         ++done;
+        if (done == total) iy_total = g.iy;
         if (done > cap) throw std::runtime_error("random: shuffle table did not converge");
     }
     // after `done` draws idum2 == g.idum2; the next draw (index done) uses lcg2 of it
+    if (total > done) {
+        const int64_t sv = (int64_t)(((unsigned __int128)powmod(IA2, total - done, IM2) *
+                                      (unsigned __int128)g.idum2) %
+                                     (unsigned __int128)IM2);
+        iy_total = g.c - sv;
+        if (iy_total < 1) iy_total += IMM1;
+    }
     const int64_t lo = std::max(first, done);
     const int64_t hi = first + count;
-    if (hi <= lo) return;
-    // state after draw index (done-1) is g.idum2; draw index i uses ia2^(i-done+1)*g.idum2
-    closed_form(g.c, g.idum2, done, lo, hi - lo, f);
+    if (hi > lo)  // draw index i uses ia2^(i-done+1)*g.idum2
+        closed_form(g.c, g.idum2, done, lo, hi - lo, f);
+    return iy_total;
 }
 
 }  // namespace
@@ -162,6 +176,80 @@ void gaussian_slot(int n, double rn1, int c, double* out) {
         const int64_t e = i - (c + 3) * per;
         out[e] = std::sqrt(-2.0 * std::log(a[e])) * std::sin(two_pi * v);
     });
+}
+
+int64_t velocity_rng_iy(int n, double rn1) {
+    const int64_t per = (int64_t)(n + 2) * n * n;
+    return draws(rn1, 0, 0, [](int64_t, double) {}, 0, 6 * per);
+}
+
+int64_t scalar_gaussian_slot(int n, double rn2, int64_t iy_prev, double* out) {
+    const int64_t per = (int64_t)(n + 2) * n * n;
+    std::vector<double> a(per);
+    const int64_t iy = draws(rn2, 0, per, [&](int64_t i, double v) { a[i] = v; }, iy_prev,
+                             2 * per);
+    const double pi = 4.0 * std::atan(1.0);
+    const double two_pi = 2.0 * pi;
+    draws(rn2, per, per, [&](int64_t i, double v) {
+        const int64_t e = i - per;
+        out[e] = std::sqrt(-2.0 * std::log(a[e])) * std::sin(two_pi * v);
+    }, iy_prev);
+    return iy;
+}
+
+void scalar_space_slot(int n, int ic_type, double peak, double* out) {
+    const double pi = 4.0 * std::atan(1.0);
+    const double dx = 2.0 * pi / (double)n;
+    const int64_t row = n + 2;
+    std::fill(out, out + row * n * n, 0.0);
+    if (ic_type == 11) {  // single slab in z, init_scalars.f90:311-330
+        const double h = std::max(2. * dx, 2.0 * pi / peak);
+        for (int k = 0; k < n; ++k) {
+            const double zloc = (double)k * dx;
+            const double v = std::tanh((zloc - pi * 0.5) / h) - std::tanh((zloc - pi * 1.5) / h) - 1.0;
+            for (int j = 0; j < n; ++j)
+                for (int i = 0; i < n; ++i) out[((int64_t)k * n + j) * row + i] = v;
+        }
+    } else if (ic_type == 12) {  // two slabs in x, :335-356
+        const double h = std::max(8. * dx, pi / 8.0);
+        for (int i = 0; i < n; ++i) {
+            const double xx = (double)i * dx;
+            const double v = std::tanh((xx - pi * 0.25) / h) - std::tanh((xx - pi * 0.75) / h) +
+                             std::tanh((xx - pi * 1.25) / h) - std::tanh((xx - pi * 1.75) / h);
+            for (int64_t r = 0; r < (int64_t)n * n; ++r) out[r * row + i] = v - 1.0;
+        }
+    } else if (ic_type == 13) {  // sinusoidal waves in x, :362-372
+        const double pk = std::min(peak, (double)(n / 8));
+        for (int i = 0; i < n; ++i) {
+            const double v = std::sin(pk * dx * (double)i);
+            for (int64_t r = 0; r < (int64_t)n * n; ++r) out[r * row + i] = v;
+        }
+    } else {
+        throw std::runtime_error("INIT_SCALARS: UNEXPECTED SCALAR TYPE");
+    }
+}
+
+std::vector<double> scalar_target_spectrum(int kmax, int64_t sc_type, double peak) {
+    std::vector<double> e(kmax);
+    const double pi = 4.0 * std::atan(1.0);
+    for (int k = 1; k <= kmax; ++k) {
+        const double wmag = (double)k;
+        const double ratio = wmag / peak;
+        double v;
+        if (sc_type == 0) v = std::pow(wmag, -5.0 / 3.0);
+        else if (sc_type == 1 || sc_type == 3) v = std::pow(ratio, 3) / peak * std::exp(-3.0 * ratio);
+        else if (sc_type == 2) {
+            const double fac = 2.0 * pi * ratio;
+            v = std::pow(fac, 4) / std::pow(1.0 + fac * fac, 3);
+        } else {
+            throw std::runtime_error("INIT_SCALARS: WRONG INITIAL SPECTRUM TYPE");
+        }
+        e[k - 1] = v;
+    }
+    double s = 0.0;
+    for (double v : e) s += v;
+    for (double& v : e) v /= s;
+    return e;
 }
 
 void taylor_green_slot(int n, int c, double* out) {

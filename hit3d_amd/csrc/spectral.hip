@@ -676,6 +676,75 @@ void launch_minmax(const Geo& g, const double* a, const double* b, const double*
 }
 
 // ---------------------------------------------------------------------------
+// scalar reactions
+// ---------------------------------------------------------------------------
+namespace {
+
+__global__ void k_reaction(Geo g, const double* a, double* out, int rtype, double rrate,
+                           const double* mean_src) {
+    const double mean = mean_src[0] / ((double)g.n * (double)g.n * (double)g.n);
+    const int64_t total = (int64_t)g.n * g.n * g.n;
+    GRID_STRIDE(i, total) {
+        const int64_t o = (i / g.n) * 2 * g.P + i % g.n;
+        const double w = a[o];
+        const double base = rrate * (1.0 - w * w);
+        out[o] = rtype == 1 ? base : (rtype == 2 ? base * w : base * (w - mean));
+    }
+}
+
+__global__ void k_double_delta(Geo g, double* a) {
+    const int64_t total = (int64_t)g.n * g.n * g.n;
+    GRID_STRIDE(i, total) {
+        const int64_t o = (i / g.n) * 2 * g.P + i % g.n;
+        a[o] = copysign(1.0, a[o]) * 0.9;
+    }
+}
+
+__global__ void k_shell_cut(Geo g, double* f, int kcut) {
+    GRID_STRIDE(i, nspec(g)) {
+        SIdx s = sdecode(g, i);
+        const int sh = shell_of(s.kx * s.kx + s.kyv * s.kyv + s.kzv * s.kzv);
+        if (sh == 0 || sh >= kcut) C(f)[s.off] = {0.0, 0.0};
+    }
+}
+
+__global__ void k_sphere_cut(Geo g, double* f, int kmax) {
+    const double rk2 = (double)kmax * (double)kmax;
+    GRID_STRIDE(i, nspec(g)) {
+        SIdx s = sdecode(g, i);
+        const double kx = s.kx, ky = s.kyv, kz = s.kzv;
+        if (kx * kx + ky * ky + kz * kz > rk2) C(f)[s.off] = {0.0, 0.0};
+    }
+}
+
+__global__ void k_pack_f32(Geo g, const double* a, float* out) {
+    const int64_t total = (int64_t)g.n * g.n * g.n;
+    GRID_STRIDE(i, total) out[i] = (float)a[(i / g.n) * 2 * g.P + i % g.n];
+}
+
+}  // namespace
+
+void launch_sphere_cut(const Geo& g, double* slot, int kmax, hipStream_t s) {
+    hipLaunchKernelGGL(k_sphere_cut, dim3(grid_for(nspec_h(g))), dim3(TB), 0, s, g, slot, kmax);
+}
+void launch_pack_f32(const Geo& g, const double* phys, float* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_pack_f32, dim3(grid_for(nphys_h(g))), dim3(TB), 0, s, g, phys, out);
+}
+
+void launch_double_delta(const Geo& g, double* a, hipStream_t s) {
+    hipLaunchKernelGGL(k_double_delta, dim3(grid_for(nphys_h(g))), dim3(TB), 0, s, g, a);
+}
+void launch_shell_cut(const Geo& g, double* slot, int kcut, hipStream_t s) {
+    hipLaunchKernelGGL(k_shell_cut, dim3(grid_for(nspec_h(g))), dim3(TB), 0, s, g, slot, kcut);
+}
+
+void launch_reaction(const Geo& g, const double* a, double* out, int rtype, double rrate,
+                     const double* mean_src, hipStream_t s) {
+    hipLaunchKernelGGL(k_reaction, dim3(grid_for(nphys_h(g))), dim3(TB), 0, s, g, a, out, rtype,
+                       rrate, mean_src);
+}
+
+// ---------------------------------------------------------------------------
 // Smagorinsky model
 // ---------------------------------------------------------------------------
 namespace {

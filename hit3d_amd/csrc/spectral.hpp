@@ -88,6 +88,24 @@ void launch_phys_sums(const Geo& g, const double* a, const double* b, const doub
 void launch_minmax(const Geo& g, const double* a, const double* b, const double* c, double* sum,
                    unsigned long long* keys, hipStream_t s);
 
+// Scalar reaction rate in physical space (rhs_scalars.f90:405-456 and the
+// type-311 form :352-356), x < n: out = r (1 - a^2) [rtype 1],
+// r (1 - a^2) a [2], r (1 - a^2) (a - mean) [3] with mean = Re F(0,0,0)/n^3
+// read from the spectral slot mean_src.  out may alias a.
+void launch_reaction(const Geo& g, const double* a, double* out, int rtype, double rrate,
+                     const double* mean_src, hipStream_t s);
+
+// init_scalar_spectrum type 3 (init_scalars.f90:239-262): physical
+// a = sign(1, a) * 0.9 (x < n); spectral zeroing of shells 0 and >= kcut
+// (float32 shell rule, all kx).
+void launch_double_delta(const Geo& g, double* a, hipStream_t s);
+void launch_shell_cut(const Geo& g, double* slot, int kcut, hipStream_t s);
+
+// io_write_4 (io_write_4.f90:16-37): zero entries with kx^2+ky^2+kz^2 > kmax^2;
+// and the float32 copy of a physical slot, packed (x fastest, no padding).
+void launch_sphere_cut(const Geo& g, double* slot, int kmax, hipStream_t s);
+void launch_pack_f32(const Geo& g, const double* phys, float* out, hipStream_t s);
+
 // Smagorinsky model (m_les.f90:902-1215).
 // Strain components of group 1 (S11, S12, S13) or group 2 (S22, S23, S33)
 // from the spectral velocity (m_les.f90:1154-1196).

@@ -42,6 +42,8 @@ struct Lib {
     int flow_type = 0, kfmax = 0;
     std::vector<double> pe;
     std::vector<int64_t> stype;
+    std::vector<double> reac;  // reaction rates reac_sc (m_parameters.f90:445)
+    int64_t rng_iy = 0;        // the generator's saved iy (RANDu.f:22-24) across inits
 
     std::vector<double*> F;  // fields, index 1..nf (0 unused)
     std::vector<double*> W;  // wrk, index 0..nw

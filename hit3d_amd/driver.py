@@ -12,7 +12,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .lib import Hit3D
+from .lib import FIELDS, Hit3D
 
 
 @dataclass
@@ -265,6 +265,51 @@ def write_scalar_stats(st: dict, itime: int, time: float, directory: str):
 
 
 # ----------------------------------------------------------------------------
+# restart files and float32 snapshots
+# ----------------------------------------------------------------------------
+
+def write_restart(path: str, fields, n: int, nums_out: int, n_les: int, time: float, dt: float):
+    """restart_write_parallel (restart_io.f90:240-331): 36-byte header of int32
+    nx, ny, nz, nums, nles and FP64 TIME, DT; then each field as n^3 FP64
+    values in the spectral (i = 1..n, kz, ky) order, the Nyquist pair
+    i = n+1, n+2 dropped.  fields: spectral slots [z(ky), y(kz), n+2]."""
+    hdr = np.array([n, n, n, nums_out, n_les], "<i4").tobytes() + np.array([time, dt], "<f8").tobytes()
+    with open(path, "wb") as f:
+        f.write(hdr)
+        for a in fields:
+            f.write(np.ascontiguousarray(np.asarray(a)[:, :, :n], "<f8").tobytes())
+
+
+def read_restart(path: str, n: int):
+    """restart_read_parallel (restart_io.f90:337-506): returns (header dict,
+    fields [count, n, n, n+2] with the Nyquist pair zero)."""
+    with open(path, "rb") as f:
+        hdr = f.read(36)
+        if len(hdr) != 36:
+            raise ValueError(f"*** error: Cannot read restart header of {path}")
+        nx1, ny1, nz1, nums1, nles1 = (int(v) for v in np.frombuffer(hdr[:20], "<i4"))
+        time, dt = (float(v) for v in np.frombuffer(hdr[20:36], "<f8"))
+        if (nx1, ny1, nz1) != (n, n, n):
+            raise ValueError(f"*** error: Dimensions are different: {(nx1, ny1, nz1)} vs {n}")
+        cnt = 3 + nums1 + nles1
+        data = np.frombuffer(f.read(cnt * n ** 3 * 8), "<f8")
+        if data.size != cnt * n ** 3:
+            raise ValueError(f"*** error: restart file {path} is truncated")
+    out = np.zeros((cnt, n, n, n + 2))
+    out[:, :, :, :n] = data.reshape(cnt, n, n, n)
+    return dict(nx=nx1, ny=ny1, nz=nz1, nums=nums1, nles=nles1, time=time, dt=dt), out
+
+
+def write_tmp4(path: str, a: np.ndarray):
+    """write_tmp4 (write_tmp4.f90:14-30): int32 (nx, ny, nz) then the float32
+    array, x fastest, as one stream."""
+    nz, ny, nx = a.shape
+    with open(path, "wb") as f:
+        f.write(np.array([nx, ny, nz], "<i4").tobytes())
+        f.write(np.ascontiguousarray(a, "<f4").tobytes())
+
+
+# ----------------------------------------------------------------------------
 # particles: initial positions and the binary particle file
 # ----------------------------------------------------------------------------
 
@@ -346,6 +391,8 @@ class Run:
         self.h = Hit3D(p.nx, n_scalars=p.n_scalars, dealias=p.dealias)
         self.h.set_physics(p.nu, flow_type=p.flow_type, kfmax=p.kfmax, famp=p.famp, pe=p.pe,
                            scalar_type=p.scalar_type)
+        if p.n_scalars > 0:
+            self.h.set_reaction(p.reac_sc if len(p.reac_sc) else [0.0] * p.n_scalars)
         if p.les_model == 1:
             self.h.set_les(1, 0.18)   # m_les.f90:1148
         self.ipart = None
@@ -359,14 +406,54 @@ class Run:
         self.courant = 0.0
 
     def begin_new(self):
-        """begin_new.f90:10-31 + main.f90:71-76 (velocity; type-0 scalars are zero)."""
+        """begin_new.f90:10-31 + main.f90:71-76: velocity, scalars (integrated
+        from the start when TSCALAR <= 0), dealias_all."""
         p = self.p
+        self.time = 0.0
+        self.int_scalars = p.tscalar <= 0.0 and p.n_scalars > 0
         self.h.init_velocity(p.rn1, p.isp_type, p.ir_exp, p.peak_wavenum)
-        for s in p.scalar_type:
-            if s % 100 != 0:
-                raise ValueError("only scalar type 0 (zero field, mean gradient) is supported")
+        if p.n_scalars > 0:
+            self.h.init_scalars(p.rn2, p.peak_wavenum_sc)
         self.fov = self.fos = True
         self.h.dealias_all()
+
+    def restart_write(self, run_name: str, directory: str = "."):
+        """restart_write_parallel (restart_io.f90:240-331) to <run>.64.<itime>."""
+        p = self.p
+        nums_out = p.n_scalars if self.int_scalars else 0
+        fields = [self.h.get(FIELDS, c) for c in range(1, 4 + nums_out)]
+        path = os.path.join(directory, f"{run_name}.64.{self.itime:06d}")
+        write_restart(path, fields, p.nx, nums_out, 0, self.time, self.dt)
+        return path
+
+    def begin_restart(self, run_name: str, directory: str = "."):
+        """begin_restart (begin_restart.f90:1-90) with restart_read_parallel:
+        fields from <run>.64.<itmin>; scalars missing from the file are
+        initialised (init_scalar, restart_io.f90:420-431), extra ones dropped;
+        dt halved for variable dt; scalars integrated when TIME > TSCALAR."""
+        p = self.p
+        self.itime = p.itmin
+        hdr, f = read_restart(os.path.join(directory, f"{run_name}.64.{p.itmin:06d}"), p.nx)
+        self.time, self.dt = hdr["time"], hdr["dt"]
+        nums_read = min(p.n_scalars, hdr["nums"])
+        for c in range(3 + nums_read):
+            self.h.put(FIELDS, c + 1, f[c])
+        for s in range(nums_read + 1, p.n_scalars + 1):
+            self.h.init_scalar(s, p.rn2, p.peak_wavenum_sc[s - 1])
+        if p.variable_dt:
+            self.dt = 0.5 * self.dt
+        self.int_scalars = p.n_scalars > 0 and self.time > p.tscalar
+        self.fov = self.fos = True
+
+    def write_snapshot(self, directory: str = "."):
+        """io_write_4 (io_write_4.f90:1-85): u., v., w.<itime> and scNN.<itime>
+        as float32 with the k > kmax modes removed."""
+        ext = f"{self.itime:06d}"
+        for c, name in ((1, "u"), (2, "v"), (3, "w")):
+            write_tmp4(os.path.join(directory, f"{name}.{ext}"), self.h.snapshot(c))
+        if self.int_scalars:
+            for s in range(1, self.p.n_scalars + 1):
+                write_tmp4(os.path.join(directory, f"sc{s:02d}.{ext}"), self.h.snapshot(3 + s))
 
     def init_particles(self, run_name: str | None = None, directory: str = "."):
         """particles_init (m_particles.f90:157-337): pt.<itmin> binary restart
@@ -412,7 +499,8 @@ class Run:
             self.dt = my_dt(self.dt, self.courant, self.int_scalars, p.sc)
         if (self.itime % p.iprint1 == 0 or self.itime % p.iwrite4 == 0):
             if p.n_scalars > 0 and not self.int_scalars and self.time > p.tscalar:
-                self.int_scalars = True  # main.f90:199-204 (type-0 scalars start at zero)
+                self.int_scalars = True  # main.f90:199-204
+                self.h.init_scalars(p.rn2, p.peak_wavenum_sc)
 
     def scalar_stats(self) -> dict:
         """stat_scalars (m_stats.f90:295-366); the reference runs it only once

@@ -27,7 +27,7 @@ program hit3d_gpu
   real*8 :: nu, time, courant, energy, enstrophy
   logical :: variable_dt, int_scalars
   integer, allocatable :: scalar_type(:)
-  real*8, allocatable :: sc(:), pe(:), e_spec(:)
+  real*8, allocatable :: sc(:), pe(:), e_spec(:), ir_sc(:), peak_sc(:), reac_sc(:)
   real*8 :: dummy(3)
   integer(c_int64_t) :: fov, fos, isc, prec
 
@@ -90,13 +90,14 @@ program hit3d_gpu
   read(10,*)
   read(10,*) n_scalars
   read(10,*)
-  allocate(scalar_type(max(n_scalars,1)), sc(max(n_scalars,1)), pe(max(n_scalars,1)))
-  scalar_type = 0; sc = 1.d0
+  allocate(scalar_type(max(n_scalars,1)), sc(max(n_scalars,1)), pe(max(n_scalars,1)), &
+       ir_sc(max(n_scalars,1)), peak_sc(max(n_scalars,1)), reac_sc(max(n_scalars,1)))
+  scalar_type = 0; sc = 1.d0; ir_sc = 0.d0; peak_sc = 1.d0; reac_sc = 0.d0
   nu = 1.d0 / re
   if (n_scalars > 0) then
      read(10,*)
-     do n = 1, n_scalars
-        read(10,*) scalar_type(n), sc(n)
+     do n = 1, n_scalars                     ! m_parameters.f90:440-448
+        read(10,*) scalar_type(n), sc(n), ir_sc(n), peak_sc(n), reac_sc(n)
         pe(n) = nu / sc(n)
      end do
   end if
@@ -118,11 +119,15 @@ program hit3d_gpu
        pe, int(scalar_type, c_int64_t)), 'set_physics')
   call h3d_check(c_init_velocity(rn1, int(isp_type, c_int64_t), int(ir_exp, c_int64_t), peak), &
        'init_velocity')
+  if (n_scalars > 0) then                    ! begin_new.f90:14,24
+     call h3d_check(c_set_reaction(reac_sc), 'set_reaction')
+     call h3d_check(c_init_scalars(rn2, peak_sc), 'init_scalars')
+  end if
   call dealias_all
   call divergence                            ! main.f90:66
   time = 0.d0
   fov = 1; fos = 1
-  int_scalars = .false.
+  int_scalars = (tscalar <= 0.d0 .and. n_scalars > 0)
   prec = precision
 
   ! ---------------------------------------------------------- main loop
@@ -139,7 +144,10 @@ program hit3d_gpu
      end if
      if (mod(itime, iprint1) == 0 .or. mod(itime, iwrite4) == 0) then
         if (time > tmax) exit
-        if (n_scalars > 0 .and. .not. int_scalars .and. time > tscalar) int_scalars = .true.
+        if (n_scalars > 0 .and. .not. int_scalars .and. time > tscalar) then
+           int_scalars = .true.                ! main.f90:199-204
+           call h3d_check(c_init_scalars(rn2, peak_sc), 'init_scalars')
+        end if
      end if
   end do
 

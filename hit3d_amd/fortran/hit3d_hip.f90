@@ -101,6 +101,14 @@ module hit3d_hip
        import :: c_int, c_double
        real(c_double), intent(out) :: dmin, dmax
      end function c_divergence
+     integer(c_int) function c_init_scalars(rn2, peak_sc) bind(C, name="hit3d_init_scalars")
+       import :: c_int, c_double
+       real(c_double), intent(in) :: rn2, peak_sc(*)
+     end function c_init_scalars
+     integer(c_int) function c_set_reaction(reac_sc) bind(C, name="hit3d_set_reaction")
+       import :: c_int, c_double
+       real(c_double), intent(in) :: reac_sc(*)
+     end function c_set_reaction
      integer(c_int) function c_init_velocity(rn1, isp_type, ir_exp, peak) &
           bind(C, name="hit3d_init_velocity")
        import :: c_int, c_int64_t, c_double

@@ -27,6 +27,7 @@ _F64P = ctypes.POINTER(ctypes.c_double)
 _SIGNATURES = {
     "hit3d_init": [_I64P] * 6,
     "hit3d_set_physics": [_F64P, _I64P, _I64P, _F64P, _F64P, _I64P],
+    "hit3d_set_reaction": [_F64P],
     "hit3d_finalize": [],
     "hit3d_put": [_I64P, _I64P, _F64P],
     "hit3d_get": [_I64P, _I64P, _F64P],
@@ -53,6 +54,12 @@ _SIGNATURES = {
     "hit3d_init_velocity": [_F64P, _I64P, _I64P, _F64P],
     "hit3d_random_draws": [_F64P, _I64P, _I64P, _F64P],
     "hit3d_init_gaussian": [_I64P, _F64P, _I64P, _F64P],
+    "hit3d_init_scalars": [_F64P, _F64P],
+    "hit3d_init_scalar": [_I64P, _F64P, _F64P],
+    "hit3d_rng_state": [_I64P],
+    "hit3d_snapshot": [_I64P, ctypes.POINTER(ctypes.c_float)],
+    "hit3d_scalar_gaussian": [_I64P, _F64P, _I64P, _F64P, _I64P],
+    "hit3d_velocity_rng_state": [_I64P, _F64P, _I64P],
     "hit3d_get_ialias": [ctypes.POINTER(ctypes.c_int8)],
     "hit3d_timing_enable": [_I64P],
     "hit3d_timing_read": [_I64P, _F64P, _I64P],
@@ -130,6 +137,23 @@ def init_gaussian(n: int, rn1: float, c: int) -> np.ndarray:
     return out
 
 
+def scalar_gaussian(n: int, rn2: float, iy_prev: int):
+    """init_scalars.f90:105-133 Gaussian field and the generator's saved iy
+    afterwards, from a generator whose saved iy is iy_prev."""
+    lib = load()
+    out = np.zeros((n, n, n + 2))
+    nxt = ctypes.c_int64(0)
+    _check(lib.hit3d_scalar_gaussian(_i(n), _d(rn2), _i(iy_prev), _dp(out), ctypes.byref(nxt)))
+    return out, nxt.value
+
+
+def velocity_rng_state(n: int, rn1: float) -> int:
+    lib = load()
+    iy = ctypes.c_int64(0)
+    _check(lib.hit3d_velocity_rng_state(_i(n), _d(rn1), ctypes.byref(iy)))
+    return iy.value
+
+
 class Hit3D:
     """The device-resident state of one run (one per process)."""
 
@@ -158,6 +182,11 @@ class Hit3D:
         st_a = np.ascontiguousarray(scalar_type, np.int64) if len(scalar_type) else np.zeros(1, np.int64)
         _check(self.lib.hit3d_set_physics(_d(nu), _i(flow_type), _i(kfmax), _d(famp), _dp(pe_a),
                                           st_a.ctypes.data_as(_I64P)))
+
+    def set_reaction(self, reac_sc):
+        """Reaction rates of scalars of type >= 100 (m_parameters.f90:445)."""
+        r = np.ascontiguousarray(reac_sc, np.float64) if len(reac_sc) else np.zeros(1)
+        _check(self.lib.hit3d_set_reaction(_dp(r)))
 
     def put(self, array_id: int, slot: int, a: np.ndarray):
         a = np.ascontiguousarray(a, np.float64)
@@ -285,6 +314,26 @@ class Hit3D:
 
     def init_velocity(self, rn1: float, isp_type: int, ir_exp: int, peak: float):
         _check(self.lib.hit3d_init_velocity(_d(rn1), _i(isp_type), _i(ir_exp), _d(peak)))
+
+    def init_scalars(self, rn2: float, peak_sc=()):
+        """init_scalars.f90:6-383 for every scalar (types from set_physics)."""
+        pk = np.ascontiguousarray(peak_sc, np.float64) if len(peak_sc) else np.zeros(1)
+        _check(self.lib.hit3d_init_scalars(_d(rn2), _dp(pk)))
+
+    def init_scalar(self, n_scalar: int, rn2: float, peak: float):
+        """init_scalar(n) (init_scalars.f90:47-79), no ialias mask."""
+        _check(self.lib.hit3d_init_scalar(_i(n_scalar), _d(rn2), _d(peak)))
+
+    def snapshot(self, n_field: int) -> np.ndarray:
+        """io_write_4 for one field: float32 physical array [z, y, x]."""
+        out = np.zeros((self.n, self.n, self.n), np.float32)
+        _check(self.lib.hit3d_snapshot(_i(n_field), out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out
+
+    def rng_state(self) -> int:
+        iy = ctypes.c_int64(0)
+        _check(self.lib.hit3d_rng_state(ctypes.byref(iy)))
+        return iy.value
 
     # -- timing -------------------------------------------------------------------
     def timing_enable(self, on: bool):

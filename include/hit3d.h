@@ -42,6 +42,12 @@ int hit3d_init(const int64_t *nx, const int64_t *ny, const int64_t *nz_all,
 int hit3d_set_physics(const double *nu, const int64_t *flow_type, const int64_t *kfmax,
                       const double *famp, const double *pe, const int64_t *scalar_type);
 
+/* Reaction rates reac_sc(1:n_scalars) (m_parameters.f90:445) for scalar types
+ * >= 100, whose hundreds digit selects the rate (add_reaction,
+ * rhs_scalars.f90:405-456: 1 KPP, 2 symmetric bistable, 3 self-adjusting
+ * bistable; with the phase shift only type 311, :340-377).  Defaults to 0. */
+int hit3d_set_reaction(const double *reac_sc);
+
 /* x_fftw_allocate(-1) (x_fftw.f90:134-142): frees everything. */
 int hit3d_finalize(void);
 
@@ -71,12 +77,14 @@ int hit3d_x_derivative(const int64_t *n, const int64_t *axis, const int64_t *nto
 /* dealias_all (dealias_all.f90:1-35). */
 int hit3d_dealias_all(void);
 
-/* rhs_scalars (rhs_scalars.f90:1-402, reaction and LES hooks excluded):
- * leaves the physical velocity in wrk(1:3) and, when scalars are active
- * (*int_scalars != 0), the scalar RHS in wrk(4:3+ns). */
+/* rhs_scalars (rhs_scalars.f90:1-402, with the reactions of hit3d_set_reaction
+ * and the Smagorinsky scalar term of hit3d_set_les; LES-only scalars n_les > 0
+ * are rejected): leaves the physical velocity in wrk(1:3) and, when scalars
+ * are active (*int_scalars != 0), the scalar RHS in wrk(4:3+ns). */
 int hit3d_rhs_scalars(const int64_t *int_scalars);
 
-/* rhs_velocity (rhs_velocity.f90:1-419, LES hook excluded): reads the
+/* rhs_velocity (rhs_velocity.f90:1-419, with the Smagorinsky term of
+ * hit3d_set_les, :414-416): reads the
  * physical velocity in wrk(1:3), writes the spectral RHS to wrk(1:3) and
  * returns courant = max(|u|+|v|+|w|)*dt/dx (:31-38). */
 int hit3d_rhs_velocity(const double *dt, double *courant);
@@ -163,6 +171,42 @@ int hit3d_divergence(double *dmin, double *dmax);
  * an addition) into fields(1:3).  isp_type -1 = Taylor-Green (:252-293). */
 int hit3d_init_velocity(const double *rn1, const int64_t *isp_type, const int64_t *ir_exp,
                         const double *peak);
+
+/* init_scalars (init_scalars.f90:6-383) for every scalar, by the last two
+ * digits of its type (hit3d_set_physics): 0 zero field; 1-9 init_scalar_spectrum
+ * (type 1 or 3 exponential, 2 von Karman spectrum with peak peak_sc(n); 3 then
+ * made double-delta +-0.9 and smoothed) from the reference generator seeded
+ * with rn2, continuing from the generator state left by hit3d_init_velocity
+ * (or by the previous spectrum scalar) as the reference's module state does;
+ * 11 tanh slab in z, 12 two tanh slabs in x, 13 sin waves in x.  Ends by
+ * zeroing every mode with ialias > 0 (:27-36). */
+int hit3d_init_scalars(const double *rn2, const double *peak_sc);
+
+/* init_scalar(n) (init_scalars.f90:47-79) for one scalar, without the final
+ * ialias mask: what restart_read_parallel calls for scalars the restart file
+ * lacks (restart_io.f90:426-429). */
+int hit3d_init_scalar(const int64_t *n_scalar, const double *rn2, const double *peak);
+
+/* io_write_4 for one field (io_write_4.f90:16-50): fields(n_field) with every
+ * mode of k^2 > kmax^2 removed, transformed to physical space and rounded to
+ * float32 into out(nx, ny, nz) (x fastest, no padding) -- the tmp4 array
+ * write_tmp4 writes after its int32 (nx, ny, nz) header
+ * (write_tmp4.f90:14-30).  Uses wrk(0) as work space. */
+int hit3d_snapshot(const int64_t *n_field, float *out);
+
+/* The generator's saved iy (RANDu.f:22-24) as the library holds it after its
+ * last initialisation (0 before any). */
+int hit3d_rng_state(int64_t *iy);
+
+/* The Gaussian field of init_scalar_spectrum (init_scalars.f90:105-133) from
+ * a generator whose saved iy is *iy_prev, in the reference layout; *iy_next
+ * receives the saved iy afterwards.  Host-only, like hit3d_init_gaussian. */
+int hit3d_scalar_gaussian(const int64_t *nx, const double *rn2, const int64_t *iy_prev,
+                          double *out, int64_t *iy_next);
+
+/* The saved iy after init_velocity's draws on a fresh generator
+ * (random(-rn1) and 6 (nx+2) nx^2 draws, init_velocity.f90:67-103). */
+int hit3d_velocity_rng_state(const int64_t *nx, const double *rn1, int64_t *iy);
 
 /* Draws of the reference generator (RANDu.f:12-52): a fresh generator,
  * random(-seed), `skip` draws discarded, then `count` draws into out. */

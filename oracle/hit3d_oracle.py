@@ -83,6 +83,10 @@ class RanduGenerator:
         _lib().randu_draw(ctypes.byref(self._s), float(seed), count, _dp(out))
         return out
 
+    @property
+    def iy(self) -> int:
+        return int(self._s.iy)
+
 
 def box_muller(a: np.ndarray, b: np.ndarray) -> np.ndarray:
     """init_velocity.f90:113 with glibc log/sin (elementwise, same shapes)."""
@@ -291,10 +295,32 @@ def _pad(p, n):
     return out
 
 
-def rhs_scalars(fields: np.ndarray, n_scalars: int, pe, scalar_type, dealias: int, n: int):
-    """rhs_scalars.f90:48-134 (2/3 rule), :144-271 (phase shift) and the
-    mean-gradient source :387-389 (scalar type 0 -> RHS -= u-hat).  Reaction
-    types (>=100) are not restated.  Returns spectral RHS[n_scalars]."""
+def reaction_rate(phi: np.ndarray, rtype: int, rrate: float, mean: float) -> np.ndarray:
+    """add_reaction (rhs_scalars.f90:405-456) in physical space: rtype 1 KPP
+    r(1-phi^2), 2 symmetric bistable r(1-phi^2)phi, 3 self-adjusting
+    bistable r(1-phi^2)(phi-mean)."""
+    if rtype == 1:
+        return rrate * (1.0 - phi ** 2)
+    if rtype == 2:
+        return rrate * (1.0 - phi ** 2) * phi
+    if rtype == 3:
+        return rrate * (1.0 - phi ** 2) * (phi - mean)
+    raise ValueError("Unknown reaction rate")
+
+
+def _cube_mask(n: int, kcut: int) -> np.ndarray:
+    kx, ky, kz = _kgrid(n)
+    return (np.abs(kx) > kcut) | (np.abs(ky) > kcut) | (np.abs(kz) > kcut)
+
+
+def rhs_scalars(fields: np.ndarray, n_scalars: int, pe, scalar_type, dealias: int, n: int,
+                reac=None):
+    """rhs_scalars.f90:48-134 (2/3 rule, with add_reaction :405-456 and
+    dealias_rhs :462-496 for types >= 100), :144-271 (phase shift, with the
+    type-311 reaction of :280-380) and the mean-gradient source :387-389
+    (scalar type 0 -> RHS -= u-hat).  Returns spectral RHS[n_scalars]."""
+    reac = [0.0] * n_scalars if reac is None else list(reac)
+    nxyz = float(n) ** 3
     kx, ky, kz = _kgrid(n)
     ia = ialias(n)[:, :, 0::2]
     k2 = (kx ** 2 + ky ** 2) + kz ** 2
@@ -307,7 +333,13 @@ def rhs_scalars(fields: np.ndarray, n_scalars: int, pe, scalar_type, dealias: in
             Q = [to_complex(fft_forward(_pad(phi * q, n))) for q in (u, v, w)]
             conv = (kx * Q[0] + ky * Q[1]) + kz * Q[2]
             r = -1j * conv - pe[s] * k2 * to_complex(fields[3 + s])
-            out.append(np.where(ia > 0, 0.0, r))
+            r = np.where(ia > 0, 0.0, r)
+            if scalar_type[s] >= 100:
+                rate = reaction_rate(phi, scalar_type[s] // 100, reac[s],
+                                     fields[3 + s][0, 0, 0] / nxyz)
+                r = r + to_complex(fft_forward(_pad(rate, n)))
+                r = np.where(_cube_mask(n, kmax_of(n, 0)), 0.0, r)
+            out.append(r)
     else:
         e = phase_factors(n)
         ce = np.conj(e)
@@ -321,7 +353,15 @@ def rhs_scalars(fields: np.ndarray, n_scalars: int, pe, scalar_type, dealias: in
             phi = fft_inverse(fields[3 + s])[:, :, :n]
             Q = [to_complex(fft_forward(_pad(phi * q, n))) for q in (u, v, w)]
             half2 = -0.5j * ((kx * Q[0] + ky * Q[1]) + kz * Q[2]) - pe[s] * k2 * F
-            out.append(np.where(ia > 1, 0.0, half1 + half2))
+            r = np.where(ia > 1, 0.0, half1 + half2)
+            if scalar_type[s] > 300:
+                if scalar_type[s] != 311:
+                    raise ValueError("reaction type not supported with phase shifting")
+                trunc = np.where(_cube_mask(n, n // 4), 0.0, F)
+                ph = fft_inverse(to_interleaved(trunc))[:, :, :n]
+                rate = reaction_rate(ph, 3, reac[s], fields[3 + s][0, 0, 0] / nxyz)
+                r = r + to_complex(fft_forward(_pad(rate, n)))
+            out.append(r)
     for s in range(n_scalars):
         if scalar_type[s] == 0:
             out[s] = out[s] - U[0]
@@ -696,6 +736,84 @@ def init_velocity(n: int, rn1: float, kmax: int, isp_type: int, peak: float, ir_
     scale = np.where(ok, np.sqrt(np.where(ok, t_at / np.where(ok, e_at, 1.0), 0.0)), 0.0)
     f = np.where(ok[None], f * scale[None], 0.0)
     return f, g, e
+
+
+def scalar_target_spectrum(kmax: int, sc_type: int, peak: float) -> np.ndarray:
+    """init_scalars.f90:170-196: type 0 k^-5/3, 1 and 3 exponential, 2 von
+    Karman, normalised to unit sum."""
+    e = np.empty(kmax)
+    for k in range(1, kmax + 1):
+        wmag = float(k)
+        ratio = wmag / peak
+        if sc_type == 0:
+            e[k - 1] = wmag ** (-5.0 / 3.0)
+        elif sc_type in (1, 3):
+            e[k - 1] = ratio ** 3 / peak * np.exp(-3.0 * ratio)
+        elif sc_type == 2:
+            fac = 2.0 * np.pi * ratio
+            e[k - 1] = fac ** 4 / (1.0 + fac ** 2) ** 3
+        else:
+            raise ValueError("INIT_SCALARS: WRONG INITIAL SPECTRUM TYPE")
+    return e / np.sum(e)
+
+
+def scalar_space(n: int, ic_type: int, peak: float) -> np.ndarray:
+    """init_scalar_space (init_scalars.f90:278-383), physical slot."""
+    dx = 2.0 * np.pi / n
+    out = np.zeros((n, n, n + 2))
+    if ic_type == 11:
+        h = max(2.0 * dx, 2.0 * np.pi / peak)
+        zloc = np.arange(n) * dx
+        v = np.tanh((zloc - np.pi * 0.5) / h) - np.tanh((zloc - np.pi * 1.5) / h) - 1.0
+        out[:, :, :n] = v[:, None, None]
+    elif ic_type == 12:
+        h = max(8.0 * dx, np.pi / 8.0)
+        xx = np.arange(n) * dx
+        v = (np.tanh((xx - np.pi * 0.25) / h) - np.tanh((xx - np.pi * 0.75) / h)
+             + np.tanh((xx - np.pi * 1.25) / h) - np.tanh((xx - np.pi * 1.75) / h))
+        out[:, :, :n] = (v - 1.0)[None, None, :]
+    elif ic_type == 13:
+        pk = min(peak, float(n // 8))
+        out[:, :, :n] = np.sin(pk * dx * np.arange(n))[None, None, :]
+    else:
+        raise ValueError("INIT_SCALARS: UNEXPECTED SCALAR TYPE")
+    return out
+
+
+def init_scalars(n: int, kmax: int, scalar_type, peak_sc, rn2: float, gen: RanduGenerator):
+    """init_scalars (init_scalars.f90:6-40) with init_scalar_spectrum
+    (:84-270) and init_scalar_space (:278-383) on one rank.  gen carries the
+    generator state left by the velocity initialisation.  Returns the
+    spectral scalar fields [n_scalars]."""
+    out = []
+    sh32 = shell_index_f32(_k2_interleaved(n))
+    for s, st in enumerate(scalar_type):
+        ic = st - (st // 100) * 100
+        if ic == 0:
+            f = np.zeros((n, n, n + 2))
+        elif ic < 10:
+            gen.random(-rn2)
+            w = gen.draw(rn2, 2 * (n + 2) * n * n).reshape(2, n, n, n + 2)
+            g = fft_forward(box_muller(w[0], w[1]))
+            e = shell_spectrum(np.stack([g, np.zeros_like(g), np.zeros_like(g)]), n, kmax)[1:]
+            tgt = scalar_target_spectrum(kmax, st, peak_sc[s])
+            ok = (sh32 > 0) & (sh32 <= kmax)
+            e_at = np.where(ok, e[np.clip(sh32, 1, kmax) - 1], 0.0)
+            t_at = np.where(ok, tgt[np.clip(sh32, 1, kmax) - 1], 0.0)
+            ok &= e_at > 0.0
+            scale = np.sqrt(np.where(ok, t_at / np.where(ok, e_at, 1.0), 0.0))
+            f = np.where(ok, g * scale, 0.0)
+            if st == 3:
+                ph = fft_inverse(f)[:, :, :n]
+                f = fft_forward(_pad(np.copysign(1.0, ph) * 0.9, n))
+                f = np.where((sh32 == 0) | (sh32 >= kmax * 2 // 3), 0.0, f)
+        else:
+            f = fft_forward(scalar_space(n, ic, peak_sc[s]))
+            if ic in (11, 12):
+                f[0, 0, 0] = 0.0
+        out.append(f)
+    ia = ialias(n)
+    return [np.where(ia > 0, 0.0, f) for f in out]
 
 
 def taylor_green(n: int) -> np.ndarray:

@@ -7,6 +7,9 @@ COUNT x random(RN2).
 
 Stored: the first HEAD values of each run and every STRIDE-th value after
 that (index arrays alongside), so the fixture stays small.
+
+randu_chain16.npz: every value of the chain of calls a 16^3 run with two
+spectrum-initialised scalars makes (velocity, then two scalars on RN2).
 """
 import os
 import subprocess
@@ -39,6 +42,17 @@ def main():
     i2, v2 = pick(b)
     np.savez_compressed(os.path.join(HERE, "randu_golden.npz"), seeds=np.array([RN1, RN2]),
                         counts=np.array([COUNT1, COUNT2]), idx1=i1, bits1=v1, idx2=i2, bits2=v2)
+
+    # the chain a 16^3 run with two spectrum scalars makes: init_velocity
+    # (random(-RN1), 6 (n+2) n^2 draws), then per scalar random(-RN2) and
+    # 2 (n+2) n^2 draws (init_scalars.f90:105-125), all values kept
+    per = 18 * 16 * 16
+    out = subprocess.run([REF, repr(RN1), str(6 * per), repr(RN2), str(2 * per), repr(RN2),
+                          str(2 * per)], check=True, capture_output=True, text=True).stdout.split()
+    bits = np.array([int(v) for v in out], dtype=np.int64)
+    assert len(bits) == 3 + 10 * per
+    np.savez_compressed(os.path.join(HERE, "randu_chain16.npz"), seeds=np.array([RN1, RN2]),
+                        n=np.array([16]), bits=bits)
 
 
 if __name__ == "__main__":

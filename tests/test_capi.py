@@ -65,6 +65,23 @@ def test_random_draws_match_golden_fixture():
         assert v.view(np.int64)[0] == b, i
 
 
+def test_scalar_gaussian_chain_bit_exact(oracle):
+    """The scalar Gaussian fields of init_scalar_spectrum continue the
+    generator state left by init_velocity, bit for bit with the oracle."""
+    n = 16
+    gen = oracle.RanduGenerator()
+    oracle.init_velocity_uniforms(n, RN1, gen)
+    iy = hl.velocity_rng_state(n, RN1)
+    assert iy == gen.iy
+    for _ in range(2):
+        gen.random(-93712139.0)
+        w = gen.draw(93712139.0, 2 * (n + 2) * n * n).reshape(2, n, n, n + 2)
+        want = oracle.box_muller(w[0], w[1])
+        got, iy = hl.scalar_gaussian(n, 93712139.0, iy)
+        assert np.array_equal(got, want)
+        assert iy == gen.iy
+
+
 def test_gaussian_field_bit_exact(oracle):
     """init_velocity.f90:87-113 on one rank: the product's Gaussian field for
     each component equals the oracle's (same draws, glibc Box-Muller)."""

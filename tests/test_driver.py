@@ -77,3 +77,31 @@ def test_fortran_e_editing():
     assert f(9.9999996) == "   0.100000E+02"
     assert f(1e-120) == "   0.100000-119"
     assert f(0.5, 17, 8) == "   0.50000000E+00"
+
+
+def test_restart_file_layout_and_round_trip(tmp_path):
+    """36-byte header (int32 nx, ny, nz, nums, nles; FP64 time, dt), then
+    each field's n^3 spectral values with the Nyquist pair dropped."""
+    n = 8
+    rng = np.random.default_rng(11)
+    f = rng.standard_normal((4, n, n, n + 2))
+    path = tmp_path / "abcdefghij.64.000020"
+    driver.write_restart(str(path), f, n, 1, 0, 0.75, 0.002)
+    raw = path.read_bytes()
+    assert len(raw) == 36 + 4 * n ** 3 * 8
+    assert list(np.frombuffer(raw[:20], "<i4")) == [n, n, n, 1, 0]
+    assert list(np.frombuffer(raw[20:36], "<f8")) == [0.75, 0.002]
+    assert np.array_equal(np.frombuffer(raw[36:36 + 8 * n], "<f8"), f[0, 0, 0, :n])
+    hdr, g = driver.read_restart(str(path), n)
+    assert hdr["nums"] == 1 and hdr["time"] == 0.75 and hdr["dt"] == 0.002
+    assert np.array_equal(g[:, :, :, :n], f[:, :, :, :n])
+    assert not g[:, :, :, n:].any()
+
+
+def test_tmp4_layout(tmp_path):
+    a = np.arange(2 * 3 * 4, dtype=np.float32).reshape(2, 3, 4)   # [z, y, x]
+    path = tmp_path / "u.000000"
+    driver.write_tmp4(str(path), a)
+    raw = path.read_bytes()
+    assert list(np.frombuffer(raw[:12], "<i4")) == [4, 3, 2]
+    assert np.array_equal(np.frombuffer(raw[12:], "<f4"), a.ravel())

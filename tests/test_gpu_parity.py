@@ -252,6 +252,30 @@ def test_rhs_scalars(oracle, dealias):
     h.close()
 
 
+@pytest.mark.parametrize("dealias,types", [(0, (100, 200)), (0, (300, 0)), (1, (311, 105))])
+def test_rhs_scalars_reactions(oracle, dealias, types):
+    """Reacting scalars: add_reaction + dealias_rhs on the 2/3 rule, the
+    type-311 reaction with the phase shift (types 1xx-3xx other than 311 get
+    no reaction there, as in the reference)."""
+    n = 32
+    note(f"rhs_scalars reactions dealias={dealias} types={types}")
+    h = hit3d_amd.Hit3D(n, n_scalars=2, dealias=dealias)
+    nu = 0.02
+    pe = (nu / 0.7, nu / 1.3)
+    reac = (1.7, 0.6)
+    h.set_physics(nu, pe=pe, scalar_type=types)
+    h.set_reaction(reac)
+    f = solenoidal_fields(oracle, n, 41, dealias, count=5)
+    f[3:] *= 0.02   # scalar of order one in physical space
+    for c in range(5):
+        h.put(FIELDS, c + 1, f[c])
+    h.rhs_scalars(True)
+    ref = oracle.rhs_scalars(f, 2, pe, types, dealias, n, reac=reac)
+    for s in range(2):
+        assert rel(h.get(WRK, 4 + s), ref[s]) < 1e-12
+    h.close()
+
+
 def test_force_velocity(oracle):
     n = 32
     note("force_velocity")

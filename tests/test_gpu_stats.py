@@ -101,3 +101,81 @@ def test_divergence_matches_oracle(oracle, n):
         assert max(abs(lo), abs(hi)) < 1e-11 * scale
     finally:
         h.close()
+
+
+@pytest.mark.parametrize("n", [16, 32])
+def test_init_scalars_match_oracle(oracle, n):
+    """init_scalars after init_velocity on the library against the oracle with
+    the same generator chain: types 0, 1, 2, 3 (spectrum, continuing the
+    generator state), 11, 12, 13 (physical space)."""
+    note(f"init_scalars n={n}")
+    rn1, rn2 = 67748937.0, 93712139.0
+    types = (1, 0, 2, 3, 11, 12, 13)
+    peaks = (3.0, 1.0, 2.0, 3.0, 2.0, 2.0, 3.0)
+    ns = len(types)
+    h = hit3d_amd.Hit3D(n, n_scalars=ns)
+    try:
+        h.set_physics(0.01, pe=[0.01] * ns, scalar_type=types)
+        h.init_velocity(rn1, 2, 4, 3.0)
+        h.init_scalars(rn2, peaks)
+        kmax = h.kmax
+        gen = oracle.RanduGenerator()
+        oracle.init_velocity_uniforms(n, rn1, gen)
+        want = oracle.init_scalars(n, kmax, types, peaks, rn2, gen)
+        assert h.rng_state() == gen.iy
+        for s in range(ns):
+            got = h.get(FIELDS, 4 + s)
+            scale = max(np.abs(want[s]).max(), 1e-300)
+            assert np.abs(got - want[s]).max() <= 1e-12 * scale, f"scalar {s + 1} type {types[s]}"
+    finally:
+        h.close()
+
+
+def test_snapshot_matches_oracle(oracle):
+    """io_write_4: spherical cut at kmax, inverse transform, float32."""
+    n = 32
+    f = random_spec(oracle, n, 5, 1)[0]
+    h = hit3d_amd.Hit3D(n)
+    try:
+        h.put(FIELDS, 2, f)
+        got = h.snapshot(2)
+        kx, ky, kz = oracle._kgrid(n)
+        kmax = h.kmax
+        c = np.where((kx ** 2 + ky ** 2) + kz ** 2 > float(kmax) ** 2, 0.0, oracle.to_complex(f))
+        want = oracle.fft_inverse(oracle.to_interleaved(c))[:, :, :n]
+        assert got.dtype == np.float32 and got.shape == (n, n, n)
+        ulp = np.spacing(np.abs(want).astype(np.float32))
+        assert (np.abs(got.astype(np.float64) - want) <= ulp).all()
+    finally:
+        h.close()
+
+
+def test_restart_round_trip_through_run(tmp_path):
+    """A run with a spectrum scalar writes its restart file; a second run
+    reading it holds the same fields bit for bit, dt halved, scalars active."""
+    from hit3d_amd.driver import Params, Run
+    p = Params(nx=16, ny=16, nz_all=16, flow_type=1, re=200.0, dt=0.002, variable_dt=True,
+               isp_type=1, peak_wavenum=3.0, n_scalars=1, scalar_type=[2], sc=[0.7],
+               ir_exp_sc=[0.0], peak_wavenum_sc=[2.0], reac_sc=[0.0], tscalar=-1.0)
+    r = Run(p)
+    try:
+        r.begin_new()
+        for _ in range(3):
+            r.step()
+        path = r.restart_write("abcdefghij", str(tmp_path))
+        want = [r.h.get(FIELDS, c) for c in (1, 2, 3, 4)]
+        t, dt = r.time, r.dt
+    finally:
+        r.h.close()
+    p.itmin = 3
+    r2 = Run(p)
+    try:
+        r2.begin_restart("abcdefghij", str(tmp_path))
+        assert r2.time == t and r2.dt == 0.5 * dt and r2.int_scalars
+        for c in (1, 2, 3, 4):
+            g = r2.h.get(FIELDS, c)
+            assert np.array_equal(g[:, :, :16], want[c - 1][:, :, :16])
+        r2.step()
+    finally:
+        r2.h.close()
+    assert path.endswith("abcdefghij.64.000003")

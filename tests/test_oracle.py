@@ -61,6 +61,20 @@ def test_rng_matches_golden_fixture(oracle):
     assert np.array_equal(run2[g["idx2"]].view(np.int64), g["bits2"])
 
 
+def test_rng_chain_matches_golden_fixture(oracle):
+    """Every value of the call chain of a 16^3 run with two spectrum scalars
+    (init_velocity on RN1, then random(-RN2) and its draws twice), from the
+    reference's own RANDu.f build: pins that re-initialisation keeps iy."""
+    g = np.load(os.path.join(os.path.dirname(GOLDEN), "randu_chain16.npz"))
+    rn1, rn2 = g["seeds"]
+    per = 18 * 16 * 16
+    gen = oracle.RanduGenerator()
+    vals = [gen.random(-rn1)] + list(gen.draw(rn1, 6 * per))
+    for _ in range(2):
+        vals += [gen.random(-rn2)] + list(gen.draw(rn2, 2 * per))
+    assert np.array_equal(np.array(vals).view(np.int64), g["bits"])
+
+
 def test_rng_known_values(oracle):
     """First values for the reference seed RN1 (as printed by the reference
     build: 0.65056227208012396 for random(-RN1))."""
@@ -261,6 +275,53 @@ def _random_spec(oracle, n, seed, count=3):
     return out
 
 
+def _spec_at(c, kx):
+    """complex entry (kx, kz=0, ky=0) of an interleaved spectral slot."""
+    return c[0, 0, 2 * kx] + 1j * c[0, 0, 2 * kx + 1]
+
+
+@pytest.mark.parametrize("st", [100, 200, 300])
+def test_scalar_reaction_closed_forms(oracle, st):
+    """Zero velocity, Pe = 0, phi = a cos 3x on n = 16 (kmax 5): the RHS is the
+    transformed reaction rate.  KPP r(1 - phi^2) = r(1 - a^2/2) - (r a^2/2) cos 6x
+    (cos 6x is outside the cube and is cut by dealias_rhs); the bistable forms
+    r(1-phi^2)phi give (r a - 3/4 r a^3) cos 3x, and their cos 9x part aliases
+    to kx = 7 on this grid, which dealias_rhs also removes."""
+    n, a, r = 16, 0.6, 2.5
+    n3 = float(n) ** 3
+    z = np.zeros((n, n, n + 2))
+    phi = oracle.fft_forward(_phys(n, lambda x, y, zz: a * np.cos(3 * x)))
+    f = np.stack([z, z, z, phi])
+    out = oracle.rhs_scalars(f, 1, [0.0], [st], 0, n, reac=[r])[0]
+    c = oracle.to_complex(out)
+    if st == 100:
+        want0, want3 = r * (1 - a * a / 2), 0.0
+    else:
+        want0, want3 = 0.0, (r * a - 0.75 * r * a ** 3) / 2
+    assert abs(_spec_at(out, 0) - n3 * want0) < 1e-9
+    assert abs(_spec_at(out, 3) - n3 * want3) < 1e-9
+    c[0, 0, 0] = 0.0
+    c[0, 0, 3] = 0.0
+    assert np.abs(c).max() < 1e-9
+
+
+def test_scalar_reaction_311_phase_shift(oracle):
+    """Type 311 with the phase shift: the scalar is cut at |k_i| > n/4 before
+    the rate r(1-phi^2)(phi-mean) is formed, and nothing is cut after it:
+    a cos 3x gives (r a - 3/4 r a^3)/2 at kx = 3 and the aliased -r a^3/8 at
+    kx = 7; a cos 5x is cut entirely, so the RHS vanishes."""
+    n, a, r = 16, 0.6, 2.5
+    n3 = float(n) ** 3
+    z = np.zeros((n, n, n + 2))
+    phi = oracle.fft_forward(_phys(n, lambda x, y, zz: a * np.cos(3 * x)))
+    out = oracle.rhs_scalars(np.stack([z, z, z, phi]), 1, [0.0], [311], 1, n, reac=[r])[0]
+    assert abs(_spec_at(out, 3) - n3 * (r * a - 0.75 * r * a ** 3) / 2) < 1e-9
+    assert abs(_spec_at(out, 7) - n3 * (-r * a ** 3 / 8)) < 1e-9
+    phi5 = oracle.fft_forward(_phys(n, lambda x, y, zz: a * np.cos(5 * x)))
+    out = oracle.rhs_scalars(np.stack([z, z, z, phi5]), 1, [0.0], [311], 1, n, reac=[r])[0]
+    assert np.abs(out).max() < 1e-9
+
+
 def test_pressure_projection(oracle):
     """k.u = 0 after projection, projection is idempotent and removes a pure
     gradient (u = grad phi) entirely.  Catches k_y/k_z swaps and sign errors."""
@@ -366,6 +427,59 @@ def test_divergence_closed_form(oracle):
     assert abs(lo + m) < 1e-14 and abs(hi - m) < 1e-14
     lo, hi = oracle.divergence(oracle.pressure(_random_spec(oracle, n, 3), n), n)
     assert max(abs(lo), abs(hi)) < 1e-12
+
+
+# ------------------------------------------------------------- scalar init
+
+def test_init_scalars_spectrum_shapes(oracle):
+    """Spectrum scalars come out with the normalised target spectrum: the
+    scalar spectrum of stat_scalars (2 x shell sums) equals 2 x target in every
+    shell (2/3 rule, where the ialias mask leaves shells <= kmax intact)."""
+    n = 32
+    kmax = oracle.kmax_of(n, 0)
+    gen = oracle.RanduGenerator()
+    oracle.init_velocity_uniforms(n, RN1, gen)
+    f = oracle.init_scalars(n, kmax, [1, 2], [3.0, 2.0], RN2, gen)
+    for s, st in enumerate((1, 2)):
+        es = oracle.scalar_stats(f[s], n, kmax, 1.0)["es_spec"]
+        tgt = oracle.scalar_target_spectrum(kmax, st, [3.0, 2.0][s])
+        assert np.abs(es - 2 * tgt).max() < 1e-12 * tgt.max()
+        assert abs(tgt.sum() - 1.0) < 1e-14
+
+
+def test_init_scalars_space_types(oracle):
+    """Type 13 is exactly sin(k0 x) with k0 = min(peak, n/8); type 11 varies in z
+    only and type 12 in x only, both with zero mean and values in [-1, 1]."""
+    n = 32
+    kmax = oracle.kmax_of(n, 0)
+    f = oracle.init_scalars(n, kmax, [13, 11, 12], [7.0, 2.0, 2.0], RN2, oracle.RanduGenerator())
+    x = np.arange(n) * 2 * np.pi / n
+    p13 = oracle.fft_inverse(f[0])[:, :, :n]
+    assert np.abs(p13 - np.sin(4.0 * x)[None, None, :]).max() < 1e-14
+    c11 = oracle.to_complex(f[1])
+    nz = np.argwhere(np.abs(c11) > 1e-9 * np.abs(c11).max())
+    assert (nz[:, 0] == 0).all() and (nz[:, 2] == 0).all()        # ky = kx = 0: z only
+    c12 = oracle.to_complex(f[2])
+    nz = np.argwhere(np.abs(c12) > 1e-9 * np.abs(c12).max())
+    assert (nz[:, 0] == 0).all() and (nz[:, 1] == 0).all()        # ky = kz = 0: x only
+    for c in (c11, c12):
+        assert c[0, 0, 0] == 0.0
+    p11 = oracle.fft_inverse(f[1])[:, :, :n]
+    assert np.abs(p11).max() < 1.2
+
+
+def test_init_scalars_double_delta(oracle):
+    """Type 3: shells 0 and >= 2 kmax/3 are empty after smoothing, and the field
+    is far from the Gaussian it started as: its flatness <phi^4>/<phi^2>^2 is
+    near 2 (a smoothed two-valued field), where type 1 gives 3."""
+    n = 32
+    kmax = oracle.kmax_of(n, 0)
+    gen = oracle.RanduGenerator()
+    f = oracle.init_scalars(n, kmax, [3], [3.0], RN2, gen)[0]
+    sh = oracle.shell_index_f32(oracle._k2_interleaved(n))
+    assert np.abs(np.where((sh == 0) | (sh >= kmax * 2 // 3), f, 0.0)).max() == 0.0
+    p = oracle.fft_inverse(f)[:, :, :n]
+    assert np.mean(p ** 4) / np.mean(p ** 2) ** 2 < 2.4
 
 
 # ------------------------------------------------------------------ particles
