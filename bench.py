@@ -101,6 +101,35 @@ def kernel_bytes(n, cls, precision):
     return table.get(cls)
 
 
+def calib_copy(n):
+    """Three device-to-device copies of one slot (wrk(2) = wrk(1)): a plain
+    copy kernel of known size for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import hit3d_amd
+    h = hit3d_amd.Hit3D(n)
+    for _ in range(3):
+        h.copy(hit3d_amd.WRK, 2, hit3d_amd.WRK, 1)
+    h.synchronize()
+    p = ((n // 2 + 1 + 15) // 16) * 16
+    log(f"[bench] calibration copy: 3 x {n * n * p * 16} bytes")
+    h.close()
+
+
+def measured_traffic(n, cls):
+    """HBM bytes per launch of a kernel class from the committed PMC summary
+    (profiles/pmc_traffic.json: FETCH_SIZE + WRITE_SIZE per dispatch, scaled by
+    the same counters on a copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if d.get("n") != n or cls not in d.get("classes", {}):
+        return None
+    c = d["classes"][cls]
+    return c["fetch_bytes"] + c["write_bytes"]
+
+
 def cpu_baseline(n_target):
     """The oracle (numpy/C port of the reference step) on the host, on a
     bounded sample: one step at 128^3 or 256^3, scaled by N^3 log2 N."""
@@ -132,6 +161,9 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib", action="store_true",
+                    help="after the measurement, copy one slot 3 times (a plain copy of known "
+                         "size, the reference for PMC byte counters)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs the whole box on one GPU; --gpus {a.gpus} is not "
@@ -152,6 +184,8 @@ def main():
                      "after_steps": a.steps + a.warmup}
         except Exception as e:  # reported, never silently replaced
             mixed = {"error": str(e)}
+    if a.calib:
+        calib_copy(n)
     # dominant kernel class of the FP64 step
     best, best_ms = None, -1.0
     for cls, (ms, cnt) in tim64.items():
@@ -164,7 +198,7 @@ def main():
         byt = kernel_bytes(n, best, 0)
         ach = byt / avg_s / 1e9
         roof = {"bound": "hbm", "kernel_class": best, "achieved": ach, "peak": PEAK_HBM_GBS,
-                "unit": "GB/s", "frac": ach / PEAK_HBM_GBS, "traffic": None,
+                "unit": "GB/s", "frac": ach / PEAK_HBM_GBS, "traffic": measured_traffic(n, best),
                 "alg_bytes_per_launch": byt, "avg_launch_ms": avg_s * 1e3,
                 "launches_per_step": cnt / a.steps}
     s = slot_bytes(n)
