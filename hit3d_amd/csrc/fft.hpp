@@ -181,8 +181,11 @@ __device__ __forceinline__ void lds_sync() {
 // the thread mapping (and so the twiddles), then recursion to the next.
 // v[f][s]: transform f, slot s.  lds: exchange buffer; element j of
 // transform f sits at lds[f*FS + padi(j*LS)].  t: thread index in the transform.
+// TWREC: the R-1 twiddles of a radix-8/4/2 butterfly come from one table load
+// w and its powers w^2..w^7 by complex products (a few ulp of extra rounding,
+// seven loads fewer); used where twiddle loads dominate the wait time.
 template <int N, int E, int DIR, typename T, int LS, int NF, int FS, int PADSH, bool WAVE,
-          int Pspan>
+          int Pspan, bool TWREC = false>
 struct Stages {
     static constexpr int R = (E < N / Pspan) ? E : N / Pspan;
     static constexpr int TPF = N / E;
@@ -193,7 +196,25 @@ struct Stages {
         for (int q = 0; q < NB; ++q) {
             const int b = t + q * TPF;
             const int k = b & (Pspan - 1);
-            if constexpr (Pspan > 1) {
+            if constexpr (Pspan > 1 && TWREC && (R == 8 || R == 4 || R == 2)) {
+                cpx<T> w[R];
+                w[1] = ldtw(tw, k * (N / (Pspan * R)));
+                if constexpr (DIR > 0) w[1].y = -w[1].y;
+                if constexpr (R >= 4) {
+                    w[2] = cmul(w[1], w[1]);
+                    w[3] = cmul(w[2], w[1]);
+                }
+                if constexpr (R == 8) {
+                    w[4] = cmul(w[2], w[2]);
+                    w[5] = cmul(w[4], w[1]);
+                    w[6] = cmul(w[3], w[3]);
+                    w[7] = cmul(w[6], w[1]);
+                }
+#pragma unroll
+                for (int m = 1; m < R; ++m)
+#pragma unroll
+                    for (int f = 0; f < NF; ++f) v[f][q + m * NB] = cmul(v[f][q + m * NB], w[m]);
+            } else if constexpr (Pspan > 1) {
 #pragma unroll
                 for (int m = 1; m < R; ++m) {
                     cpx<T> w = ldtw(tw, m * k * (N / (Pspan * R)));
@@ -223,17 +244,18 @@ struct Stages {
             for (int s = 0; s < E; ++s)
 #pragma unroll
                 for (int f = 0; f < NF; ++f) v[f][s] = lds[f * FS + padi<PADSH>((t + s * TPF) * LS)];
-            Stages<N, E, DIR, T, LS, NF, FS, PADSH, WAVE, Pspan * R>::run(v, t, lds, tw);
+            Stages<N, E, DIR, T, LS, NF, FS, PADSH, WAVE, Pspan * R, TWREC>::run(v, t, lds, tw);
         }
     }
 };
 
 // NF transforms at once.  Every thread taking part in the exchange must call
 // this (it synchronises), also threads whose data is masked out.
-template <int N, int E, int DIR, typename T, int LS, int NF, int FS, int PADSH, bool WAVE>
+template <int N, int E, int DIR, typename T, int LS, int NF, int FS, int PADSH, bool WAVE,
+          bool TWREC = false>
 __device__ __forceinline__ void multi_fft(cpx<T> (&v)[NF][E], int t, cpx<T>* lds,
                                           const cpx<T>* __restrict__ tw) {
-    Stages<N, E, DIR, T, LS, NF, FS, PADSH, WAVE, 1>::run(v, t, lds, tw);
+    Stages<N, E, DIR, T, LS, NF, FS, PADSH, WAVE, 1, TWREC>::run(v, t, lds, tw);
 }
 
 // One transform with block-wide synchronisation; element j at lds[padi(j*LS)].

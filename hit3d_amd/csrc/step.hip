@@ -56,6 +56,13 @@ struct XP {
     static constexpr int THREADS = ROWS * TPF;
 };
 
+// The fused step runs the 2/3 rule only, where kmax = N/3 (m_parameters.f90:124).
+template <int N>
+struct Kc {
+    static constexpr int K = N / 3;
+    static constexpr int NK = K + 1;
+};
+
 __device__ __forceinline__ bool kept(int j, int K, int n) { return j <= K || j >= n - K; }
 __device__ __forceinline__ int kept_row(int kp, int K, int n) { return kp <= K ? kp : n - 2 * K - 1 + kp; }
 
@@ -81,8 +88,8 @@ __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_inv_z(SG g, P3 F, W6<T>
     __shared__ cpx<T> lds[padded_size<PS>(N * B)];
     const int c = threadIdx.x % B, t = threadIdx.x / B;
     const int kx = blockIdx.x * B + c;
-    const bool valid = kx < g.NK;
-    const int jy = kept_row(blockIdx.y, g.K, N);
+    const bool valid = kx < Kc<N>::NK;
+    const int jy = kept_row(blockIdx.y, Kc<N>::K, N);
     const int comp = blockIdx.z;
     const cpx<double>* src = reinterpret_cast<const cpx<double>*>(F.p[comp]) + (int64_t)jy * N * g.P + kx;
     cpx<T>* dst = reinterpret_cast<cpx<T>*>(A.p[comp]) + (int64_t)jy * g.PA + kx;
@@ -90,7 +97,7 @@ __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_inv_z(SG g, P3 F, W6<T>
 #pragma unroll
     for (int s = 0; s < E; ++s) {
         const int e = t + s * TPF;
-        v[s] = (valid && kept(e, g.K, N)) ? to<T>(src[(int64_t)e * g.P]) : cpx<T>{(T)0, (T)0};
+        v[s] = (valid && kept(e, Kc<N>::K, N)) ? to<T>(src[(int64_t)e * g.P]) : cpx<T>{(T)0, (T)0};
     }
     block_fft<N, E, 1, T, B, PS>(v, t, lds + c, tw + N);
     if (valid) {
@@ -110,14 +117,14 @@ __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_y(SG g, W6<T> A,
     __shared__ cpx<T> lds[padded_size<PS>(N * B)];
     const int c = threadIdx.x % B, t = threadIdx.x / B;
     const int kx = blockIdx.x * B + c;
-    const bool valid = kx < g.NK;
+    const bool valid = kx < Kc<N>::NK;
     const int z = blockIdx.y;
     cpx<T>* base = reinterpret_cast<cpx<T>*>(A.p[blockIdx.z]) + (int64_t)z * N * g.PA + kx;
     cpx<T> v[E];
 #pragma unroll
     for (int s = 0; s < E; ++s) {
         const int e = t + s * TPF;
-        const bool ld = valid && (DIR < 0 || kept(e, g.K, N));
+        const bool ld = valid && (DIR < 0 || kept(e, Kc<N>::K, N));
         v[s] = ld ? base[(int64_t)e * g.PA] : cpx<T>{(T)0, (T)0};
     }
     block_fft<N, E, DIR, T, B, PS>(v, t, lds + c, tw + N);
@@ -125,7 +132,7 @@ __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_y(SG g, W6<T> A,
 #pragma unroll
         for (int s = 0; s < E; ++s) {
             const int e = t + s * TPF;
-            if (DIR > 0 || kept(e, g.K, N)) base[(int64_t)e * g.PA] = v[s];
+            if (DIR > 0 || kept(e, Kc<N>::K, N)) base[(int64_t)e * g.PA] = v[s];
         }
     }
 }
@@ -136,8 +143,11 @@ __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_y(SG g, W6<T> A,
 // as the reference's c2r does, x_fftw.f90:1066-1073); z_n = u_n + i v_n.
 // w rides alone as Z_k = W_k.  Forward: z = p + i q for the product pairs
 // (uu,uv), (uw,vv), (vw,ww); P_k = (Z_k + conj Z_{N-k})/2,
-// Q_k = -i (Z_k - conj Z_{N-k})/2, kept for kx <= kmax.  One LDS row of N
-// entries per row, exchanges synchronised per wavefront.
+// Q_k = -i (Z_k - conj Z_{N-k})/2, kept for kx <= kmax.  Output k sits in
+// lane t, slot s (k = t + s*TPF); N-k sits in lane (TPF-t) mod TPF, slot
+// E-1-s (slot (E-s) mod E for t = 0), so the split is a lane shuffle.  One
+// LDS row of N entries per row for the transform exchanges, synchronised per
+// wavefront; twiddles by recurrence from one table load per butterfly.
 template <int N, typename T>
 __device__ __forceinline__ cpx<T> herm_load(const cpx<T>* rowp, int k, int nk) {
     // entry k of the Hermitian-extended half spectrum (zero outside kx < nk)
@@ -158,13 +168,16 @@ __global__ __launch_bounds__((XP<N, T>::THREADS)) void k_xp(SG g, W6<T> Wb,
     constexpr int E = C::E, TPF = C::TPF;
     constexpr int PS = PadFor<T>::value;
     constexpr int FS = padded_size<PS>(N);
+    constexpr int NK = Kc<N>::NK;
     static_assert(64 % TPF == 0 || TPF % 64 == 0, "row transforms must not straddle waves");
-    constexpr bool WAVE = TPF <= 64;
+    static_assert(TPF <= 64, "the output split shuffles within a wavefront");
+    constexpr bool WAVE = true;
     __shared__ cpx<T> lds[C::ROWS][FS];
     __shared__ double redm[C::THREADS / 64 + 1];
     const int t = threadIdx.x % TPF;
     const int rr = threadIdx.x / TPF;
     const int64_t row = (int64_t)blockIdx.x * C::ROWS + rr;
+    const int partner = (TPF - t) & (TPF - 1);
     cpx<T>* L = lds[rr];
     T u[E], v[E], w[E];
     {
@@ -174,10 +187,10 @@ __global__ __launch_bounds__((XP<N, T>::THREADS)) void k_xp(SG g, W6<T> Wb,
 #pragma unroll
         for (int s = 0; s < E; ++s) {
             const int k = t + s * TPF;
-            const cpx<T> a = herm_load<N>(U, k, g.NK), b = herm_load<N>(V, k, g.NK);
+            const cpx<T> a = herm_load<N>(U, k, NK), b = herm_load<N>(V, k, NK);
             z[0][s] = cpx<T>{a.x - b.y, a.y + b.x};
         }
-        multi_fft<N, E, 1, T, 1, 1, FS, PS, WAVE>(z, t, L, tw + N);
+        multi_fft<N, E, 1, T, 1, 1, FS, PS, WAVE, true>(z, t, L, tw + N);
 #pragma unroll
         for (int s = 0; s < E; ++s) {
             u[s] = z[0][s].x * norm;
@@ -188,9 +201,9 @@ __global__ __launch_bounds__((XP<N, T>::THREADS)) void k_xp(SG g, W6<T> Wb,
         const cpx<T>* Wr = reinterpret_cast<const cpx<T>*>(Wb.p[2]) + row * g.PA;
         cpx<T> z[1][E];
 #pragma unroll
-        for (int s = 0; s < E; ++s) z[0][s] = herm_load<N>(Wr, t + s * TPF, g.NK);
+        for (int s = 0; s < E; ++s) z[0][s] = herm_load<N>(Wr, t + s * TPF, NK);
         lds_sync<WAVE>();  // the previous transform's readers are done
-        multi_fft<N, E, 1, T, 1, 1, FS, PS, WAVE>(z, t, L, tw + N);
+        multi_fft<N, E, 1, T, 1, 1, FS, PS, WAVE, true>(z, t, L, tw + N);
 #pragma unroll
         for (int s = 0; s < E; ++s) w[s] = z[0][s].x * norm;
     }
@@ -212,21 +225,21 @@ __global__ __launch_bounds__((XP<N, T>::THREADS)) void k_xp(SG g, W6<T> Wb,
             if (pr == 2) z[0][s] = cpx<T>{v[s] * w[s], w[s] * w[s]};
         }
         lds_sync<WAVE>();
-        multi_fft<N, E, -1, T, 1, 1, FS, PS, WAVE>(z, t, L, tw + N);
-        lds_sync<WAVE>();
-#pragma unroll
-        for (int s = 0; s < E; ++s) L[padi<PS>(t + s * TPF)] = z[0][s];
-        lds_sync<WAVE>();
+        multi_fft<N, E, -1, T, 1, 1, FS, PS, WAVE, true>(z, t, L, tw + N);
         cpx<T>* outp = reinterpret_cast<cpx<T>*>(Wb.p[2 * pr]) + row * g.PA;
         cpx<T>* outq = reinterpret_cast<cpx<T>*>(Wb.p[2 * pr + 1]) + row * g.PA;
 #pragma unroll
         for (int s = 0; s < E; ++s) {
-            const int k = t + s * TPF;
-            if (k < g.NK) {
-                const cpx<T> zk = L[padi<PS>(k)];
-                const cpx<T> zm = conj(L[padi<PS>((N - k) & (N - 1))]);
-                outp[k] = scale(zk + zm, (T)0.5);
-                outq[k] = scale(mul_mi(zk - zm), (T)0.5);
+            if (s * TPF < NK) {  // the same for every lane
+                const cpx<T> mine = (t == 0) ? z[0][(E - s) & (E - 1)] : z[0][E - 1 - s];
+                const cpx<T> zm = conj(cpx<T>{__shfl(mine.x, partner, TPF),
+                                              __shfl(mine.y, partner, TPF)});
+                const int k = t + s * TPF;
+                if (k < NK) {
+                    const cpx<T> zk = z[0][s];
+                    outp[k] = scale(zk + zm, (T)0.5);
+                    outq[k] = scale(mul_mi(zk - zm), (T)0.5);
+                }
             }
         }
     }
@@ -263,8 +276,8 @@ __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_fwd_z_update(SG g, W6<T
     __shared__ cpx<T> lds[padded_size<PS>(N * B)];
     const int c = threadIdx.x % B, t = threadIdx.x / B;
     const int kx = blockIdx.x * B + c;
-    const bool valid = kx < g.NK;
-    const int jy = kept_row(blockIdx.y, g.K, N);
+    const bool valid = kx < Kc<N>::NK;
+    const int jy = kept_row(blockIdx.y, Kc<N>::K, N);
     const int ky = wavenum(jy, N);
     const T kxT = (T)kx, kyT = (T)ky;
     // t_i = k_j P_ij accumulated in the reference's order (rhs_velocity.f90:89-91)
@@ -288,6 +301,7 @@ __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_fwd_z_update(SG g, W6<T
         block_fft<N, E, -1, T, B, PS>(v, t, lds + c, tw + N);
 #pragma unroll
         for (int s = 0; s < E; ++s) {
+            if (!kept(t + s * TPF, Kc<N>::K, N)) continue;  // slots never written out
             const T kzT = (T)wavenum(t + s * TPF, N);
             // p: 0 uu, 1 uv, 2 uw, 3 vv, 4 vw, 5 ww
             if (p == 0) acc[0][s] = scale(v[s], kxT);
@@ -320,7 +334,7 @@ __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_fwd_z_update(SG g, W6<T
 #pragma unroll
     for (int s = 0; s < E; ++s) {
         const int e = t + s * TPF;
-        if (!kept(e, g.K, N)) continue;
+        if (!kept(e, Kc<N>::K, N)) continue;
         const int kz = wavenum(e, N);
         const double kzd = kz;
         const int64_t off = ((int64_t)jy * N + e) * g.P + kx;
@@ -377,6 +391,7 @@ static void dispatch_step(int n, F&& f) {
 
 template <int N, typename T>
 static void run(Lib& L, double dt, bool first, const cpx<T>* tw) {
+    if (L.kmax != Kc<N>::K) throw std::runtime_error("the fused step expects kmax = n/3");
     SG g;
     g.K = L.kmax;
     g.NK = L.kmax + 1;
