@@ -169,6 +169,11 @@ static void free_all() {
     if (L.dshell) (void)hipFree(L.dshell);
     if (L.dpart) (void)hipFree(L.dpart);
     if (L.tv) (void)hipFree(L.tv);
+    for (void* q : {(void*)L.mlp.p, (void*)L.mlp.wb, (void*)L.mlp.m, (void*)L.mlp.v,
+                    (void*)L.mlp.g, (void*)L.mlp.partial, (void*)L.mlp.lossp, (void*)L.mlp.loss,
+                    (void*)L.mlp.X, (void*)L.mlp.T, (void*)L.mlp.bad})
+        if (q) (void)hipFree(q);
+    for (auto* q : L.mlp.buf) if (q) (void)hipFree(q);
     if (L.xyzp) (void)hipFree(L.xyzp);
     if (L.uvwp) (void)hipFree(L.uvwp);
     if (L.dscal) (void)hipFree(L.dscal);
@@ -212,9 +217,15 @@ static void les_turb_visc() {
     }
 }
 
+static void mlp_velocity();
+
 // les_rhs_velocity for model 1 (m_les.f90:636-690 with les_rhsv_turb_visc
 // :902-1018): rhs += -d_j tau_ij, tau_ij = -2 nu_t S_ij, then the ialias mask.
 static void les_velocity() {
+    if (lib().les_model == 11) {
+        mlp_velocity();
+        return;
+    }
     Lib& L = lib();
     auto& W = L.W;
     const double* U[3] = {L.F[1], L.F[2], L.F[3]};
@@ -264,6 +275,65 @@ static void les_scalar(int n) {
     Timed t(TC_POINTWISE);
     launch_add(L.g, W[n], W[0], true, L.st);
     launch_mask_alias(L.g, W[n], 0, L.st);
+}
+
+// nine physical velocity gradients du_i/dx_j (i-major) of fields(1:3) into
+// the closure's buffers 0..8
+static void mlp_gradients() {
+    Lib& L = lib();
+    auto& B = L.mlp.buf;
+    if (B.empty()) {
+        B.assign(15, nullptr);
+        for (auto& q : B) q = dalloc(slot_bytes());
+    }
+    for (int c = 0; c < 3; ++c)
+        for (int a = 0; a < 3; ++a) {
+            {
+                Timed t(TC_POINTWISE);
+                launch_derivative(L.g, L.F[c + 1], B[3 * c + a], a + 1, L.st);
+            }
+            fft_inverse(B[3 * c + a]);
+        }
+}
+
+// The learned closure's stress divergence: tau_ij from the MLP at every point,
+// rhs_i += -i k_j tau_ij on unaliased modes as for model 1.  A step whose
+// network output is not finite everywhere uses the Smagorinsky stress
+// instead (the fallback) and is counted.
+static void mlp_velocity() {
+    Lib& L = lib();
+    auto& M = L.mlp;
+    mlp_gradients();
+    const double* gr[9];
+    for (int i = 0; i < 9; ++i) gr[i] = M.buf[i];
+    double* tau[6];
+    for (int i = 0; i < 6; ++i) tau[i] = M.buf[9 + i];
+    check(hipMemsetAsync(M.bad, 0, sizeof(unsigned int), L.st), "hipMemsetAsync");
+    {
+        Timed t(TC_POINTWISE);
+        launch_mlp_les(L.g, gr, M.delta, M.norm, M.wb, M.p, tau, M.bad, L.st);
+    }
+    unsigned int bad = 0;
+    check(hipMemcpyAsync(&bad, M.bad, sizeof(bad), hipMemcpyDeviceToHost, L.st), "hipMemcpyAsync");
+    check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
+    if (bad != 0) {
+        ++M.fallbacks;
+        if (!L.tv) L.tv = dalloc(slot_bytes());
+        const int keep = L.les_model;
+        L.les_model = 1;
+        les_turb_visc();
+        les_velocity();
+        L.les_model = keep;
+        return;
+    }
+    for (int i = 9; i < 15; ++i) fft_forward(M.buf[i]);
+    double* rhs[3] = {L.W[1], L.W[2], L.W[3]};
+    const double* t1[3] = {M.buf[9], M.buf[10], M.buf[11]};   // tau11, tau12, tau13
+    const double* t2[3] = {M.buf[12], M.buf[13], M.buf[14]};  // tau22, tau23, tau33
+    Timed t(TC_POINTWISE);
+    launch_les_vel_accum(L.g, t1, rhs, 1, L.st);
+    launch_les_vel_accum(L.g, t2, rhs, 2, L.st);
+    for (int c = 1; c <= 3; ++c) launch_mask_alias(L.g, L.W[c], 0, L.st);
 }
 
 static void do_rhs_scalars(bool int_scalars) {
@@ -440,7 +510,7 @@ static void do_rhs_velocity() {
         Timed t(TC_POINTWISE);
         for (int c = 1; c <= 3; ++c) launch_mask_ps(g, W[c], L.st);
     }
-    if (L.les_model == 1) les_velocity();  // rhs_velocity.f90:414-416
+    if (L.les_model == 1 || L.les_model == 11) les_velocity();  // rhs_velocity.f90:414-416
 }
 
 static void do_force(double* energy) {
@@ -1183,14 +1253,235 @@ int hit3d_set_les(const int64_t* les_model, const double* c_smag) {
     return guarded([&] {
         require_ready();
         Lib& L = lib();
-        if (*les_model != 0 && *les_model != 1)
-            throw std::runtime_error("only les_model 0 (none) and 1 (Smagorinsky) are supported");
-        if (*les_model == 1 && L.dealias != 0)
+        if (*les_model != 0 && *les_model != 1 && *les_model != 11)
+            throw std::runtime_error(
+                "only les_model 0 (none), 1 (Smagorinsky) and 11 (learned closure) are supported");
+        if (*les_model == 11 && !(L.mlp.have_params && L.mlp.have_norm))
+            throw std::runtime_error("les_model 11: set the closure's parameters and scaling first");
+        if (*les_model == 11 && L.ns > 0)
+            throw std::runtime_error("les_model 11 closes the velocity only (no scalars)");
+        if (*les_model != 0 && L.dealias != 0)
             throw std::runtime_error("LES needs the 2/3 rule (m_parameters.f90:415-420)");
         L.les_model = (int)*les_model;
         L.c_smag = *c_smag;
-        if (L.les_model == 1 && !L.tv) L.tv = dalloc(slot_bytes());
+        if (L.les_model != 0 && !L.tv) L.tv = dalloc(slot_bytes());
+        if (L.les_model == 11) L.mlp.delta = L.dx;  // filter width of the LES grid
     });
+}
+
+// ------------------------------------------------------------ learned closure
+static void mlp_alloc_params() {
+    auto& M = lib().mlp;
+    if (M.p) return;
+    const size_t b = sizeof(float) * mlp::NPARAM;
+    M.p = reinterpret_cast<float*>(dalloc(b));
+    M.m = reinterpret_cast<float*>(dalloc(b));
+    M.v = reinterpret_cast<float*>(dalloc(b));
+    M.g = reinterpret_cast<float*>(dalloc(b));
+    M.wb = reinterpret_cast<uint16_t*>(dalloc(sizeof(uint16_t) * mlp::NPARAM));
+    M.bad = reinterpret_cast<unsigned int*>(dalloc(sizeof(unsigned int) * 4));
+}
+
+static void mlp_reserve(int64_t tiles, int64_t steps) {
+    auto& M = lib().mlp;
+    if (tiles > M.cap_tiles) {
+        if (M.partial) check(hipFree(M.partial), "hipFree");
+        if (M.lossp) check(hipFree(M.lossp), "hipFree");
+        M.partial = reinterpret_cast<float*>(dalloc(sizeof(float) * mlp::NPARAM * tiles));
+        M.lossp = reinterpret_cast<float*>(dalloc(sizeof(float) * tiles));
+        M.cap_tiles = tiles;
+    }
+    if (steps > M.cap_steps) {
+        if (M.loss) check(hipFree(M.loss), "hipFree");
+        M.loss = reinterpret_cast<float*>(dalloc(sizeof(float) * steps));
+        M.cap_steps = steps;
+    }
+}
+
+int hit3d_mlp_set_params(const float* params) {
+    return guarded([&] {
+        require_ready();
+        Lib& L = lib();
+        mlp_alloc_params();
+        auto& M = L.mlp;
+        const size_t b = sizeof(float) * mlp::NPARAM;
+        check(hipMemcpyAsync(M.p, params, b, hipMemcpyHostToDevice, L.st), "hipMemcpyAsync");
+        check(hipMemsetAsync(M.m, 0, b, L.st), "hipMemsetAsync");
+        check(hipMemsetAsync(M.v, 0, b, L.st), "hipMemsetAsync");
+        launch_mlp_to_bf16(M.p, M.wb, L.st);
+        check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
+        M.step = 0;
+        M.have_params = true;
+    });
+}
+
+int hit3d_mlp_get_params(float* params) {
+    return guarded([&] {
+        require_ready();
+        Lib& L = lib();
+        if (!L.mlp.have_params) throw std::runtime_error("mlp: no parameters set");
+        check(hipMemcpyAsync(params, L.mlp.p, sizeof(float) * mlp::NPARAM, hipMemcpyDeviceToHost,
+                             L.st),
+              "hipMemcpyAsync");
+        check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
+    });
+}
+
+int hit3d_mlp_set_norm(const double* xmean, const double* xstd, const double* ymean,
+                       const double* ystd) {
+    return guarded([&] {
+        auto& N = lib().mlp.norm;
+        for (int i = 0; i < mlp::NIN; ++i) {
+            if (!(xstd[i] > 0.0)) throw std::runtime_error("mlp: input scales must be positive");
+            N.xmean[i] = (float)xmean[i];
+            N.xstd[i] = (float)xstd[i];
+        }
+        for (int i = 0; i < mlp::NOUT; ++i) {
+            if (!(ystd[i] > 0.0)) throw std::runtime_error("mlp: output scales must be positive");
+            N.ymean[i] = (float)ymean[i];
+            N.ystd[i] = (float)ystd[i];
+        }
+        lib().mlp.have_norm = true;
+    }, false);
+}
+
+int hit3d_mlp_set_data(const int64_t* count, const float* X, const float* T) {
+    return guarded([&] {
+        require_ready();
+        Lib& L = lib();
+        auto& M = L.mlp;
+        if (*count < 1) throw std::runtime_error("mlp: empty training set");
+        if (M.X) check(hipFree(M.X), "hipFree");
+        if (M.T) check(hipFree(M.T), "hipFree");
+        M.X = M.T = nullptr;
+        M.X = reinterpret_cast<float*>(dalloc(sizeof(float) * mlp::NIN * *count));
+        M.T = reinterpret_cast<float*>(dalloc(sizeof(float) * mlp::NOUT * *count));
+        check(hipMemcpyAsync(M.X, X, sizeof(float) * mlp::NIN * *count, hipMemcpyHostToDevice, L.st),
+              "hipMemcpyAsync");
+        check(hipMemcpyAsync(M.T, T, sizeof(float) * mlp::NOUT * *count, hipMemcpyHostToDevice, L.st),
+              "hipMemcpyAsync");
+        check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
+        M.count = *count;
+    });
+}
+
+int hit3d_mlp_train(const int64_t* steps, const int64_t* batch, const double* lr, double* losses) {
+    return guarded([&] {
+        require_ready();
+        Lib& L = lib();
+        auto& M = L.mlp;
+        if (!M.have_params || !M.X) throw std::runtime_error("mlp: parameters and data first");
+        if (*batch < 32 || *batch % 32 != 0) throw std::runtime_error("mlp: batch must be a multiple of 32");
+        if (*steps < 1) return;
+        const int64_t tiles = *batch / 32;
+        mlp_reserve(tiles, *steps);
+        for (int64_t i = 0; i < *steps; ++i) {
+            const int64_t start = ((int64_t)M.step * *batch) % M.count;
+            launch_mlp_train_tiles(M.X, M.T, M.count, start, (int)tiles, *batch, M.wb, M.p,
+                                   M.partial, M.lossp, L.st);
+            launch_mlp_reduce(M.partial, M.lossp, (int)tiles, M.g, M.loss + i, L.st);
+            ++M.step;
+            launch_mlp_adam(M.p, M.wb, M.g, M.m, M.v, M.step, (float)*lr, L.st);
+        }
+        std::vector<float> ls(*steps);
+        check(hipMemcpyAsync(ls.data(), M.loss, sizeof(float) * *steps, hipMemcpyDeviceToHost, L.st),
+              "hipMemcpyAsync");
+        check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
+        if (losses)
+            for (int64_t i = 0; i < *steps; ++i) losses[i] = (double)ls[i] / (double)(*batch * mlp::NOUT);
+    });
+}
+
+int hit3d_mlp_grads(const int64_t* count, const float* X, const float* T, float* grads,
+                    double* loss) {
+    return guarded([&] {
+        require_ready();
+        Lib& L = lib();
+        auto& M = L.mlp;
+        if (!M.have_params) throw std::runtime_error("mlp: parameters first");
+        const int64_t n = *count;
+        if (n < 32 || n % 32 != 0) throw std::runtime_error("mlp: count must be a multiple of 32");
+        float* dX = reinterpret_cast<float*>(dalloc(sizeof(float) * mlp::NIN * n));
+        float* dT = reinterpret_cast<float*>(dalloc(sizeof(float) * mlp::NOUT * n));
+        check(hipMemcpyAsync(dX, X, sizeof(float) * mlp::NIN * n, hipMemcpyHostToDevice, L.st), "hipMemcpyAsync");
+        check(hipMemcpyAsync(dT, T, sizeof(float) * mlp::NOUT * n, hipMemcpyHostToDevice, L.st), "hipMemcpyAsync");
+        const int64_t tiles = n / 32;
+        mlp_reserve(tiles, 1);
+        launch_mlp_train_tiles(dX, dT, n, 0, (int)tiles, n, M.wb, M.p, M.partial, M.lossp, L.st);
+        launch_mlp_reduce(M.partial, M.lossp, (int)tiles, M.g, M.loss, L.st);
+        float ls = 0.f;
+        check(hipMemcpyAsync(grads, M.g, sizeof(float) * mlp::NPARAM, hipMemcpyDeviceToHost, L.st), "hipMemcpyAsync");
+        check(hipMemcpyAsync(&ls, M.loss, sizeof(float), hipMemcpyDeviceToHost, L.st), "hipMemcpyAsync");
+        check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
+        check(hipFree(dX), "hipFree");
+        check(hipFree(dT), "hipFree");
+        if (loss) *loss = (double)ls / (double)(n * mlp::NOUT);
+    });
+}
+
+int hit3d_mlp_forward(const int64_t* count, const float* X, float* Y) {
+    return guarded([&] {
+        require_ready();
+        Lib& L = lib();
+        auto& M = L.mlp;
+        if (!M.have_params) throw std::runtime_error("mlp: parameters first");
+        const int64_t n = *count;
+        if (n < 1) return;
+        float* dX = reinterpret_cast<float*>(dalloc(sizeof(float) * mlp::NIN * n));
+        float* dY = reinterpret_cast<float*>(dalloc(sizeof(float) * mlp::NOUT * n));
+        check(hipMemcpyAsync(dX, X, sizeof(float) * mlp::NIN * n, hipMemcpyHostToDevice, L.st), "hipMemcpyAsync");
+        launch_mlp_forward(dX, n, M.wb, M.p, dY, L.st);
+        check(hipMemcpyAsync(Y, dY, sizeof(float) * mlp::NOUT * n, hipMemcpyDeviceToHost, L.st), "hipMemcpyAsync");
+        check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
+        check(hipFree(dX), "hipFree");
+        check(hipFree(dY), "hipFree");
+    });
+}
+
+int hit3d_box_filter(const int64_t* ratio, double* ubar, double* tau) {
+    return guarded([&] {
+        require_ready();
+        Lib& L = lib();
+        const int r = (int)*ratio;
+        if (r < 1 || L.n % r != 0) throw std::runtime_error("box filter: ratio must divide n");
+        if (9 * L.n * (int64_t)sizeof(double) > 65536)
+            throw std::runtime_error("box filter: grids up to 512 points per direction");
+        const int m = L.n / r;
+        const size_t m3 = (size_t)m * m * m;
+        double* d = dalloc(sizeof(double) * 9 * m3);
+        {
+            Timed t(TC_POINTWISE);
+            launch_box_filter(L.g, L.W[1], L.W[2], L.W[3], r, d, d + 3 * m3, L.st);
+        }
+        check(hipMemcpyAsync(ubar, d, sizeof(double) * 3 * m3, hipMemcpyDeviceToHost, L.st), "hipMemcpyAsync");
+        check(hipMemcpyAsync(tau, d + 3 * m3, sizeof(double) * 6 * m3, hipMemcpyDeviceToHost, L.st),
+              "hipMemcpyAsync");
+        check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
+        check(hipFree(d), "hipFree");
+    });
+}
+
+int hit3d_sgs_features(const double* delta, float* X) {
+    return guarded([&] {
+        require_ready();
+        Lib& L = lib();
+        mlp_gradients();
+        const double* gr[9];
+        for (int i = 0; i < 9; ++i) gr[i] = L.mlp.buf[i];
+        const size_t n3 = (size_t)L.n * L.n * L.n;
+        float* d = reinterpret_cast<float*>(dalloc(sizeof(float) * mlp::NIN * n3));
+        {
+            Timed t(TC_POINTWISE);
+            launch_sgs_features(L.g, gr, *delta, d, L.st);
+        }
+        check(hipMemcpyAsync(X, d, sizeof(float) * mlp::NIN * n3, hipMemcpyDeviceToHost, L.st), "hipMemcpyAsync");
+        check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
+        check(hipFree(d), "hipFree");
+    });
+}
+
+int hit3d_mlp_fallbacks(int64_t* count) {
+    return guarded([&] { *count = lib().mlp.fallbacks; }, false);
 }
 
 int hit3d_set_fusion(const int64_t* on) {

@@ -6,6 +6,7 @@
 
 #include "common.hpp"
 #include "particles.hpp"
+#include "sgs_mlp.hpp"
 #include "spectral.hpp"
 
 namespace hit3d {
@@ -72,6 +73,26 @@ struct Lib {
     int64_t np = 0, np_cap = 0;
     double* xyzp = nullptr;
     double* uvwp = nullptr;
+
+    // learned closure (les_model 11): parameters, Adam state, training data
+    struct Mlp {
+        float* p = nullptr;       // FP32 master parameters (mlp::NPARAM)
+        uint16_t* wb = nullptr;   // bfloat16 copy
+        float *m = nullptr, *v = nullptr, *g = nullptr;
+        float* partial = nullptr; // per-tile gradient partials
+        float* lossp = nullptr;
+        float* loss = nullptr;    // per-step losses of the last train call
+        int64_t cap_tiles = 0, cap_steps = 0;
+        int step = 0;
+        float *X = nullptr, *T = nullptr;
+        int64_t count = 0;
+        MlpNorm norm{};
+        bool have_params = false, have_norm = false;
+        std::vector<double*> buf;  // 9 gradient + 6 stress slots
+        unsigned int* bad = nullptr;
+        int64_t fallbacks = 0;     // steps that used Smagorinsky instead
+        double delta = 0.0;
+    } mlp;
 
     bool timing = false;
     struct Ev {

@@ -43,6 +43,17 @@ _SIGNATURES = {
     "hit3d_pressure": [],
     "hit3d_step": [_F64P, _I64P, _I64P, _I64P, _I64P, _F64P, _F64P],
     "hit3d_set_fusion": [_I64P],
+    "hit3d_mlp_set_params": [ctypes.POINTER(ctypes.c_float)],
+    "hit3d_mlp_get_params": [ctypes.POINTER(ctypes.c_float)],
+    "hit3d_mlp_set_norm": [_F64P] * 4,
+    "hit3d_mlp_set_data": [_I64P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)],
+    "hit3d_mlp_train": [_I64P, _I64P, _F64P, _F64P],
+    "hit3d_mlp_grads": [_I64P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                        ctypes.POINTER(ctypes.c_float), _F64P],
+    "hit3d_mlp_forward": [_I64P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)],
+    "hit3d_mlp_fallbacks": [_I64P],
+    "hit3d_box_filter": [_I64P, _F64P, _F64P],
+    "hit3d_sgs_features": [_F64P, ctypes.POINTER(ctypes.c_float)],
     "hit3d_set_les": [_I64P, _F64P],
     "hit3d_stats": [_F64P, _F64P],
     "hit3d_stat_scalars": [_F64P] * 6,
@@ -329,6 +340,71 @@ class Hit3D:
         out = np.zeros((self.n, self.n, self.n), np.float32)
         _check(self.lib.hit3d_snapshot(_i(n_field), out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
         return out
+
+    # -- learned closure --------------------------------------------------------
+    NPARAM = 6288
+
+    @staticmethod
+    def _fp(a):
+        return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+    def mlp_set_params(self, params: np.ndarray):
+        p = np.ascontiguousarray(params, np.float32).ravel()
+        if p.size != self.NPARAM:
+            raise ValueError(f"expected {self.NPARAM} parameters")
+        _check(self.lib.hit3d_mlp_set_params(self._fp(p)))
+
+    def mlp_get_params(self) -> np.ndarray:
+        p = np.zeros(self.NPARAM, np.float32)
+        _check(self.lib.hit3d_mlp_get_params(self._fp(p)))
+        return p
+
+    def mlp_set_norm(self, xmean, xstd, ymean, ystd):
+        a = [np.ascontiguousarray(v, np.float64) for v in (xmean, xstd, ymean, ystd)]
+        _check(self.lib.hit3d_mlp_set_norm(*[_dp(v) for v in a]))
+
+    def mlp_set_data(self, X: np.ndarray, T: np.ndarray):
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        self._mlp_data = (X, T)
+        _check(self.lib.hit3d_mlp_set_data(_i(X.shape[0]), self._fp(X), self._fp(T)))
+
+    def mlp_train(self, steps: int, batch: int, lr: float) -> np.ndarray:
+        losses = np.zeros(steps)
+        _check(self.lib.hit3d_mlp_train(_i(steps), _i(batch), _d(lr), _dp(losses)))
+        return losses
+
+    def mlp_grads(self, X: np.ndarray, T: np.ndarray):
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        g = np.zeros(self.NPARAM, np.float32)
+        loss = ctypes.c_double(0.0)
+        _check(self.lib.hit3d_mlp_grads(_i(X.shape[0]), self._fp(X), self._fp(T), self._fp(g),
+                                         ctypes.byref(loss)))
+        return g, loss.value
+
+    def mlp_forward(self, X: np.ndarray) -> np.ndarray:
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.zeros((X.shape[0], 6), np.float32)
+        _check(self.lib.hit3d_mlp_forward(_i(X.shape[0]), self._fp(X), self._fp(Y)))
+        return Y
+
+    def mlp_fallbacks(self) -> int:
+        c = ctypes.c_int64(0)
+        _check(self.lib.hit3d_mlp_fallbacks(ctypes.byref(c)))
+        return c.value
+
+    def box_filter(self, ratio: int):
+        m = self.n // ratio
+        ubar = np.zeros((3, m, m, m))
+        tau = np.zeros((6, m, m, m))
+        _check(self.lib.hit3d_box_filter(_i(ratio), _dp(ubar), _dp(tau)))
+        return ubar, tau
+
+    def sgs_features(self, delta: float) -> np.ndarray:
+        X = np.zeros((self.n ** 3, 10), np.float32)
+        _check(self.lib.hit3d_sgs_features(_d(delta), self._fp(X)))
+        return X
 
     def rng_state(self) -> int:
         iy = ctypes.c_int64(0)

@@ -1,0 +1,245 @@
+"""The learned subgrid-stress closure's workflow on the GPU library: training
+data from box-filtered DNS, standardisation, training, and coarse LES runs
+with the closure or with Smagorinsky, judged by the filtered-DNS spectrum.
+
+No counterpart in the reference: the closure stands in for the eddy
+viscosity of m_les.f90:1142-1215 (kept as the fallback).  Every field
+operation runs on the GPU through the C ABI; this module only moves
+host-side arrays between the two grids (one library instance holds one grid
+at a time).
+"""
+from __future__ import annotations
+
+import math
+import sys
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .driver import Params, Run, my_dt
+from .lib import FIELDS, WRK, Hit3D
+
+# tau components in the closure's order and the velocity pairs they belong to
+TAU_PAIRS = ((0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2))
+
+
+def log(msg: str):
+    print(msg, file=sys.stderr, flush=True)
+
+
+@dataclass
+class Snapshot:
+    time: float
+    ubar: np.ndarray   # [3, m, m, m] physical, box-filtered
+    tau: np.ndarray    # [6, m, m, m] avg(u_i u_j) - ubar_i ubar_j
+
+
+@dataclass
+class Dataset:
+    X: np.ndarray      # [P, 10] raw inputs
+    T: np.ndarray      # [P, 6] deviatoric stress
+    m: int
+    delta: float
+    ek_filtered: list = field(default_factory=list)  # E(k) of each filtered snapshot
+
+
+def dns_params(n: int, re: float = 800.0) -> Params:
+    """Forced DNS as in the 512^3 configuration: band forcing KFMAX=2,
+    FAMP=0.5, 2/3 rule, variable dt, seeded k^4 exp(-2(k/3)^2) field."""
+    return Params(nx=n, ny=n, nz_all=n, flow_type=1, re=re, dt=1e-3, variable_dt=True,
+                  isp_type=2, ir_exp=4, peak_wavenum=3.0, kfmax=2, famp=0.5, dealias=0)
+
+
+def run_dns(n: int, ratio: int, spinup: int, nsnap: int, every: int, re: float = 800.0,
+            precision: int = 0) -> list:
+    """Box-filtered snapshots of a forced DNS: spinup steps, then nsnap
+    snapshots every `every` steps (the filter ratio sets the coarse grid)."""
+    r = Run(dns_params(n, re), precision=precision)
+    snaps = []
+    try:
+        r.begin_new()
+        t0 = time.time()
+        for i in range(spinup):
+            r.step()
+            if (i + 1) % max(1, spinup // 10) == 0:
+                log(f"[sgs] dns n={n} spin-up {i + 1}/{spinup} t={r.time:.3f} "
+                    f"courant={r.courant:.3f} ({time.time() - t0:.0f}s)")
+        for s in range(nsnap):
+            for _ in range(every):
+                r.step()
+            r.h.fields_to_parts()
+            ubar, tau = r.h.box_filter(ratio)
+            snaps.append(Snapshot(r.time, ubar, tau))
+            log(f"[sgs] dns snapshot {s + 1}/{nsnap} t={r.time:.3f} energy={r.stats().energy:.4f}")
+    finally:
+        r.h.close()
+    return snaps
+
+
+def shell_spectrum_host(ubar: np.ndarray) -> np.ndarray:
+    """E(k) of a physical field on the coarse grid, shells 1..m/2 (the
+    reference's convention: 1/2 sum |u_k|^2 / m^6 over a sphere shell)."""
+    m = ubar.shape[1]
+    F = [np.fft.fftn(ubar[c]) for c in range(3)]
+    k = np.fft.fftfreq(m, 1.0 / m)
+    kz, ky, kx = np.meshgrid(k, k, k, indexing="ij")
+    sh = np.floor(np.sqrt(kx ** 2 + ky ** 2 + kz ** 2) + 0.5).astype(int)
+    e = 0.5 * sum(np.abs(f) ** 2 for f in F) / float(m) ** 6
+    return np.bincount(sh.ravel(), weights=e.ravel(), minlength=m)[1:m // 2 + 1]
+
+
+def features(snaps: list, m: int, delta: float) -> Dataset:
+    """Inputs and targets on the coarse grid: the filtered velocity
+    transformed and truncated to the 2/3 cube as the LES holds it, its nine
+    gradients (spectral derivatives, as the LES closure computes them) and
+    the filter width; targets the deviatoric part of tau."""
+    h = Hit3D(m)
+    Xs, Ts, eks = [], [], []
+    try:
+        for sn in snaps:
+            for c in range(3):
+                a = np.zeros((m, m, m + 2))
+                a[:, :, :m] = sn.ubar[c]
+                h.put(WRK, 1, a)
+                h.xFFT3d(1, 1)
+                h.copy(FIELDS, c + 1, WRK, 1)
+            h.dealias_all()
+            Xs.append(h.sgs_features(delta))
+            t = sn.tau.reshape(6, -1)
+            tr = (t[0] + t[3] + t[5]) / 3.0
+            dev = t.copy()
+            dev[0] -= tr
+            dev[3] -= tr
+            dev[5] -= tr
+            Ts.append(dev.T.astype(np.float32))
+            eks.append(shell_spectrum_host(sn.ubar))
+    finally:
+        h.close()
+    return Dataset(np.concatenate(Xs), np.concatenate(Ts), m, delta, eks)
+
+
+def standardise(ds: Dataset):
+    xm, xs = ds.X.mean(0).astype(np.float64), ds.X.std(0).astype(np.float64)
+    xs = np.where(xs > 1e-12 * np.maximum(1.0, np.abs(xm)), xs, 1.0)
+    ym, ys = ds.T.mean(0).astype(np.float64), ds.T.std(0).astype(np.float64)
+    ys = np.where(ys > 0, ys, 1.0)
+    return xm, xs, ym, ys
+
+
+def init_params(seed: int = 0) -> np.ndarray:
+    """He-style initial weights in the closure's parameter layout."""
+    rng = np.random.default_rng(seed)
+    W1 = np.zeros((16, 64))
+    W1[:10] = rng.standard_normal((10, 64)) * math.sqrt(2.0 / 10)
+    W2 = rng.standard_normal((64, 64)) * math.sqrt(2.0 / 64)
+    W3 = np.zeros((64, 16))
+    W3[:, :6] = rng.standard_normal((64, 6)) * math.sqrt(1.0 / 64)
+    return np.concatenate([W1.ravel(), np.zeros(64), W2.ravel(), np.zeros(64), W3.ravel(),
+                           np.zeros(16)]).astype(np.float32)
+
+
+@dataclass
+class Trained:
+    params: np.ndarray
+    norm: tuple
+    losses: np.ndarray
+    val_loss: float
+    smag_val_loss: float
+
+
+def smagorinsky_tau(X: np.ndarray, delta: float, cs: float) -> np.ndarray:
+    """Deviatoric Smagorinsky stress -2 (cs delta)^2 |S| S_ij from the raw
+    gradient inputs (the a-priori comparison of the fallback model)."""
+    g = X[:, :9].astype(np.float64).reshape(-1, 3, 3)   # g[p, i, j] = du_i/dx_j
+    S = 0.5 * (g + g.transpose(0, 2, 1))
+    smag = np.sqrt(2.0 * np.einsum("pij,pij->p", S, S))
+    nut = (cs * delta) ** 2 * smag
+    tr = (S[:, 0, 0] + S[:, 1, 1] + S[:, 2, 2]) / 3.0
+    out = np.stack([S[:, i, j] for (i, j) in TAU_PAIRS], axis=1)
+    out[:, 0] -= tr
+    out[:, 3] -= tr
+    out[:, 5] -= tr
+    return -2.0 * nut[:, None] * out
+
+
+def train(ds: Dataset, steps: int = 2000, batch: int = 65536, lr: float = 1e-3, seed: int = 0,
+          val_frac: float = 0.1) -> Trained:
+    """Adam on the GPU (the library's own forward/backward/Adam kernels) on a
+    shuffled training split; returns the parameters, the scaling and the
+    validation loss next to the Smagorinsky (Cs = 0.17) stress's."""
+    norm = standardise(ds)
+    xm, xs, ym, ys = norm
+    rng = np.random.default_rng(seed)
+    idx = rng.permutation(ds.X.shape[0])
+    nval = int(len(idx) * val_frac)
+    val, tr = idx[:nval], idx[nval:]
+    Xn = ((ds.X - xm) / xs).astype(np.float32)
+    Tn = ((ds.T - ym) / ys).astype(np.float32)
+    h = Hit3D(16)   # a small grid: the closure's kernels do not use it
+    try:
+        h.mlp_set_params(init_params(seed))
+        h.mlp_set_norm(xm, xs, ym, ys)
+        h.mlp_set_data(Xn[tr], Tn[tr])
+        losses = h.mlp_train(steps, min(batch, (len(tr) // 32) * 32), lr)
+        p = h.mlp_get_params()
+        Yv = h.mlp_forward(Xn[val])
+    finally:
+        h.close()
+    val_loss = float(np.mean((Yv - Tn[val]) ** 2))
+    sm = (smagorinsky_tau(ds.X[val], ds.delta, 0.17) - ym) / ys
+    smag_val = float(np.mean((sm - Tn[val]) ** 2))
+    return Trained(p, norm, losses, val_loss, smag_val)
+
+
+def les_run(ubar0: np.ndarray, model: str, steps: int, trained: Trained | None = None,
+            cs: float = 0.17, re: float = 800.0, sample_every: int = 50, dt0: float = 2e-3):
+    """Forced coarse LES from the filtered field ubar0 (physical, [3, m, m, m])
+    with the learned closure (model 'mlp') or Smagorinsky ('smag').  Returns
+    the time-averaged E(k) (shells 1..kmax), the energy history and the count
+    of fallback steps."""
+    m = ubar0.shape[1]
+    p = dns_params(m, re)
+    p.dt = dt0
+    h = Hit3D(m)
+    try:
+        h.set_physics(p.nu, flow_type=1, kfmax=p.kfmax, famp=p.famp)
+        for c in range(3):
+            a = np.zeros((m, m, m + 2))
+            a[:, :, :m] = ubar0[c]
+            h.put(WRK, 1, a)
+            h.xFFT3d(1, 1)
+            h.copy(FIELDS, c + 1, WRK, 1)
+        h.dealias_all()
+        h.pressure()
+        if model == "mlp":
+            h.mlp_set_params(trained.params)
+            h.mlp_set_norm(*trained.norm)
+            h.set_les(11, cs)
+        elif model == "smag":
+            h.set_les(1, cs)
+        else:
+            raise ValueError("model must be 'mlp' or 'smag'")
+        dt, fov, fos = p.dt, True, True
+        eks, energy = [], []
+        for i in range(steps):
+            courant, en, fov, fos = h.step(dt, fov, fos)
+            dt = my_dt(dt, courant)
+            if not np.isfinite(courant) or courant > 10.0:
+                raise FloatingPointError(f"LES ({model}) diverged at step {i}")
+            if (i + 1) % sample_every == 0:
+                e, _ = h.stats()
+                eks.append(e)
+                energy.append(float(e.sum()))
+        fb = h.mlp_fallbacks() if model == "mlp" else 0
+    finally:
+        h.close()
+    half = len(eks) // 2
+    return np.mean(eks[half:], axis=0), np.array(energy), fb
+
+
+def spectrum_error(ek_les: np.ndarray, ek_ref: np.ndarray, kmin: int = 1) -> float:
+    """Mean relative E(k) error over shells kmin..kmax of the LES."""
+    k = min(len(ek_les), len(ek_ref))
+    a, b = ek_les[kmin - 1:k], ek_ref[kmin - 1:k]
+    return float(np.mean(np.abs(a - b) / b))

@@ -112,12 +112,56 @@ int hit3d_step(const double *dt, int64_t *fov, int64_t *fos, const int64_t *int_
                const int64_t *precision, double *courant, double *energy);
 
 /* LES model selection (m_les.f90:69-104, dispatch :636-690, :700-895):
- * les_model 0 = none, 1 = Smagorinsky, nu_t = (c_smag dx)^2 sqrt(2 S_ij S_ij)
+ * les_model 0 = none, 1 = Smagorinsky, 11 = the learned closure below (not a
+ * reference model number), nu_t = (c_smag dx)^2 sqrt(2 S_ij S_ij)
  * (les_get_turb_visc_smag, m_les.f90:1142-1215) applied to velocities
  * (les_rhsv_turb_visc :902-1018) and scalars (m_les_rhss_turb_visc1
  * :1041-1091).  The reference hard-codes c_smag = 0.18 (:1148).  Needs the
  * 2/3 rule, as the reference forces (m_parameters.f90:415-420). */
 int hit3d_set_les(const int64_t *les_model, const double *c_smag);
+
+/* Learned subgrid-stress closure (no counterpart in the reference; it stands
+ * in for the eddy viscosity of m_les.f90:1142-1215 in coarse LES, with
+ * Smagorinsky as its fallback).  A pointwise MLP 10 -> 64 -> 64 -> 6 with GELU
+ * maps the standardised inputs (du_i/dx_j in i-major order, then the filter
+ * width) to the standardised deviatoric stress (tau_11, tau_12, tau_13,
+ * tau_22, tau_23, tau_33); products run on bfloat16 matrix cores with FP32
+ * accumulation.  Parameters are one float array of 6288 entries: W1[16][64]
+ * (input rows 10..15 zero), b1[64], W2[64][64], b2[64], W3[64][16] (output
+ * columns 6..15 zero), b3[16]; W[k][n] multiplies input k into output n.
+ *
+ * set_params also resets the Adam state; set_norm gives the input/output
+ * means and scales (10, 10, 6, 6 doubles; scales > 0).  set_data uploads a
+ * standardised training set X[count][10], T[count][6] (floats).  train runs
+ * `steps` Adam steps (lr, betas 0.9/0.999, eps 1e-8) on consecutive batches
+ * of `batch` samples (a multiple of 32; indices wrap), the loss being the
+ * mean squared error over batch x 6 outputs; losses[i] receives the loss of
+ * step i before its update.  grads returns the gradient of that loss over
+ * the count samples given (a multiple of 32) without updating; forward the
+ * network's standardised outputs.  hit3d_set_les(11, ...) selects the
+ * closure for the LES right-hand side (velocity only); a step whose network
+ * output is not finite everywhere uses the Smagorinsky stress instead and is
+ * counted by hit3d_mlp_fallbacks. */
+int hit3d_mlp_set_params(const float *params);
+int hit3d_mlp_get_params(float *params);
+int hit3d_mlp_set_norm(const double *xmean, const double *xstd, const double *ymean,
+                       const double *ystd);
+int hit3d_mlp_set_data(const int64_t *count, const float *X, const float *T);
+int hit3d_mlp_train(const int64_t *steps, const int64_t *batch, const double *lr, double *losses);
+int hit3d_mlp_grads(const int64_t *count, const float *X, const float *T, float *grads,
+                    double *loss);
+int hit3d_mlp_forward(const int64_t *count, const float *X, float *Y);
+int hit3d_mlp_fallbacks(int64_t *count);
+
+/* Training data for the closure.  box_filter: top-hat average over ratio^3
+ * blocks of the physical velocity in wrk(1:3) (as hit3d_fields_to_parts
+ * leaves it) onto the grid m = nx/ratio: ubar[3][m^3] and the subgrid stress
+ * tau[6][m^3] = avg(u_i u_j) - ubar_i ubar_j (11,12,13,22,23,33), coarse
+ * index (z*m + y)*m + x; grids up to 512.  sgs_features: the closure's raw
+ * inputs at every point of this grid from fields(1:3): X[p][10] =
+ * (du_i/dx_j, delta), p = (z*nx + y)*nx + x. */
+int hit3d_box_filter(const int64_t *ratio, double *ubar, double *tau);
+int hit3d_sgs_features(const double *delta, float *X);
 
 /* Selects how hit3d_step runs: *on != 0 (default) the fused pipeline where
  * it applies (2/3 rule without active scalars), 0 the unfused sequence of

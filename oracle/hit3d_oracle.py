@@ -419,6 +419,29 @@ def les_rhs_velocity(fields: np.ndarray, rhs: np.ndarray, tv: np.ndarray, n: int
     return np.stack([to_interleaved(x) for x in r])
 
 
+def velocity_gradients(fields: np.ndarray, n: int):
+    """du_i/dx_j in physical space, i-major (11, 12, 13, 21, ..., 33):
+    x_derivative then the inverse transform."""
+    return [fft_inverse(x_derivative(fields[c], ax))[:, :, :n] for c in range(3) for ax in ("x", "y", "z")]
+
+
+def stress_divergence(rhs: np.ndarray, tau6, n: int) -> np.ndarray:
+    """rhs_i += -i k_j tau_ij (tau physical: 11, 12, 13, 22, 23, 33) on
+    ialias == 0 entries, others zeroed -- the stress divergence and mask of
+    les_rhsv_turb_visc / les_rhs_velocity (m_les.f90:937-1015, :682-690)
+    for an arbitrary stress."""
+    kx, ky, kz = _kgrid(n)
+    ia = ialias(n)[:, :, 0::2]
+    tau = [to_complex(fft_forward(_pad(t, n))) for t in tau6]
+    r = [to_complex(rhs[c]) for c in range(3)]
+    keep = ia == 0
+    r[0] = np.where(keep, r[0] - 1j * ((kx * tau[0] + ky * tau[1]) + kz * tau[2]), r[0])
+    r[1] = np.where(keep, r[1] - 1j * ((kx * tau[1] + ky * tau[3]) + kz * tau[4]), r[1])
+    r[2] = np.where(keep, r[2] - 1j * ((kx * tau[2] + ky * tau[4]) + kz * tau[5]), r[2])
+    r = [np.where(keep, x, 0.0) for x in r]
+    return np.stack([to_interleaved(x) for x in r])
+
+
 def les_rhs_scalar(field_s: np.ndarray, rhs_s: np.ndarray, tv: np.ndarray, n: int) -> np.ndarray:
     """m_les_rhss_turb_visc1 (m_les.f90:1041-1091): rhs += sum_dir
     d_dir(nu_t d_dir phi) on entries with ialias == 0."""

@@ -1,0 +1,195 @@
+"""GPU checks of the learned closure against its oracle (oracle/sgs_mlp.py):
+forward and gradients with the same bfloat16 rounding points (tolerances set
+by FP32 accumulation: 2e-3 relative in norm), Adam, deterministic training,
+the box filter and the gradient features (1e-12 / 1e-13), and the closure's
+stress divergence in the LES right-hand side."""
+import sys
+
+import numpy as np
+import pytest
+
+import hit3d_amd
+from hit3d_amd import FIELDS, WRK
+from oracle import sgs_mlp as om
+
+pytestmark = pytest.mark.gpu
+
+
+def note(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def flat(p):
+    return np.concatenate([p["W1"].ravel(), p["b1"], p["W2"].ravel(), p["b2"], p["W3"].ravel(), p["b3"]])
+
+
+def unflat(v):
+    v = np.asarray(v, np.float64)
+    out, o = {}, 0
+    for k, shape in (("W1", (16, 64)), ("b1", (64,)), ("W2", (64, 64)), ("b2", (64,)), ("W3", (64, 16)),
+                     ("b3", (16,))):
+        size = int(np.prod(shape))
+        out[k] = v[o:o + size].reshape(shape)
+        o += size
+    return out
+
+
+def params(seed):
+    rng = np.random.default_rng(seed)
+    p = om.init_params(seed)
+    p["b1"] = rng.standard_normal(64) * 0.1
+    p["b2"] = rng.standard_normal(64) * 0.1
+    p["b3"][:6] = rng.standard_normal(6) * 0.1
+    # the device keeps float32 masters: compare against those values
+    return unflat(flat(p).astype(np.float32))
+
+
+def relnorm(a, b):
+    return np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-300)
+
+
+@pytest.fixture
+def h16():
+    h = hit3d_amd.Hit3D(16)
+    yield h
+    h.close()
+
+
+def test_forward_matches_oracle(h16):
+    note("mlp forward")
+    p = params(1)
+    h16.mlp_set_params(flat(p))
+    rng = np.random.default_rng(2)
+    X = rng.standard_normal((200, 10)).astype(np.float32)   # not a multiple of the tile
+    Y = h16.mlp_forward(X)
+    Yo, _ = om.forward(p, X.astype(np.float64), exact=False)
+    assert relnorm(Y, Yo) < 2e-3
+    assert np.abs(Y - Yo).max() < 1e-2 * np.abs(Yo).max()
+
+
+def test_gradients_match_oracle(h16):
+    note("mlp gradients")
+    p = params(3)
+    h16.mlp_set_params(flat(p))
+    rng = np.random.default_rng(4)
+    X = rng.standard_normal((256, 10)).astype(np.float32)
+    T = rng.standard_normal((256, 6)).astype(np.float32)
+    g, loss = h16.mlp_grads(X, T)
+    lo, go = om.loss_and_grads(p, X.astype(np.float64), T.astype(np.float64), exact=False)
+    assert abs(loss - lo) < 1e-4 * lo
+    gg = unflat(g)
+    for k in ("W1", "b1", "W2", "b2", "W3", "b3"):
+        assert relnorm(gg[k], go[k]) < 2e-3, k
+    # padded inputs/outputs get exactly zero gradient
+    assert not gg["W1"][10:].any() and not gg["W3"][:, 6:].any() and not gg["b3"][6:].any()
+
+
+def test_adam_and_deterministic_training(h16):
+    """One step equals the oracle's Adam on the device gradients; a longer run
+    lowers the loss of a smooth target and repeats bit for bit."""
+    note("mlp training")
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal((4096, 10)).astype(np.float32)
+    A = rng.standard_normal((10, 6)) * 0.5
+    T = np.tanh(X.astype(np.float64) @ A).astype(np.float32)
+    p0 = params(6)
+    h16.mlp_set_params(flat(p0))
+    h16.mlp_set_data(X, T)
+    g, _ = h16.mlp_grads(X[:1024], T[:1024])
+    h16.mlp_train(1, 1024, 1e-3)
+    p1 = unflat(h16.mlp_get_params())
+    want = {k: v.copy() for k, v in p0.items()}
+    om.adam(want, unflat(g), {k: np.zeros_like(v) for k, v in p0.items()},
+            {k: np.zeros_like(v) for k, v in p0.items()}, 1, lr=1e-3)
+    for k in want:
+        assert np.abs(p1[k] - want[k]).max() < 2e-6, k
+    runs = []
+    for _ in range(2):
+        h16.mlp_set_params(flat(p0))
+        losses = h16.mlp_train(300, 1024, 3e-3)
+        runs.append((losses, h16.mlp_get_params()))
+    assert np.array_equal(runs[0][1], runs[1][1]) and np.array_equal(runs[0][0], runs[1][0])
+    assert runs[0][0][-20:].mean() < 0.2 * runs[0][0][:5].mean()
+
+
+def test_box_filter_and_features(oracle):
+    note("box filter / features")
+    n, r = 64, 4
+    rng = np.random.default_rng(7)
+    phys = np.zeros((3, n, n, n + 2))
+    phys[:, :, :, :n] = rng.standard_normal((3, n, n, n))
+    h = hit3d_amd.Hit3D(n)
+    try:
+        for c in range(3):
+            h.put(WRK, c + 1, phys[c])
+        ubar, tau = h.box_filter(r)
+        ub = [om.box_filter(phys[c], r) for c in range(3)]
+        uu = {(i, j): om.box_filter(phys[i] * phys[j], r) for i in range(3) for j in range(i, 3)}
+        want_tau = [uu[(i, j)] - ub[i] * ub[j] for (i, j) in ((0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2))]
+        for c in range(3):
+            assert np.abs(ubar[c] - ub[c]).max() < 1e-13
+        for c in range(6):
+            assert np.abs(tau[c] - want_tau[c]).max() < 1e-12
+        spec = np.stack([oracle.fft_forward(p) for p in phys])
+        for c in range(3):
+            h.put(FIELDS, c + 1, spec[c])
+        X = h.sgs_features(0.25)
+        grads = oracle.velocity_gradients(spec, n)
+        for c in range(9):
+            want = grads[c].ravel().astype(np.float32)
+            assert np.abs(X[:, c] - want).max() <= 1e-5 * np.abs(want).max()
+        assert (X[:, 9] == np.float32(0.25)).all()
+    finally:
+        h.close()
+
+
+def test_les_rhs_with_closure_matches_oracle(oracle):
+    """rhs_velocity with les_model 11: the stress from the network at every
+    grid point, then -i k_j tau_ij on unaliased modes (oracle: the same
+    network evaluated in numpy with the kernels' rounding points)."""
+    note("les closure rhs")
+    n = 32
+    rng = np.random.default_rng(8)
+    p = params(9)
+    xmean, xstd = rng.standard_normal(10) * 0.1, 1.0 + rng.random(10)
+    ymean, ystd = rng.standard_normal(6) * 0.01, 0.05 + 0.1 * rng.random(6)
+    h = hit3d_amd.Hit3D(n)
+    try:
+        h.set_physics(0.005)
+        phys = np.zeros((3, n, n, n + 2))
+        phys[:, :, :, :n] = rng.standard_normal((3, n, n, n))
+        spec = np.stack([oracle.fft_forward(q) for q in phys])
+        spec = oracle.dealias_all(oracle.pressure(spec, n), n // 3, n)
+        for c in range(3):
+            h.put(FIELDS, c + 1, spec[c])
+        h.mlp_set_params(flat(p))
+        h.mlp_set_norm(xmean, xstd, ymean, ystd)
+        h.set_les(11)
+        h.rhs_scalars(False)
+        h.rhs_velocity(1e-3)
+        got = np.stack([h.get(WRK, c) for c in (1, 2, 3)])
+        assert h.mlp_fallbacks() == 0
+        # oracle: base RHS, then the closure's stress divergence
+        base = oracle.rhs_velocity(spec, 0.005, 0, n)[0]
+        grads = oracle.velocity_gradients(spec, n)
+        delta = 2 * np.pi / n
+        raw = np.stack([g.ravel() for g in grads] + [np.full(n ** 3, delta)], axis=1)
+        X = (raw.astype(np.float32) - xmean.astype(np.float32)) / xstd.astype(np.float32)
+        Y, _ = om.forward(p, X.astype(np.float64), exact=False)
+        tau = [(Y[:, c] * ystd[c] + ymean[c]).reshape(n, n, n) for c in range(6)]
+        want = oracle.stress_divergence(base, tau, n)
+        assert relnorm(got - base, want - base) < 5e-3
+        # non-finite network output: the step falls back to Smagorinsky
+        bad = flat(p)
+        bad[0] = np.nan
+        h.mlp_set_params(bad)
+        h.rhs_scalars(False)
+        h.rhs_velocity(1e-3)
+        assert h.mlp_fallbacks() == 1
+        got_sm = np.stack([h.get(WRK, c) for c in (1, 2, 3)])
+        tv = oracle.turb_visc_smag(spec, n)
+        want_sm = oracle.les_rhs_velocity(spec, base, tv, n)
+        assert relnorm(got_sm, want_sm) < 1e-12
+    finally:
+        h.close()
