@@ -44,6 +44,68 @@ def slot_bytes(n, word=8):
     return (n + 2) * n * n * word
 
 
+# the other configurations of the measurement plan, run with --workload
+WORKLOADS = {
+    "c1": dict(desc="32^3 decaying, RE=100, fixed DT=0.01, 2/3 rule", n=32, n_fft=9, ns=0,
+               kw=dict(flow_type=0, re=100.0, dt=0.01, variable_dt=False, dealias=0)),
+    "c2": dict(desc="256^3 forced, RE=400, phase shift, KFMAX=2, FAMP=0.5, 2 scalars "
+                    "(Sc 0.7, 1.0, mean gradient)", n=256, n_fft=37, ns=2,
+               kw=dict(flow_type=1, re=400.0, dt=0.001, variable_dt=True, kfmax=2, famp=0.5,
+                       dealias=1, n_scalars=2, scalar_type=[0, 0], sc=[0.7, 1.0],
+                       ir_exp_sc=[0, 0], peak_wavenum_sc=[1.0, 1.0], reac_sc=[0.0, 0.0],
+                       tscalar=-1.0)),
+    "c5": dict(desc="1024^3 forced, RE=1600, phase shift, 3 scalars (Sc 0.5, 1, 2, mean "
+                    "gradient), 1,048,576 particles (CINT)", n=1024, n_fft=45, ns=3,
+               kw=dict(flow_type=1, re=1600.0, dt=5e-4, variable_dt=True, kfmax=2, famp=0.5,
+                       dealias=1, n_scalars=3, scalar_type=[0, 0, 0], sc=[0.5, 1.0, 2.0],
+                       ir_exp_sc=[0, 0, 0], peak_wavenum_sc=[1.0] * 3, reac_sc=[0.0] * 3,
+                       tscalar=-1.0, nptot=1048576, particles_tracking_scheme=1)),
+}
+
+
+def run_workload(name, steps, warmup):
+    """One of the other configurations: FP64 driver steps (unfused where the
+    phase shift or scalars apply), with particles moved every step when the
+    configuration has them.  Prints its own JSON line."""
+    from hit3d_amd.driver import Params, Run
+    w = WORKLOADS[name]
+    n = w["n"]
+    kw = dict(w["kw"])
+    nptot = kw.pop("nptot", 0)
+    scheme = kw.pop("particles_tracking_scheme", 0)
+    p = Params(nx=n, ny=n, nz_all=n, isp_type=2, ir_exp=4, peak_wavenum=3.0, **kw)
+    r = Run(p)
+    t0 = time.time()
+    r.begin_new()
+    if nptot:
+        rng = np.random.default_rng(12345)
+        r.ipart = np.arange(1, nptot + 1, dtype=np.int32)
+        r.h.particles_set(rng.uniform(1.0, n + 1.0, (nptot, 3)))
+        p.nptot, p.particles_tracking_scheme = nptot, scheme
+    r.h.synchronize()
+    log(f"[bench] {name} init {time.time() - t0:.1f}s")
+    for _ in range(warmup):
+        r.step()
+    r.h.synchronize()
+    r.h.timing_enable(True)
+    t0 = time.perf_counter()
+    for i in range(steps):
+        r.step()
+        log(f"[bench] {name} step {i + 1}/{steps}")
+    r.h.synchronize()
+    t1 = time.perf_counter()
+    timers = r.h.timing_read()
+    r.h.close()
+    t = (t1 - t0) / steps
+    s = slot_bytes(n)
+    b_alg = 6 * w["n_fft"] * s + 4 * (3 + w["ns"]) * s
+    out = {"workload": name, "config": w["desc"], "s_per_step": t, "ns_per_point_step": t / n ** 3 * 1e9,
+           "step_alg_bytes": b_alg, "step_hbm_roofline_frac": b_alg / t / (PEAK_HBM_GBS * 1e9),
+           "particles": nptot, "steps": steps, "warmup": warmup,
+           "kernel_ms_per_step": {k: v[0] / steps for k, v in timers.items() if v[1]}}
+    print(json.dumps(out), flush=True)
+
+
 def run_form(n, precision, steps, warmup):
     from hit3d_amd.driver import Run
     p = params(n)
@@ -161,6 +223,9 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="c3", choices=["c3"] + sorted(WORKLOADS),
+                    help="c3 (default) prints the benchmark line; c1, c2, c5 time the other "
+                         "configurations")
     ap.add_argument("--calib", action="store_true",
                     help="after the measurement, copy one slot 3 times (a plain copy of known "
                          "size, the reference for PMC byte counters)")
@@ -169,6 +234,9 @@ def main():
         print(f"bench.py: this build runs the whole box on one GPU; --gpus {a.gpus} is not "
               "supported (multi-GPU sharding is design only)", file=sys.stderr)
         sys.exit(2)
+    if a.workload != "c3":
+        run_workload(a.workload, a.steps, a.warmup)
+        return
     n = a.n
     t64, tim64, st64, _ = run_form(n, 0, a.steps, a.warmup)
     mixed = None

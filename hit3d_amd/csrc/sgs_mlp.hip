@@ -295,11 +295,17 @@ struct Ptr6 {
     double* p[6];
 };
 
+// Inference at every grid point.  The stress is then cleared of local
+// backscatter: with S the strain rate of the resolved velocity, a point where
+// the subgrid dissipation -tau:S is negative gets tau + ((-tau:S)/(S:S)) S,
+// whose dissipation is zero (the part of tau orthogonal to S is kept).
 __global__ __launch_bounds__(NT) void k_les(Geo g, Ptr9 gr, float delta, MlpNorm nrm,
                                             const uint16_t* __restrict__ wb, const float* __restrict__ p,
                                             Ptr6 tau, unsigned int* bad) {
     __shared__ Lds s;
     __shared__ int64_t offs[TS];
+    __shared__ float graw[TS][9];
+    __shared__ float tl[TS][NOUT];
     const int64_t total = (int64_t)g.n * g.n * g.n;
     const int64_t first = (int64_t)blockIdx.x * TS;
     if (threadIdx.x < TS) {
@@ -313,6 +319,7 @@ __global__ __launch_bounds__(NT) void k_les(Geo g, Ptr9 gr, float delta, MlpNorm
         float x = 0.f;
         if (o >= 0 && c < NIN) {
             const float raw = c < 9 ? (float)gr.p[c][o] : delta;
+            if (c < 9) graw[row][c] = raw;
             x = (raw - nrm.xmean[c]) / nrm.xstd[c];
         }
         s.xb[i] = c < NIN ? to_bf16(x) : (uint16_t)0;
@@ -320,13 +327,35 @@ __global__ __launch_bounds__(NT) void k_les(Geo g, Ptr9 gr, float delta, MlpNorm
     stage_weights(s, wb, p);
     __syncthreads();
     hidden_layers(s);
-    output_layer(s, [&](int i, int n, float y) {
+    output_layer(s, [&](int i, int n, float y) { tl[i][n] = y * nrm.ystd[n] + nrm.ymean[n]; });
+    __syncthreads();
+    if (threadIdx.x < TS) {
+        const int i = threadIdx.x;
         const int64_t o = offs[i];
-        if (o < 0) return;
-        const float t = y * nrm.ystd[n] + nrm.ymean[n];
-        if (!isfinite(t)) atomicAdd(bad, 1u);
-        tau.p[n][o] = (double)t;
-    });
+        if (o >= 0) {
+            const float* gg = graw[i];  // du_i/dx_j at gg[3 i + j]
+            const float S[6] = {gg[0], 0.5f * (gg[1] + gg[3]), 0.5f * (gg[2] + gg[6]), gg[4],
+                                0.5f * (gg[5] + gg[7]), gg[8]};
+            const float w[6] = {1.f, 2.f, 2.f, 1.f, 2.f, 1.f};  // symmetric double sum
+            float t[6], ts = 0.f, ss = 0.f;
+            bool fin = true;
+#pragma unroll
+            for (int c = 0; c < NOUT; ++c) {
+                t[c] = tl[i][c];
+                fin = fin && isfinite(t[c]);
+                ts += w[c] * t[c] * S[c];
+                ss += w[c] * S[c] * S[c];
+            }
+            if (!fin) atomicAdd(bad, 1u);
+            if (ts > 0.f && ss > 0.f) {  // -tau:S < 0: backscatter
+                const float a = -ts / ss;
+#pragma unroll
+                for (int c = 0; c < NOUT; ++c) t[c] += a * S[c];
+            }
+#pragma unroll
+            for (int c = 0; c < NOUT; ++c) tau.p[c][o] = (double)t[c];
+        }
+    }
 }
 
 __global__ void k_features(Geo g, Ptr9 gr, float delta, float* X) {

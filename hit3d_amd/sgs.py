@@ -89,31 +89,54 @@ def shell_spectrum_host(ubar: np.ndarray) -> np.ndarray:
     return np.bincount(sh.ravel(), weights=e.ravel(), minlength=m)[1:m // 2 + 1]
 
 
+def _truncated(h: Hit3D, phys3: list, m: int) -> list:
+    """The 2/3-rule truncation (dealias_all) of up to three physical coarse
+    fields, on the GPU: transform, truncate, transform back."""
+    for c, f in enumerate(phys3):
+        a = np.zeros((m, m, m + 2))
+        a[:, :, :m] = f
+        h.put(WRK, 1, a)
+        h.xFFT3d(1, 1)
+        h.copy(FIELDS, c + 1, WRK, 1)
+    h.dealias_all()
+    out = []
+    for c in range(len(phys3)):
+        h.copy(WRK, 1, FIELDS, c + 1)
+        h.xFFT3d(-1, 1)
+        out.append(h.get(WRK, 1)[:, :, :m])
+    return out
+
+
 def features(snaps: list, m: int, delta: float) -> Dataset:
-    """Inputs and targets on the coarse grid: the filtered velocity
-    transformed and truncated to the 2/3 cube as the LES holds it, its nine
-    gradients (spectral derivatives, as the LES closure computes them) and
-    the filter width; targets the deviatoric part of tau."""
+    """Inputs and targets on the coarse grid for the filter the LES actually
+    applies: the box filter B followed by the 2/3-rule truncation T of the
+    coarse grid.  Inputs: the nine gradients of T(B(u)) (spectral
+    derivatives, as the LES closure computes them) and the filter width;
+    targets: the deviatoric part of T(B(u_i u_j)) - T(B(u_i)) T(B(u_j))."""
     h = Hit3D(m)
     Xs, Ts, eks = [], [], []
     try:
         for sn in snaps:
+            ut = _truncated(h, [sn.ubar[0], sn.ubar[1], sn.ubar[2]], m)
+            bprod = [sn.tau[k] + sn.ubar[i] * sn.ubar[j] for k, (i, j) in enumerate(TAU_PAIRS)]
+            tprod = _truncated(h, bprod[:3], m) + _truncated(h, bprod[3:], m)
+            t = np.stack([tprod[k] - ut[i] * ut[j] for k, (i, j) in enumerate(TAU_PAIRS)]).reshape(6, -1)
+            # the LES state: T(B(u)) as spectral fields
             for c in range(3):
                 a = np.zeros((m, m, m + 2))
-                a[:, :, :m] = sn.ubar[c]
+                a[:, :, :m] = ut[c]
                 h.put(WRK, 1, a)
                 h.xFFT3d(1, 1)
                 h.copy(FIELDS, c + 1, WRK, 1)
             h.dealias_all()
             Xs.append(h.sgs_features(delta))
-            t = sn.tau.reshape(6, -1)
             tr = (t[0] + t[3] + t[5]) / 3.0
             dev = t.copy()
             dev[0] -= tr
             dev[3] -= tr
             dev[5] -= tr
             Ts.append(dev.T.astype(np.float32))
-            eks.append(shell_spectrum_host(sn.ubar))
+            eks.append(shell_spectrum_host(np.stack(ut)))
     finally:
         h.close()
     return Dataset(np.concatenate(Xs), np.concatenate(Ts), m, delta, eks)
@@ -243,3 +266,68 @@ def spectrum_error(ek_les: np.ndarray, ek_ref: np.ndarray, kmin: int = 1) -> flo
     k = min(len(ek_les), len(ek_ref))
     a, b = ek_les[kmin - 1:k], ek_ref[kmin - 1:k]
     return float(np.mean(np.abs(a - b) / b))
+
+
+def main(argv=None):
+    """Train the closure on box-filtered DNS and compare the coarse LES with
+    it and with Smagorinsky against the filtered-DNS spectrum; prints one
+    JSON line (and saves the parameters with --out)."""
+    import argparse
+    import json
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--dns", type=int, default=512)
+    ap.add_argument("--ratio", type=int, default=8)
+    ap.add_argument("--re", type=float, default=800.0)
+    ap.add_argument("--spinup", type=int, default=6000)
+    ap.add_argument("--snaps", type=int, default=8)
+    ap.add_argument("--every", type=int, default=500)
+    ap.add_argument("--train-steps", type=int, default=2000)
+    ap.add_argument("--batch", type=int, default=65536)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--les-steps", type=int, default=5000)
+    ap.add_argument("--cs", type=float, default=0.17)
+    ap.add_argument("--out", default="")
+    a = ap.parse_args(argv)
+    m = a.dns // a.ratio
+    delta = 2.0 * math.pi / m
+    t0 = time.time()
+    snaps = run_dns(a.dns, a.ratio, a.spinup, a.snaps, a.every, a.re)
+    t_dns = time.time() - t0
+    ds = features(snaps, m, delta)
+    ek_ref = np.mean(ds.ek_filtered, axis=0)
+    log(f"[sgs] dataset {ds.X.shape[0]} samples on {m}^3 ({t_dns:.0f}s of DNS)")
+    t0 = time.time()
+    tr = train(ds, a.train_steps, a.batch, a.lr)
+    t_train = time.time() - t0
+    log(f"[sgs] trained: loss {tr.losses[0]:.4f} -> {tr.losses[-20:].mean():.4f}, "
+        f"validation {tr.val_loss:.4f} vs Smagorinsky a priori {tr.smag_val_loss:.4f} ({t_train:.1f}s)")
+    res = {"dns": a.dns, "ratio": a.ratio, "les_grid": m, "samples": int(ds.X.shape[0]),
+           "train_steps": a.train_steps, "batch": a.batch, "lr": a.lr,
+           "train_loss_first": float(tr.losses[0]), "train_loss_last": float(tr.losses[-20:].mean()),
+           "val_loss_mlp": tr.val_loss, "val_loss_smagorinsky": tr.smag_val_loss,
+           "train_seconds": t_train, "dns_seconds": t_dns}
+    u0 = snaps[-1].ubar
+    kmax = m // 3
+    for model in ("smag", "mlp"):
+        t0 = time.time()
+        try:
+            ek, energy, fb = les_run(u0, model, a.les_steps, tr, a.cs, a.re)
+            res[f"ek_err_{model}"] = spectrum_error(ek[:kmax], ek_ref[:kmax])
+            res[f"energy_{model}"] = float(energy[len(energy) // 2:].mean())
+            res[f"fallback_steps_{model}"] = int(fb)
+            res[f"ek_{model}"] = [float(v) for v in ek[:kmax]]
+        except FloatingPointError as e:
+            res[f"ek_err_{model}"] = None
+            res[f"error_{model}"] = str(e)
+        res[f"les_seconds_{model}"] = time.time() - t0
+        log(f"[sgs] LES {model}: E(k) error {res.get(f'ek_err_{model}')} ({res[f'les_seconds_{model}']:.1f}s)")
+    res["ek_filtered_dns"] = [float(v) for v in ek_ref[:kmax]]
+    res["energy_filtered_dns"] = float(np.mean([e.sum() for e in ds.ek_filtered]))
+    if a.out:
+        np.savez(a.out, params=tr.params, xmean=tr.norm[0], xstd=tr.norm[1], ymean=tr.norm[2],
+                 ystd=tr.norm[3], losses=tr.losses)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

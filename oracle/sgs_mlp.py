@@ -114,3 +114,19 @@ def box_filter(u: np.ndarray, ratio: int):
     m = n // ratio
     a = np.asarray(u)[:, :, :n].reshape(m, ratio, m, ratio, m, ratio)
     return a.mean(axis=(1, 3, 5))
+
+
+def clip_backscatter(tau: np.ndarray, grads: np.ndarray) -> np.ndarray:
+    """Where the subgrid dissipation -tau:S is negative (S the strain rate
+    from grads[p, 3i+j] = du_i/dx_j), add (-tau:S)/(S:S) S: the dissipation
+    becomes zero and the part of tau orthogonal to S is kept.  tau[p, 6] in
+    the order 11, 12, 13, 22, 23, 33."""
+    g = grads.reshape(-1, 3, 3)
+    S = np.stack([g[:, 0, 0], 0.5 * (g[:, 0, 1] + g[:, 1, 0]), 0.5 * (g[:, 0, 2] + g[:, 2, 0]),
+                  g[:, 1, 1], 0.5 * (g[:, 1, 2] + g[:, 2, 1]), g[:, 2, 2]], axis=1)
+    w = np.array([1.0, 2.0, 2.0, 1.0, 2.0, 1.0])
+    ts = (w * tau * S).sum(1)
+    ss = (w * S * S).sum(1)
+    clip = (ts > 0) & (ss > 0)
+    a = np.where(clip, -ts / np.where(ss > 0, ss, 1.0), 0.0)
+    return tau + a[:, None] * S

@@ -177,7 +177,9 @@ def test_les_rhs_with_closure_matches_oracle(oracle):
         raw = np.stack([g.ravel() for g in grads] + [np.full(n ** 3, delta)], axis=1)
         X = (raw.astype(np.float32) - xmean.astype(np.float32)) / xstd.astype(np.float32)
         Y, _ = om.forward(p, X.astype(np.float64), exact=False)
-        tau = [(Y[:, c] * ystd[c] + ymean[c]).reshape(n, n, n) for c in range(6)]
+        t6 = np.stack([Y[:, c] * ystd[c] + ymean[c] for c in range(6)], axis=1)
+        t6 = om.clip_backscatter(t6, raw[:, :9])
+        tau = [t6[:, c].reshape(n, n, n) for c in range(6)]
         want = oracle.stress_divergence(base, tau, n)
         assert relnorm(got - base, want - base) < 5e-3
         # non-finite network output: the step falls back to Smagorinsky
@@ -193,3 +195,25 @@ def test_les_rhs_with_closure_matches_oracle(oracle):
         assert relnorm(got_sm, want_sm) < 1e-12
     finally:
         h.close()
+
+
+def test_closure_gain_over_smagorinsky():
+    """What the trained network gains over the method without it: on a small
+    forced case (128^3 DNS, filter ratio 4, 32^3 LES) the closure trained by
+    the library's own kernels has a lower validation error than the
+    Smagorinsky stress (Cs = 0.17) a priori, and its LES spectrum is closer to
+    the filtered DNS than the Smagorinsky LES's a posteriori.  Every kernel is
+    deterministic, so the comparison repeats exactly."""
+    from hit3d_amd import sgs
+    note("closure gain (128^3 -> 32^3)")
+    snaps = sgs.run_dns(128, 4, 6000, 8, 400)
+    ds = sgs.features(snaps, 32, 2 * np.pi / 32)
+    tr = sgs.train(ds, steps=1000, batch=8192, lr=1e-3)
+    assert tr.val_loss < 0.85 * tr.smag_val_loss
+    ref = np.mean(ds.ek_filtered, axis=0)[:10]
+    ek_s, _, _ = sgs.les_run(snaps[-1].ubar, "smag", 4000, tr)
+    ek_m, _, fb = sgs.les_run(snaps[-1].ubar, "mlp", 4000, tr)
+    err_s, err_m = sgs.spectrum_error(ek_s[:10], ref), sgs.spectrum_error(ek_m[:10], ref)
+    note(f"E(k) error: closure {err_m:.3f}, Smagorinsky {err_s:.3f}, fallback steps {fb}")
+    assert fb == 0
+    assert err_m < err_s

@@ -75,3 +75,23 @@ def test_box_filter():
     # average of sin over cell [x0, x0 + 4 dx) at the grid points
     c0 = np.mean(np.sin(x[:4]))
     assert abs(f[0, 0, 0] - 2.0 - c0) < 1e-14 and abs(f[3, 2, 0] - 2.0 - c0) < 1e-14
+
+
+def test_backscatter_clip_zeroes_negative_dissipation():
+    """After the clip the subgrid dissipation -tau:S is >= 0 everywhere, points
+    that dissipated keep their stress, and the clipped stress keeps its part
+    orthogonal to S."""
+    rng = np.random.default_rng(9)
+    g = rng.standard_normal((500, 9))
+    tau = rng.standard_normal((500, 6))
+    out = m.clip_backscatter(tau, g)
+    G = g.reshape(-1, 3, 3)
+    S = np.stack([G[:, 0, 0], 0.5 * (G[:, 0, 1] + G[:, 1, 0]), 0.5 * (G[:, 0, 2] + G[:, 2, 0]),
+                  G[:, 1, 1], 0.5 * (G[:, 1, 2] + G[:, 2, 1]), G[:, 2, 2]], axis=1)
+    w = np.array([1.0, 2.0, 2.0, 1.0, 2.0, 1.0])
+    d0 = -(w * tau * S).sum(1)
+    d1 = -(w * out * S).sum(1)
+    assert (d1 >= -1e-12).all()
+    assert np.array_equal(out[d0 >= 0], tau[d0 >= 0])
+    assert np.abs(d1[d0 < 0]).max() < 1e-12
+    assert (d0 < 0).sum() > 100
