@@ -163,6 +163,33 @@ def kernel_bytes(n, cls, precision):
     return table.get(cls)
 
 
+def accuracy_run(n, steps, every):
+    """Untimed: FP64 and mixed from the same field for `steps` steps, E(k)
+    sampled every `every` steps and averaged; skewness, energy and
+    dissipation at the end.  Returns the errors of the mixed form against
+    FP64."""
+    from hit3d_amd.driver import Run
+    res = {}
+    for prec in (0, 1):
+        r = Run(params(n), precision=prec)
+        r.begin_new()
+        eks = []
+        for i in range(steps):
+            r.step()
+            if (i + 1) % every == 0:
+                eks.append(r.stats().e_spec)
+        st = r.stats()
+        res[prec] = (np.mean(eks, axis=0), st, r.h.skewness())
+        r.h.close()
+    e64, e32 = res[0][0], res[1][0]
+    sig = e64 > 1e-12 * e64.max()
+    return {"steps": steps, "samples": steps // every,
+            "ek_time_avg_max_rel_err_vs_fp64": float(np.max(np.abs(e32[sig] - e64[sig]) / e64[sig])),
+            "eps_rel_err_vs_fp64": abs(res[1][1].eps_v - res[0][1].eps_v) / res[0][1].eps_v,
+            "skewness_fp64": res[0][2], "skewness_mixed": res[1][2],
+            "skewness_rel_err_vs_fp64": abs(res[1][2] - res[0][2]) / abs(res[0][2])}
+
+
 def calib_copy(n):
     """Three device-to-device copies of one slot (wrk(2) = wrk(1)): a plain
     copy kernel of known size for scaling the FETCH_SIZE / WRITE_SIZE counters."""
@@ -226,6 +253,9 @@ def main():
     ap.add_argument("--workload", default="c3", choices=["c3"] + sorted(WORKLOADS),
                     help="c3 (default) prints the benchmark line; c1, c2, c5 time the other "
                          "configurations")
+    ap.add_argument("--accuracy-steps", type=int, default=100,
+                    help="untimed FP64 vs mixed run for time-averaged E(k) and skewness errors "
+                         "(0 skips it)")
     ap.add_argument("--calib", action="store_true",
                     help="after the measurement, copy one slot 3 times (a plain copy of known "
                          "size, the reference for PMC byte counters)")
@@ -250,6 +280,8 @@ def main():
                      "energy_rel_err_vs_fp64": abs(st32.energy - st64.energy) / st64.energy,
                      "eps_rel_err_vs_fp64": abs(st32.eps_v - st64.eps_v) / st64.eps_v,
                      "after_steps": a.steps + a.warmup}
+            if a.accuracy_steps > 0:
+                mixed["accuracy"] = accuracy_run(n, a.accuracy_steps, 10)
         except Exception as e:  # reported, never silently replaced
             mixed = {"error": str(e)}
     if a.calib:

@@ -1038,6 +1038,33 @@ int hit3d_stat_scalars(double* sc_diss, double* mean, double* variance, double* 
     });
 }
 
+int hit3d_skewness(double* skew) {
+    return guarded([&] {
+        require_ready();
+        Lib& L = lib();
+        double tot = 0.0;
+        for (int c = 1; c <= 3; ++c) {
+            {
+                Timed t(TC_POINTWISE);
+                launch_derivative(L.g, L.F[c], L.W[0], c, L.st);
+            }
+            fft_inverse(L.W[0]);
+            {
+                Timed t(TC_REDUCE);
+                launch_phys_moments23(L.g, L.W[0], L.dpart, L.dscal + 4, L.st);
+            }
+            check(hipMemcpyAsync(L.hpin + 4, L.dscal + 4, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                                 L.st),
+                  "hipMemcpyAsync");
+            check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
+            const double n3 = (double)L.n * L.n * L.n;
+            const double m2 = L.hpin[4] / n3, m3 = L.hpin[5] / n3;
+            tot += m3 / std::pow(m2, 1.5);
+        }
+        *skew = tot / 3.0;
+    });
+}
+
 int hit3d_divergence(double* dmin, double* dmax) {
     return guarded([&] {
         require_ready();

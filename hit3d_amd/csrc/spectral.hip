@@ -594,6 +594,25 @@ __global__ void k_phys_plane_sums(Geo g, const double* a, const double* b, const
     }
 }
 
+__global__ void k_phys_plane_m23(Geo g, const double* a, double* partial) {
+    __shared__ double sh[TB / 64];
+    const int z = blockIdx.x;
+    double s2 = 0.0, s3 = 0.0;
+    for (int p = threadIdx.x; p < g.n * g.n; p += blockDim.x) {
+        const int x = p % g.n, y = p / g.n;
+        const double va = a[((int64_t)z * g.n + y) * 2 * g.P + x];
+        const double q = va * va;
+        s2 += q;
+        s3 += q * va;
+    }
+    const double t2 = block_sum(s2, sh);
+    const double t3 = block_sum(s3, sh);
+    if (threadIdx.x == 0) {
+        partial[2 * z] = t2;
+        partial[2 * z + 1] = t3;
+    }
+}
+
 __global__ void k_sum2_serial(const double* partial, int cnt, double* out) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         double s0 = 0.0, s1 = 0.0;
@@ -667,6 +686,11 @@ void launch_shell_sums1(const Geo& g, const double* f, int s1, int ns, double* o
 void launch_phys_sums(const Geo& g, const double* a, const double* b, const double* c,
                       double* partial, double* out, hipStream_t s) {
     hipLaunchKernelGGL(k_phys_plane_sums, dim3(g.n), dim3(TB), 0, s, g, a, b, c, partial);
+    hipLaunchKernelGGL(k_sum2_serial, dim3(1), dim3(64), 0, s, partial, g.n, out);
+}
+void launch_phys_moments23(const Geo& g, const double* a, double* partial, double* out,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(k_phys_plane_m23, dim3(g.n), dim3(TB), 0, s, g, a, partial);
     hipLaunchKernelGGL(k_sum2_serial, dim3(1), dim3(64), 0, s, partial, g.n, out);
 }
 void launch_minmax(const Geo& g, const double* a, const double* b, const double* c, double* sum,

@@ -82,6 +82,9 @@ void launch_shell_sums1(const Geo& g, const double* f, int s1, int ns, double* o
 // partial must hold 2n doubles.
 void launch_phys_sums(const Geo& g, const double* a, const double* b, const double* c,
                       double* partial, double* out, hipStream_t s);
+// out[0] = sum(a^2), out[1] = sum(a^3) over x < n, deterministic (partial: 2n).
+void launch_phys_moments23(const Geo& g, const double* a, double* partial, double* out,
+                           hipStream_t s);
 // min and max over x < n of a (b, c null) or (a + b) + c, into keys[0..1] as
 // order-preserving unsigned keys; the combined field is stored into sum when
 // sum is non-null.

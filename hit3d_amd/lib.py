@@ -58,6 +58,7 @@ _SIGNATURES = {
     "hit3d_stats": [_F64P, _F64P],
     "hit3d_stat_scalars": [_F64P] * 6,
     "hit3d_divergence": [_F64P, _F64P],
+    "hit3d_skewness": [_F64P],
     "hit3d_particles_set": [_I64P, _F64P, _F64P],
     "hit3d_particles_get": [_F64P, _F64P],
     "hit3d_fields_to_parts": [],
@@ -292,6 +293,12 @@ class Hit3D:
                                                _dp(out["sc_max"]), _dp(es)))
         out["es_spec"] = es
         return out
+
+    def skewness(self) -> float:
+        """Mean longitudinal velocity-derivative skewness."""
+        v = ctypes.c_double(0.0)
+        _check(self.lib.hit3d_skewness(ctypes.byref(v)))
+        return v.value
 
     def divergence(self):
         """(dmin, dmax) of div u in physical space -- pressure.f90:53-93."""

@@ -204,6 +204,11 @@ int hit3d_fields_to_parts(void);
  * slab exchange of :1236-1300 on one rank).  Particles keep their order. */
 int hit3d_particles_move(const double *dt, const int64_t *scheme);
 
+/* Velocity-derivative skewness, the mean over i of
+ * <(du_i/dx_i)^3> / <(du_i/dx_i)^2>^(3/2) in physical space (no reference
+ * counterpart; a statistic for comparing the precision forms).  Uses wrk(0). */
+int hit3d_skewness(double *skew);
+
 /* divergence (pressure.f90:53-93): min and max over the grid of
  * du/dx + dv/dy + dw/dz in physical space; leaves wrk(1:3) = fields(1:3),
  * wrk(4:6) = the three derivatives in physical space and wrk(0) = their sum. */

@@ -573,6 +573,15 @@ def scalar_stats(phi: np.ndarray, n: int, kmax: int, pe: float) -> dict:
                 sc_max=float(p.max()), es_spec=es)
 
 
+def skewness(fields: np.ndarray, n: int) -> float:
+    """Mean over i of <(du_i/dx_i)^3>/<(du_i/dx_i)^2>^(3/2) (a build statistic)."""
+    out = 0.0
+    for c, ax in ((0, "x"), (1, "y"), (2, "z")):
+        d = fft_inverse(x_derivative(fields[c], ax))[:, :, :n]
+        out += float(np.mean(d ** 3)) / float(np.mean(d ** 2)) ** 1.5
+    return out / 3.0
+
+
 def divergence(fields: np.ndarray, n: int):
     """pressure.f90:53-93: min and max of du/dx + dv/dy + dw/dz, each
     derivative taken spectrally and brought to physical space separately."""

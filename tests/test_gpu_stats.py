@@ -179,3 +179,17 @@ def test_restart_round_trip_through_run(tmp_path):
     finally:
         r2.h.close()
     assert path.endswith("abcdefghij.64.000003")
+
+
+def test_skewness_matches_oracle(oracle):
+    n = 32
+    f = random_spec(oracle, n, 21, 3)
+    f = oracle.dealias_all(oracle.pressure(f, n), n // 3, n)
+    h = hit3d_amd.Hit3D(n)
+    try:
+        for c in range(3):
+            h.put(FIELDS, c + 1, f[c])
+        got, want = h.skewness(), oracle.skewness(f, n)
+        assert abs(got - want) <= 1e-12 * max(1.0, abs(want))
+    finally:
+        h.close()

@@ -415,6 +415,20 @@ def test_scalar_spectrum_sums_to_variance(oracle):
     assert abs(st["es_spec"].sum() - st["variance"]) < 1e-12 * st["variance"]
 
 
+def test_skewness_closed_form(oracle):
+    """u = (sin x + a sin 2x, 0, 0)-type fields: du/dx = cos x + 2a cos 2x has
+    <d^2> = 1/2 + 2a^2 and <d^3> = 3a/2 (only the cos^2 x cos 2x term
+    survives), the other components have zero derivative variance and are
+    given a symmetric profile (zero skewness)."""
+    n, a = 32, 0.3
+    f = np.stack([oracle.fft_forward(_phys(n, g)) for g in (
+        lambda x, y, z: np.sin(x) + a * np.sin(2 * x),
+        lambda x, y, z: np.sin(y), lambda x, y, z: np.sin(z))])
+    sk = oracle.skewness(f, n)
+    want = ((1.5 * a) / (0.5 + 2 * a * a) ** 1.5 + 0.0 + 0.0) / 3.0
+    assert abs(sk - want) < 1e-13
+
+
 def test_divergence_closed_form(oracle):
     """u = (a sin x, b sin 2y, c sin 4z): div u = a cos x + 2b cos 2y + 4c cos 4z,
     extremes -+(a + 2b + 4c) at grid points; zero after projection."""
