@@ -64,40 +64,62 @@ static void pass_y(double* slot, int dir) {
     launch_cols<double>(L.n, dir, c, c, n * P, n * P, P, P, L.tw64, L.st);
 }
 
-void fft_forward(double*& slot) {
+// y and z passes of a forward transform whose x pass is done; ph = -1
+// multiplies the spectral result by exp(-i theta) (the phase shift back).
+static void fft_forward_yz(double*& slot, int ph) {
     Lib& L = lib();
     const int64_t n = L.n, P = L.g.P;
-    {
-        Timed t(TC_FFT_X);
-        launch_x_r2c<double>(L.n, slot, P, L.tw64, L.st);
-    }
     pass_y(slot, -1);
     {
         Timed t(TC_FFT_Z);
         launch_cols<double>(L.n, -1, reinterpret_cast<cpx<double>*>(slot),
                             reinterpret_cast<cpx<double>*>(L.spare), P, n * P, n * P, P, L.tw64,
-                            L.st);
+                            L.st, ph, L.ps_tab);
     }
     std::swap(slot, L.spare);
 }
 
-void fft_inverse(double*& slot) {
+void fft_forward(double*& slot, int ph) {
+    Lib& L = lib();
+    {
+        Timed t(TC_FFT_X);
+        launch_x_r2c<double>(L.n, slot, L.g.P, L.tw64, L.st);
+    }
+    fft_forward_yz(slot, ph);
+}
+
+// dst = forward transform of the physical product a*b (dst may be a or b),
+// times exp(-i theta) when ph = -1
+static void fft_forward_product(const double* a, const double* b, double*& dst, int ph) {
+    Lib& L = lib();
+    {
+        Timed t(TC_FFT_X);
+        launch_x_r2c_mul<double>(L.n, a, b, dst, L.g.P, L.tw64, L.st);
+    }
+    fft_forward_yz(dst, ph);
+}
+
+// dst = inverse transform of src (src unchanged unless it is dst), the
+// spectral input multiplied by exp(+i theta) first when ph = +1
+static void fft_inverse_from(const double* src, double*& dst, int ph) {
     Lib& L = lib();
     const int64_t n = L.n, P = L.g.P;
     {
         Timed t(TC_FFT_Z);
-        launch_cols<double>(L.n, 1, reinterpret_cast<cpx<double>*>(slot),
+        launch_cols<double>(L.n, 1, reinterpret_cast<const cpx<double>*>(src),
                             reinterpret_cast<cpx<double>*>(L.spare), n * P, P, P, n * P, L.tw64,
-                            L.st);
+                            L.st, ph, L.ps_tab);
     }
-    std::swap(slot, L.spare);
-    pass_y(slot, 1);
+    std::swap(dst, L.spare);
+    pass_y(dst, 1);
     {
         Timed t(TC_FFT_X);
         const double norm = 1.0 / ((double)n * (double)n * (double)n);
-        launch_x_c2r<double>(L.n, slot, P, L.tw64, norm, L.st);
+        launch_x_c2r<double>(L.n, dst, P, L.tw64, norm, L.st);
     }
 }
+
+void fft_inverse(double*& slot) { fft_inverse_from(slot, slot, 0); }
 
 // ---------------------------------------------------------------------------
 // helpers
@@ -135,12 +157,6 @@ static void zero_slot(double* dst) {
     Lib& L = lib();
     Timed t(TC_POINTWISE);
     check(hipMemsetAsync(dst, 0, slot_bytes(), L.st), "hipMemsetAsync");
-}
-
-static void shift(const double* src, double* dst, int sign) {
-    Lib& L = lib();
-    Timed t(TC_POINTWISE);
-    launch_shift(L.g, src, dst, sign, L.ps_tab, L.st);
 }
 
 static void pmul(const double* a, const double* b, double* out) {
@@ -344,24 +360,15 @@ static void do_rhs_scalars(bool int_scalars) {
     if (L.nles > 0) throw std::runtime_error("LES scalars are not supported");
     if (L.les_model == 1) les_turb_visc();  // rhs_scalars.f90:18
     if (!int_scalars || L.ns == 0) {  // rhs_scalars.f90:32-39
-        for (int c = 1; c <= 3; ++c) {
-            copy_slot(W[c], F[c]);
-            fft_inverse(W[c]);
-        }
+        for (int c = 1; c <= 3; ++c) fft_inverse_from(F[c], W[c], 0);
         return;
     }
     if (L.dealias == 0) {  // rhs_scalars.f90:48-134
-        for (int c = 1; c <= 3; ++c) {
-            copy_slot(W[c], F[c]);
-            fft_inverse(W[c]);
-        }
+        for (int c = 1; c <= 3; ++c) fft_inverse_from(F[c], W[c], 0);
         for (int s = 1; s <= L.ns; ++s) {
-            copy_slot(W[0], F[3 + s]);
-            fft_inverse(W[0]);
-            for (int i = 1; i <= 3; ++i) {
-                pmul(W[0], W[i], W[s + 2 + i]);
-                fft_forward(W[s + 2 + i]);
-            }
+            fft_inverse_from(F[3 + s], W[0], 0);
+            // products phi u_i straight into the forward x pass
+            for (int i = 1; i <= 3; ++i) fft_forward_product(W[0], W[i], W[s + 2 + i], 0);
             const double* Q[3] = {W[s + 3], W[s + 4], W[s + 5]};
             {
                 Timed t(TC_POINTWISE);
@@ -380,38 +387,24 @@ static void do_rhs_scalars(bool int_scalars) {
             }
         }
     } else {  // rhs_scalars.f90:144-271
-        for (int c = 1; c <= 3 + L.ns; ++c) {
-            shift(F[c], W[c], +1);
-            fft_inverse(W[c]);
-        }
+        // the shift exp(i theta) on the first inverse pass, the products in
+        // the forward x pass, the shift back on the last forward pass
+        for (int c = 1; c <= 3 + L.ns; ++c) fft_inverse_from(F[c], W[c], +1);
         const int n1 = L.ns + 4, n2 = L.ns + 5;
         for (int c = 4; c <= 3 + L.ns; ++c) {
-            pmul(W[c], W[1], W[n1]);
-            pmul(W[c], W[2], W[n2]);
-            pmul(W[c], W[3], W[c]);
-            fft_forward(W[n1]);
-            fft_forward(W[n2]);
-            fft_forward(W[c]);
-            shift(W[n1], W[n1], -1);
-            shift(W[n2], W[n2], -1);
-            shift(W[c], W[c], -1);
+            fft_forward_product(W[c], W[1], W[n1], -1);
+            fft_forward_product(W[c], W[2], W[n2], -1);
+            fft_forward_product(W[c], W[3], W[c], -1);
             const double* Q[3] = {W[n1], W[n2], W[c]};
             Timed t(TC_POINTWISE);
             launch_scalar_rhs_ps(g, Q, F[c], W[c], 0.0, false, L.st);
         }
-        for (int c = 1; c <= 3; ++c) {
-            copy_slot(W[c], F[c]);
-            fft_inverse(W[c]);
-        }
+        for (int c = 1; c <= 3; ++c) fft_inverse_from(F[c], W[c], 0);
         for (int c = 4; c <= 3 + L.ns; ++c) {
-            copy_slot(W[0], F[c]);
-            fft_inverse(W[0]);
-            pmul(W[0], W[1], W[n1]);
-            pmul(W[0], W[2], W[n2]);
-            pmul(W[0], W[3], W[0]);
-            fft_forward(W[n1]);
-            fft_forward(W[n2]);
-            fft_forward(W[0]);
+            fft_inverse_from(F[c], W[0], 0);
+            fft_forward_product(W[0], W[1], W[n1], 0);
+            fft_forward_product(W[0], W[2], W[n2], 0);
+            fft_forward_product(W[0], W[3], W[0], 0);
             const double* Q[3] = {W[n1], W[n2], W[0]};
             Timed t(TC_POINTWISE);
             launch_scalar_rhs_ps(g, Q, F[c], W[c], L.pe[c - 4], true, L.st);
@@ -476,32 +469,21 @@ static void do_rhs_velocity() {
         launch_rhs_assemble(g, P, U, out, L.nu, L.dealias == 1, L.st);
     }
     if (L.dealias == 1) {  // rhs_velocity.f90:242-408
-        shift(F[1], W[4], +1);
-        shift(F[2], W[5], +1);
-        fft_inverse(W[4]);
-        fft_inverse(W[5]);
-        pmul(W[4], W[4], W[6]);
-        fft_forward(W[6]);
-        shift(W[6], W[6], -1);
+        // shift on the first inverse pass, products in the forward x pass,
+        // shift back on the last forward pass
+        fft_inverse_from(F[1], W[4], +1);
+        fft_inverse_from(F[2], W[5], +1);
+        fft_forward_product(W[4], W[4], W[6], -1);
         ps_acc(W[6], W[1], 1);
-        pmul(W[4], W[5], W[6]);
-        fft_forward(W[6]);
-        shift(W[6], W[6], -1);
+        fft_forward_product(W[4], W[5], W[6], -1);
         ps_acc(W[6], W[1], 2);
         ps_acc(W[6], W[2], 1);
-        pmul(W[5], W[5], W[6]);
-        fft_forward(W[6]);
-        shift(W[6], W[6], -1);
+        fft_forward_product(W[5], W[5], W[6], -1);
         ps_acc(W[6], W[2], 2);
-        shift(F[3], W[6], +1);
-        fft_inverse(W[6]);
-        pmul(W[4], W[6], W[4]);
-        pmul(W[5], W[6], W[5]);
-        pmul(W[6], W[6], W[6]);
-        for (int c = 4; c <= 6; ++c) {
-            fft_forward(W[c]);
-            shift(W[c], W[c], -1);
-        }
+        fft_inverse_from(F[3], W[6], +1);
+        fft_forward_product(W[4], W[6], W[4], -1);
+        fft_forward_product(W[5], W[6], W[5], -1);
+        fft_forward_product(W[6], W[6], W[6], -1);
         ps_acc(W[4], W[1], 3);
         ps_acc(W[5], W[2], 3);
         ps_acc(W[4], W[3], 1);

@@ -3,6 +3,7 @@
 // plane/stick loops and the slab transpose of xFFT3d (x_fftw.f90:852-1085).
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 
 #include "fft.hpp"
 #include "passes.hpp"
@@ -30,11 +31,24 @@ struct ColE {
     static constexpr int E = (N < 8) ? N : ((B * N / 8 <= 1024) ? 8 : 16);
 };
 
-template <int N, int E, int B, int DIR, typename T>
+// PH = +1: the input (spectral kx = column, kz = element, ky = group)
+// is multiplied by exp(+i theta) as it is loaded; PH = -1: the output (same
+// indexing) by exp(-i theta) before it is stored; theta = (kx+ky+kz) dx/2,
+// tab[2(kx+ky+kz+N)] = cos, sin (the phase shift of rhs_velocity.f90:248-278).
+template <int N, int PH, typename T>
+__device__ __forceinline__ cpx<T> phase(cpx<T> a, int kx, int jy, int jz, const double* tab) {
+    const int t = kx + wavenum(jy, N) + wavenum(jz, N) + N;
+    const T c = (T)tab[2 * t], sn = (T)tab[2 * t + 1];
+    if constexpr (PH > 0) return cpx<T>{a.x * c - a.y * sn, a.y * c + a.x * sn};
+    else return cpx<T>{a.x * c + a.y * sn, a.y * c - a.x * sn};
+}
+
+template <int N, int E, int B, int DIR, typename T, int PH = 0>
 __global__ __launch_bounds__(B* N / E) void k_cols(const cpx<T>* __restrict__ in, cpx<T>* out,
                                                    int64_t a_in, int64_t a_out, int64_t s_in,
                                                    int64_t s_out, int ncols,
-                                                   const cpx<T>* __restrict__ tw) {
+                                                   const cpx<T>* __restrict__ tw,
+                                                   const double* __restrict__ tab) {
     constexpr int TPF = N / E;
     constexpr int PS = PadFor<T>::value;
     __shared__ cpx<T> lds[padded_size<PS>(N * B)];
@@ -47,29 +61,50 @@ __global__ __launch_bounds__(B* N / E) void k_cols(const cpx<T>* __restrict__ in
     cpx<T>* dst = out + a * a_out + col;
     cpx<T> v[E];
 #pragma unroll
-    for (int s = 0; s < E; ++s)
+    for (int s = 0; s < E; ++s) {
         v[s] = valid ? src[(int64_t)(t + s * TPF) * s_in] : cpx<T>{(T)0, (T)0};
+        if constexpr (PH > 0)
+            if (valid) v[s] = phase<N, 1>(v[s], col, (int)a, t + s * TPF, tab);
+    }
     block_fft<N, E, DIR, T, B, PS>(v, t, lds + c, tw + N);
     if (valid) {
 #pragma unroll
-        for (int s = 0; s < E; ++s) dst[(int64_t)(t + s * TPF) * s_out] = v[s];
+        for (int s = 0; s < E; ++s) {
+            if constexpr (PH < 0) v[s] = phase<N, -1>(v[s], col, (int)a, t + s * TPF, tab);
+            dst[(int64_t)(t + s * TPF) * s_out] = v[s];
+        }
     }
 }
 
 template <int N, typename T>
 static void cols_n(int dir, const cpx<T>* in, cpx<T>* out, int64_t a_in, int64_t a_out,
-                   int64_t s_in, int64_t s_out, const cpx<T>* tw, hipStream_t s) {
+                   int64_t s_in, int64_t s_out, const cpx<T>* tw, hipStream_t s, int ph,
+                   const double* tab) {
     constexpr int B = ColCfg<T>::B;
     constexpr int E = ColE<N, T>::E;
     const int ncols = N / 2 + 1;
     dim3 grid((ncols + B - 1) / B, N);
     dim3 block(B * N / E);
+    if (ph != 0 && !tab) throw std::runtime_error("column pass: phase table missing");
+    if constexpr (std::is_same_v<T, double>) {
+        if (dir < 0 && ph < 0) {
+            hipLaunchKernelGGL((k_cols<N, E, B, -1, T, -1>), grid, block, 0, s, in, out, a_in,
+                               a_out, s_in, s_out, ncols, tw, tab);
+            return;
+        }
+        if (dir > 0 && ph > 0) {
+            hipLaunchKernelGGL((k_cols<N, E, B, 1, T, 1>), grid, block, 0, s, in, out, a_in, a_out,
+                               s_in, s_out, ncols, tw, tab);
+            return;
+        }
+    }
+    if (ph != 0) throw std::runtime_error("column pass: unsupported phase option");
     if (dir < 0)
         hipLaunchKernelGGL((k_cols<N, E, B, -1, T>), grid, block, 0, s, in, out, a_in, a_out, s_in,
-                           s_out, ncols, tw);
+                           s_out, ncols, tw, tab);
     else
         hipLaunchKernelGGL((k_cols<N, E, B, 1, T>), grid, block, 0, s, in, out, a_in, a_out, s_in,
-                           s_out, ncols, tw);
+                           s_out, ncols, tw, tab);
 }
 
 // ---------------------------------------------------------------------------
@@ -96,6 +131,47 @@ __global__ __launch_bounds__((RowCfg<N, T>::ROWS * RowCfg<N, T>::TPF)) void k_r2
     cpx<T> v[E];
 #pragma unroll
     for (int s = 0; s < E; ++s) v[s] = rowp[t + s * TPF];
+    block_fft<M, E, -1, T, 1>(v, t, lds[r], tw + M);
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < E; ++s) lds[r][t + s * TPF] = v[s];
+    __syncthreads();
+    const cpx<T>* twN = tw + N;
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+        const int k = t + s * TPF;
+        cpx<T> zk = lds[r][k];
+        cpx<T> zm = conj(lds[r][(M - k) & (M - 1)]);
+        cpx<T> e = scale(zk + zm, (T)0.5);
+        cpx<T> o = scale(mul_mi(zk - zm), (T)0.5);
+        rowp[k] = e + cmul(twN[k], o);
+    }
+    if (t == 0) {
+        cpx<T> z0 = lds[r][0];
+        rowp[M] = cpx<T>{z0.x - z0.y, (T)0};
+    }
+}
+
+// forward x pass of the pointwise product of two physical slots a*b into out
+// (out may be a or b: each row is read before it is written)
+template <int N, typename T>
+__global__ __launch_bounds__((RowCfg<N, T>::ROWS * RowCfg<N, T>::TPF)) void k_r2c_mul(
+    const T* a, const T* b, T* out, int64_t P, const cpx<T>* __restrict__ tw) {
+    using C = RowCfg<N, T>;
+    constexpr int M = C::M, E = C::E, TPF = C::TPF;
+    __shared__ cpx<T> lds[C::ROWS][M + 1];
+    const int t = threadIdx.x % TPF;
+    const int r = threadIdx.x / TPF;
+    const int64_t row = (int64_t)blockIdx.x * C::ROWS + r;
+    const cpx<T>* ap = reinterpret_cast<const cpx<T>*>(a) + row * P;
+    const cpx<T>* bp = reinterpret_cast<const cpx<T>*>(b) + row * P;
+    cpx<T>* rowp = reinterpret_cast<cpx<T>*>(out) + row * P;
+    cpx<T> v[E];
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+        const cpx<T> x = ap[t + s * TPF], y = bp[t + s * TPF];
+        v[s] = cpx<T>{x.x * y.x, x.y * y.y};
+    }
     block_fft<M, E, -1, T, 1>(v, t, lds[r], tw + M);
     __syncthreads();
 #pragma unroll
@@ -160,6 +236,14 @@ static void r2c_n(T* slot, int64_t P, const cpx<T>* tw, hipStream_t s) {
 }
 
 template <int N, typename T>
+static void r2c_mul_n(const T* a, const T* b, T* out, int64_t P, const cpx<T>* tw, hipStream_t s) {
+    using C = RowCfg<N, T>;
+    const int64_t rows = (int64_t)N * N;
+    hipLaunchKernelGGL((k_r2c_mul<N, T>), dim3(rows / C::ROWS), dim3(C::ROWS * C::TPF), 0, s, a, b,
+                       out, P, tw);
+}
+
+template <int N, typename T>
 static void c2r_n(T* slot, int64_t P, const cpx<T>* tw, T norm, hipStream_t s) {
     using C = RowCfg<N, T>;
     const int64_t rows = (int64_t)N * N;
@@ -194,20 +278,30 @@ void launch_x_c2r(int n, T* slot, int64_t P, const cpx<T>* tw, T norm, hipStream
     dispatch(n, [&](auto NC) { c2r_n<decltype(NC)::value, T>(slot, P, tw, norm, s); });
 }
 template <typename T>
+void launch_x_r2c_mul(int n, const T* a, const T* b, T* out, int64_t P, const cpx<T>* tw,
+                      hipStream_t s) {
+    dispatch(n, [&](auto NC) { r2c_mul_n<decltype(NC)::value, T>(a, b, out, P, tw, s); });
+}
+template <typename T>
 void launch_cols(int n, int dir, const cpx<T>* in, cpx<T>* out, int64_t a_in, int64_t a_out,
-                 int64_t s_in, int64_t s_out, const cpx<T>* tw, hipStream_t s) {
+                 int64_t s_in, int64_t s_out, const cpx<T>* tw, hipStream_t s, int ph,
+                 const double* tab) {
     dispatch(n, [&](auto NC) {
-        cols_n<decltype(NC)::value, T>(dir, in, out, a_in, a_out, s_in, s_out, tw, s);
+        cols_n<decltype(NC)::value, T>(dir, in, out, a_in, a_out, s_in, s_out, tw, s, ph, tab);
     });
 }
 
+template void launch_x_r2c_mul<double>(int, const double*, const double*, double*, int64_t,
+                                       const cpx<double>*, hipStream_t);
 template void launch_x_r2c<double>(int, double*, int64_t, const cpx<double>*, hipStream_t);
 template void launch_x_r2c<float>(int, float*, int64_t, const cpx<float>*, hipStream_t);
 template void launch_x_c2r<double>(int, double*, int64_t, const cpx<double>*, double, hipStream_t);
 template void launch_x_c2r<float>(int, float*, int64_t, const cpx<float>*, float, hipStream_t);
 template void launch_cols<double>(int, int, const cpx<double>*, cpx<double>*, int64_t, int64_t,
-                                  int64_t, int64_t, const cpx<double>*, hipStream_t);
+                                  int64_t, int64_t, const cpx<double>*, hipStream_t, int,
+                                  const double*);
 template void launch_cols<float>(int, int, const cpx<float>*, cpx<float>*, int64_t, int64_t,
-                                 int64_t, int64_t, const cpx<float>*, hipStream_t);
+                                 int64_t, int64_t, const cpx<float>*, hipStream_t, int,
+                                 const double*);
 
 }  // namespace hit3d

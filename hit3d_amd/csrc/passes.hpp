@@ -23,8 +23,19 @@ void launch_x_c2r(int n, T* slot, int64_t P, const cpx<T>* tw, T norm, hipStream
 // n/2+1 columns kx of each of the n groups a; element e of column kx of
 // group a is read at in[a*a_in + kx + e*s_in] and written at
 // out[a*a_out + kx + e*s_out].  dir = -1 forward, +1 inverse.
+// ph = +1 (inverse pass only): the spectral input (kx = column, kz =
+// element, ky = group) is multiplied by exp(+i theta) on load; ph = -1
+// (forward pass only): the spectral output by exp(-i theta) on store;
+// theta = (kx+ky+kz) dx/2 from tab (cos, sin per kx+ky+kz+n).  FP64 only.
 template <typename T>
 void launch_cols(int n, int dir, const cpx<T>* in, cpx<T>* out, int64_t a_in, int64_t a_out,
-                 int64_t s_in, int64_t s_out, const cpx<T>* tw, hipStream_t s);
+                 int64_t s_in, int64_t s_out, const cpx<T>* tw, hipStream_t s, int ph = 0,
+                 const double* tab = nullptr);
+
+// x pass, forward, of the pointwise product of two physical slots:
+// out = r2c(a * b) row by row (out may alias a or b).
+template <typename T>
+void launch_x_r2c_mul(int n, const T* a, const T* b, T* out, int64_t P, const cpx<T>* tw,
+                      hipStream_t s);
 
 }  // namespace hit3d

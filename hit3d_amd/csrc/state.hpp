@@ -118,7 +118,8 @@ struct Timed {
 
 // internal transforms on a slot pointer (the pointer may be swapped with
 // the spare buffer, which is why it is passed by reference)
-void fft_forward(double*& slot);
+// ph = -1: the spectral result is multiplied by exp(-i theta), theta = (kx+ky+kz) dx/2
+void fft_forward(double*& slot, int ph = 0);
 void fft_inverse(double*& slot);
 
 // fused FP64 / mixed step (step.hip)
