@@ -99,6 +99,29 @@ static void fft_forward_product(const double* a, const double* b, double*& dst, 
     fft_forward_yz(dst, ph);
 }
 
+// Forward transform of the physical product a*b, shifted back by exp(-i theta),
+// accumulated into the RHS as rhs_j += -1/2 i k_axj (a*b)^ for the given
+// targets instead of being stored; `work` is overwritten (x and y passes).
+static void fft_forward_product_acc(const double* a, const double* b, double* work, double* r0,
+                                    int ax0, double* r1 = nullptr, int ax1 = 0) {
+    Lib& L = lib();
+    const int64_t n = L.n, P = L.g.P;
+    {
+        Timed t(TC_FFT_X);
+        launch_x_r2c_mul<double>(L.n, a, b, work, P, L.tw64, L.st);
+    }
+    pass_y(work, -1);
+    ColAcc acc;
+    acc.r[0] = reinterpret_cast<cpx<double>*>(r0);
+    acc.ax[0] = ax0;
+    acc.r[1] = reinterpret_cast<cpx<double>*>(r1);
+    acc.ax[1] = ax1;
+    Timed t(TC_FFT_Z);
+    launch_cols<double>(L.n, -1, reinterpret_cast<cpx<double>*>(work),
+                        reinterpret_cast<cpx<double>*>(L.spare), P, n * P, n * P, P, L.tw64, L.st,
+                        -1, L.ps_tab, acc);
+}
+
 // dst = inverse transform of src (src unchanged unless it is dst), the
 // spectral input multiplied by exp(+i theta) first when ph = +1
 static void fft_inverse_from(const double* src, double*& dst, int ph) {
@@ -163,12 +186,6 @@ static void pmul(const double* a, const double* b, double* out) {
     Lib& L = lib();
     Timed t(TC_POINTWISE);
     launch_phys_mul(L.g, a, b, out, L.st);
-}
-
-static void ps_acc(const double* Ps, double* rhs, int axis) {
-    Lib& L = lib();
-    Timed t(TC_POINTWISE);
-    launch_ps_accum(L.g, Ps, rhs, axis, L.st);
 }
 
 static void free_all() {
@@ -471,24 +488,16 @@ static void do_rhs_velocity() {
     if (L.dealias == 1) {  // rhs_velocity.f90:242-408
         // shift on the first inverse pass, products in the forward x pass,
         // shift back on the last forward pass
+        // and the half-RHS accumulation in the last forward pass's epilogue
         fft_inverse_from(F[1], W[4], +1);
         fft_inverse_from(F[2], W[5], +1);
-        fft_forward_product(W[4], W[4], W[6], -1);
-        ps_acc(W[6], W[1], 1);
-        fft_forward_product(W[4], W[5], W[6], -1);
-        ps_acc(W[6], W[1], 2);
-        ps_acc(W[6], W[2], 1);
-        fft_forward_product(W[5], W[5], W[6], -1);
-        ps_acc(W[6], W[2], 2);
+        fft_forward_product_acc(W[4], W[4], W[6], W[1], 1);
+        fft_forward_product_acc(W[4], W[5], W[6], W[1], 2, W[2], 1);
+        fft_forward_product_acc(W[5], W[5], W[6], W[2], 2);
         fft_inverse_from(F[3], W[6], +1);
-        fft_forward_product(W[4], W[6], W[4], -1);
-        fft_forward_product(W[5], W[6], W[5], -1);
-        fft_forward_product(W[6], W[6], W[6], -1);
-        ps_acc(W[4], W[1], 3);
-        ps_acc(W[5], W[2], 3);
-        ps_acc(W[4], W[3], 1);
-        ps_acc(W[5], W[3], 2);
-        ps_acc(W[6], W[3], 3);
+        fft_forward_product_acc(W[4], W[6], W[4], W[1], 3, W[3], 1);
+        fft_forward_product_acc(W[5], W[6], W[5], W[2], 3, W[3], 2);
+        fft_forward_product_acc(W[6], W[6], W[6], W[3], 3);
         Timed t(TC_POINTWISE);
         for (int c = 1; c <= 3; ++c) launch_mask_ps(g, W[c], L.st);
     }

@@ -43,12 +43,12 @@ __device__ __forceinline__ cpx<T> phase(cpx<T> a, int kx, int jy, int jz, const 
     else return cpx<T>{a.x * c + a.y * sn, a.y * c - a.x * sn};
 }
 
-template <int N, int E, int B, int DIR, typename T, int PH = 0>
+template <int N, int E, int B, int DIR, typename T, int PH = 0, bool ACC = false>
 __global__ __launch_bounds__(B* N / E) void k_cols(const cpx<T>* __restrict__ in, cpx<T>* out,
                                                    int64_t a_in, int64_t a_out, int64_t s_in,
                                                    int64_t s_out, int ncols,
                                                    const cpx<T>* __restrict__ tw,
-                                                   const double* __restrict__ tab) {
+                                                   const double* __restrict__ tab, ColAcc acc) {
     constexpr int TPF = N / E;
     constexpr int PS = PadFor<T>::value;
     __shared__ cpx<T> lds[padded_size<PS>(N * B)];
@@ -71,7 +71,21 @@ __global__ __launch_bounds__(B* N / E) void k_cols(const cpx<T>* __restrict__ in
 #pragma unroll
         for (int s = 0; s < E; ++s) {
             if constexpr (PH < 0) v[s] = phase<N, -1>(v[s], col, (int)a, t + s * TPF, tab);
-            dst[(int64_t)(t + s * TPF) * s_out] = v[s];
+            if constexpr (ACC) {
+                // rhs += -1/2 i k_axis v for each target (rhs_velocity.f90:282-398)
+                const int64_t off = a * a_out + col + (int64_t)(t + s * TPF) * s_out;
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    if (!acc.r[j]) continue;
+                    const int ax = acc.ax[j];
+                    const double k = ax == 1 ? (double)col
+                                             : (double)wavenum(ax == 2 ? (int)a : t + s * TPF, N);
+                    const cpx<double> r = acc.r[j][off];
+                    acc.r[j][off] = {r.x + 0.5 * k * (double)v[s].y, r.y - 0.5 * k * (double)v[s].x};
+                }
+            } else {
+                dst[(int64_t)(t + s * TPF) * s_out] = v[s];
+            }
         }
     }
 }
@@ -79,7 +93,7 @@ __global__ __launch_bounds__(B* N / E) void k_cols(const cpx<T>* __restrict__ in
 template <int N, typename T>
 static void cols_n(int dir, const cpx<T>* in, cpx<T>* out, int64_t a_in, int64_t a_out,
                    int64_t s_in, int64_t s_out, const cpx<T>* tw, hipStream_t s, int ph,
-                   const double* tab) {
+                   const double* tab, ColAcc acc) {
     constexpr int B = ColCfg<T>::B;
     constexpr int E = ColE<N, T>::E;
     const int ncols = N / 2 + 1;
@@ -87,24 +101,29 @@ static void cols_n(int dir, const cpx<T>* in, cpx<T>* out, int64_t a_in, int64_t
     dim3 block(B * N / E);
     if (ph != 0 && !tab) throw std::runtime_error("column pass: phase table missing");
     if constexpr (std::is_same_v<T, double>) {
+        if (dir < 0 && ph < 0 && acc.r[0]) {
+            hipLaunchKernelGGL((k_cols<N, E, B, -1, T, -1, true>), grid, block, 0, s, in, out, a_in,
+                               a_out, s_in, s_out, ncols, tw, tab, acc);
+            return;
+        }
         if (dir < 0 && ph < 0) {
             hipLaunchKernelGGL((k_cols<N, E, B, -1, T, -1>), grid, block, 0, s, in, out, a_in,
-                               a_out, s_in, s_out, ncols, tw, tab);
+                               a_out, s_in, s_out, ncols, tw, tab, acc);
             return;
         }
         if (dir > 0 && ph > 0) {
             hipLaunchKernelGGL((k_cols<N, E, B, 1, T, 1>), grid, block, 0, s, in, out, a_in, a_out,
-                               s_in, s_out, ncols, tw, tab);
+                               s_in, s_out, ncols, tw, tab, acc);
             return;
         }
     }
-    if (ph != 0) throw std::runtime_error("column pass: unsupported phase option");
+    if (ph != 0 || acc.r[0]) throw std::runtime_error("column pass: unsupported option");
     if (dir < 0)
         hipLaunchKernelGGL((k_cols<N, E, B, -1, T>), grid, block, 0, s, in, out, a_in, a_out, s_in,
-                           s_out, ncols, tw, tab);
+                           s_out, ncols, tw, tab, acc);
     else
         hipLaunchKernelGGL((k_cols<N, E, B, 1, T>), grid, block, 0, s, in, out, a_in, a_out, s_in,
-                           s_out, ncols, tw, tab);
+                           s_out, ncols, tw, tab, acc);
 }
 
 // ---------------------------------------------------------------------------
@@ -285,9 +304,9 @@ void launch_x_r2c_mul(int n, const T* a, const T* b, T* out, int64_t P, const cp
 template <typename T>
 void launch_cols(int n, int dir, const cpx<T>* in, cpx<T>* out, int64_t a_in, int64_t a_out,
                  int64_t s_in, int64_t s_out, const cpx<T>* tw, hipStream_t s, int ph,
-                 const double* tab) {
+                 const double* tab, ColAcc acc) {
     dispatch(n, [&](auto NC) {
-        cols_n<decltype(NC)::value, T>(dir, in, out, a_in, a_out, s_in, s_out, tw, s, ph, tab);
+        cols_n<decltype(NC)::value, T>(dir, in, out, a_in, a_out, s_in, s_out, tw, s, ph, tab, acc);
     });
 }
 
@@ -299,9 +318,9 @@ template void launch_x_c2r<double>(int, double*, int64_t, const cpx<double>*, do
 template void launch_x_c2r<float>(int, float*, int64_t, const cpx<float>*, float, hipStream_t);
 template void launch_cols<double>(int, int, const cpx<double>*, cpx<double>*, int64_t, int64_t,
                                   int64_t, int64_t, const cpx<double>*, hipStream_t, int,
-                                  const double*);
+                                  const double*, ColAcc);
 template void launch_cols<float>(int, int, const cpx<float>*, cpx<float>*, int64_t, int64_t,
                                  int64_t, int64_t, const cpx<float>*, hipStream_t, int,
-                                 const double*);
+                                 const double*, ColAcc);
 
 }  // namespace hit3d

@@ -27,10 +27,17 @@ void launch_x_c2r(int n, T* slot, int64_t P, const cpx<T>* tw, T norm, hipStream
 // element, ky = group) is multiplied by exp(+i theta) on load; ph = -1
 // (forward pass only): the spectral output by exp(-i theta) on store;
 // theta = (kx+ky+kz) dx/2 from tab (cos, sin per kx+ky+kz+n).  FP64 only.
+// acc (forward pass with ph = -1 only): instead of storing the result v,
+// add -1/2 i k_ax[j] v to each non-null r[j] (same spectral layout as out),
+// ax 1 = kx, 2 = ky, 3 = kz.
+struct ColAcc {
+    cpx<double>* r[2] = {nullptr, nullptr};
+    int ax[2] = {0, 0};
+};
 template <typename T>
 void launch_cols(int n, int dir, const cpx<T>* in, cpx<T>* out, int64_t a_in, int64_t a_out,
                  int64_t s_in, int64_t s_out, const cpx<T>* tw, hipStream_t s, int ph = 0,
-                 const double* tab = nullptr);
+                 const double* tab = nullptr, ColAcc acc = ColAcc{});
 
 // x pass, forward, of the pointwise product of two physical slots:
 // out = r2c(a * b) row by row (out may alias a or b).

@@ -190,28 +190,6 @@ __global__ void k_rhs_assemble(Geo g, CPtr6 P, CPtr3 U, Ptr3 out, double nu, int
     }
 }
 
-__global__ void k_shift(Geo g, const double* src, double* dst, int sign, const double* tab) {
-    GRID_STRIDE(i, nspec(g)) {
-        SIdx s = sdecode(g, i);
-        const int t = s.kx + s.kyv + s.kzv + g.n;
-        const double c = tab[2 * t], sn = tab[2 * t + 1];
-        cpx<double> a = C(src)[s.off];
-        if (sign > 0)
-            C(dst)[s.off] = {a.x * c - a.y * sn, a.y * c + a.x * sn};
-        else
-            C(dst)[s.off] = {a.x * c + a.y * sn, a.y * c - a.x * sn};
-    }
-}
-
-__global__ void k_ps_accum(Geo g, const double* Ps, double* rhs, int axis) {
-    GRID_STRIDE(i, nspec(g)) {
-        SIdx s = sdecode(g, i);
-        const double k = axis == 1 ? (double)s.kx : (axis == 2 ? (double)s.kyv : (double)s.kzv);
-        cpx<double> a = C(Ps)[s.off], r = C(rhs)[s.off];
-        C(rhs)[s.off] = {r.x + 0.5 * k * a.y, r.y - 0.5 * k * a.x};
-    }
-}
-
 __global__ void k_mask_ps(Geo g, double* slot) {
     GRID_STRIDE(i, nspec(g)) {
         SIdx s = sdecode(g, i);
@@ -456,13 +434,6 @@ void launch_rhs_assemble(const Geo& g, const double* const P[6], const double* c
     }
     hipLaunchKernelGGL(k_rhs_assemble, dim3(grid_for(nspec_h(g))), dim3(TB), 0, s, g, p, u, o, nu,
                        phase_shift ? 1 : 0);
-}
-void launch_shift(const Geo& g, const double* src, double* dst, int sign, const double* tab,
-                  hipStream_t s) {
-    hipLaunchKernelGGL(k_shift, dim3(grid_for(nspec_h(g))), dim3(TB), 0, s, g, src, dst, sign, tab);
-}
-void launch_ps_accum(const Geo& g, const double* Ps, double* rhs, int axis, hipStream_t s) {
-    hipLaunchKernelGGL(k_ps_accum, dim3(grid_for(nspec_h(g))), dim3(TB), 0, s, g, Ps, rhs, axis);
 }
 void launch_mask_ps(const Geo& g, double* slot, hipStream_t s) {
     hipLaunchKernelGGL(k_mask_ps, dim3(grid_for(nspec_h(g))), dim3(TB), 0, s, g, slot);

@@ -31,12 +31,6 @@ void launch_products(const Geo& g, const double* u, const double* v, const doubl
 // mask: alias_count == 0 (2/3 rule) or <= 1 (phase shift).
 void launch_rhs_assemble(const Geo& g, const double* const P[6], const double* const U[3],
                          double* const out[3], double nu, bool phase_shift, hipStream_t s);
-// dst = src * exp(sign * i * theta), theta = (kx+ky+kz)*dx/2 (rhs_velocity.f90:248-257,276-278);
-// tab holds cos/sin per kx+ky+kz.
-void launch_shift(const Geo& g, const double* src, double* dst, int sign, const double* tab,
-                  hipStream_t s);
-// rhs += -1/2 i k_axis Ps (Ps already shifted back), rhs_velocity.f90:282-336,389-398.
-void launch_ps_accum(const Geo& g, const double* Ps, double* rhs, int axis, hipStream_t s);
 // zero entries with alias_count >= 2 (rhs_velocity.f90:400-404).
 void launch_mask_ps(const Geo& g, double* slot, hipStream_t s);
 // physical out = a * b (x < n only).
