@@ -201,6 +201,7 @@ static void free_all() {
     if (L.ps_tab) (void)hipFree(L.ps_tab);
     if (L.dshell) (void)hipFree(L.dshell);
     if (L.dpart) (void)hipFree(L.dpart);
+    if (L.dmax) (void)hipFree(L.dmax);
     if (L.tv) (void)hipFree(L.tv);
     for (void* q : {(void*)L.mlp.p, (void*)L.mlp.wb, (void*)L.mlp.m, (void*)L.mlp.v,
                     (void*)L.mlp.g, (void*)L.mlp.partial, (void*)L.mlp.lossp, (void*)L.mlp.loss,
@@ -470,12 +471,10 @@ static void do_rhs_velocity() {
     auto& F = L.F;
     auto& W = L.W;
     const Geo& g = L.g;
-    check(hipMemsetAsync(L.dscal, 0, sizeof(double), L.st), "hipMemsetAsync");
     {
         double* out[6] = {W[1], W[2], W[3], W[4], W[5], W[6]};
         Timed t(TC_POINTWISE);
-        launch_products(g, W[1], W[2], W[3], out, reinterpret_cast<unsigned long long*>(L.dscal),
-                        L.st);
+        launch_products(g, W[1], W[2], W[3], out, L.dmax, L.dscal, L.st);
     }
     for (int c = 1; c <= 6; ++c) fft_forward(W[c]);
     {
@@ -726,6 +725,7 @@ int hit3d_init(const int64_t* nx, const int64_t* ny, const int64_t* nz_all,
               "hipMemcpy");
         L.dshell = dalloc((size_t)(L.n + 2) * sizeof(double) * 2);
         L.dpart = dalloc((size_t)L.n * 2 * sizeof(double));
+        L.dmax = dalloc(std::max<size_t>(65536, (size_t)L.n * L.n) * sizeof(double));
         L.dscal = dalloc(16 * sizeof(double));
         check(hipHostMalloc((void**)&L.hpin, 16 * sizeof(double), hipHostMallocDefault),
               "hipHostMalloc");

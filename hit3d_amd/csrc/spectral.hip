@@ -129,7 +129,7 @@ struct Ptr3 {
 };
 
 __global__ void k_products(Geo g, const double* u, const double* v, const double* w, Ptr6 out,
-                           unsigned long long* maxbits) {
+                           double* blockmax) {
     __shared__ double sh[TB / 64];
     const int64_t total = (int64_t)g.n * g.n * g.n;
     double m = 0.0;
@@ -157,7 +157,26 @@ __global__ void k_products(Geo g, const double* u, const double* v, const double
     if (threadIdx.x == 0) {
         double mm = 0.0;
         for (int i = 0; i < (int)(blockDim.x >> 6); ++i) mm = sh[i] > mm ? sh[i] : mm;
-        atomicMax(maxbits, (unsigned long long)__double_as_longlong(mm));
+        blockmax[blockIdx.x] = mm;
+    }
+}
+
+// out[0] = max of count non-negative partials (one block of 1024 threads)
+__global__ __launch_bounds__(1024) void k_max_fold(const double* part, int64_t count, double* out) {
+    __shared__ double sh[16];
+    double m = 0.0;
+    for (int64_t i = threadIdx.x; i < count; i += 1024) m = part[i] > m ? part[i] : m;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double q = __shfl_down(m, o, 64);
+        m = q > m ? q : m;
+    }
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double mm = 0.0;
+        for (int i = 0; i < 16; ++i) mm = sh[i] > mm ? sh[i] : mm;
+        out[0] = mm;
     }
 }
 
@@ -415,12 +434,16 @@ void launch_advance(const Geo& g, double* f, const double* r, double* ro, double
     hipLaunchKernelGGL(k_advance, dim3(grid_for(nspec_h(g))), dim3(TB), 0, s, g, f, r, ro, dt,
                        first ? 1 : 0);
 }
+void launch_max_fold(const double* part, int64_t count, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_max_fold, dim3(1), dim3(1024), 0, s, part, count, out);
+}
 void launch_products(const Geo& g, const double* u, const double* v, const double* w,
-                     double* const out[6], unsigned long long* maxbits, hipStream_t s) {
+                     double* const out[6], double* blockmax, double* maxout, hipStream_t s) {
     Ptr6 o;
     for (int i = 0; i < 6; ++i) o.p[i] = out[i];
-    hipLaunchKernelGGL(k_products, dim3(grid_for(nphys_h(g))), dim3(TB), 0, s, g, u, v, w, o,
-                       maxbits);
+    const int nb = grid_for(nphys_h(g));  // <= 65536, the smallest size of blockmax
+    hipLaunchKernelGGL(k_products, dim3(nb), dim3(TB), 0, s, g, u, v, w, o, blockmax);
+    launch_max_fold(blockmax, nb, maxout, s);
 }
 void launch_rhs_assemble(const Geo& g, const double* const P[6], const double* const U[3],
                          double* const out[3], double nu, bool phase_shift, hipStream_t s) {

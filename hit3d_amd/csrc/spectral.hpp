@@ -21,11 +21,13 @@ void launch_pressure(const Geo& g, double* u, double* v, double* w, hipStream_t 
 // main.f90:137-146,156-164: f += dt*r (first) or f += dt*(1.5 r - 0.5 ro); ro = r.
 void launch_advance(const Geo& g, double* f, const double* r, double* ro, double dt, bool first,
                     hipStream_t s);
-// rhs_velocity.f90:31-32,56-72: max(|u|+|v|+|w|) into *maxbits (as ordered
-// bits of a non-negative double) and the six products u_i u_j (written to
-// out[0..5], which may alias u, v, w).
+// rhs_velocity.f90:31-32,56-72: max(|u|+|v|+|w|) into *maxout (per-block
+// maxima staged in blockmax, >= 65536 doubles) and the six products u_i u_j
+// (written to out[0..5], which may alias u, v, w).
 void launch_products(const Geo& g, const double* u, const double* v, const double* w,
-                     double* const out[6], unsigned long long* maxbits, hipStream_t s);
+                     double* const out[6], double* blockmax, double* maxout, hipStream_t s);
+// out[0] = max(part[0..count)), part non-negative.
+void launch_max_fold(const double* part, int64_t count, double* out, hipStream_t s);
 // rhs_velocity.f90:85-142 (half = false) or :183-232 (half = true):
 // r_i = mask * (c * i * t_i - nu k^2 U_i), t_i = -k_j P_ij; c = 1 or 1/2;
 // mask: alias_count == 0 (2/3 rule) or <= 1 (phase shift).

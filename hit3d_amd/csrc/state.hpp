@@ -56,6 +56,7 @@ struct Lib {
     double* ps_tab = nullptr;        // cos/sin(0.5*(kx+ky+kz)*dx), index (kx+ky+kz+n)
     double* dshell = nullptr;        // shell sums (kmax+2 doubles)
     double* dpart = nullptr;         // per-plane partial sums (2n doubles)
+    double* dmax = nullptr;          // per-block maxima of |u|+|v|+|w| (max(65536, n*n) doubles)
     double* dscal = nullptr;         // [0] courant max bits, [1] band energy, [2] enstrophy,
                                      // [4..5] physical sums, [8..9] min/max keys
     double* hpin = nullptr;          // pinned host mirror of dscal

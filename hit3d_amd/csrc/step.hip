@@ -163,7 +163,7 @@ __device__ __forceinline__ cpx<T> herm_load(const cpx<T>* rowp, int k, int nk) {
 template <int N, typename T>
 __global__ __launch_bounds__((XP<N, T>::THREADS)) void k_xp(SG g, W6<T> Wb,
                                                           const cpx<T>* __restrict__ tw, T norm,
-                                                          unsigned long long* maxbits) {
+                                                          double* __restrict__ blockmax) {
     using C = XP<N, T>;
     constexpr int E = C::E, TPF = C::TPF;
     constexpr int PS = PadFor<T>::value;
@@ -243,7 +243,8 @@ __global__ __launch_bounds__((XP<N, T>::THREADS)) void k_xp(SG g, W6<T> Wb,
             }
         }
     }
-    // block max -> one atomic
+    // block max -> one partial per block (folded by the next launch: one
+    // same-address atomic per block serialises at the L2)
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         const double q = __shfl_down(m, o, 64);
@@ -254,7 +255,7 @@ __global__ __launch_bounds__((XP<N, T>::THREADS)) void k_xp(SG g, W6<T> Wb,
     if (threadIdx.x == 0) {
         double mm = 0.0;
         for (int i = 0; i < (C::THREADS + 63) / 64; ++i) mm = redm[i] > mm ? redm[i] : mm;
-        atomicMax(maxbits, (unsigned long long)__double_as_longlong(mm));
+        blockmax[blockIdx.x] = mm;
     }
 }
 
@@ -372,8 +373,6 @@ __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_fwd_z_update(SG g, W6<T
     }
 }
 
-__global__ void k_zero_bits(unsigned long long* p) { *p = 0ull; }
-
 // ---------------------------------------------------------------- host
 template <typename F>
 static void dispatch_step(int n, F&& f) {
@@ -404,12 +403,10 @@ static void run(Lib& L, double dt, bool first, const cpx<T>* tw) {
     const int ncb = (g.NK + B - 1) / B;
     const int nky = 2 * g.K + 1;
     const bool forced = L.flow_type == 1;
-    auto* maxbits = reinterpret_cast<unsigned long long*>(L.dscal);
     if (forced) {
         Timed tm(TC_REDUCE);
         launch_shell_sums(L.g, L.F[1], L.F[2], L.F[3], 1, L.kfmax, L.dshell, L.st);
     }
-    hipLaunchKernelGGL(k_zero_bits, dim3(1), dim3(1), 0, L.st, maxbits);
     {
         Timed tm(TC_STEP_INV_Z);
         hipLaunchKernelGGL((k_inv_z<N, T>), dim3(ncb, nky, 3), dim3(CE<N, T>::THREADS), 0, L.st,
@@ -423,8 +420,10 @@ static void run(Lib& L, double dt, bool first, const cpx<T>* tw) {
     {
         Timed tm(TC_STEP_X);
         const T norm = (T)(1.0 / ((double)N * (double)N * (double)N));
-        hipLaunchKernelGGL((k_xp<N, T>), dim3((int64_t)N * N / XP<N, T>::ROWS),
-                           dim3(XP<N, T>::THREADS), 0, L.st, g, Wb, tw, norm, maxbits);
+        const int64_t nb = (int64_t)N * N / XP<N, T>::ROWS;  // <= n*n, the size of dmax
+        hipLaunchKernelGGL((k_xp<N, T>), dim3(nb), dim3(XP<N, T>::THREADS), 0, L.st, g, Wb, tw,
+                           norm, L.dmax);
+        launch_max_fold(L.dmax, nb, L.dscal, L.st);
     }
     {
         Timed tm(TC_STEP_FWD_Y);
