@@ -63,6 +63,73 @@ WORKLOADS = {
 }
 
 
+# configuration C4: 64^3 LES
+C4_DESC = ("64^3 forced LES, RE=800, 2/3 rule, KFMAX=2, FAMP=0.5: Smagorinsky (C_s 0.18) and the "
+           "learned closure (les_model 11, random-init weights of the 10-64-64-6 MLP); closure "
+           "training step at batch 65,536 (Adam, lr 1e-3)")
+# multiply-adds per sample of one training step: forward 16x64 + 64x64 + 64x16,
+# the same again for the weight gradients, 64x64 + 64x16 for the input gradients
+MLP_TRAIN_FLOP_PER_SAMPLE = 2 * (2 * 6144 + 5120)
+PEAK_BF16_TFLOPS = 2500.0
+
+
+def run_c4(steps, warmup, train_steps=50, batch=65536):
+    """C4 timing: LES steps with each closure, then closure training steps on
+    synthetic standardised samples.  Prints its own JSON line."""
+    from hit3d_amd.driver import Params, Run
+    from hit3d_amd.sgs import init_params
+    n = 64
+    out = {"workload": "c4", "config": C4_DESC, "steps": steps, "warmup": warmup}
+    for model in ("smagorinsky", "mlp"):
+        p = Params(nx=n, ny=n, nz_all=n, flow_type=1, re=800.0, dt=0.002, variable_dt=True,
+                   isp_type=2, ir_exp=4, peak_wavenum=3.0, kfmax=2, famp=0.5, dealias=0,
+                   les_model=1)
+        r = Run(p)
+        r.begin_new()
+        if model == "mlp":
+            r.h.mlp_set_params(init_params(0))
+            # inputs as they come (unit scale), outputs scaled to a small stress
+            r.h.mlp_set_norm(np.zeros(10), np.ones(10), np.zeros(6), np.full(6, 1e-3))
+            r.h.set_les(11, 0.18)
+        for _ in range(warmup):
+            r.step()
+        r.h.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            r.step()
+        r.h.synchronize()
+        t = (time.perf_counter() - t0) / steps
+        out[f"{model}_s_per_step"] = t
+        if model == "mlp":
+            out["mlp_fallback_steps"] = r.h.mlp_fallbacks()
+        r.h.close()
+        log(f"[bench] c4 {model} {t * 1e3:.3f} ms/step")
+    from hit3d_amd import Hit3D
+    rng = np.random.default_rng(12345)
+    X = rng.standard_normal((4 * batch, 10)).astype(np.float32)
+    T = (0.1 * rng.standard_normal((4 * batch, 6))).astype(np.float32)
+    h = Hit3D(n)
+    try:
+        h.mlp_set_params(init_params(0))
+        h.mlp_set_norm(np.zeros(10), np.ones(10), np.zeros(6), np.ones(6))
+        h.mlp_set_data(X, T)
+        h.mlp_train(5, batch, 1e-3)
+        h.synchronize()
+        t0 = time.perf_counter()
+        losses = h.mlp_train(train_steps, batch, 1e-3)
+        h.synchronize()
+        t = (time.perf_counter() - t0) / train_steps
+    finally:
+        h.close()
+    flop = MLP_TRAIN_FLOP_PER_SAMPLE * batch
+    out["train"] = {"batch": batch, "steps": train_steps, "s_per_step": t,
+                    "samples_per_s": batch / t, "flop_per_step": flop,
+                    "achieved_tflops": flop / t / 1e12, "peak_bf16_tflops": PEAK_BF16_TFLOPS,
+                    "frac": flop / t / 1e12 / PEAK_BF16_TFLOPS,
+                    "loss_first_last": [float(losses[0]), float(losses[-1])]}
+    print(json.dumps(out), flush=True)
+
+
 def run_workload(name, steps, warmup):
     """One of the other configurations: FP64 driver steps (unfused where the
     phase shift or scalars apply), with particles moved every step when the
@@ -190,17 +257,27 @@ def accuracy_run(n, steps, every):
             "skewness_rel_err_vs_fp64": abs(res[1][2] - res[0][2]) / abs(res[0][2])}
 
 
-def calib_copy(n):
-    """Three device-to-device copies of one slot (wrk(2) = wrk(1)): a plain
-    copy kernel of known size for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+def copy_bandwidth(n, reps=10):
+    """Device-to-device copies of one slot (wrk(2) = wrk(1)) with the runtime's
+    copy kernel: the HBM rate a plain copy reaches (bytes read + written per
+    second, GB/s), and a kernel of known size for scaling the FETCH_SIZE /
+    WRITE_SIZE counters."""
     import hit3d_amd
     h = hit3d_amd.Hit3D(n)
-    for _ in range(3):
+    try:
         h.copy(hit3d_amd.WRK, 2, hit3d_amd.WRK, 1)
-    h.synchronize()
+        h.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            h.copy(hit3d_amd.WRK, 2, hit3d_amd.WRK, 1)
+        h.synchronize()
+        t = (time.perf_counter() - t0) / reps
+    finally:
+        h.close()
     p = ((n // 2 + 1 + 15) // 16) * 16
-    log(f"[bench] calibration copy: 3 x {n * n * p * 16} bytes")
-    h.close()
+    byt = n * n * p * 16
+    log(f"[bench] copy: {reps + 1} x {byt} bytes, {t * 1e3:.3f} ms each")
+    return 2 * byt / t / 1e9
 
 
 def measured_traffic(n, cls):
@@ -250,20 +327,23 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="c3", choices=["c3"] + sorted(WORKLOADS),
-                    help="c3 (default) prints the benchmark line; c1, c2, c5 time the other "
+    ap.add_argument("--workload", default="c3", choices=["c3", "c4"] + sorted(WORKLOADS),
+                    help="c3 (default) prints the benchmark line; c1, c2, c4, c5 time the other "
                          "configurations")
     ap.add_argument("--accuracy-steps", type=int, default=100,
                     help="untimed FP64 vs mixed run for time-averaged E(k) and skewness errors "
                          "(0 skips it)")
     ap.add_argument("--calib", action="store_true",
-                    help="after the measurement, copy one slot 3 times (a plain copy of known "
-                         "size, the reference for PMC byte counters)")
+                    help="accepted for older command lines; the copy of known size (the "
+                         "reference for PMC byte counters) now always runs")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs the whole box on one GPU; --gpus {a.gpus} is not "
               "supported (multi-GPU sharding is design only)", file=sys.stderr)
         sys.exit(2)
+    if a.workload == "c4":
+        run_c4(a.steps, a.warmup)
+        return
     if a.workload != "c3":
         run_workload(a.workload, a.steps, a.warmup)
         return
@@ -284,8 +364,7 @@ def main():
                 mixed["accuracy"] = accuracy_run(n, a.accuracy_steps, 10)
         except Exception as e:  # reported, never silently replaced
             mixed = {"error": str(e)}
-    if a.calib:
-        calib_copy(n)
+    copy_gbs = copy_bandwidth(n)
     # dominant kernel class of the FP64 step
     best, best_ms = None, -1.0
     for cls, (ms, cnt) in tim64.items():
@@ -300,7 +379,8 @@ def main():
         roof = {"bound": "hbm", "kernel_class": best, "achieved": ach, "peak": PEAK_HBM_GBS,
                 "unit": "GB/s", "frac": ach / PEAK_HBM_GBS, "traffic": measured_traffic(n, best),
                 "alg_bytes_per_launch": byt, "avg_launch_ms": avg_s * 1e3,
-                "launches_per_step": cnt / a.steps}
+                "launches_per_step": cnt / a.steps, "copy_kernel_GBs": copy_gbs,
+                "frac_of_copy": ach / copy_gbs}
     s = slot_bytes(n)
     b_alg = 6 * 9 * s + 4 * 3 * s  # B_alg of the measurement plan, n_fft = 9, no scalars
     out = {

@@ -205,6 +205,7 @@ static void free_all() {
     if (L.tv) (void)hipFree(L.tv);
     for (void* q : {(void*)L.mlp.p, (void*)L.mlp.wb, (void*)L.mlp.m, (void*)L.mlp.v,
                     (void*)L.mlp.g, (void*)L.mlp.partial, (void*)L.mlp.lossp, (void*)L.mlp.loss,
+                    (void*)L.mlp.red,
                     (void*)L.mlp.X, (void*)L.mlp.T, (void*)L.mlp.bad})
         if (q) (void)hipFree(q);
     for (auto* q : L.mlp.buf) if (q) (void)hipFree(q);
@@ -1309,6 +1310,9 @@ static void mlp_reserve(int64_t tiles, int64_t steps) {
         M.lossp = reinterpret_cast<float*>(dalloc(sizeof(float) * tiles));
         M.cap_tiles = tiles;
     }
+    if (!M.red)
+        M.red = reinterpret_cast<float*>(
+            dalloc(sizeof(float) * kReduceGroups * (mlp::NPARAM + 1)));
     if (steps > M.cap_steps) {
         if (M.loss) check(hipFree(M.loss), "hipFree");
         M.loss = reinterpret_cast<float*>(dalloc(sizeof(float) * steps));
@@ -1397,7 +1401,7 @@ int hit3d_mlp_train(const int64_t* steps, const int64_t* batch, const double* lr
             const int64_t start = ((int64_t)M.step * *batch) % M.count;
             launch_mlp_train_tiles(M.X, M.T, M.count, start, (int)tiles, *batch, M.wb, M.p,
                                    M.partial, M.lossp, L.st);
-            launch_mlp_reduce(M.partial, M.lossp, (int)tiles, M.g, M.loss + i, L.st);
+            launch_mlp_reduce(M.partial, M.lossp, (int)tiles, M.red, M.g, M.loss + i, L.st);
             ++M.step;
             launch_mlp_adam(M.p, M.wb, M.g, M.m, M.v, M.step, (float)*lr, L.st);
         }
@@ -1426,7 +1430,7 @@ int hit3d_mlp_grads(const int64_t* count, const float* X, const float* T, float*
         const int64_t tiles = n / 32;
         mlp_reserve(tiles, 1);
         launch_mlp_train_tiles(dX, dT, n, 0, (int)tiles, n, M.wb, M.p, M.partial, M.lossp, L.st);
-        launch_mlp_reduce(M.partial, M.lossp, (int)tiles, M.g, M.loss, L.st);
+        launch_mlp_reduce(M.partial, M.lossp, (int)tiles, M.red, M.g, M.loss, L.st);
         float ls = 0.f;
         check(hipMemcpyAsync(grads, M.g, sizeof(float) * mlp::NPARAM, hipMemcpyDeviceToHost, L.st), "hipMemcpyAsync");
         check(hipMemcpyAsync(&ls, M.loss, sizeof(float), hipMemcpyDeviceToHost, L.st), "hipMemcpyAsync");

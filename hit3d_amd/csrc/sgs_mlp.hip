@@ -87,7 +87,9 @@ __device__ void stage_weights(Lds& s, const uint16_t* wb, const float* p) {
     for (int i = threadIdx.x; i < KOUT; i += NT) s.b3[i] = p[OB3 + i];
 }
 
-// forward through the two hidden layers for the TS samples staged in s.xb
+// forward through the two hidden layers for the TS samples staged in s.xb;
+// GRAD also keeps gelu'(Z) in s.g1, s.g2 for the backward pass
+template <bool GRAD>
 __device__ void hidden_layers(Lds& s) {
     const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, r = l & 15, gq = l >> 4;
     const int n0 = 16 * wv;
@@ -97,7 +99,7 @@ __device__ void hidden_layers(Lds& s) {
         for (int j = 0; j < 4; ++j) {
             const int i = 16 * mt + 4 * gq + j, n = n0 + r;
             const float z = acc[j] + s.b1[n];
-            s.g1[i * NH + n] = gelu_grad(z);
+            if (GRAD) s.g1[i * NH + n] = gelu_grad(z);
             s.h1[i * NH + n] = to_bf16(gelu(z));
         }
     }
@@ -108,7 +110,7 @@ __device__ void hidden_layers(Lds& s) {
         for (int j = 0; j < 4; ++j) {
             const int i = 16 * mt + 4 * gq + j, n = n0 + r;
             const float z = acc[j] + s.b2[n];
-            s.g2[i * NH + n] = gelu_grad(z);
+            if (GRAD) s.g2[i * NH + n] = gelu_grad(z);
             s.h2[i * NH + n] = to_bf16(gelu(z));
         }
     }
@@ -129,7 +131,7 @@ __global__ __launch_bounds__(NT) void k_train(const float* __restrict__ X, const
     }
     stage_weights(s, wb, p);
     __syncthreads();
-    hidden_layers(s);
+    hidden_layers<true>(s);
     float* part = partial + (int64_t)blockIdx.x * NPARAM;
     // output layer Y = H2 W3 + b3, loss and dY (waves 0, 1: one row tile each)
     const float inv = 2.0f / (float)(nb_total * NOUT);
@@ -223,19 +225,31 @@ __global__ __launch_bounds__(NT) void k_train(const float* __restrict__ X, const
     }
 }
 
-__global__ void k_reduce(const float* __restrict__ partial, const float* __restrict__ lossp, int tiles,
-                         float* __restrict__ grads, float* __restrict__ loss) {
+// Gradient reduction in two fixed-order levels: the tiles fall into
+// kReduceGroups contiguous groups, each summed in tile order (level 1, one
+// block row per group), then the group sums in group order (level 2).
+__global__ void k_reduce_groups(const float* __restrict__ partial, const float* __restrict__ lossp,
+                                int tiles, float* __restrict__ red) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int chunk = (tiles + kReduceGroups - 1) / kReduceGroups;
+    const int t0 = blockIdx.y * chunk, t1 = min(tiles, t0 + chunk);
+    float s = 0.f;
     if (k < NPARAM) {
-        float s = 0.f;
-        for (int t = 0; t < tiles; ++t) s += partial[(int64_t)t * NPARAM + k];
-        grads[k] = s;
+        for (int t = t0; t < t1; ++t) s += partial[(int64_t)t * NPARAM + k];
+    } else if (k == NPARAM) {
+        for (int t = t0; t < t1; ++t) s += lossp[t];
     }
-    if (k == NPARAM) {
-        float s = 0.f;
-        for (int t = 0; t < tiles; ++t) s += lossp[t];
-        *loss = s;
-    }
+    if (k <= NPARAM) red[(int64_t)blockIdx.y * (NPARAM + 1) + k] = s;
+}
+
+__global__ void k_reduce_final(const float* __restrict__ red, float* __restrict__ grads,
+                               float* __restrict__ loss) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > NPARAM) return;
+    float s = 0.f;
+    for (int gi = 0; gi < kReduceGroups; ++gi) s += red[(int64_t)gi * (NPARAM + 1) + k];
+    if (k < NPARAM) grads[k] = s;
+    else *loss = s;
 }
 
 __global__ void k_adam(float* p, uint16_t* wb, const float* g, float* m, float* v, float bc1, float bc2,
@@ -270,22 +284,29 @@ __device__ void output_layer(Lds& s, F&& f) {
     }
 }
 
+// inference kernels stage the weights once per block and run this many tiles
+constexpr int FWD_TILES = 8;
+
 __global__ __launch_bounds__(NT) void k_forward(const float* __restrict__ X, int64_t count,
                                                 const uint16_t* __restrict__ wb, const float* __restrict__ p,
                                                 float* __restrict__ Y) {
     __shared__ Lds s;
-    const int64_t first = (int64_t)blockIdx.x * TS;
-    for (int i = threadIdx.x; i < TS * KIN; i += NT) {
-        const int row = i / KIN, c = i % KIN;
-        const int64_t q = first + row;
-        s.xb[i] = (c < NIN && q < count) ? to_bf16(X[q * NIN + c]) : (uint16_t)0;
-    }
     stage_weights(s, wb, p);
-    __syncthreads();
-    hidden_layers(s);
-    output_layer(s, [&](int i, int n, float y) {
-        if (first + i < count) Y[(first + i) * NOUT + n] = y;
-    });
+    for (int it = 0; it < FWD_TILES; ++it) {
+        const int64_t first = ((int64_t)blockIdx.x * FWD_TILES + it) * TS;
+        if (first >= count) break;  // the same for the whole block
+        for (int i = threadIdx.x; i < TS * KIN; i += NT) {
+            const int row = i / KIN, c = i % KIN;
+            const int64_t q = first + row;
+            s.xb[i] = (c < NIN && q < count) ? to_bf16(X[q * NIN + c]) : (uint16_t)0;
+        }
+        __syncthreads();
+        hidden_layers<false>(s);
+        output_layer(s, [&](int i, int n, float y) {
+            if (first + i < count) Y[(first + i) * NOUT + n] = y;
+        });
+        __syncthreads();  // s.xb and s.h2 are rewritten by the next tile
+    }
 }
 
 struct Ptr9 {
@@ -307,54 +328,58 @@ __global__ __launch_bounds__(NT) void k_les(Geo g, Ptr9 gr, float delta, MlpNorm
     __shared__ float graw[TS][9];
     __shared__ float tl[TS][NOUT];
     const int64_t total = (int64_t)g.n * g.n * g.n;
-    const int64_t first = (int64_t)blockIdx.x * TS;
-    if (threadIdx.x < TS) {
-        const int64_t q = first + threadIdx.x;
-        offs[threadIdx.x] = q < total ? (q / g.n) * 2 * g.P + q % g.n : -1;
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < TS * KIN; i += NT) {
-        const int row = i / KIN, c = i % KIN;
-        const int64_t o = offs[row];
-        float x = 0.f;
-        if (o >= 0 && c < NIN) {
-            const float raw = c < 9 ? (float)gr.p[c][o] : delta;
-            if (c < 9) graw[row][c] = raw;
-            x = (raw - nrm.xmean[c]) / nrm.xstd[c];
-        }
-        s.xb[i] = c < NIN ? to_bf16(x) : (uint16_t)0;
-    }
     stage_weights(s, wb, p);
-    __syncthreads();
-    hidden_layers(s);
-    output_layer(s, [&](int i, int n, float y) { tl[i][n] = y * nrm.ystd[n] + nrm.ymean[n]; });
-    __syncthreads();
-    if (threadIdx.x < TS) {
-        const int i = threadIdx.x;
-        const int64_t o = offs[i];
-        if (o >= 0) {
-            const float* gg = graw[i];  // du_i/dx_j at gg[3 i + j]
-            const float S[6] = {gg[0], 0.5f * (gg[1] + gg[3]), 0.5f * (gg[2] + gg[6]), gg[4],
-                                0.5f * (gg[5] + gg[7]), gg[8]};
-            const float w[6] = {1.f, 2.f, 2.f, 1.f, 2.f, 1.f};  // symmetric double sum
-            float t[6], ts = 0.f, ss = 0.f;
-            bool fin = true;
-#pragma unroll
-            for (int c = 0; c < NOUT; ++c) {
-                t[c] = tl[i][c];
-                fin = fin && isfinite(t[c]);
-                ts += w[c] * t[c] * S[c];
-                ss += w[c] * S[c] * S[c];
-            }
-            if (!fin) atomicAdd(bad, 1u);
-            if (ts > 0.f && ss > 0.f) {  // -tau:S < 0: backscatter
-                const float a = -ts / ss;
-#pragma unroll
-                for (int c = 0; c < NOUT; ++c) t[c] += a * S[c];
-            }
-#pragma unroll
-            for (int c = 0; c < NOUT; ++c) tau.p[c][o] = (double)t[c];
+    for (int it = 0; it < FWD_TILES; ++it) {
+        const int64_t first = ((int64_t)blockIdx.x * FWD_TILES + it) * TS;
+        if (first >= total) break;  // the same for the whole block
+        if (threadIdx.x < TS) {
+            const int64_t q = first + threadIdx.x;
+            offs[threadIdx.x] = q < total ? (q / g.n) * 2 * g.P + q % g.n : -1;
         }
+        __syncthreads();
+        for (int i = threadIdx.x; i < TS * KIN; i += NT) {
+            const int row = i / KIN, c = i % KIN;
+            const int64_t o = offs[row];
+            float x = 0.f;
+            if (o >= 0 && c < NIN) {
+                const float raw = c < 9 ? (float)gr.p[c][o] : delta;
+                if (c < 9) graw[row][c] = raw;
+                x = (raw - nrm.xmean[c]) / nrm.xstd[c];
+            }
+            s.xb[i] = c < NIN ? to_bf16(x) : (uint16_t)0;
+        }
+        __syncthreads();
+        hidden_layers<false>(s);
+        output_layer(s, [&](int i, int n, float y) { tl[i][n] = y * nrm.ystd[n] + nrm.ymean[n]; });
+        __syncthreads();
+        if (threadIdx.x < TS) {
+            const int i = threadIdx.x;
+            const int64_t o = offs[i];
+            if (o >= 0) {
+                const float* gg = graw[i];  // du_i/dx_j at gg[3 i + j]
+                const float S[6] = {gg[0], 0.5f * (gg[1] + gg[3]), 0.5f * (gg[2] + gg[6]), gg[4],
+                                    0.5f * (gg[5] + gg[7]), gg[8]};
+                const float w[6] = {1.f, 2.f, 2.f, 1.f, 2.f, 1.f};  // symmetric double sum
+                float t[6], ts = 0.f, ss = 0.f;
+                bool fin = true;
+#pragma unroll
+                for (int c = 0; c < NOUT; ++c) {
+                    t[c] = tl[i][c];
+                    fin = fin && isfinite(t[c]);
+                    ts += w[c] * t[c] * S[c];
+                    ss += w[c] * S[c] * S[c];
+                }
+                if (!fin) atomicAdd(bad, 1u);
+                if (ts > 0.f && ss > 0.f) {  // -tau:S < 0: backscatter
+                    const float a = -ts / ss;
+#pragma unroll
+                    for (int c = 0; c < NOUT; ++c) t[c] += a * S[c];
+                }
+#pragma unroll
+                for (int c = 0; c < NOUT; ++c) tau.p[c][o] = (double)t[c];
+            }
+        }
+        __syncthreads();  // offs, graw, tl, s.xb are rewritten by the next tile
     }
 }
 
@@ -427,10 +452,12 @@ void launch_mlp_train_tiles(const float* X, const float* T, int64_t count, int64
     hipLaunchKernelGGL(k_train, dim3(tiles), dim3(NT), 0, s, X, T, count, start, nb_total, wb, p,
                        partial, lossp);
 }
-void launch_mlp_reduce(const float* partial, const float* lossp, int tiles, float* grads,
-                       float* loss, hipStream_t s) {
-    hipLaunchKernelGGL(k_reduce, dim3((NPARAM + 1 + 255) / 256), dim3(256), 0, s, partial, lossp,
-                       tiles, grads, loss);
+void launch_mlp_reduce(const float* partial, const float* lossp, int tiles, float* red,
+                       float* grads, float* loss, hipStream_t s) {
+    const unsigned nb = (NPARAM + 1 + 255) / 256;
+    hipLaunchKernelGGL(k_reduce_groups, dim3(nb, kReduceGroups), dim3(256), 0, s, partial, lossp,
+                       tiles, red);
+    hipLaunchKernelGGL(k_reduce_final, dim3(nb), dim3(256), 0, s, red, grads, loss);
 }
 void launch_mlp_adam(float* p, uint16_t* wb, const float* g, float* m, float* v, int t, float lr,
                      hipStream_t s) {
@@ -443,8 +470,9 @@ void launch_mlp_to_bf16(const float* p, uint16_t* wb, hipStream_t s) {
 }
 void launch_mlp_forward(const float* X, int64_t count, const uint16_t* wb, const float* p, float* Y,
                         hipStream_t s) {
-    const int64_t tiles = (count + TS - 1) / TS;
-    hipLaunchKernelGGL(k_forward, dim3((unsigned)tiles), dim3(NT), 0, s, X, count, wb, p, Y);
+    const int64_t per = (int64_t)TS * FWD_TILES;
+    hipLaunchKernelGGL(k_forward, dim3((unsigned)((count + per - 1) / per)), dim3(NT), 0, s, X,
+                       count, wb, p, Y);
 }
 void launch_mlp_les(const Geo& g, const double* const grad[9], double delta, const MlpNorm& nrm,
                     const uint16_t* wb, const float* p, double* const tau[6], unsigned int* bad,
@@ -454,7 +482,8 @@ void launch_mlp_les(const Geo& g, const double* const grad[9], double delta, con
     Ptr6 tp;
     for (int i = 0; i < 6; ++i) tp.p[i] = tau[i];
     const int64_t total = (int64_t)g.n * g.n * g.n;
-    hipLaunchKernelGGL(k_les, dim3((unsigned)((total + TS - 1) / TS)), dim3(NT), 0, s, g, gr,
+    const int64_t per = (int64_t)TS * FWD_TILES;
+    hipLaunchKernelGGL(k_les, dim3((unsigned)((total + per - 1) / per)), dim3(NT), 0, s, g, gr,
                        (float)delta, nrm, wb, p, tp, bad);
 }
 void launch_sgs_features(const Geo& g, const double* const grad[9], double delta, float* X,

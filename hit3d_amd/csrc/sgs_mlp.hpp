@@ -31,10 +31,13 @@ struct MlpNorm {
 void launch_mlp_train_tiles(const float* X, const float* T, int64_t count, int64_t start, int tiles,
                             int64_t nb_total, const uint16_t* wb, const float* p, float* partial,
                             float* lossp, hipStream_t s);
-// grads[k] = sum over tiles of partial[t][k] and *loss = sum of lossp, in
-// tile order (deterministic).
-void launch_mlp_reduce(const float* partial, const float* lossp, int tiles, float* grads,
-                       float* loss, hipStream_t s);
+// grads[k] = sum over tiles of partial[t][k] and *loss = sum of lossp, in a
+// fixed order (deterministic): kReduceGroups contiguous tile groups, each in
+// tile order, then the groups in order; red holds kReduceGroups * (NPARAM + 1)
+// floats of group sums.
+constexpr int kReduceGroups = 64;
+void launch_mlp_reduce(const float* partial, const float* lossp, int tiles, float* red,
+                       float* grads, float* loss, hipStream_t s);
 // Adam step t (1-based) on the FP32 master copy, refreshing the bfloat16 copy.
 void launch_mlp_adam(float* p, uint16_t* wb, const float* g, float* m, float* v, int t, float lr,
                      hipStream_t s);

@@ -82,6 +82,7 @@ struct Lib {
         float *m = nullptr, *v = nullptr, *g = nullptr;
         float* partial = nullptr; // per-tile gradient partials
         float* lossp = nullptr;
+        float* red = nullptr;     // group sums of the two-level gradient reduction
         float* loss = nullptr;    // per-step losses of the last train call
         int64_t cap_tiles = 0, cap_steps = 0;
         int step = 0;

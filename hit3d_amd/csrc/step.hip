@@ -64,6 +64,12 @@ struct Kc {
 };
 
 __device__ __forceinline__ bool kept(int j, int K, int n) { return j <= K || j >= n - K; }
+// Whether any element t + s*TPF (0 <= t < TPF) of register slot s is kept;
+// constant once the slot loop is unrolled, so dead slots take no registers.
+template <int N, int TPF>
+__device__ __forceinline__ constexpr bool slot_live(int s) {
+    return s * TPF <= Kc<N>::K || s * TPF + TPF - 1 >= N - Kc<N>::K;
+}
 __device__ __forceinline__ int kept_row(int kp, int K, int n) { return kp <= K ? kp : n - 2 * K - 1 + kp; }
 
 template <typename T>
@@ -302,7 +308,7 @@ __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_fwd_z_update(SG g, W6<T
         block_fft<N, E, -1, T, B, PS>(v, t, lds + c, tw + N);
 #pragma unroll
         for (int s = 0; s < E; ++s) {
-            if (!kept(t + s * TPF, Kc<N>::K, N)) continue;  // slots never written out
+            if (!slot_live<N, TPF>(s) || !kept(t + s * TPF, Kc<N>::K, N)) continue;  // never written out
             const T kzT = (T)wavenum(t + s * TPF, N);
             // p: 0 uu, 1 uv, 2 uw, 3 vv, 4 vw, 5 ww
             if (p == 0) acc[0][s] = scale(v[s], kxT);
@@ -335,7 +341,7 @@ __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_fwd_z_update(SG g, W6<T
 #pragma unroll
     for (int s = 0; s < E; ++s) {
         const int e = t + s * TPF;
-        if (!kept(e, Kc<N>::K, N)) continue;
+        if (!slot_live<N, TPF>(s) || !kept(e, Kc<N>::K, N)) continue;
         const int kz = wavenum(e, N);
         const double kzd = kz;
         const int64_t off = ((int64_t)jy * N + e) * g.P + kx;
