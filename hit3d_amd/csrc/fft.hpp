@@ -259,11 +259,11 @@ __device__ __forceinline__ void multi_fft(cpx<T> (&v)[NF][E], int t, cpx<T>* lds
 }
 
 // One transform with block-wide synchronisation; element j at lds[padi(j*LS)].
-template <int N, int E, int DIR, typename T, int LS, int PADSH = 0>
+template <int N, int E, int DIR, typename T, int LS, int PADSH = 0, bool TWREC = false>
 __device__ __forceinline__ void block_fft(cpx<T> (&v)[E], int t, cpx<T>* lds,
                                           const cpx<T>* __restrict__ tw) {
     cpx<T>(&vv)[1][E] = *reinterpret_cast<cpx<T>(*)[1][E]>(&v);
-    Stages<N, E, DIR, T, LS, 1, 0, PADSH, false, 1>::run(vv, t, lds, tw);
+    Stages<N, E, DIR, T, LS, 1, 0, PADSH, false, 1, TWREC>::run(vv, t, lds, tw);
 }
 
 }  // namespace hit3d

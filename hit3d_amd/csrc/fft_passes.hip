@@ -67,12 +67,28 @@ __global__ __launch_bounds__(B* N / E) void k_cols(const cpx<T>* __restrict__ in
             if (valid) v[s] = phase<N, 1>(v[s], col, (int)a, t + s * TPF, tab);
     }
     block_fft<N, E, DIR, T, B, PS>(v, t, lds + c, tw + N);
-    if (valid) {
+    if (!valid) return;
+    if constexpr (ACC) {
+        // rhs += -1/2 i k_axis v for each target (rhs_velocity.f90:282-398).
+        // Every target load is issued before the first store: the compiler
+        // cannot tell the targets apart, and a store between two loads would
+        // cost one memory round trip per load.  Blocks over 512 threads (at
+        // most 128 registers) do it in two halves.
+        constexpr int NCH = (B * N / E > 512) ? 2 : 1, H = E / NCH;
 #pragma unroll
-        for (int s = 0; s < E; ++s) {
-            if constexpr (PH < 0) v[s] = phase<N, -1>(v[s], col, (int)a, t + s * TPF, tab);
-            if constexpr (ACC) {
-                // rhs += -1/2 i k_axis v for each target (rhs_velocity.f90:282-398)
+        for (int h = 0; h < NCH; ++h) {
+            cpx<double> r[2][H];
+#pragma unroll
+            for (int q = 0; q < H; ++q) {
+                const int64_t off = a * a_out + col + (int64_t)(t + (h * H + q) * TPF) * s_out;
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    if (acc.r[j]) r[j][q] = acc.r[j][off];
+            }
+#pragma unroll
+            for (int q = 0; q < H; ++q) {
+                const int s = h * H + q;
+                if constexpr (PH < 0) v[s] = phase<N, -1>(v[s], col, (int)a, t + s * TPF, tab);
                 const int64_t off = a * a_out + col + (int64_t)(t + s * TPF) * s_out;
 #pragma unroll
                 for (int j = 0; j < 2; ++j) {
@@ -80,12 +96,16 @@ __global__ __launch_bounds__(B* N / E) void k_cols(const cpx<T>* __restrict__ in
                     const int ax = acc.ax[j];
                     const double k = ax == 1 ? (double)col
                                              : (double)wavenum(ax == 2 ? (int)a : t + s * TPF, N);
-                    const cpx<double> r = acc.r[j][off];
-                    acc.r[j][off] = {r.x + 0.5 * k * (double)v[s].y, r.y - 0.5 * k * (double)v[s].x};
+                    acc.r[j][off] = {r[j][q].x + 0.5 * k * (double)v[s].y,
+                                     r[j][q].y - 0.5 * k * (double)v[s].x};
                 }
-            } else {
-                dst[(int64_t)(t + s * TPF) * s_out] = v[s];
             }
+        }
+    } else {
+#pragma unroll
+        for (int s = 0; s < E; ++s) {
+            if constexpr (PH < 0) v[s] = phase<N, -1>(v[s], col, (int)a, t + s * TPF, tab);
+            dst[(int64_t)(t + s * TPF) * s_out] = v[s];
         }
     }
 }

@@ -291,8 +291,10 @@ __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_fwd_z_update(SG g, W6<T
     cpx<T> acc[3][E];
 #pragma unroll
     for (int s = 0; s < E; ++s) acc[0][s] = acc[1][s] = acc[2][s] = cpx<T>{(T)0, (T)0};
-#pragma unroll 1
-    for (int p = 0; p < 6; ++p) {
+    // FP64: the column of product p + 1 is loaded while product p is
+    // transformed (FP32 blocks have no registers to spare for it)
+    constexpr bool PREFETCH = sizeof(T) == 8;
+    auto load = [&](int p, cpx<T>(&dst)[E]) {
         // select the buffer without indexing the argument array at run time
         const T* qp = Q.p[0];
         if (p == 1) qp = Q.p[1];
@@ -301,11 +303,28 @@ __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_fwd_z_update(SG g, W6<T
         if (p == 4) qp = Q.p[4];
         if (p == 5) qp = Q.p[5];
         const cpx<T>* src = reinterpret_cast<const cpx<T>*>(qp) + (int64_t)jy * g.PA + kx;
-        cpx<T> v[E];
 #pragma unroll
         for (int s = 0; s < E; ++s)
-            v[s] = valid ? src[(int64_t)(t + s * TPF) * N * g.PA] : cpx<T>{(T)0, (T)0};
-        block_fft<N, E, -1, T, B, PS>(v, t, lds + c, tw + N);
+            dst[s] = valid ? src[(int64_t)(t + s * TPF) * N * g.PA] : cpx<T>{(T)0, (T)0};
+    };
+    cpx<T> nxt[E];
+    if (PREFETCH) load(0, nxt);
+#pragma unroll 1
+    for (int p = 0; p < 6; ++p) {
+        cpx<T> v[E];
+        if (PREFETCH) {
+#pragma unroll
+            for (int s = 0; s < E; ++s) v[s] = nxt[s];
+            if (p < 5) load(p + 1, nxt);
+        } else {
+            load(p, v);
+        }
+        // twiddles by recurrence from a pointer the compiler cannot see is
+        // loop-invariant: they are recomputed per product instead of holding
+        // fourteen complex values in registers across the loop
+        const cpx<T>* twp = tw + N;
+        asm volatile("" : "+s"(twp));
+        block_fft<N, E, -1, T, B, PS, true>(v, t, lds + c, twp);
 #pragma unroll
         for (int s = 0; s < E; ++s) {
             if (!slot_live<N, TPF>(s) || !kept(t + s * TPF, Kc<N>::K, N)) continue;  // never written out
@@ -338,44 +357,65 @@ __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_fwd_z_update(SG g, W6<T
     }
     if (!valid) return;
     const double kxd = kx, kyd = ky;
+    // The slots in NCH chunks; in each, every fields / rhs_old load is issued
+    // before the first store (the stores may alias later loads as far as the
+    // compiler knows, which would otherwise serialise one memory round trip
+    // per load).  FP32 blocks are twice as wide, so they get half the
+    // registers per thread: smaller chunks.
+    constexpr int NCH = sizeof(T) == 8 ? 2 : E, H = E / NCH;
 #pragma unroll
-    for (int s = 0; s < E; ++s) {
-        const int e = t + s * TPF;
-        if (!slot_live<N, TPF>(s) || !kept(e, Kc<N>::K, N)) continue;
-        const int kz = wavenum(e, N);
-        const double kzd = kz;
-        const int64_t off = ((int64_t)jy * N + e) * g.P + kx;
-        const double k2 = kxd * kxd + kyd * kyd + kzd * kzd;
-        const double lap = -u.nu * k2;
-        const bool fnode = u.forced && kx < N / 2 && [&] {
-            const int sh = shell_of(kx * kx + ky * ky + kz * kz);
-            return sh > 0 && sh <= u.kfmax;
-        }();
-        cpx<double> fn[3];
+    for (int h = 0; h < NCH; ++h) {
+        cpx<double> f[3][H], o[3][H];
 #pragma unroll
-        for (int cc = 0; cc < 3; ++cc) {
-            const cpx<double> f = reinterpret_cast<const cpx<double>*>(u.F[cc])[off];
-            // r = i * (-t) + lap * f, t = k.P  (rhs_velocity.f90:89-136)
-            const double tre = -(double)acc[cc][s].x, tim = -(double)acc[cc][s].y;
-            cpx<double> r = {-tim + lap * f.x, tre + lap * f.y};
-            if (fnode) r = {r.x + fac * f.x, r.y + fac * f.y};
-            cpx<double>* ro = reinterpret_cast<cpx<double>*>(u.R[cc]) + off;
-            if (u.first) {
-                fn[cc] = {f.x + u.dt * r.x, f.y + u.dt * r.y};
-            } else {
-                const cpx<double> o = *ro;
-                fn[cc] = {f.x + u.dt * (1.5 * r.x - 0.5 * o.x), f.y + u.dt * (1.5 * r.y - 0.5 * o.y)};
+        for (int q = 0; q < H; ++q) {
+            const int s = h * H + q, e = t + s * TPF;
+            if (!slot_live<N, TPF>(s) || !kept(e, Kc<N>::K, N)) continue;
+            const int64_t off = ((int64_t)jy * N + e) * g.P + kx;
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) {
+                f[cc][q] = reinterpret_cast<const cpx<double>*>(u.F[cc])[off];
+                if (!u.first) o[cc][q] = reinterpret_cast<const cpx<double>*>(u.R[cc])[off];
             }
-            *ro = r;
         }
-        // projection, pressure.f90:21-43
-        const double dre = kxd * fn[0].x + kyd * fn[1].x + kzd * fn[2].x;
-        const double dim = kxd * fn[0].y + kyd * fn[1].y + kzd * fn[2].y;
-        const double lapl = k2 == 0.0 ? 9e20 : k2;
-        const double pr = dre / lapl, pi = dim / lapl;
-        reinterpret_cast<cpx<double>*>(u.F[0])[off] = {fn[0].x - pr * kxd, fn[0].y - pi * kxd};
-        reinterpret_cast<cpx<double>*>(u.F[1])[off] = {fn[1].x - pr * kyd, fn[1].y - pi * kyd};
-        reinterpret_cast<cpx<double>*>(u.F[2])[off] = {fn[2].x - pr * kzd, fn[2].y - pi * kzd};
+#pragma unroll
+        for (int q = 0; q < H; ++q) {
+            const int s = h * H + q, e = t + s * TPF;
+            if (!slot_live<N, TPF>(s) || !kept(e, Kc<N>::K, N)) continue;
+            const int kz = wavenum(e, N);
+            const double kzd = kz;
+            const int64_t off = ((int64_t)jy * N + e) * g.P + kx;
+            const double k2 = kxd * kxd + kyd * kyd + kzd * kzd;
+            const double lap = -u.nu * k2;
+            const bool fnode = u.forced && kx < N / 2 && [&] {
+                const int sh = shell_of(kx * kx + ky * ky + kz * kz);
+                return sh > 0 && sh <= u.kfmax;
+            }();
+            cpx<double> fn[3];
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) {
+                const cpx<double> fc = f[cc][q];
+                // r = i * (-t) + lap * f, t = k.P  (rhs_velocity.f90:89-136)
+                const double tre = -(double)acc[cc][s].x, tim = -(double)acc[cc][s].y;
+                cpx<double> r = {-tim + lap * fc.x, tre + lap * fc.y};
+                if (fnode) r = {r.x + fac * fc.x, r.y + fac * fc.y};
+                if (u.first) {
+                    fn[cc] = {fc.x + u.dt * r.x, fc.y + u.dt * r.y};
+                } else {
+                    const cpx<double> oc = o[cc][q];
+                    fn[cc] = {fc.x + u.dt * (1.5 * r.x - 0.5 * oc.x),
+                              fc.y + u.dt * (1.5 * r.y - 0.5 * oc.y)};
+                }
+                reinterpret_cast<cpx<double>*>(u.R[cc])[off] = r;
+            }
+            // projection, pressure.f90:21-43
+            const double dre = kxd * fn[0].x + kyd * fn[1].x + kzd * fn[2].x;
+            const double dim = kxd * fn[0].y + kyd * fn[1].y + kzd * fn[2].y;
+            const double lapl = k2 == 0.0 ? 9e20 : k2;
+            const double pr = dre / lapl, pi = dim / lapl;
+            reinterpret_cast<cpx<double>*>(u.F[0])[off] = {fn[0].x - pr * kxd, fn[0].y - pi * kxd};
+            reinterpret_cast<cpx<double>*>(u.F[1])[off] = {fn[1].x - pr * kyd, fn[1].y - pi * kyd};
+            reinterpret_cast<cpx<double>*>(u.F[2])[off] = {fn[2].x - pr * kzd, fn[2].y - pi * kzd};
+        }
     }
 }
 
