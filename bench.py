@@ -154,13 +154,19 @@ def run_workload(name, steps, warmup):
     for _ in range(warmup):
         r.step()
     r.h.synchronize()
-    r.h.timing_enable(True)
     t0 = time.perf_counter()
     for i in range(steps):
         r.step()
         log(f"[bench] {name} step {i + 1}/{steps}")
     r.h.synchronize()
     t1 = time.perf_counter()
+    graphs, replays = r.h.graph_stats()
+    # per-kernel-class times: as many further steps with HIP events around
+    # the launches (direct launches, no graph replay)
+    r.h.timing_enable(True)
+    for i in range(steps):
+        r.step()
+    r.h.synchronize()
     timers = r.h.timing_read()
     r.h.close()
     t = (t1 - t0) / steps
@@ -169,6 +175,7 @@ def run_workload(name, steps, warmup):
     out = {"workload": name, "config": w["desc"], "s_per_step": t, "ns_per_point_step": t / n ** 3 * 1e9,
            "step_alg_bytes": b_alg, "step_hbm_roofline_frac": b_alg / t / (PEAK_HBM_GBS * 1e9),
            "particles": nptot, "steps": steps, "warmup": warmup,
+           "graph_replays": replays,
            "kernel_ms_per_step": {k: v[0] / steps for k, v in timers.items() if v[1]}}
     print(json.dumps(out), flush=True)
 

@@ -179,7 +179,7 @@ static void copy_slot(double* dst, const double* src) {
 static void zero_slot(double* dst) {
     Lib& L = lib();
     Timed t(TC_POINTWISE);
-    check(hipMemsetAsync(dst, 0, slot_bytes(), L.st), "hipMemsetAsync");
+    launch_fill_zero(dst, (int64_t)(slot_bytes() / sizeof(double)), L.st);
 }
 
 static void pmul(const double* a, const double* b, double* out) {
@@ -188,9 +188,18 @@ static void pmul(const double* a, const double* b, double* out) {
     launch_phys_mul(L.g, a, b, out, L.st);
 }
 
+static void clear_graphs() {
+    Lib& L = lib();
+    for (auto& g : L.graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    L.graphs.clear();
+    L.graph_seen.clear();
+}
+
 static void free_all() {
     Lib& L = lib();
     (void)hipDeviceSynchronize();
+    clear_graphs();
     for (auto* p : L.F) if (p) (void)hipFree(p);
     for (auto* p : L.W) if (p) (void)hipFree(p);
     for (auto* p : L.R) if (p) (void)hipFree(p);
@@ -294,7 +303,7 @@ static void les_scalar(int n) {
     auto& W = L.W;
     const int t1 = 3 + L.ns + L.nles + 1, t2 = t1 + 1;
     copy_slot(W[t1], L.F[n]);
-    check(hipMemsetAsync(W[0], 0, slot_bytes(), L.st), "hipMemsetAsync");
+    zero_slot(W[0]);
     for (int axis = 1; axis <= 3; ++axis) {
         {
             Timed t(TC_POINTWISE);
@@ -314,13 +323,18 @@ static void les_scalar(int n) {
 
 // nine physical velocity gradients du_i/dx_j (i-major) of fields(1:3) into
 // the closure's buffers 0..8
-static void mlp_gradients() {
-    Lib& L = lib();
-    auto& B = L.mlp.buf;
+static void mlp_alloc_scratch() {
+    auto& B = lib().mlp.buf;
     if (B.empty()) {
         B.assign(15, nullptr);
         for (auto& q : B) q = dalloc(slot_bytes());
     }
+}
+
+static void mlp_gradients() {
+    Lib& L = lib();
+    auto& B = L.mlp.buf;
+    mlp_alloc_scratch();
     for (int c = 0; c < 3; ++c)
         for (int a = 0; a < 3; ++a) {
             {
@@ -504,7 +518,8 @@ static void do_rhs_velocity() {
     if (L.les_model == 1 || L.les_model == 11) les_velocity();  // rhs_velocity.f90:414-416
 }
 
-static void do_force(double* energy) {
+// band energy into dscal[1] and the forcing added to the RHS (no host sync)
+static void force_enqueue() {
     Lib& L = lib();
     if (L.kfmax < 1) throw std::runtime_error("force_velocity needs kfmax >= 1");
     Timed t(TC_REDUCE);
@@ -512,6 +527,11 @@ static void do_force(double* energy) {
     const double* f[3] = {L.F[1], L.F[2], L.F[3]};
     double* r[3] = {L.W[1], L.W[2], L.W[3]};
     launch_force_apply(L.g, f, r, L.dshell, L.kfmax, L.famp, L.dscal + 1, L.st);
+}
+
+static void do_force(double* energy) {
+    Lib& L = lib();
+    force_enqueue();
     if (energy) {
         check(hipMemcpyAsync(L.hpin + 1, L.dscal + 1, sizeof(double), hipMemcpyDeviceToHost, L.st),
               "hipMemcpyAsync");
@@ -520,7 +540,7 @@ static void do_force(double* energy) {
     }
 }
 
-static void do_advance(int which, bool first, double dt) {
+static void do_advance(int which, bool first, double dt, const double* dtp = nullptr) {
     Lib& L = lib();
     int lo = 1, hi = 3;
     if (which == 2) {
@@ -530,7 +550,8 @@ static void do_advance(int which, bool first, double dt) {
         throw std::runtime_error("advance: which must be 1 (velocity) or 2 (scalars)");
     }
     Timed t(TC_POINTWISE);
-    for (int c = lo; c <= hi; ++c) launch_advance(L.g, L.F[c], L.W[c], L.R[c], dt, first, L.st);
+    for (int c = lo; c <= hi; ++c)
+        launch_advance(L.g, L.F[c], L.W[c], L.R[c], dt, dtp, first, L.st);
 }
 
 static void do_pressure() {
@@ -741,6 +762,7 @@ int hit3d_set_physics(const double* nu, const int64_t* flow_type, const int64_t*
     return guarded([&] {
         require_ready();
         Lib& L = lib();
+        clear_graphs();  // captured steps hold the old parameters as kernel arguments
         L.nu = *nu;
         L.flow_type = (int)*flow_type;
         L.kfmax = (int)*kfmax;
@@ -769,6 +791,7 @@ int hit3d_set_reaction(const double* reac_sc) {
         require_ready();
         Lib& L = lib();
         if ((int)L.reac.size() != L.ns) throw std::runtime_error("set_reaction: set_physics first");
+        clear_graphs();
         for (int s = 0; s < L.ns; ++s) L.reac[s] = reac_sc[s];
     }, false);
 }
@@ -892,6 +915,108 @@ int hit3d_pressure(void) {
     });
 }
 
+// The device work of one driver step (main.f90:124-167), enqueued without any
+// host synchronisation; the step size is read from dscal[12].
+static void step_body(bool fused, bool mixed, bool scal, bool first_v, bool first_s) {
+    Lib& L = lib();
+    const double* dtp = L.dscal + 12;
+    if (fused) {
+        // rhs_scalars + rhs_velocity + force + update + pressure in one pipeline
+        fused_step(dtp, first_v, mixed);
+        return;
+    }
+    do_rhs_scalars(scal);
+    if (scal) do_advance(2, first_s, 0.0, dtp);
+    do_rhs_velocity();
+    if (L.flow_type == 1) force_enqueue();
+    do_advance(1, first_v, 0.0, dtp);
+    do_pressure();
+}
+
+static std::vector<void*> slot_state() {
+    Lib& L = lib();
+    std::vector<void*> s;
+    for (auto* p : L.F) s.push_back(p);
+    for (auto* p : L.R) s.push_back(p);
+    for (auto* p : L.W) s.push_back(p);
+    for (auto* p : L.W32) s.push_back(p);
+    for (auto* p : L.mlp.buf) s.push_back(p);  // closure scratch, transformed in place
+    s.push_back(L.spare);
+    return s;
+}
+
+static void set_slot_state(const std::vector<void*>& s) {
+    Lib& L = lib();
+    size_t i = 0;
+    for (auto*& p : L.F) p = static_cast<double*>(s[i++]);
+    for (auto*& p : L.R) p = static_cast<double*>(s[i++]);
+    for (auto*& p : L.W) p = static_cast<double*>(s[i++]);
+    for (auto*& p : L.W32) p = static_cast<float*>(s[i++]);
+    for (auto*& p : L.mlp.buf) p = static_cast<double*>(s[i++]);
+    L.spare = static_cast<double*>(s[i++]);
+}
+
+// Puts the work-slot, closure-scratch and spare pointers back to the state s
+// (from slot_state); fields and rhs_old must not have moved.
+static void restore_work_slots(const std::vector<void*>& s) {
+    Lib& L = lib();
+    if (s.size() != slot_state().size())
+        throw std::runtime_error("internal: scratch slots were allocated during the step");
+    size_t i = 0;
+    for (auto* p : L.F)
+        if (p != s[i++]) throw std::runtime_error("internal: a field slot moved during the step");
+    for (auto* p : L.R)
+        if (p != s[i++]) throw std::runtime_error("internal: an rhs_old slot moved during the step");
+    for (auto*& p : L.W) p = static_cast<double*>(s[i++]);
+    for (auto*& p : L.W32) p = static_cast<float*>(s[i++]);
+    for (auto*& p : L.mlp.buf) p = static_cast<double*>(s[i++]);
+    L.spare = static_cast<double*>(s[i++]);
+}
+
+// step_body from a captured graph: replayed when one was captured from the
+// same slot pointers and flags, captured when this combination was already
+// run once eagerly (which also makes every lazy allocation happen outside a
+// capture), run eagerly otherwise.
+static void step_graph(bool fused, bool mixed, bool scal, bool first_v, bool first_s) {
+    Lib& L = lib();
+    const unsigned flags = (fused ? 1u : 0u) | (mixed ? 2u : 0u) | (scal ? 4u : 0u) |
+                           (first_v ? 8u : 0u) | (first_s ? 16u : 0u);
+    const std::vector<void*> before = slot_state();
+    for (auto& g : L.graphs)
+        if (g.flags == flags && g.before == before) {
+            check(hipGraphLaunch(g.exec, L.st), "hipGraphLaunch");
+            set_slot_state(g.after);
+            ++L.graph_replays;
+            return;
+        }
+    bool seen = false;
+    for (auto& k : L.graph_seen) seen = seen || (k.second == flags && k.first == before);
+    if (!seen || L.graphs.size() >= 16) {
+        if (!seen && L.graph_seen.size() < 64) L.graph_seen.push_back({before, flags});
+        step_body(fused, mixed, scal, first_v, first_s);
+        return;
+    }
+    hipGraph_t graph = nullptr;
+    check(hipStreamBeginCapture(L.st, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
+    try {
+        step_body(fused, mixed, scal, first_v, first_s);
+    } catch (...) {
+        (void)hipStreamEndCapture(L.st, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        set_slot_state(before);
+        throw;
+    }
+    check(hipStreamEndCapture(L.st, &graph), "hipStreamEndCapture");
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) set_slot_state(before);
+    check(e, "hipGraphInstantiate");
+    L.graphs.push_back({before, slot_state(), flags, exec});
+    check(hipGraphLaunch(exec, L.st), "hipGraphLaunch");
+    ++L.graph_replays;
+}
+
 int hit3d_step(const double* dt, int64_t* fov, int64_t* fos, const int64_t* int_scalars,
                const int64_t* precision, double* courant, double* energy) {
     return guarded([&] {
@@ -901,32 +1026,38 @@ int hit3d_step(const double* dt, int64_t* fov, int64_t* fos, const int64_t* int_
         if (*precision != 0 && *precision != 1)
             throw std::runtime_error("precision must be 0 (FP64) or 1 (mixed)");
         const bool scal = (*int_scalars != 0) && L.ns > 0;
-        double cmax = 0.0;
-        if (!scal && L.fuse && L.les_model == 0 && fused_step_supported(mixed)) {
-            // rhs_scalars + rhs_velocity + force + update + pressure in one pipeline
-            fused_step(*dt, *fov != 0, mixed, &cmax);
-            *fov = 0;
-            if (energy && L.flow_type == 1) {
-                check(hipMemcpyAsync(L.hpin + 1, L.dscal + 1, sizeof(double),
-                                     hipMemcpyDeviceToHost, L.st),
-                      "hipMemcpyAsync");
-                check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
-                *energy = L.hpin[1];
-            }
-        } else {
-            if (mixed) throw std::runtime_error("mixed precision needs the fused path");
-            do_rhs_scalars(scal);
-            if (scal) {
-                do_advance(2, *fos != 0, *dt);
-                *fos = 0;
-            }
-            do_rhs_velocity();
-            cmax = read_courant_max();
-            if (L.flow_type == 1) do_force(energy);
-            do_advance(1, *fov != 0, *dt);
-            *fov = 0;
-            do_pressure();
-        }
+        const bool fused = !scal && L.fuse && L.les_model == 0 && fused_step_supported(mixed);
+        if (!fused && mixed) throw std::runtime_error("mixed precision needs the fused path");
+        const bool first_v = *fov != 0, first_s = *fos != 0;
+        // the step size goes to the device (dscal[12]) ahead of the step's
+        // work, so a captured step replays with the current dt
+        L.hpin[12] = *dt;
+        check(hipMemcpyAsync(L.dscal + 12, L.hpin + 12, sizeof(double), hipMemcpyHostToDevice,
+                             L.st),
+              "hipMemcpyAsync");
+        // The out-of-place passes swap work slots with the spare; fields and
+        // rhs_old never move.  wrk is scratch of the step (undefined after
+        // it), so the work-slot labels are put back afterwards: every step
+        // then starts from the same pointers and one captured graph serves
+        // them all.
+        if (L.les_model == 11) mlp_alloc_scratch();  // before the pointer snapshot
+        const std::vector<void*> s0 = slot_state();
+        // the learned closure checks its output on the host mid-step
+        if (L.graph_mode && !L.timing && L.les_model != 11)
+            step_graph(fused, mixed, scal, first_v, first_s);
+        else
+            step_body(fused, mixed, scal, first_v, first_s);
+        restore_work_slots(s0);
+        *fov = 0;
+        if (scal) *fos = 0;
+        // Courant max (dscal[0], order-free bits of a non-negative double)
+        // and band energy (dscal[1]) back in one copy
+        check(hipMemcpyAsync(L.hpin, L.dscal, 2 * sizeof(double), hipMemcpyDeviceToHost, L.st),
+              "hipMemcpyAsync");
+        check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
+        double cmax;
+        std::memcpy(&cmax, L.hpin, sizeof(cmax));
+        if (energy && L.flow_type == 1) *energy = L.hpin[1];
         if (courant) *courant = cmax * (*dt) / L.dx;
     });
 }
@@ -1281,6 +1412,7 @@ int hit3d_set_les(const int64_t* les_model, const double* c_smag) {
             throw std::runtime_error("les_model 11 closes the velocity only (no scalars)");
         if (*les_model != 0 && L.dealias != 0)
             throw std::runtime_error("LES needs the 2/3 rule (m_parameters.f90:415-420)");
+        clear_graphs();
         L.les_model = (int)*les_model;
         L.c_smag = *c_smag;
         if (L.les_model != 0 && !L.tv) L.tv = dalloc(slot_bytes());
@@ -1509,7 +1641,24 @@ int hit3d_mlp_fallbacks(int64_t* count) {
 int hit3d_set_fusion(const int64_t* on) {
     return guarded([&] {
         require_ready();
+        clear_graphs();
         lib().fuse = *on != 0;
+    });
+}
+
+int hit3d_set_graphs(const int64_t* on) {
+    return guarded([&] {
+        require_ready();
+        clear_graphs();
+        lib().graph_mode = *on != 0;
+    });
+}
+
+int hit3d_graph_stats(int64_t* captured, int64_t* replays) {
+    return guarded([&] {
+        require_ready();
+        if (captured) *captured = (int64_t)lib().graphs.size();
+        if (replays) *replays = lib().graph_replays;
     });
 }
 

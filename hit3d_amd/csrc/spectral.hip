@@ -100,7 +100,9 @@ __global__ void k_pressure(Geo g, double* u, double* v, double* w) {
     }
 }
 
-__global__ void k_advance(Geo g, double* f, const double* r, double* ro, double dt, int first) {
+__global__ void k_advance(Geo g, double* f, const double* r, double* ro, double dt,
+                          const double* dtp, int first) {
+    if (dtp) dt = *dtp;
     GRID_STRIDE(i, nspec(g)) {
         SIdx s = sdecode(g, i);
         cpx<double> a = C(f)[s.off], b = C(r)[s.off];
@@ -113,6 +115,12 @@ __global__ void k_advance(Geo g, double* f, const double* r, double* ro, double 
             C(ro)[s.off] = b;
         }
     }
+}
+
+// zero fill by a kernel (also inside captured graphs, where memset nodes
+// are avoided)
+__global__ void k_fill_zero(double* p, int64_t count) {
+    GRID_STRIDE(i, count) p[i] = 0.0;
 }
 
 struct Ptr6 {
@@ -429,10 +437,13 @@ void launch_truncate(const Geo& g, double* slot, int kmax, hipStream_t s) {
 void launch_pressure(const Geo& g, double* u, double* v, double* w, hipStream_t s) {
     hipLaunchKernelGGL(k_pressure, dim3(grid_for(nspec_h(g))), dim3(TB), 0, s, g, u, v, w);
 }
-void launch_advance(const Geo& g, double* f, const double* r, double* ro, double dt, bool first,
-                    hipStream_t s) {
-    hipLaunchKernelGGL(k_advance, dim3(grid_for(nspec_h(g))), dim3(TB), 0, s, g, f, r, ro, dt,
+void launch_advance(const Geo& g, double* f, const double* r, double* ro, double dt,
+                    const double* dtp, bool first, hipStream_t s) {
+    hipLaunchKernelGGL(k_advance, dim3(grid_for(nspec_h(g))), dim3(TB), 0, s, g, f, r, ro, dt, dtp,
                        first ? 1 : 0);
+}
+void launch_fill_zero(double* p, int64_t count, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_zero, dim3(grid_for(count)), dim3(TB), 0, s, p, count);
 }
 void launch_max_fold(const double* part, int64_t count, double* out, hipStream_t s) {
     hipLaunchKernelGGL(k_max_fold, dim3(1), dim3(1024), 0, s, part, count, out);

@@ -19,8 +19,12 @@ void launch_truncate(const Geo& g, double* slot, int kmax, hipStream_t s);
 // pressure.f90:13-47.
 void launch_pressure(const Geo& g, double* u, double* v, double* w, hipStream_t s);
 // main.f90:137-146,156-164: f += dt*r (first) or f += dt*(1.5 r - 0.5 ro); ro = r.
-void launch_advance(const Geo& g, double* f, const double* r, double* ro, double dt, bool first,
-                    hipStream_t s);
+// dt is *dtp when dtp is not null (a device value, so a captured graph
+// replays with the current step size).
+void launch_advance(const Geo& g, double* f, const double* r, double* ro, double dt,
+                    const double* dtp, bool first, hipStream_t s);
+// p[0..count) = 0 by a kernel.
+void launch_fill_zero(double* p, int64_t count, hipStream_t s);
 // rhs_velocity.f90:31-32,56-72: max(|u|+|v|+|w|) into *maxout (per-block
 // maxima staged in blockmax, >= 65536 doubles) and the six products u_i u_j
 // (written to out[0..5], which may alias u, v, w).

@@ -66,6 +66,19 @@ struct Lib {
     std::vector<float*> W32;
 
     bool fuse = true;  // hit3d_step uses the fused pipeline where it applies
+
+    // hit3d_step replayed from captured HIP graphs (graph mode).  A capture is
+    // valid for the slot pointers it started from (the out-of-place passes
+    // swap them) and the step's flags; it records the pointers it leaves.
+    struct StepGraph {
+        std::vector<void*> before, after;
+        unsigned flags = 0;
+        hipGraphExec_t exec = nullptr;
+    };
+    bool graph_mode = true;
+    int64_t graph_replays = 0;  // steps run from a captured graph
+    std::vector<StepGraph> graphs;
+    std::vector<std::pair<std::vector<void*>, unsigned>> graph_seen;  // run once eagerly first
     int les_model = 0;       // 0 none, 1 Smagorinsky (m_les.f90:94-104)
     double c_smag = 0.18;    // m_les.f90:1148
     double* tv = nullptr;    // turb_visc, physical slot layout
@@ -125,7 +138,10 @@ void fft_forward(double*& slot, int ph = 0);
 void fft_inverse(double*& slot);
 
 // fused FP64 / mixed step (step.hip)
-void fused_step(double dt, bool first, bool mixed, double* courant_max);
+// Enqueues one fused step (2/3 rule): RHS, forcing, update and projection;
+// the step size is read from device memory at dtp; the Courant max and the
+// band energy land in dscal[0], dscal[1].  No host synchronisation.
+void fused_step(const double* dtp, bool first, bool mixed);
 bool fused_step_supported(bool mixed);
 
 }  // namespace hit3d

@@ -271,7 +271,8 @@ struct Upd {
     double* R[3];
     const double* eshell;  // band energy per shell 1..kfmax (device)
     double* energy_out;
-    double dt, nu, famp;
+    const double* dtp;  // the step size (device value)
+    double nu, famp;
     int kfmax, forced, first;
 };
 
@@ -356,6 +357,7 @@ __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_fwd_z_update(SG g, W6<T
         if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *u.energy_out = en;
     }
     if (!valid) return;
+    const double dt = *u.dtp;
     const double kxd = kx, kyd = ky;
     // The slots in NCH chunks; in each, every fields / rhs_old load is issued
     // before the first store (the stores may alias later loads as far as the
@@ -399,11 +401,11 @@ __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_fwd_z_update(SG g, W6<T
                 cpx<double> r = {-tim + lap * fc.x, tre + lap * fc.y};
                 if (fnode) r = {r.x + fac * fc.x, r.y + fac * fc.y};
                 if (u.first) {
-                    fn[cc] = {fc.x + u.dt * r.x, fc.y + u.dt * r.y};
+                    fn[cc] = {fc.x + dt * r.x, fc.y + dt * r.y};
                 } else {
                     const cpx<double> oc = o[cc][q];
-                    fn[cc] = {fc.x + u.dt * (1.5 * r.x - 0.5 * oc.x),
-                              fc.y + u.dt * (1.5 * r.y - 0.5 * oc.y)};
+                    fn[cc] = {fc.x + dt * (1.5 * r.x - 0.5 * oc.x),
+                              fc.y + dt * (1.5 * r.y - 0.5 * oc.y)};
                 }
                 reinterpret_cast<cpx<double>*>(u.R[cc])[off] = r;
             }
@@ -435,7 +437,7 @@ static void dispatch_step(int n, F&& f) {
 }
 
 template <int N, typename T>
-static void run(Lib& L, double dt, bool first, const cpx<T>* tw) {
+static void run(Lib& L, const double* dtp, bool first, const cpx<T>* tw) {
     if (L.kmax != Kc<N>::K) throw std::runtime_error("the fused step expects kmax = n/3");
     SG g;
     g.K = L.kmax;
@@ -485,7 +487,7 @@ static void run(Lib& L, double dt, bool first, const cpx<T>* tw) {
         }
         u.eshell = L.dshell;
         u.energy_out = L.dscal + 1;
-        u.dt = dt;
+        u.dtp = dtp;
         u.nu = L.nu;
         u.famp = L.famp;
         u.kfmax = L.kfmax;
@@ -503,20 +505,12 @@ bool fused_step_supported(bool) {
     return L.dealias == 0 && L.nles == 0 && L.n >= 16 && L.kmax >= 1;
 }
 
-void fused_step(double dt, bool first, bool mixed, double* courant_max) {
+void fused_step(const double* dtp, bool first, bool mixed) {
     Lib& L = lib();
     if (mixed)
-        dispatch_step(L.n, [&](auto NC) { run<decltype(NC)::value, float>(L, dt, first, L.tw32); });
+        dispatch_step(L.n, [&](auto NC) { run<decltype(NC)::value, float>(L, dtp, first, L.tw32); });
     else
-        dispatch_step(L.n, [&](auto NC) { run<decltype(NC)::value, double>(L, dt, first, L.tw64); });
-    check(hipMemcpyAsync(L.hpin, L.dscal, 2 * sizeof(double), hipMemcpyDeviceToHost, L.st),
-          "hipMemcpyAsync");
-    check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
-    uint64_t bits;
-    __builtin_memcpy(&bits, L.hpin, sizeof(bits));
-    double m;
-    __builtin_memcpy(&m, &bits, sizeof(m));
-    *courant_max = m;
+        dispatch_step(L.n, [&](auto NC) { run<decltype(NC)::value, double>(L, dtp, first, L.tw64); });
 }
 
 }  // namespace hit3d

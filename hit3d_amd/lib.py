@@ -43,6 +43,8 @@ _SIGNATURES = {
     "hit3d_pressure": [],
     "hit3d_step": [_F64P, _I64P, _I64P, _I64P, _I64P, _F64P, _F64P],
     "hit3d_set_fusion": [_I64P],
+    "hit3d_set_graphs": [_I64P],
+    "hit3d_graph_stats": [_I64P, _I64P],
     "hit3d_mlp_set_params": [ctypes.POINTER(ctypes.c_float)],
     "hit3d_mlp_get_params": [ctypes.POINTER(ctypes.c_float)],
     "hit3d_mlp_set_norm": [_F64P] * 4,
@@ -272,6 +274,16 @@ class Hit3D:
 
     def set_fusion(self, on: bool):
         _check(self.lib.hit3d_set_fusion(_i(1 if on else 0)))
+
+    def set_graphs(self, on: bool):
+        """hit3d_step from captured HIP graphs (default on) or direct launches."""
+        _check(self.lib.hit3d_set_graphs(_i(1 if on else 0)))
+
+    def graph_stats(self):
+        """(step graphs held, steps run from a graph so far)."""
+        c, r = ctypes.c_int64(0), ctypes.c_int64(0)
+        _check(self.lib.hit3d_graph_stats(ctypes.byref(c), ctypes.byref(r)))
+        return c.value, r.value
 
     def stats(self):
         """(e_spec[1..kmax], enstrophy) -- m_stats.f90:195-289."""

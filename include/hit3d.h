@@ -169,6 +169,16 @@ int hit3d_sgs_features(const double *delta, float *X);
  * checks).  The mixed form needs the fused pipeline. */
 int hit3d_set_fusion(const int64_t *on);
 
+/* Selects whether hit3d_step replays captured HIP graphs: *on != 0
+ * (default) captures a step's kernels once per combination of slot pointers
+ * and step flags (after running that combination once directly) and replays
+ * it, cutting the launch cost of small grids; 0 launches every kernel
+ * directly.  Results are identical.  Steps with timing enabled or with
+ * les_model 11 always launch directly.  No reference counterpart. */
+int hit3d_set_graphs(const int64_t *on);
+/* Number of step graphs held now and of steps run from one so far. */
+int hit3d_graph_stats(int64_t *captured, int64_t *replays);
+
 /* Statistics of m_stats.f90:195-289: e_spec(1:kmax) (FP64 shell sums) and
  * the mean enstrophy <|omega|^2>. */
 int hit3d_stats(double *e_spec, double *enstrophy);
