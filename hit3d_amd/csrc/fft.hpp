@@ -184,20 +184,22 @@ __device__ __forceinline__ void lds_sync() {
 // TWREC: the R-1 twiddles of a radix-8/4/2 butterfly come from one table load
 // w and its powers w^2..w^7 by complex products (a few ulp of extra rounding,
 // seven loads fewer); used where twiddle loads dominate the wait time.
+// TW: precision of the twiddle table and of that recurrence (FP64 for FP32
+// data keeps the twiddles within an ulp of FP32).
 template <int N, int E, int DIR, typename T, int LS, int NF, int FS, int PADSH, bool WAVE,
-          int Pspan, bool TWREC = false>
+          int Pspan, bool TWREC = false, typename TW = T>
 struct Stages {
     static constexpr int R = (E < N / Pspan) ? E : N / Pspan;
     static constexpr int TPF = N / E;
     static constexpr int NB = E / R;  // butterflies per thread
     __device__ static __forceinline__ void run(cpx<T> (&v)[NF][E], int t, cpx<T>* lds,
-                                               const cpx<T>* __restrict__ tw) {
+                                               const cpx<TW>* __restrict__ tw) {
 #pragma unroll
         for (int q = 0; q < NB; ++q) {
             const int b = t + q * TPF;
             const int k = b & (Pspan - 1);
             if constexpr (Pspan > 1 && TWREC && (R == 8 || R == 4 || R == 2)) {
-                cpx<T> w[R];
+                cpx<TW> w[R];
                 w[1] = ldtw(tw, k * (N / (Pspan * R)));
                 if constexpr (DIR > 0) w[1].y = -w[1].y;
                 if constexpr (R >= 4) {
@@ -211,13 +213,16 @@ struct Stages {
                     w[7] = cmul(w[6], w[1]);
                 }
 #pragma unroll
-                for (int m = 1; m < R; ++m)
+                for (int m = 1; m < R; ++m) {
+                    const cpx<T> wm{(T)w[m].x, (T)w[m].y};
 #pragma unroll
-                    for (int f = 0; f < NF; ++f) v[f][q + m * NB] = cmul(v[f][q + m * NB], w[m]);
+                    for (int f = 0; f < NF; ++f) v[f][q + m * NB] = cmul(v[f][q + m * NB], wm);
+                }
             } else if constexpr (Pspan > 1) {
 #pragma unroll
                 for (int m = 1; m < R; ++m) {
-                    cpx<T> w = ldtw(tw, m * k * (N / (Pspan * R)));
+                    const cpx<TW> wt = ldtw(tw, m * k * (N / (Pspan * R)));
+                    cpx<T> w{(T)wt.x, (T)wt.y};
                     if constexpr (DIR > 0) w.y = -w.y;
 #pragma unroll
                     for (int f = 0; f < NF; ++f) v[f][q + m * NB] = cmul(v[f][q + m * NB], w);
@@ -244,7 +249,7 @@ struct Stages {
             for (int s = 0; s < E; ++s)
 #pragma unroll
                 for (int f = 0; f < NF; ++f) v[f][s] = lds[f * FS + padi<PADSH>((t + s * TPF) * LS)];
-            Stages<N, E, DIR, T, LS, NF, FS, PADSH, WAVE, Pspan * R, TWREC>::run(v, t, lds, tw);
+            Stages<N, E, DIR, T, LS, NF, FS, PADSH, WAVE, Pspan * R, TWREC, TW>::run(v, t, lds, tw);
         }
     }
 };
@@ -252,18 +257,19 @@ struct Stages {
 // NF transforms at once.  Every thread taking part in the exchange must call
 // this (it synchronises), also threads whose data is masked out.
 template <int N, int E, int DIR, typename T, int LS, int NF, int FS, int PADSH, bool WAVE,
-          bool TWREC = false>
+          bool TWREC = false, typename TW = T>
 __device__ __forceinline__ void multi_fft(cpx<T> (&v)[NF][E], int t, cpx<T>* lds,
-                                          const cpx<T>* __restrict__ tw) {
-    Stages<N, E, DIR, T, LS, NF, FS, PADSH, WAVE, 1, TWREC>::run(v, t, lds, tw);
+                                          const cpx<TW>* __restrict__ tw) {
+    Stages<N, E, DIR, T, LS, NF, FS, PADSH, WAVE, 1, TWREC, TW>::run(v, t, lds, tw);
 }
 
 // One transform with block-wide synchronisation; element j at lds[padi(j*LS)].
-template <int N, int E, int DIR, typename T, int LS, int PADSH = 0, bool TWREC = false>
+template <int N, int E, int DIR, typename T, int LS, int PADSH = 0, bool TWREC = false,
+          typename TW = T>
 __device__ __forceinline__ void block_fft(cpx<T> (&v)[E], int t, cpx<T>* lds,
-                                          const cpx<T>* __restrict__ tw) {
+                                          const cpx<TW>* __restrict__ tw) {
     cpx<T>(&vv)[1][E] = *reinterpret_cast<cpx<T>(*)[1][E]>(&v);
-    Stages<N, E, DIR, T, LS, 1, 0, PADSH, false, 1, TWREC>::run(vv, t, lds, tw);
+    Stages<N, E, DIR, T, LS, 1, 0, PADSH, false, 1, TWREC, TW>::run(vv, t, lds, tw);
 }
 
 }  // namespace hit3d

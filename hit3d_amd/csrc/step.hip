@@ -153,7 +153,8 @@ __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_y(SG g, W6<T> A,
 // lane t, slot s (k = t + s*TPF); N-k sits in lane (TPF-t) mod TPF, slot
 // E-1-s (slot (E-s) mod E for t = 0), so the split is a lane shuffle.  One
 // LDS row of N entries per row for the transform exchanges, synchronised per
-// wavefront; twiddles by recurrence from one table load per butterfly.
+// wavefront; twiddles by recurrence from one FP64 table load per butterfly
+// (computed in FP64 for the FP32 form as well, then rounded once).
 template <int N, typename T>
 __device__ __forceinline__ cpx<T> herm_load(const cpx<T>* rowp, int k, int nk) {
     // entry k of the Hermitian-extended half spectrum (zero outside kx < nk)
@@ -168,8 +169,8 @@ __device__ __forceinline__ cpx<T> herm_load(const cpx<T>* rowp, int k, int nk) {
 
 template <int N, typename T>
 __global__ __launch_bounds__((XP<N, T>::THREADS)) void k_xp(SG g, W6<T> Wb,
-                                                          const cpx<T>* __restrict__ tw, T norm,
-                                                          double* __restrict__ blockmax) {
+                                                          const cpx<double>* __restrict__ twd,
+                                                          T norm, double* __restrict__ blockmax) {
     using C = XP<N, T>;
     constexpr int E = C::E, TPF = C::TPF;
     constexpr int PS = PadFor<T>::value;
@@ -196,7 +197,7 @@ __global__ __launch_bounds__((XP<N, T>::THREADS)) void k_xp(SG g, W6<T> Wb,
             const cpx<T> a = herm_load<N>(U, k, NK), b = herm_load<N>(V, k, NK);
             z[0][s] = cpx<T>{a.x - b.y, a.y + b.x};
         }
-        multi_fft<N, E, 1, T, 1, 1, FS, PS, WAVE, true>(z, t, L, tw + N);
+        multi_fft<N, E, 1, T, 1, 1, FS, PS, WAVE, true, double>(z, t, L, twd + N);
 #pragma unroll
         for (int s = 0; s < E; ++s) {
             u[s] = z[0][s].x * norm;
@@ -209,7 +210,7 @@ __global__ __launch_bounds__((XP<N, T>::THREADS)) void k_xp(SG g, W6<T> Wb,
 #pragma unroll
         for (int s = 0; s < E; ++s) z[0][s] = herm_load<N>(Wr, t + s * TPF, NK);
         lds_sync<WAVE>();  // the previous transform's readers are done
-        multi_fft<N, E, 1, T, 1, 1, FS, PS, WAVE, true>(z, t, L, tw + N);
+        multi_fft<N, E, 1, T, 1, 1, FS, PS, WAVE, true, double>(z, t, L, twd + N);
 #pragma unroll
         for (int s = 0; s < E; ++s) w[s] = z[0][s].x * norm;
     }
@@ -231,7 +232,7 @@ __global__ __launch_bounds__((XP<N, T>::THREADS)) void k_xp(SG g, W6<T> Wb,
             if (pr == 2) z[0][s] = cpx<T>{v[s] * w[s], w[s] * w[s]};
         }
         lds_sync<WAVE>();
-        multi_fft<N, E, -1, T, 1, 1, FS, PS, WAVE, true>(z, t, L, tw + N);
+        multi_fft<N, E, -1, T, 1, 1, FS, PS, WAVE, true, double>(z, t, L, twd + N);
         cpx<T>* outp = reinterpret_cast<cpx<T>*>(Wb.p[2 * pr]) + row * g.PA;
         cpx<T>* outq = reinterpret_cast<cpx<T>*>(Wb.p[2 * pr + 1]) + row * g.PA;
 #pragma unroll
@@ -277,8 +278,8 @@ struct Upd {
 };
 
 template <int N, typename T>
-__global__ __launch_bounds__((CE<N, T>::THREADS)) void k_fwd_z_update(SG g, W6<T> Q, Upd u,
-                                                                    const cpx<T>* __restrict__ tw) {
+__global__ __launch_bounds__((CE<N, T>::THREADS)) void k_fwd_z_update(
+    SG g, W6<T> Q, Upd u, const cpx<double>* __restrict__ twd) {
     constexpr int E = CE<N, T>::E, B = CE<N, T>::B, TPF = CE<N, T>::TPF;
     constexpr int PS = 0;
     __shared__ cpx<T> lds[padded_size<PS>(N * B)];
@@ -320,12 +321,14 @@ __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_fwd_z_update(SG g, W6<T
         } else {
             load(p, v);
         }
-        // twiddles by recurrence from a pointer the compiler cannot see is
-        // loop-invariant: they are recomputed per product instead of holding
-        // fourteen complex values in registers across the loop
-        const cpx<T>* twp = tw + N;
+        // Twiddles by recurrence from one FP64 table load per butterfly (in
+        // FP64 for the FP32 form too, then rounded once), from a pointer the
+        // compiler cannot see is loop-invariant, so they are recomputed per
+        // product instead of holding fourteen complex values in registers
+        // across the loop.
+        const cpx<double>* twp = twd + N;
         asm volatile("" : "+s"(twp));
-        block_fft<N, E, -1, T, B, PS, true>(v, t, lds + c, twp);
+        block_fft<N, E, -1, T, B, PS, true, double>(v, t, lds + c, twp);
 #pragma unroll
         for (int s = 0; s < E; ++s) {
             if (!slot_live<N, TPF>(s) || !kept(t + s * TPF, Kc<N>::K, N)) continue;  // never written out
@@ -469,8 +472,9 @@ static void run(Lib& L, const double* dtp, bool first, const cpx<T>* tw) {
         Timed tm(TC_STEP_X);
         const T norm = (T)(1.0 / ((double)N * (double)N * (double)N));
         const int64_t nb = (int64_t)N * N / XP<N, T>::ROWS;  // <= n*n, the size of dmax
-        hipLaunchKernelGGL((k_xp<N, T>), dim3(nb), dim3(XP<N, T>::THREADS), 0, L.st, g, Wb, tw,
-                           norm, L.dmax);
+        // FP64 twiddles in both forms (the FP32 form's recurrence in FP64)
+        hipLaunchKernelGGL((k_xp<N, T>), dim3(nb), dim3(XP<N, T>::THREADS), 0, L.st, g, Wb,
+                           L.tw64, norm, L.dmax);
         launch_max_fold(L.dmax, nb, L.dscal, L.st);
     }
     {
@@ -494,7 +498,7 @@ static void run(Lib& L, const double* dtp, bool first, const cpx<T>* tw) {
         u.forced = forced ? 1 : 0;
         u.first = first ? 1 : 0;
         hipLaunchKernelGGL((k_fwd_z_update<N, T>), dim3(ncb, nky), dim3(CE<N, T>::THREADS), 0,
-                           L.st, g, Wb, u, tw);
+                           L.st, g, Wb, u, L.tw64);
     }
 }
 

@@ -1,0 +1,149 @@
+"""GPU checks at the benchmark's full size (512^3) through properties that do
+not depend on the grid size: the transform's round trip, linearity, Parseval
+and a single-mode known answer; and, for the fused step, a divergence-free
+2/3-truncated result, the shell spectrum against the oracle's shell sums of
+the same fields, fused vs unfused agreement and the mixed form's single-RHS
+bound.
+
+Tolerances: round trip 1e-14, linearity and the known answer 1e-13,
+Parseval and shell sums 1e-12 (relative); divergence |k.u| <= 1e-12 |k||u|
+elementwise scale; truncation exact; fused vs unfused 1e-13; mixed single
+RHS 1e-5 (the stated bound of the mixed form).
+"""
+import numpy as np
+import pytest
+
+import hit3d_amd
+from hit3d_amd import FIELDS, RHS_OLD, WRK
+
+pytestmark = pytest.mark.gpu
+
+N = 512
+RN1 = 67748937.0
+
+
+def note(msg):
+    print(msg, flush=True)
+
+
+def rel(a, b):
+    d = np.abs(np.asarray(a) - np.asarray(b)).max()
+    s = np.abs(np.asarray(b)).max()
+    return d / s if s > 0 else d
+
+
+def spectral(a):
+    """(ky, kz, kx re/im) slot -> complex (ky, kz, kx)."""
+    return a[..., 0::2] + 1j * a[..., 1::2]
+
+
+def kgrid(n):
+    k = np.fft.fftfreq(n, 1.0 / n)
+    return k[:, None, None], k[None, :, None], np.arange(n // 2 + 1)[None, None, :]
+
+
+@pytest.fixture(scope="module")
+def h():
+    lib = hit3d_amd.Hit3D(N, dealias=0)
+    yield lib
+    lib.close()
+
+
+def test_fft_roundtrip_linearity_parseval_512(h):
+    note("fft properties n=512")
+    rng = np.random.default_rng(512)
+    a = np.zeros((N, N, N + 2))
+    a[:, :, :N] = rng.uniform(-1, 1, (N, N, N))
+    b = np.zeros_like(a)
+    b[:, :, :N] = rng.uniform(-1, 1, (N, N, N))
+    h.put(WRK, 1, a)
+    h.put(WRK, 2, b)
+    h.put(WRK, 3, a + 2.0 * b)
+    for s in (1, 2, 3):
+        h.xFFT3d(1, s)
+    fa, fb, fab = (h.get(WRK, s) for s in (1, 2, 3))
+    assert rel(fab, fa + 2.0 * fb) < 1e-13
+    # Parseval for the unnormalised forward transform: sum u^2 = sum |u_k|^2 / N^3,
+    # the half spectrum counting kx = 1 .. N/2-1 twice
+    c = spectral(fa)
+    w = np.full(N // 2 + 1, 2.0)
+    w[0] = w[-1] = 1.0
+    energy_k = np.sum(w * np.abs(c) ** 2) / N ** 3
+    energy_x = np.sum(a[:, :, :N] ** 2)
+    assert abs(energy_k - energy_x) < 1e-12 * energy_x
+    h.xFFT3d(-1, 1)
+    assert rel(h.get(WRK, 1)[:, :, :N], a[:, :, :N]) < 1e-14
+
+
+def test_fft_single_mode_512(h):
+    note("fft single mode n=512")
+    kx, ky, kz, amp = 37, -170, 101, 0.75
+    x = 2 * np.pi * np.arange(N) / N
+    # u(x, y, z) = amp cos(kx x + ky y + kz z), array index (z, y, x)
+    ph = (kz * x)[:, None, None] + (ky * x)[None, :, None] + (kx * x)[None, None, :]
+    a = np.zeros((N, N, N + 2))
+    a[:, :, :N] = amp * np.cos(ph)
+    h.put(WRK, 1, a)
+    h.xFFT3d(1, 1)
+    c = spectral(h.get(WRK, 1))
+    # forward sign -1, unnormalised: coefficient N^3 amp / 2 at (ky, kz, kx)
+    j, k = kz % N, ky % N
+    assert abs(c[k, j, kx] - N ** 3 * amp / 2) < 1e-9 * N ** 3
+    c[k, j, kx] = 0.0
+    assert np.abs(c).max() < 1e-11 * N ** 3
+
+
+def test_fused_step_512_invariants(h, oracle):
+    note("fused step invariants n=512")
+    h.set_physics(1 / 800.0, flow_type=1, kfmax=2, famp=0.5)
+    h.init_velocity(RN1, 2, 4, 3.0)
+    h.dealias_all()
+    f0 = np.stack([h.get(FIELDS, c) for c in (1, 2, 3)])
+    res = {}
+    for fused in (True, False):
+        for c in range(3):
+            h.put(FIELDS, c + 1, f0[c])
+        h.set_fusion(fused)
+        fov = True
+        for _ in range(2):
+            cr, en, fov, _ = h.step(5e-4, fov, True)
+        res[fused] = (cr, en, np.stack([h.get(FIELDS, c) for c in (1, 2, 3)]),
+                      np.stack([h.get(RHS_OLD, c) for c in (1, 2, 3)]))
+    h.set_fusion(True)
+    (c1, e1, f1, r1), (c2, e2, f2, r2) = res[True], res[False]
+    note(f"fused vs unfused: fields {rel(f1, f2):.2e}, rhs_old {rel(r1, r2):.2e}")
+    assert abs(c1 - c2) <= 1e-13 * c2 and abs(e1 - e2) <= 1e-13 * e2
+    assert rel(f1, f2) < 1e-13 and rel(r1, r2) < 1e-13
+    # divergence-free and 2/3-truncated
+    u = [spectral(f1[c]) for c in range(3)]
+    ky, kz, kx = kgrid(N)
+    div = np.abs(kx * u[0] + ky * u[1] + kz * u[2])
+    kmag = np.sqrt(kx ** 2 + ky ** 2 + kz ** 2)
+    umag = np.sqrt(sum(np.abs(v) ** 2 for v in u))
+    assert div.max() <= 1e-12 * (kmag * umag).max()
+    K = N // 3
+    outside = (np.abs(kx) > K) | (np.abs(ky) > K) | (np.abs(kz) > K)
+    outside = np.broadcast_to(outside, u[0].shape)
+    for v in u:
+        assert not np.any(v[outside])
+    # E(k) from the device's shell sums vs the oracle's on the same fields
+    e_dev, _ = h.stats()
+    e_ora = oracle.shell_spectrum(f1, N, K)[1:]
+    assert rel(e_dev, e_ora) < 1e-12
+
+
+def test_mixed_single_rhs_512(h):
+    note("mixed single rhs n=512")
+    h.set_physics(1 / 800.0, flow_type=1, kfmax=2, famp=0.5)
+    h.init_velocity(RN1, 2, 4, 3.0)
+    h.dealias_all()
+    f0 = np.stack([h.get(FIELDS, c) for c in (1, 2, 3)])
+    rhs = {}
+    for prec in (0, 1):
+        for c in range(3):
+            h.put(FIELDS, c + 1, f0[c])
+        h.step(1e-3, True, True, precision=prec)
+        rhs[prec] = np.stack([h.get(RHS_OLD, c) for c in (1, 2, 3)])
+    err = rel(rhs[1], rhs[0])
+    note(f"mixed single-RHS rel err at 512^3: {err:.3e}")
+    assert 1e-12 < err < 1e-5
