@@ -103,7 +103,8 @@ static void fft_forward_product(const double* a, const double* b, double*& dst, 
 // accumulated into the RHS as rhs_j += -1/2 i k_axj (a*b)^ for the given
 // targets instead of being stored; `work` is overwritten (x and y passes).
 static void fft_forward_product_acc(const double* a, const double* b, double* work, double* r0,
-                                    int ax0, double* r1 = nullptr, int ax1 = 0) {
+                                    int ax0, bool mask0, double* r1 = nullptr, int ax1 = 0,
+                                    bool mask1 = false) {
     Lib& L = lib();
     const int64_t n = L.n, P = L.g.P;
     {
@@ -116,6 +117,8 @@ static void fft_forward_product_acc(const double* a, const double* b, double* wo
     acc.ax[0] = ax0;
     acc.r[1] = reinterpret_cast<cpx<double>*>(r1);
     acc.ax[1] = ax1;
+    acc.mask[0] = mask0;
+    acc.mask[1] = mask1;
     Timed t(TC_FFT_Z);
     launch_cols<double>(L.n, -1, reinterpret_cast<cpx<double>*>(work),
                         reinterpret_cast<cpx<double>*>(L.spare), P, n * P, n * P, P, L.tw64, L.st,
@@ -392,6 +395,9 @@ static void do_rhs_scalars(bool int_scalars) {
     const Geo& g = L.g;
     if (L.nles > 0) throw std::runtime_error("LES scalars are not supported");
     if (L.les_model == 1) les_turb_visc();  // rhs_scalars.f90:18
+    // scalar s of type 0 has the mean-gradient source -u (rhs_scalars.f90:387-389),
+    // subtracted inside the kernel that completes its RHS
+    auto source = [&](int s) -> const double* { return L.stype[s - 1] == 0 ? F[1] : nullptr; };
     if (!int_scalars || L.ns == 0) {  // rhs_scalars.f90:32-39
         for (int c = 1; c <= 3; ++c) fft_inverse_from(F[c], W[c], 0);
         return;
@@ -405,7 +411,7 @@ static void do_rhs_scalars(bool int_scalars) {
             const double* Q[3] = {W[s + 3], W[s + 4], W[s + 5]};
             {
                 Timed t(TC_POINTWISE);
-                launch_scalar_rhs(g, Q, F[3 + s], W[3 + s], L.pe[s - 1], L.st);
+                launch_scalar_rhs(g, Q, F[3 + s], W[3 + s], L.pe[s - 1], source(s), L.st);
             }
             if (L.stype[s - 1] >= 100) {  // add_reaction + dealias_rhs, :124-130
                 {
@@ -430,7 +436,7 @@ static void do_rhs_scalars(bool int_scalars) {
             fft_forward_product(W[c], W[3], W[c], -1);
             const double* Q[3] = {W[n1], W[n2], W[c]};
             Timed t(TC_POINTWISE);
-            launch_scalar_rhs_ps(g, Q, F[c], W[c], 0.0, false, L.st);
+            launch_scalar_rhs_ps(g, Q, F[c], W[c], 0.0, false, nullptr, L.st);
         }
         for (int c = 1; c <= 3; ++c) fft_inverse_from(F[c], W[c], 0);
         for (int c = 4; c <= 3 + L.ns; ++c) {
@@ -440,7 +446,7 @@ static void do_rhs_scalars(bool int_scalars) {
             fft_forward_product(W[0], W[3], W[0], 0);
             const double* Q[3] = {W[n1], W[n2], W[0]};
             Timed t(TC_POINTWISE);
-            launch_scalar_rhs_ps(g, Q, F[c], W[c], L.pe[c - 4], true, L.st);
+            launch_scalar_rhs_ps(g, Q, F[c], W[c], L.pe[c - 4], true, source(c - 3), L.st);
         }
         for (int c = 4; c <= 3 + L.ns; ++c) {  // reaction_rates, rhs_scalars.f90:280-380
             if (L.stype[c - 4] <= 300) continue;
@@ -457,12 +463,6 @@ static void do_rhs_scalars(bool int_scalars) {
             fft_forward(W[n2]);
             Timed t(TC_POINTWISE);
             launch_add(g, W[c], W[n2], false, L.st);
-        }
-    }
-    for (int s = 1; s <= L.ns; ++s) {  // rhs_scalars.f90:387-389
-        if (L.stype[s - 1] == 0) {
-            Timed t(TC_POINTWISE);
-            launch_sub(g, W[3 + s], F[1], L.st);
         }
     }
     if (L.les_model == 1)  // rhs_scalars.f90:396-398
@@ -505,15 +505,15 @@ static void do_rhs_velocity() {
         // and the half-RHS accumulation in the last forward pass's epilogue
         fft_inverse_from(F[1], W[4], +1);
         fft_inverse_from(F[2], W[5], +1);
-        fft_forward_product_acc(W[4], W[4], W[6], W[1], 1);
-        fft_forward_product_acc(W[4], W[5], W[6], W[1], 2, W[2], 1);
-        fft_forward_product_acc(W[5], W[5], W[6], W[2], 2);
+        // (the last accumulation into each component applies the mask of
+        // rhs_velocity.f90:400-404)
+        fft_forward_product_acc(W[4], W[4], W[6], W[1], 1, false);
+        fft_forward_product_acc(W[4], W[5], W[6], W[1], 2, false, W[2], 1, false);
+        fft_forward_product_acc(W[5], W[5], W[6], W[2], 2, false);
         fft_inverse_from(F[3], W[6], +1);
-        fft_forward_product_acc(W[4], W[6], W[4], W[1], 3, W[3], 1);
-        fft_forward_product_acc(W[5], W[6], W[5], W[2], 3, W[3], 2);
-        fft_forward_product_acc(W[6], W[6], W[6], W[3], 3);
-        Timed t(TC_POINTWISE);
-        for (int c = 1; c <= 3; ++c) launch_mask_ps(g, W[c], L.st);
+        fft_forward_product_acc(W[4], W[6], W[4], W[1], 3, true, W[3], 1, false);
+        fft_forward_product_acc(W[5], W[6], W[5], W[2], 3, true, W[3], 2, false);
+        fft_forward_product_acc(W[6], W[6], W[6], W[3], 3, true);
     }
     if (L.les_model == 1 || L.les_model == 11) les_velocity();  // rhs_velocity.f90:414-416
 }
@@ -929,8 +929,13 @@ static void step_body(bool fused, bool mixed, bool scal, bool first_v, bool firs
     if (scal) do_advance(2, first_s, 0.0, dtp);
     do_rhs_velocity();
     if (L.flow_type == 1) force_enqueue();
-    do_advance(1, first_v, 0.0, dtp);
-    do_pressure();
+    {  // main.f90:156-164 and pressure.f90:13-47 in one pass
+        Timed t(TC_POINTWISE);
+        double* f[3] = {L.F[1], L.F[2], L.F[3]};
+        const double* r[3] = {L.W[1], L.W[2], L.W[3]};
+        double* ro[3] = {L.R[1], L.R[2], L.R[3]};
+        launch_advance_project(L.g, f, r, ro, 0.0, dtp, first_v, L.st);
+    }
 }
 
 static std::vector<void*> slot_state() {

@@ -96,8 +96,12 @@ __global__ __launch_bounds__(B* N / E) void k_cols(const cpx<T>* __restrict__ in
                     const int ax = acc.ax[j];
                     const double k = ax == 1 ? (double)col
                                              : (double)wavenum(ax == 2 ? (int)a : t + s * TPF, N);
-                    acc.r[j][off] = {r[j][q].x + 0.5 * k * (double)v[s].y,
-                                     r[j][q].y - 0.5 * k * (double)v[s].x};
+                    cpx<double> val = {r[j][q].x + 0.5 * k * (double)v[s].y,
+                                       r[j][q].y - 0.5 * k * (double)v[s].x};
+                    if (acc.mask[j] &&
+                        alias_count(col, wavenum((int)a, N), wavenum(t + s * TPF, N), N) >= 2)
+                        val = {0.0, 0.0};
+                    acc.r[j][off] = val;
                 }
             }
         }

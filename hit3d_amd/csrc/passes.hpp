@@ -29,10 +29,13 @@ void launch_x_c2r(int n, T* slot, int64_t P, const cpx<T>* tw, T norm, hipStream
 // theta = (kx+ky+kz) dx/2 from tab (cos, sin per kx+ky+kz+n).  FP64 only.
 // acc (forward pass with ph = -1 only): instead of storing the result v,
 // add -1/2 i k_ax[j] v to each non-null r[j] (same spectral layout as out),
-// ax 1 = kx, 2 = ky, 3 = kz.
+// ax 1 = kx, 2 = ky, 3 = kz; where mask[j], entries with alias_count >= 2
+// are set to zero instead (the phase-shift mask, rhs_velocity.f90:400-404,
+// applied by the last accumulation into a target).
 struct ColAcc {
     cpx<double>* r[2] = {nullptr, nullptr};
     int ax[2] = {0, 0};
+    bool mask[2] = {false, false};
 };
 template <typename T>
 void launch_cols(int n, int dir, const cpx<T>* in, cpx<T>* out, int64_t a_in, int64_t a_out,

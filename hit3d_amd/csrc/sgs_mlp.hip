@@ -328,26 +328,43 @@ __global__ __launch_bounds__(NT) void k_les(Geo g, Ptr9 gr, float delta, MlpNorm
     __shared__ float graw[TS][9];
     __shared__ float tl[TS][NOUT];
     const int64_t total = (int64_t)g.n * g.n * g.n;
+    auto offset = [&](int64_t q) { return (q / g.n) * 2 * g.P + q % g.n; };
+    // Each thread gathers entries e = threadIdx.x + j*NT of a tile, feature
+    // major (feature e / TS of sample e % TS): consecutive lanes read
+    // consecutive grid points of one gradient slot.  The next tile's values
+    // are loaded while this tile goes through the network.
+    static_assert(TS * KIN == 2 * NT, "two gathered entries per thread");
+    auto gather = [&](int64_t first, float (&raw)[2]) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int e = threadIdx.x + j * NT, c = e / TS;
+            const int64_t q = first + e % TS;
+            raw[j] = (c < 9 && q < total) ? (float)gr.p[c][offset(q)] : 0.f;
+        }
+    };
+    float cur[2];
+    gather((int64_t)blockIdx.x * FWD_TILES * TS, cur);
     stage_weights(s, wb, p);
     for (int it = 0; it < FWD_TILES; ++it) {
         const int64_t first = ((int64_t)blockIdx.x * FWD_TILES + it) * TS;
         if (first >= total) break;  // the same for the whole block
         if (threadIdx.x < TS) {
             const int64_t q = first + threadIdx.x;
-            offs[threadIdx.x] = q < total ? (q / g.n) * 2 * g.P + q % g.n : -1;
+            offs[threadIdx.x] = q < total ? offset(q) : -1;
         }
-        __syncthreads();
-        for (int i = threadIdx.x; i < TS * KIN; i += NT) {
-            const int row = i / KIN, c = i % KIN;
-            const int64_t o = offs[row];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int e = threadIdx.x + j * NT, c = e / TS, row = e % TS;
             float x = 0.f;
-            if (o >= 0 && c < NIN) {
-                const float raw = c < 9 ? (float)gr.p[c][o] : delta;
+            if (first + row < total && c < NIN) {
+                const float raw = c < 9 ? cur[j] : delta;
                 if (c < 9) graw[row][c] = raw;
                 x = (raw - nrm.xmean[c]) / nrm.xstd[c];
             }
-            s.xb[i] = c < NIN ? to_bf16(x) : (uint16_t)0;
+            s.xb[row * KIN + c] = c < NIN ? to_bf16(x) : (uint16_t)0;
         }
+        float nxt[2] = {0.f, 0.f};
+        if (it + 1 < FWD_TILES && first + TS < total) gather(first + TS, nxt);
         __syncthreads();
         hidden_layers<false>(s);
         output_layer(s, [&](int i, int n, float y) { tl[i][n] = y * nrm.ystd[n] + nrm.ymean[n]; });
@@ -380,6 +397,8 @@ __global__ __launch_bounds__(NT) void k_les(Geo g, Ptr9 gr, float delta, MlpNorm
             }
         }
         __syncthreads();  // offs, graw, tl, s.xb are rewritten by the next tile
+        cur[0] = nxt[0];
+        cur[1] = nxt[1];
     }
 }
 

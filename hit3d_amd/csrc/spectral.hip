@@ -136,6 +136,45 @@ struct Ptr3 {
     double* p[3];
 };
 
+// AB2/Euler update of the three velocity components followed by the
+// projection, per entry (k_advance then k_pressure without the round trip)
+__global__ void k_advance_project(Geo g, Ptr3 f, CPtr3 r, Ptr3 ro, double dt, const double* dtp,
+                                  int first) {
+    if (dtp) dt = *dtp;
+    GRID_STRIDE(i, nspec(g)) {
+        SIdx s = sdecode(g, i);
+        // every load before the first store (the arrays may alias as far as
+        // the compiler knows)
+        cpx<double> u[3], b[3], o[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            u[c] = C(f.p[c])[s.off];
+            b[c] = C(r.p[c])[s.off];
+            if (!first) o[c] = C(ro.p[c])[s.off];
+        }
+        cpx<double> a[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if (first)
+                a[c] = {u[c].x + dt * b[c].x, u[c].y + dt * b[c].y};
+            else
+                a[c] = {u[c].x + dt * (1.5 * b[c].x - 0.5 * o[c].x),
+                        u[c].y + dt * (1.5 * b[c].y - 0.5 * o[c].y)};
+            C(ro.p[c])[s.off] = b[c];
+        }
+        const double kx = s.kx, ky = s.kyv, kz = s.kzv;
+        const double dre = kx * a[0].x + ky * a[1].x + kz * a[2].x;
+        const double dim = kx * a[0].y + ky * a[1].y + kz * a[2].y;
+        double lap = kx * kx + ky * ky + kz * kz;
+        if (lap == 0.0) lap = 9e20;
+        const double pr = dre / lap, pi = dim / lap;
+        C(f.p[0])[s.off] = {a[0].x - pr * kx, a[0].y - pi * kx};
+        C(f.p[1])[s.off] = {a[1].x - pr * ky, a[1].y - pi * ky};
+        C(f.p[2])[s.off] = {a[2].x - pr * kz, a[2].y - pi * kz};
+    }
+}
+
+
 __global__ void k_products(Geo g, const double* u, const double* v, const double* w, Ptr6 out,
                            double* blockmax) {
     __shared__ double sh[TB / 64];
@@ -217,13 +256,6 @@ __global__ void k_rhs_assemble(Geo g, CPtr6 P, CPtr3 U, Ptr3 out, double nu, int
     }
 }
 
-__global__ void k_mask_ps(Geo g, double* slot) {
-    GRID_STRIDE(i, nspec(g)) {
-        SIdx s = sdecode(g, i);
-        if (alias_count(s.kx, s.kyv, s.kzv, g.n) >= 2) C(slot)[s.off] = {0.0, 0.0};
-    }
-}
-
 __global__ void k_phys_mul(Geo g, const double* a, const double* b, double* out) {
     const int64_t total = (int64_t)g.n * g.n * g.n;
     GRID_STRIDE(i, total) {
@@ -233,24 +265,32 @@ __global__ void k_phys_mul(Geo g, const double* a, const double* b, double* out)
     }
 }
 
-__global__ void k_scalar_rhs(Geo g, CPtr3 Q, const double* F, double* out, double pe) {
+// sub (may be null): subtracted afterwards at every entry (the mean-gradient
+// source, rhs_scalars.f90:387-389)
+__global__ void k_scalar_rhs(Geo g, CPtr3 Q, const double* F, double* out, double pe,
+                             const double* sub) {
     GRID_STRIDE(i, nspec(g)) {
         SIdx s = sdecode(g, i);
-        if (alias_count(s.kx, s.kyv, s.kzv, g.n) > 0) {
-            C(out)[s.off] = {0.0, 0.0};
-            continue;
+        cpx<double> r = {0.0, 0.0};
+        if (alias_count(s.kx, s.kyv, s.kzv, g.n) == 0) {
+            const double kx = s.kx, ky = s.kyv, kz = s.kzv;
+            const cpx<double> a = C(Q.p[0])[s.off], b = C(Q.p[1])[s.off], c = C(Q.p[2])[s.off];
+            const cpx<double> f = C(F)[s.off];
+            const double r1 = kx * a.y + ky * b.y + kz * c.y;
+            const double r2 = kx * a.x + ky * b.x + kz * c.x;
+            const double k2 = kx * kx + ky * ky + kz * kz;
+            r = {r1 - pe * k2 * f.x, -r2 - pe * k2 * f.y};
         }
-        const double kx = s.kx, ky = s.kyv, kz = s.kzv;
-        const cpx<double> a = C(Q.p[0])[s.off], b = C(Q.p[1])[s.off], c = C(Q.p[2])[s.off];
-        const cpx<double> f = C(F)[s.off];
-        const double r1 = kx * a.y + ky * b.y + kz * c.y;
-        const double r2 = kx * a.x + ky * b.x + kz * c.x;
-        const double k2 = kx * kx + ky * ky + kz * kz;
-        C(out)[s.off] = {r1 - pe * k2 * f.x, -r2 - pe * k2 * f.y};
+        if (sub) {
+            const cpx<double> u = C(sub)[s.off];
+            r = {r.x - u.x, r.y - u.y};
+        }
+        C(out)[s.off] = r;
     }
 }
 
-__global__ void k_scalar_rhs_ps(Geo g, CPtr3 Q, const double* F, double* out, double pe, int acc) {
+__global__ void k_scalar_rhs_ps(Geo g, CPtr3 Q, const double* F, double* out, double pe, int acc,
+                                const double* sub) {
     GRID_STRIDE(i, nspec(g)) {
         SIdx s = sdecode(g, i);
         const bool keep = alias_count(s.kx, s.kyv, s.kzv, g.n) <= 1;
@@ -260,19 +300,19 @@ __global__ void k_scalar_rhs_ps(Geo g, CPtr3 Q, const double* F, double* out, do
         const double r2 = kx * a.x + ky * b.x + kz * c.x;
         if (!acc) {
             C(out)[s.off] = keep ? cpx<double>{0.5 * r1, -0.5 * r2} : cpx<double>{0.0, 0.0};
-        } else if (keep) {
-            const cpx<double> f = C(F)[s.off], o = C(out)[s.off];
-            const double k2 = kx * kx + ky * ky + kz * kz;
-            C(out)[s.off] = {o.x + 0.5 * r1 - pe * k2 * f.x, o.y - 0.5 * r2 - pe * k2 * f.y};
+        } else {
+            cpx<double> o = C(out)[s.off];
+            if (keep) {
+                const cpx<double> f = C(F)[s.off];
+                const double k2 = kx * kx + ky * ky + kz * kz;
+                o = {o.x + 0.5 * r1 - pe * k2 * f.x, o.y - 0.5 * r2 - pe * k2 * f.y};
+            }
+            if (sub) {
+                const cpx<double> u = C(sub)[s.off];
+                o = {o.x - u.x, o.y - u.y};
+            }
+            C(out)[s.off] = o;
         }
-    }
-}
-
-__global__ void k_sub(Geo g, double* out, const double* a) {
-    GRID_STRIDE(i, nspec(g)) {
-        SIdx s = sdecode(g, i);
-        cpx<double> o = C(out)[s.off], b = C(a)[s.off];
-        C(out)[s.off] = {o.x - b.x, o.y - b.y};
     }
 }
 
@@ -442,6 +482,19 @@ void launch_advance(const Geo& g, double* f, const double* r, double* ro, double
     hipLaunchKernelGGL(k_advance, dim3(grid_for(nspec_h(g))), dim3(TB), 0, s, g, f, r, ro, dt, dtp,
                        first ? 1 : 0);
 }
+void launch_advance_project(const Geo& g, double* const f[3], const double* const r[3],
+                            double* const ro[3], double dt, const double* dtp, bool first,
+                            hipStream_t s) {
+    Ptr3 pf, pro;
+    CPtr3 pr;
+    for (int c = 0; c < 3; ++c) {
+        pf.p[c] = f[c];
+        pr.p[c] = r[c];
+        pro.p[c] = ro[c];
+    }
+    hipLaunchKernelGGL(k_advance_project, dim3(grid_for(nspec_h(g))), dim3(TB), 0, s, g, pf, pr, pro,
+                       dt, dtp, first ? 1 : 0);
+}
 void launch_fill_zero(double* p, int64_t count, hipStream_t s) {
     hipLaunchKernelGGL(k_fill_zero, dim3(grid_for(count)), dim3(TB), 0, s, p, count);
 }
@@ -469,27 +522,22 @@ void launch_rhs_assemble(const Geo& g, const double* const P[6], const double* c
     hipLaunchKernelGGL(k_rhs_assemble, dim3(grid_for(nspec_h(g))), dim3(TB), 0, s, g, p, u, o, nu,
                        phase_shift ? 1 : 0);
 }
-void launch_mask_ps(const Geo& g, double* slot, hipStream_t s) {
-    hipLaunchKernelGGL(k_mask_ps, dim3(grid_for(nspec_h(g))), dim3(TB), 0, s, g, slot);
-}
 void launch_phys_mul(const Geo& g, const double* a, const double* b, double* out, hipStream_t s) {
     hipLaunchKernelGGL(k_phys_mul, dim3(grid_for(nphys_h(g))), dim3(TB), 0, s, g, a, b, out);
 }
 void launch_scalar_rhs(const Geo& g, const double* const Q[3], const double* F, double* out,
-                       double pe, hipStream_t s) {
+                       double pe, const double* sub, hipStream_t s) {
     CPtr3 q;
     for (int i = 0; i < 3; ++i) q.p[i] = Q[i];
-    hipLaunchKernelGGL(k_scalar_rhs, dim3(grid_for(nspec_h(g))), dim3(TB), 0, s, g, q, F, out, pe);
+    hipLaunchKernelGGL(k_scalar_rhs, dim3(grid_for(nspec_h(g))), dim3(TB), 0, s, g, q, F, out, pe,
+                       sub);
 }
 void launch_scalar_rhs_ps(const Geo& g, const double* const Q[3], const double* F, double* out,
-                          double pe, bool accumulate, hipStream_t s) {
+                          double pe, bool accumulate, const double* sub, hipStream_t s) {
     CPtr3 q;
     for (int i = 0; i < 3; ++i) q.p[i] = Q[i];
     hipLaunchKernelGGL(k_scalar_rhs_ps, dim3(grid_for(nspec_h(g))), dim3(TB), 0, s, g, q, F, out,
-                       pe, accumulate ? 1 : 0);
-}
-void launch_sub(const Geo& g, double* out, const double* a, hipStream_t s) {
-    hipLaunchKernelGGL(k_sub, dim3(grid_for(nspec_h(g))), dim3(TB), 0, s, g, out, a);
+                       pe, accumulate ? 1 : 0, accumulate ? sub : nullptr);
 }
 void launch_shell_sums(const Geo& g, const double* u, const double* v, const double* w, int s1,
                        int ns, double* out, hipStream_t s) {

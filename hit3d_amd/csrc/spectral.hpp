@@ -23,6 +23,11 @@ void launch_pressure(const Geo& g, double* u, double* v, double* w, hipStream_t 
 // replays with the current step size).
 void launch_advance(const Geo& g, double* f, const double* r, double* ro, double dt,
                     const double* dtp, bool first, hipStream_t s);
+// The velocity update of main.f90:156-164 and then pressure.f90:13-47 in
+// one pass over the three components.
+void launch_advance_project(const Geo& g, double* const f[3], const double* const r[3],
+                            double* const ro[3], double dt, const double* dtp, bool first,
+                            hipStream_t s);
 // p[0..count) = 0 by a kernel.
 void launch_fill_zero(double* p, int64_t count, hipStream_t s);
 // rhs_velocity.f90:31-32,56-72: max(|u|+|v|+|w|) into *maxout (per-block
@@ -37,20 +42,19 @@ void launch_max_fold(const double* part, int64_t count, double* out, hipStream_t
 // mask: alias_count == 0 (2/3 rule) or <= 1 (phase shift).
 void launch_rhs_assemble(const Geo& g, const double* const P[6], const double* const U[3],
                          double* const out[3], double nu, bool phase_shift, hipStream_t s);
-// zero entries with alias_count >= 2 (rhs_velocity.f90:400-404).
-void launch_mask_ps(const Geo& g, double* slot, hipStream_t s);
 // physical out = a * b (x < n only).
 void launch_phys_mul(const Geo& g, const double* a, const double* b, double* out, hipStream_t s);
 // rhs_scalars.f90:88-121: r = mask0 * (-i k.Q - pe k^2 F); Q[3] = (u phi, v phi, w phi)^.
+// sub (may be null): afterwards out -= sub at every entry (the mean-gradient
+// source of rhs_scalars.f90:387-389).
 void launch_scalar_rhs(const Geo& g, const double* const Q[3], const double* F, double* out,
-                       double pe, hipStream_t s);
+                       double pe, const double* sub, hipStream_t s);
 // rhs_scalars.f90:196-219 (phase-shifted half, Q already shifted back):
 // out = mask1 * (-1/2 i k.Q)   and :253-269 (unshifted half):
-// out += mask1 * (-1/2 i k.Q - pe k^2 F).
+// out += mask1 * (-1/2 i k.Q - pe k^2 F), then out -= sub (if not null) at
+// every entry (rhs_scalars.f90:387-389).
 void launch_scalar_rhs_ps(const Geo& g, const double* const Q[3], const double* F, double* out,
-                          double pe, bool accumulate, hipStream_t s);
-// out -= a over all entries (rhs_scalars.f90:387-389).
-void launch_sub(const Geo& g, double* out, const double* a, hipStream_t s);
+                          double pe, bool accumulate, const double* sub, hipStream_t s);
 // Deterministic FP64 shell sums of (|u|^2+|v|^2+|w|^2)/n^6 over entries
 // kx < n/2, kx = 0 halved, for shells s1..s1+ns-1 (m_stats.f90:207-228).
 void launch_shell_sums(const Geo& g, const double* u, const double* v, const double* w, int s1,
