@@ -7,680 +7,13 @@
 #include <string>
 #include <vector>
 
+#include "guard.hpp"
 #include "hit3d.h"
+#include "host.hpp"
 #include "init_host.hpp"
 #include "passes.hpp"
-#include "state.hpp"
-
-namespace hit3d {
-
-static Lib g_lib;
-Lib& lib() { return g_lib; }
-
-void check(hipError_t e, const char* what) {
-    if (e != hipSuccess) throw HipError{std::string(what) + ": " + hipGetErrorString(e)};
-}
-
-static const char* kTimerNames =
-    "fft_x,fft_y,fft_z,pointwise,reduce,step_inv_z,step_inv_y,step_x,step_fwd_y,step_fwd_z,"
-    "step_update,particles";
-
-Timed::Timed(int c) : cls(c) {
-    Lib& L = lib();
-    if (!L.timing) return;
-    check(hipEventCreate(&a), "hipEventCreate");
-    check(hipEventCreate(&b), "hipEventCreate");
-    check(hipEventRecord(a, L.st), "hipEventRecord");
-}
-Timed::~Timed() {
-    Lib& L = lib();
-    if (!a) return;
-    if (hipEventRecord(b, L.st) == hipSuccess) L.evs.push_back({cls, a, b});
-}
-
-static void collect_timers() {
-    Lib& L = lib();
-    if (L.evs.empty()) return;
-    check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
-    for (auto& e : L.evs) {
-        float ms = 0.f;
-        check(hipEventElapsedTime(&ms, e.a, e.b), "hipEventElapsedTime");
-        L.tms[e.cls] += ms;
-        L.tcnt[e.cls] += 1;
-        (void)hipEventDestroy(e.a);
-        (void)hipEventDestroy(e.b);
-    }
-    L.evs.clear();
-}
-
-// ---------------------------------------------------------------------------
-// transforms
-// ---------------------------------------------------------------------------
-static void pass_y(double* slot, int dir) {
-    Lib& L = lib();
-    auto* c = reinterpret_cast<cpx<double>*>(slot);
-    const int64_t n = L.n, P = L.g.P;
-    Timed t(TC_FFT_Y);
-    launch_cols<double>(L.n, dir, c, c, n * P, n * P, P, P, L.tw64, L.st);
-}
-
-// y and z passes of a forward transform whose x pass is done; ph = -1
-// multiplies the spectral result by exp(-i theta) (the phase shift back).
-static void fft_forward_yz(double*& slot, int ph) {
-    Lib& L = lib();
-    const int64_t n = L.n, P = L.g.P;
-    pass_y(slot, -1);
-    {
-        Timed t(TC_FFT_Z);
-        launch_cols<double>(L.n, -1, reinterpret_cast<cpx<double>*>(slot),
-                            reinterpret_cast<cpx<double>*>(L.spare), P, n * P, n * P, P, L.tw64,
-                            L.st, ph, L.ps_tab);
-    }
-    std::swap(slot, L.spare);
-}
-
-void fft_forward(double*& slot, int ph) {
-    Lib& L = lib();
-    {
-        Timed t(TC_FFT_X);
-        launch_x_r2c<double>(L.n, slot, L.g.P, L.tw64, L.st);
-    }
-    fft_forward_yz(slot, ph);
-}
-
-// dst = forward transform of the physical product a*b (dst may be a or b),
-// times exp(-i theta) when ph = -1
-static void fft_forward_product(const double* a, const double* b, double*& dst, int ph) {
-    Lib& L = lib();
-    {
-        Timed t(TC_FFT_X);
-        launch_x_r2c_mul<double>(L.n, a, b, dst, L.g.P, L.tw64, L.st);
-    }
-    fft_forward_yz(dst, ph);
-}
-
-// Forward transform of the physical product a*b, shifted back by exp(-i theta),
-// accumulated into the RHS as rhs_j += -1/2 i k_axj (a*b)^ for the given
-// targets instead of being stored; `work` is overwritten (x and y passes).
-static void fft_forward_product_acc(const double* a, const double* b, double* work, double* r0,
-                                    int ax0, bool mask0, double* r1 = nullptr, int ax1 = 0,
-                                    bool mask1 = false) {
-    Lib& L = lib();
-    const int64_t n = L.n, P = L.g.P;
-    {
-        Timed t(TC_FFT_X);
-        launch_x_r2c_mul<double>(L.n, a, b, work, P, L.tw64, L.st);
-    }
-    pass_y(work, -1);
-    ColAcc acc;
-    acc.r[0] = reinterpret_cast<cpx<double>*>(r0);
-    acc.ax[0] = ax0;
-    acc.r[1] = reinterpret_cast<cpx<double>*>(r1);
-    acc.ax[1] = ax1;
-    acc.mask[0] = mask0;
-    acc.mask[1] = mask1;
-    Timed t(TC_FFT_Z);
-    launch_cols<double>(L.n, -1, reinterpret_cast<cpx<double>*>(work),
-                        reinterpret_cast<cpx<double>*>(L.spare), P, n * P, n * P, P, L.tw64, L.st,
-                        -1, L.ps_tab, acc);
-}
-
-// dst = inverse transform of src (src unchanged unless it is dst), the
-// spectral input multiplied by exp(+i theta) first when ph = +1
-static void fft_inverse_from(const double* src, double*& dst, int ph) {
-    Lib& L = lib();
-    const int64_t n = L.n, P = L.g.P;
-    {
-        Timed t(TC_FFT_Z);
-        launch_cols<double>(L.n, 1, reinterpret_cast<const cpx<double>*>(src),
-                            reinterpret_cast<cpx<double>*>(L.spare), n * P, P, P, n * P, L.tw64,
-                            L.st, ph, L.ps_tab);
-    }
-    std::swap(dst, L.spare);
-    pass_y(dst, 1);
-    {
-        Timed t(TC_FFT_X);
-        const double norm = 1.0 / ((double)n * (double)n * (double)n);
-        launch_x_c2r<double>(L.n, dst, P, L.tw64, norm, L.st);
-    }
-}
-
-void fft_inverse(double*& slot) { fft_inverse_from(slot, slot, 0); }
-
-// ---------------------------------------------------------------------------
-// helpers
-// ---------------------------------------------------------------------------
-static void require_ready() {
-    if (!lib().ready) throw std::runtime_error("hit3d_init has not been called");
-}
-
-static size_t slot_bytes() { return (size_t)lib().g.nc * sizeof(cpx<double>); }
-
-static double*& slot_ref(int64_t array_id, int64_t s) {
-    Lib& L = lib();
-    if (array_id == 1) {
-        if (s < 1 || s > L.nf) throw std::runtime_error("fields slot out of range");
-        return L.F[s];
-    }
-    if (array_id == 2) {
-        if (s < 0 || s > L.nw) throw std::runtime_error("wrk slot out of range");
-        return L.W[s];
-    }
-    if (array_id == 3) {
-        if (s < 1 || s > L.nf) throw std::runtime_error("rhs_old slot out of range");
-        return L.R[s];
-    }
-    throw std::runtime_error("unknown array id");
-}
-
-static void copy_slot(double* dst, const double* src) {
-    Lib& L = lib();
-    Timed t(TC_POINTWISE);
-    check(hipMemcpyAsync(dst, src, slot_bytes(), hipMemcpyDeviceToDevice, L.st), "hipMemcpyAsync");
-}
-
-static void zero_slot(double* dst) {
-    Lib& L = lib();
-    Timed t(TC_POINTWISE);
-    launch_fill_zero(dst, (int64_t)(slot_bytes() / sizeof(double)), L.st);
-}
-
-static void pmul(const double* a, const double* b, double* out) {
-    Lib& L = lib();
-    Timed t(TC_POINTWISE);
-    launch_phys_mul(L.g, a, b, out, L.st);
-}
-
-static void clear_graphs() {
-    Lib& L = lib();
-    for (auto& g : L.graphs)
-        if (g.exec) (void)hipGraphExecDestroy(g.exec);
-    L.graphs.clear();
-    L.graph_seen.clear();
-}
-
-static void free_all() {
-    Lib& L = lib();
-    (void)hipDeviceSynchronize();
-    clear_graphs();
-    for (auto* p : L.F) if (p) (void)hipFree(p);
-    for (auto* p : L.W) if (p) (void)hipFree(p);
-    for (auto* p : L.R) if (p) (void)hipFree(p);
-    for (auto* p : L.W32) if (p) (void)hipFree(p);
-    if (L.spare) (void)hipFree(L.spare);
-    if (L.tw64) (void)hipFree(L.tw64);
-    if (L.tw32) (void)hipFree(L.tw32);
-    if (L.ps_tab) (void)hipFree(L.ps_tab);
-    if (L.dshell) (void)hipFree(L.dshell);
-    if (L.dpart) (void)hipFree(L.dpart);
-    if (L.dmax) (void)hipFree(L.dmax);
-    if (L.tv) (void)hipFree(L.tv);
-    for (void* q : {(void*)L.mlp.p, (void*)L.mlp.wb, (void*)L.mlp.m, (void*)L.mlp.v,
-                    (void*)L.mlp.g, (void*)L.mlp.partial, (void*)L.mlp.lossp, (void*)L.mlp.loss,
-                    (void*)L.mlp.red,
-                    (void*)L.mlp.X, (void*)L.mlp.T, (void*)L.mlp.bad})
-        if (q) (void)hipFree(q);
-    for (auto* q : L.mlp.buf) if (q) (void)hipFree(q);
-    if (L.xyzp) (void)hipFree(L.xyzp);
-    if (L.uvwp) (void)hipFree(L.uvwp);
-    if (L.dscal) (void)hipFree(L.dscal);
-    if (L.hpin) (void)hipHostFree(L.hpin);
-    if (L.st) (void)hipStreamDestroy(L.st);
-    for (auto& e : L.evs) {
-        (void)hipEventDestroy(e.a);
-        (void)hipEventDestroy(e.b);
-    }
-    std::string keep = L.err;
-    L = Lib();
-    L.err = keep;
-}
-
-static double* dalloc(size_t bytes) {
-    void* p = nullptr;
-    check(hipMalloc(&p, bytes), "hipMalloc");
-    check(hipMemset(p, 0, bytes), "hipMemset");
-    return static_cast<double*>(p);
-}
-
-// ---------------------------------------------------------------------------
-// reference subroutines
-// ---------------------------------------------------------------------------
-// les_get_turb_visc_smag (m_les.f90:1142-1215): turb_visc from fields(1:3),
-// using wrk(4:6) as work space like the reference.
-static void les_turb_visc() {
-    Lib& L = lib();
-    auto& W = L.W;
-    const double* U[3] = {L.F[1], L.F[2], L.F[3]};
-    const double coef = (L.c_smag * L.dx) * (L.c_smag * L.dx);
-    for (int group = 1; group <= 2; ++group) {
-        {
-            double* out[3] = {W[4], W[5], W[6]};
-            Timed t(TC_POINTWISE);
-            launch_strain(L.g, U, out, group, L.st);
-        }
-        for (int c = 4; c <= 6; ++c) fft_inverse(W[c]);
-        Timed t(TC_POINTWISE);
-        launch_tv_accum(L.g, W[4], W[5], W[6], L.tv, group, coef, L.st);
-    }
-}
-
-static void mlp_velocity();
-
-// les_rhs_velocity for model 1 (m_les.f90:636-690 with les_rhsv_turb_visc
-// :902-1018): rhs += -d_j tau_ij, tau_ij = -2 nu_t S_ij, then the ialias mask.
-static void les_velocity() {
-    if (lib().les_model == 11) {
-        mlp_velocity();
-        return;
-    }
-    Lib& L = lib();
-    auto& W = L.W;
-    const double* U[3] = {L.F[1], L.F[2], L.F[3]};
-    for (int group = 1; group <= 2; ++group) {
-        {
-            double* out[3] = {W[4], W[5], W[6]};
-            Timed t(TC_POINTWISE);
-            launch_strain(L.g, U, out, group, L.st);
-        }
-        for (int c = 4; c <= 6; ++c) {
-            fft_inverse(W[c]);
-            {
-                Timed t(TC_POINTWISE);
-                launch_tau(L.g, W[c], L.tv, L.st);
-            }
-            fft_forward(W[c]);
-        }
-        const double* tau[3] = {W[4], W[5], W[6]};
-        double* rhs[3] = {W[1], W[2], W[3]};
-        Timed t(TC_POINTWISE);
-        launch_les_vel_accum(L.g, tau, rhs, group, L.st);
-    }
-    Timed t(TC_POINTWISE);
-    for (int c = 1; c <= 3; ++c) launch_mask_alias(L.g, W[c], 0, L.st);
-}
-
-// m_les_rhss_turb_visc1 (m_les.f90:1041-1091) for field slot n, then the
-// ialias mask of les_rhs_scalars (:863-871).
-static void les_scalar(int n) {
-    Lib& L = lib();
-    auto& W = L.W;
-    const int t1 = 3 + L.ns + L.nles + 1, t2 = t1 + 1;
-    copy_slot(W[t1], L.F[n]);
-    zero_slot(W[0]);
-    for (int axis = 1; axis <= 3; ++axis) {
-        {
-            Timed t(TC_POINTWISE);
-            launch_derivative(L.g, W[t1], W[t2], axis, L.st);
-        }
-        fft_inverse(W[t2]);
-        pmul(W[t2], L.tv, W[t2]);
-        fft_forward(W[t2]);
-        Timed t(TC_POINTWISE);
-        launch_derivative(L.g, W[t2], W[t2], axis, L.st);
-        launch_add(L.g, W[0], W[t2], false, L.st);
-    }
-    Timed t(TC_POINTWISE);
-    launch_add(L.g, W[n], W[0], true, L.st);
-    launch_mask_alias(L.g, W[n], 0, L.st);
-}
-
-// nine physical velocity gradients du_i/dx_j (i-major) of fields(1:3) into
-// the closure's buffers 0..8
-static void mlp_alloc_scratch() {
-    auto& B = lib().mlp.buf;
-    if (B.empty()) {
-        B.assign(15, nullptr);
-        for (auto& q : B) q = dalloc(slot_bytes());
-    }
-}
-
-static void mlp_gradients() {
-    Lib& L = lib();
-    auto& B = L.mlp.buf;
-    mlp_alloc_scratch();
-    for (int c = 0; c < 3; ++c)
-        for (int a = 0; a < 3; ++a) {
-            {
-                Timed t(TC_POINTWISE);
-                launch_derivative(L.g, L.F[c + 1], B[3 * c + a], a + 1, L.st);
-            }
-            fft_inverse(B[3 * c + a]);
-        }
-}
-
-// The learned closure's stress divergence: tau_ij from the MLP at every point,
-// rhs_i += -i k_j tau_ij on unaliased modes as for model 1.  A step whose
-// network output is not finite everywhere uses the Smagorinsky stress
-// instead (the fallback) and is counted.
-static void mlp_velocity() {
-    Lib& L = lib();
-    auto& M = L.mlp;
-    mlp_gradients();
-    const double* gr[9];
-    for (int i = 0; i < 9; ++i) gr[i] = M.buf[i];
-    double* tau[6];
-    for (int i = 0; i < 6; ++i) tau[i] = M.buf[9 + i];
-    check(hipMemsetAsync(M.bad, 0, sizeof(unsigned int), L.st), "hipMemsetAsync");
-    {
-        Timed t(TC_POINTWISE);
-        launch_mlp_les(L.g, gr, M.delta, M.norm, M.wb, M.p, tau, M.bad, L.st);
-    }
-    unsigned int bad = 0;
-    check(hipMemcpyAsync(&bad, M.bad, sizeof(bad), hipMemcpyDeviceToHost, L.st), "hipMemcpyAsync");
-    check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
-    if (bad != 0) {
-        ++M.fallbacks;
-        if (!L.tv) L.tv = dalloc(slot_bytes());
-        const int keep = L.les_model;
-        L.les_model = 1;
-        les_turb_visc();
-        les_velocity();
-        L.les_model = keep;
-        return;
-    }
-    for (int i = 9; i < 15; ++i) fft_forward(M.buf[i]);
-    double* rhs[3] = {L.W[1], L.W[2], L.W[3]};
-    const double* t1[3] = {M.buf[9], M.buf[10], M.buf[11]};   // tau11, tau12, tau13
-    const double* t2[3] = {M.buf[12], M.buf[13], M.buf[14]};  // tau22, tau23, tau33
-    Timed t(TC_POINTWISE);
-    launch_les_vel_accum(L.g, t1, rhs, 1, L.st);
-    launch_les_vel_accum(L.g, t2, rhs, 2, L.st);
-    for (int c = 1; c <= 3; ++c) launch_mask_alias(L.g, L.W[c], 0, L.st);
-}
-
-static void do_rhs_scalars(bool int_scalars) {
-    Lib& L = lib();
-    auto& F = L.F;
-    auto& W = L.W;
-    const Geo& g = L.g;
-    if (L.nles > 0) throw std::runtime_error("LES scalars are not supported");
-    if (L.les_model == 1) les_turb_visc();  // rhs_scalars.f90:18
-    // scalar s of type 0 has the mean-gradient source -u (rhs_scalars.f90:387-389),
-    // subtracted inside the kernel that completes its RHS
-    auto source = [&](int s) -> const double* { return L.stype[s - 1] == 0 ? F[1] : nullptr; };
-    if (!int_scalars || L.ns == 0) {  // rhs_scalars.f90:32-39
-        for (int c = 1; c <= 3; ++c) fft_inverse_from(F[c], W[c], 0);
-        return;
-    }
-    if (L.dealias == 0) {  // rhs_scalars.f90:48-134
-        for (int c = 1; c <= 3; ++c) fft_inverse_from(F[c], W[c], 0);
-        for (int s = 1; s <= L.ns; ++s) {
-            fft_inverse_from(F[3 + s], W[0], 0);
-            // products phi u_i straight into the forward x pass
-            for (int i = 1; i <= 3; ++i) fft_forward_product(W[0], W[i], W[s + 2 + i], 0);
-            const double* Q[3] = {W[s + 3], W[s + 4], W[s + 5]};
-            {
-                Timed t(TC_POINTWISE);
-                launch_scalar_rhs(g, Q, F[3 + s], W[3 + s], L.pe[s - 1], source(s), L.st);
-            }
-            if (L.stype[s - 1] >= 100) {  // add_reaction + dealias_rhs, :124-130
-                {
-                    Timed t(TC_POINTWISE);
-                    launch_reaction(g, W[0], W[0], (int)(L.stype[s - 1] / 100), L.reac[s - 1],
-                                    F[3 + s], L.st);
-                }
-                fft_forward(W[0]);
-                Timed t(TC_POINTWISE);
-                launch_add(g, W[3 + s], W[0], false, L.st);
-                launch_truncate(g, W[3 + s], L.kmax, L.st);
-            }
-        }
-    } else {  // rhs_scalars.f90:144-271
-        // the shift exp(i theta) on the first inverse pass, the products in
-        // the forward x pass, the shift back on the last forward pass
-        for (int c = 1; c <= 3 + L.ns; ++c) fft_inverse_from(F[c], W[c], +1);
-        const int n1 = L.ns + 4, n2 = L.ns + 5;
-        for (int c = 4; c <= 3 + L.ns; ++c) {
-            fft_forward_product(W[c], W[1], W[n1], -1);
-            fft_forward_product(W[c], W[2], W[n2], -1);
-            fft_forward_product(W[c], W[3], W[c], -1);
-            const double* Q[3] = {W[n1], W[n2], W[c]};
-            Timed t(TC_POINTWISE);
-            launch_scalar_rhs_ps(g, Q, F[c], W[c], 0.0, false, nullptr, L.st);
-        }
-        for (int c = 1; c <= 3; ++c) fft_inverse_from(F[c], W[c], 0);
-        for (int c = 4; c <= 3 + L.ns; ++c) {
-            fft_inverse_from(F[c], W[0], 0);
-            fft_forward_product(W[0], W[1], W[n1], 0);
-            fft_forward_product(W[0], W[2], W[n2], 0);
-            fft_forward_product(W[0], W[3], W[0], 0);
-            const double* Q[3] = {W[n1], W[n2], W[0]};
-            Timed t(TC_POINTWISE);
-            launch_scalar_rhs_ps(g, Q, F[c], W[c], L.pe[c - 4], true, source(c - 3), L.st);
-        }
-        for (int c = 4; c <= 3 + L.ns; ++c) {  // reaction_rates, rhs_scalars.f90:280-380
-            if (L.stype[c - 4] <= 300) continue;
-            copy_slot(W[n1], F[c]);
-            {
-                Timed t(TC_POINTWISE);
-                launch_truncate(g, W[n1], L.n / 4, L.st);
-            }
-            fft_inverse(W[n1]);
-            {
-                Timed t(TC_POINTWISE);
-                launch_reaction(g, W[n1], W[n2], 3, L.reac[c - 4], F[c], L.st);
-            }
-            fft_forward(W[n2]);
-            Timed t(TC_POINTWISE);
-            launch_add(g, W[c], W[n2], false, L.st);
-        }
-    }
-    if (L.les_model == 1)  // rhs_scalars.f90:396-398
-        for (int s = 1; s <= L.ns; ++s) les_scalar(3 + s);
-}
-
-static double read_courant_max() {
-    Lib& L = lib();
-    check(hipMemcpyAsync(L.hpin, L.dscal, sizeof(double), hipMemcpyDeviceToHost, L.st),
-          "hipMemcpyAsync");
-    check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
-    uint64_t bits;
-    std::memcpy(&bits, L.hpin, sizeof(bits));
-    double m;
-    std::memcpy(&m, &bits, sizeof(m));
-    return m;
-}
-
-static void do_rhs_velocity() {
-    Lib& L = lib();
-    auto& F = L.F;
-    auto& W = L.W;
-    const Geo& g = L.g;
-    {
-        double* out[6] = {W[1], W[2], W[3], W[4], W[5], W[6]};
-        Timed t(TC_POINTWISE);
-        launch_products(g, W[1], W[2], W[3], out, L.dmax, L.dscal, L.st);
-    }
-    for (int c = 1; c <= 6; ++c) fft_forward(W[c]);
-    {
-        const double* P[6] = {W[1], W[2], W[3], W[4], W[5], W[6]};
-        const double* U[3] = {F[1], F[2], F[3]};
-        double* out[3] = {W[1], W[2], W[3]};
-        Timed t(TC_POINTWISE);
-        launch_rhs_assemble(g, P, U, out, L.nu, L.dealias == 1, L.st);
-    }
-    if (L.dealias == 1) {  // rhs_velocity.f90:242-408
-        // shift on the first inverse pass, products in the forward x pass,
-        // shift back on the last forward pass
-        // and the half-RHS accumulation in the last forward pass's epilogue
-        fft_inverse_from(F[1], W[4], +1);
-        fft_inverse_from(F[2], W[5], +1);
-        // (the last accumulation into each component applies the mask of
-        // rhs_velocity.f90:400-404)
-        fft_forward_product_acc(W[4], W[4], W[6], W[1], 1, false);
-        fft_forward_product_acc(W[4], W[5], W[6], W[1], 2, false, W[2], 1, false);
-        fft_forward_product_acc(W[5], W[5], W[6], W[2], 2, false);
-        fft_inverse_from(F[3], W[6], +1);
-        fft_forward_product_acc(W[4], W[6], W[4], W[1], 3, true, W[3], 1, false);
-        fft_forward_product_acc(W[5], W[6], W[5], W[2], 3, true, W[3], 2, false);
-        fft_forward_product_acc(W[6], W[6], W[6], W[3], 3, true);
-    }
-    if (L.les_model == 1 || L.les_model == 11) les_velocity();  // rhs_velocity.f90:414-416
-}
-
-// band energy into dscal[1] and the forcing added to the RHS (no host sync)
-static void force_enqueue() {
-    Lib& L = lib();
-    if (L.kfmax < 1) throw std::runtime_error("force_velocity needs kfmax >= 1");
-    Timed t(TC_REDUCE);
-    launch_shell_sums(L.g, L.F[1], L.F[2], L.F[3], 1, L.kfmax, L.dshell, L.st);
-    const double* f[3] = {L.F[1], L.F[2], L.F[3]};
-    double* r[3] = {L.W[1], L.W[2], L.W[3]};
-    launch_force_apply(L.g, f, r, L.dshell, L.kfmax, L.famp, L.dscal + 1, L.st);
-}
-
-static void do_force(double* energy) {
-    Lib& L = lib();
-    force_enqueue();
-    if (energy) {
-        check(hipMemcpyAsync(L.hpin + 1, L.dscal + 1, sizeof(double), hipMemcpyDeviceToHost, L.st),
-              "hipMemcpyAsync");
-        check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
-        *energy = L.hpin[1];
-    }
-}
-
-static void do_advance(int which, bool first, double dt, const double* dtp = nullptr) {
-    Lib& L = lib();
-    int lo = 1, hi = 3;
-    if (which == 2) {
-        lo = 4;
-        hi = 3 + L.ns + L.nles;
-    } else if (which != 1) {
-        throw std::runtime_error("advance: which must be 1 (velocity) or 2 (scalars)");
-    }
-    Timed t(TC_POINTWISE);
-    for (int c = lo; c <= hi; ++c)
-        launch_advance(L.g, L.F[c], L.W[c], L.R[c], dt, dtp, first, L.st);
-}
-
-static void do_pressure() {
-    Lib& L = lib();
-    Timed t(TC_POINTWISE);
-    launch_pressure(L.g, L.F[1], L.F[2], L.F[3], L.st);
-}
-
-static double key_to_double(uint64_t k) {
-    const uint64_t b = (k >> 63) ? (k & ~(1ull << 63)) : ~k;
-    double d;
-    std::memcpy(&d, &b, sizeof d);
-    return d;
-}
-
-// min and max of (a [+ b + c]) in physical space, stored to sum if given.
-static void minmax_to_host(const double* a, const double* b, const double* c, double* sum,
-                           double* mn, double* mx) {
-    Lib& L = lib();
-    auto* keys = reinterpret_cast<unsigned long long*>(L.dscal + 8);
-    {
-        Timed t(TC_REDUCE);
-        launch_minmax(L.g, a, b, c, sum, keys, L.st);
-    }
-    check(hipMemcpyAsync(L.hpin + 8, L.dscal + 8, 2 * sizeof(double), hipMemcpyDeviceToHost, L.st),
-          "hipMemcpyAsync");
-    check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
-    uint64_t k[2];
-    std::memcpy(k, L.hpin + 8, sizeof k);
-    *mn = key_to_double(k[0]);
-    *mx = key_to_double(k[1]);
-}
-
-static void sums_to_host(const double* a, const double* b, const double* c, double out[2]) {
-    Lib& L = lib();
-    {
-        Timed t(TC_REDUCE);
-        launch_phys_sums(L.g, a, b, c, L.dpart, L.dscal + 4, L.st);
-    }
-    check(hipMemcpyAsync(L.hpin + 4, L.dscal + 4, 2 * sizeof(double), hipMemcpyDeviceToHost, L.st),
-          "hipMemcpyAsync");
-    check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
-    out[0] = L.hpin[4];
-    out[1] = L.hpin[5];
-}
-
-// m_stats.f90:295-366 (stat_scalars) with get_scalar_spectra (:371-437):
-// per scalar, 2 pe <|grad phi|^2>, mean, variance, min, max, and the variance
-// spectrum.  wrk(0:4) are used as work space like the reference.
-static void do_stat_scalars(double* diss, double* mean, double* var, double* mn, double* mx,
-                            double* es) {
-    Lib& L = lib();
-    auto& W = L.W;
-    const double fac = 1.0 / ((double)L.n * (double)L.n * (double)L.n);
-    if ((int)L.pe.size() < L.ns) throw std::runtime_error("stat_scalars: set_physics first");
-    for (int n = 1; n <= L.ns; ++n) {
-        const double* phi = L.F[3 + n];
-        if (es) {
-            {
-                Timed t(TC_REDUCE);
-                launch_shell_sums1(L.g, phi, 1, L.kmax, L.dshell, L.st);
-            }
-            std::vector<double> e(L.kmax);
-            check(hipMemcpyAsync(e.data(), L.dshell, L.kmax * sizeof(double),
-                                 hipMemcpyDeviceToHost, L.st),
-                  "hipMemcpyAsync");
-            check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
-            for (int k = 0; k < L.kmax; ++k) es[(int64_t)(n - 1) * L.kmax + k] = 2.0 * e[k];
-        }
-        copy_slot(W[0], phi);
-        {
-            Timed t(TC_POINTWISE);
-            for (int a = 1; a <= 3; ++a) launch_derivative(L.g, W[0], W[a], a, L.st);
-        }
-        for (int a = 1; a <= 3; ++a) fft_inverse(W[a]);
-        double s[2];
-        sums_to_host(W[1], W[2], W[3], s);
-        if (diss) diss[n - 1] = 2.0 * L.pe[n - 1] * s[0] * fac;
-        fft_inverse(W[0]);
-        sums_to_host(W[0], nullptr, nullptr, s);
-        const double m1 = s[0] * fac;
-        if (mean) mean[n - 1] = m1;
-        if (var) var[n - 1] = s[1] * fac - m1 * m1;
-        double lo, hi;
-        minmax_to_host(W[0], nullptr, nullptr, nullptr, &lo, &hi);
-        if (mn) mn[n - 1] = lo;
-        if (mx) mx[n - 1] = hi;
-    }
-}
-
-// pressure.f90:53-93: wrk(1:3) = fields(1:3); wrk(4:6) = d/dx_i of them in
-// physical space; wrk(0) = their sum; min and max of wrk(0).
-static void do_divergence(double* dmin, double* dmax) {
-    Lib& L = lib();
-    auto& W = L.W;
-    for (int c = 1; c <= 3; ++c) copy_slot(W[c], L.F[c]);
-    {
-        Timed t(TC_POINTWISE);
-        for (int c = 1; c <= 3; ++c) launch_derivative(L.g, W[c], W[3 + c], c, L.st);
-    }
-    for (int c = 4; c <= 6; ++c) fft_inverse(W[c]);
-    minmax_to_host(W[4], W[5], W[6], W[0], dmin, dmax);
-}
-
-}  // namespace hit3d
 
 using namespace hit3d;
-
-// ---------------------------------------------------------------------------
-// C ABI
-// ---------------------------------------------------------------------------
-template <typename Fn>
-static int guarded(Fn&& fn, bool device = true) {
-    try {
-        fn();
-        const hipError_t e = device ? hipGetLastError() : hipSuccess;
-        if (e != hipSuccess) throw HipError{std::string("kernel launch: ") + hipGetErrorString(e)};
-        return 0;
-    } catch (const HipError& e) {
-        lib().err = e.msg;
-        return 2;
-    } catch (const std::exception& e) {
-        lib().err = e.what();
-        return 1;
-    } catch (...) {
-        lib().err = "unknown error";
-        return 3;
-    }
-}
 
 extern "C" {
 
@@ -913,113 +246,6 @@ int hit3d_pressure(void) {
         require_ready();
         do_pressure();
     });
-}
-
-// The device work of one driver step (main.f90:124-167), enqueued without any
-// host synchronisation; the step size is read from dscal[12].
-static void step_body(bool fused, bool mixed, bool scal, bool first_v, bool first_s) {
-    Lib& L = lib();
-    const double* dtp = L.dscal + 12;
-    if (fused) {
-        // rhs_scalars + rhs_velocity + force + update + pressure in one pipeline
-        fused_step(dtp, first_v, mixed);
-        return;
-    }
-    do_rhs_scalars(scal);
-    if (scal) do_advance(2, first_s, 0.0, dtp);
-    do_rhs_velocity();
-    if (L.flow_type == 1) force_enqueue();
-    {  // main.f90:156-164 and pressure.f90:13-47 in one pass
-        Timed t(TC_POINTWISE);
-        double* f[3] = {L.F[1], L.F[2], L.F[3]};
-        const double* r[3] = {L.W[1], L.W[2], L.W[3]};
-        double* ro[3] = {L.R[1], L.R[2], L.R[3]};
-        launch_advance_project(L.g, f, r, ro, 0.0, dtp, first_v, L.st);
-    }
-}
-
-static std::vector<void*> slot_state() {
-    Lib& L = lib();
-    std::vector<void*> s;
-    for (auto* p : L.F) s.push_back(p);
-    for (auto* p : L.R) s.push_back(p);
-    for (auto* p : L.W) s.push_back(p);
-    for (auto* p : L.W32) s.push_back(p);
-    for (auto* p : L.mlp.buf) s.push_back(p);  // closure scratch, transformed in place
-    s.push_back(L.spare);
-    return s;
-}
-
-static void set_slot_state(const std::vector<void*>& s) {
-    Lib& L = lib();
-    size_t i = 0;
-    for (auto*& p : L.F) p = static_cast<double*>(s[i++]);
-    for (auto*& p : L.R) p = static_cast<double*>(s[i++]);
-    for (auto*& p : L.W) p = static_cast<double*>(s[i++]);
-    for (auto*& p : L.W32) p = static_cast<float*>(s[i++]);
-    for (auto*& p : L.mlp.buf) p = static_cast<double*>(s[i++]);
-    L.spare = static_cast<double*>(s[i++]);
-}
-
-// Puts the work-slot, closure-scratch and spare pointers back to the state s
-// (from slot_state); fields and rhs_old must not have moved.
-static void restore_work_slots(const std::vector<void*>& s) {
-    Lib& L = lib();
-    if (s.size() != slot_state().size())
-        throw std::runtime_error("internal: scratch slots were allocated during the step");
-    size_t i = 0;
-    for (auto* p : L.F)
-        if (p != s[i++]) throw std::runtime_error("internal: a field slot moved during the step");
-    for (auto* p : L.R)
-        if (p != s[i++]) throw std::runtime_error("internal: an rhs_old slot moved during the step");
-    for (auto*& p : L.W) p = static_cast<double*>(s[i++]);
-    for (auto*& p : L.W32) p = static_cast<float*>(s[i++]);
-    for (auto*& p : L.mlp.buf) p = static_cast<double*>(s[i++]);
-    L.spare = static_cast<double*>(s[i++]);
-}
-
-// step_body from a captured graph: replayed when one was captured from the
-// same slot pointers and flags, captured when this combination was already
-// run once eagerly (which also makes every lazy allocation happen outside a
-// capture), run eagerly otherwise.
-static void step_graph(bool fused, bool mixed, bool scal, bool first_v, bool first_s) {
-    Lib& L = lib();
-    const unsigned flags = (fused ? 1u : 0u) | (mixed ? 2u : 0u) | (scal ? 4u : 0u) |
-                           (first_v ? 8u : 0u) | (first_s ? 16u : 0u);
-    const std::vector<void*> before = slot_state();
-    for (auto& g : L.graphs)
-        if (g.flags == flags && g.before == before) {
-            check(hipGraphLaunch(g.exec, L.st), "hipGraphLaunch");
-            set_slot_state(g.after);
-            ++L.graph_replays;
-            return;
-        }
-    bool seen = false;
-    for (auto& k : L.graph_seen) seen = seen || (k.second == flags && k.first == before);
-    if (!seen || L.graphs.size() >= 16) {
-        if (!seen && L.graph_seen.size() < 64) L.graph_seen.push_back({before, flags});
-        step_body(fused, mixed, scal, first_v, first_s);
-        return;
-    }
-    hipGraph_t graph = nullptr;
-    check(hipStreamBeginCapture(L.st, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
-    try {
-        step_body(fused, mixed, scal, first_v, first_s);
-    } catch (...) {
-        (void)hipStreamEndCapture(L.st, &graph);
-        if (graph) (void)hipGraphDestroy(graph);
-        set_slot_state(before);
-        throw;
-    }
-    check(hipStreamEndCapture(L.st, &graph), "hipStreamEndCapture");
-    hipGraphExec_t exec = nullptr;
-    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    if (e != hipSuccess) set_slot_state(before);
-    check(e, "hipGraphInstantiate");
-    L.graphs.push_back({before, slot_state(), flags, exec});
-    check(hipGraphLaunch(exec, L.st), "hipGraphLaunch");
-    ++L.graph_replays;
 }
 
 int hit3d_step(const double* dt, int64_t* fov, int64_t* fos, const int64_t* int_scalars,
@@ -1423,224 +649,6 @@ int hit3d_set_les(const int64_t* les_model, const double* c_smag) {
         if (L.les_model != 0 && !L.tv) L.tv = dalloc(slot_bytes());
         if (L.les_model == 11) L.mlp.delta = L.dx;  // filter width of the LES grid
     });
-}
-
-// ------------------------------------------------------------ learned closure
-static void mlp_alloc_params() {
-    auto& M = lib().mlp;
-    if (M.p) return;
-    const size_t b = sizeof(float) * mlp::NPARAM;
-    M.p = reinterpret_cast<float*>(dalloc(b));
-    M.m = reinterpret_cast<float*>(dalloc(b));
-    M.v = reinterpret_cast<float*>(dalloc(b));
-    M.g = reinterpret_cast<float*>(dalloc(b));
-    M.wb = reinterpret_cast<uint16_t*>(dalloc(sizeof(uint16_t) * mlp::NPARAM));
-    M.bad = reinterpret_cast<unsigned int*>(dalloc(sizeof(unsigned int) * 4));
-}
-
-static void mlp_reserve(int64_t tiles, int64_t steps) {
-    auto& M = lib().mlp;
-    if (tiles > M.cap_tiles) {
-        if (M.partial) check(hipFree(M.partial), "hipFree");
-        if (M.lossp) check(hipFree(M.lossp), "hipFree");
-        M.partial = reinterpret_cast<float*>(dalloc(sizeof(float) * mlp::NPARAM * tiles));
-        M.lossp = reinterpret_cast<float*>(dalloc(sizeof(float) * tiles));
-        M.cap_tiles = tiles;
-    }
-    if (!M.red)
-        M.red = reinterpret_cast<float*>(
-            dalloc(sizeof(float) * kReduceGroups * (mlp::NPARAM + 1)));
-    if (steps > M.cap_steps) {
-        if (M.loss) check(hipFree(M.loss), "hipFree");
-        M.loss = reinterpret_cast<float*>(dalloc(sizeof(float) * steps));
-        M.cap_steps = steps;
-    }
-}
-
-int hit3d_mlp_set_params(const float* params) {
-    return guarded([&] {
-        require_ready();
-        Lib& L = lib();
-        mlp_alloc_params();
-        auto& M = L.mlp;
-        const size_t b = sizeof(float) * mlp::NPARAM;
-        check(hipMemcpyAsync(M.p, params, b, hipMemcpyHostToDevice, L.st), "hipMemcpyAsync");
-        check(hipMemsetAsync(M.m, 0, b, L.st), "hipMemsetAsync");
-        check(hipMemsetAsync(M.v, 0, b, L.st), "hipMemsetAsync");
-        launch_mlp_to_bf16(M.p, M.wb, L.st);
-        check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
-        M.step = 0;
-        M.have_params = true;
-    });
-}
-
-int hit3d_mlp_get_params(float* params) {
-    return guarded([&] {
-        require_ready();
-        Lib& L = lib();
-        if (!L.mlp.have_params) throw std::runtime_error("mlp: no parameters set");
-        check(hipMemcpyAsync(params, L.mlp.p, sizeof(float) * mlp::NPARAM, hipMemcpyDeviceToHost,
-                             L.st),
-              "hipMemcpyAsync");
-        check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
-    });
-}
-
-int hit3d_mlp_set_norm(const double* xmean, const double* xstd, const double* ymean,
-                       const double* ystd) {
-    return guarded([&] {
-        auto& N = lib().mlp.norm;
-        for (int i = 0; i < mlp::NIN; ++i) {
-            if (!(xstd[i] > 0.0)) throw std::runtime_error("mlp: input scales must be positive");
-            N.xmean[i] = (float)xmean[i];
-            N.xstd[i] = (float)xstd[i];
-        }
-        for (int i = 0; i < mlp::NOUT; ++i) {
-            if (!(ystd[i] > 0.0)) throw std::runtime_error("mlp: output scales must be positive");
-            N.ymean[i] = (float)ymean[i];
-            N.ystd[i] = (float)ystd[i];
-        }
-        lib().mlp.have_norm = true;
-    }, false);
-}
-
-int hit3d_mlp_set_data(const int64_t* count, const float* X, const float* T) {
-    return guarded([&] {
-        require_ready();
-        Lib& L = lib();
-        auto& M = L.mlp;
-        if (*count < 1) throw std::runtime_error("mlp: empty training set");
-        if (M.X) check(hipFree(M.X), "hipFree");
-        if (M.T) check(hipFree(M.T), "hipFree");
-        M.X = M.T = nullptr;
-        M.X = reinterpret_cast<float*>(dalloc(sizeof(float) * mlp::NIN * *count));
-        M.T = reinterpret_cast<float*>(dalloc(sizeof(float) * mlp::NOUT * *count));
-        check(hipMemcpyAsync(M.X, X, sizeof(float) * mlp::NIN * *count, hipMemcpyHostToDevice, L.st),
-              "hipMemcpyAsync");
-        check(hipMemcpyAsync(M.T, T, sizeof(float) * mlp::NOUT * *count, hipMemcpyHostToDevice, L.st),
-              "hipMemcpyAsync");
-        check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
-        M.count = *count;
-    });
-}
-
-int hit3d_mlp_train(const int64_t* steps, const int64_t* batch, const double* lr, double* losses) {
-    return guarded([&] {
-        require_ready();
-        Lib& L = lib();
-        auto& M = L.mlp;
-        if (!M.have_params || !M.X) throw std::runtime_error("mlp: parameters and data first");
-        if (*batch < 32 || *batch % 32 != 0) throw std::runtime_error("mlp: batch must be a multiple of 32");
-        if (*steps < 1) return;
-        const int64_t tiles = *batch / 32;
-        mlp_reserve(tiles, *steps);
-        for (int64_t i = 0; i < *steps; ++i) {
-            const int64_t start = ((int64_t)M.step * *batch) % M.count;
-            launch_mlp_train_tiles(M.X, M.T, M.count, start, (int)tiles, *batch, M.wb, M.p,
-                                   M.partial, M.lossp, L.st);
-            launch_mlp_reduce(M.partial, M.lossp, (int)tiles, M.red, M.g, M.loss + i, L.st);
-            ++M.step;
-            launch_mlp_adam(M.p, M.wb, M.g, M.m, M.v, M.step, (float)*lr, L.st);
-        }
-        std::vector<float> ls(*steps);
-        check(hipMemcpyAsync(ls.data(), M.loss, sizeof(float) * *steps, hipMemcpyDeviceToHost, L.st),
-              "hipMemcpyAsync");
-        check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
-        if (losses)
-            for (int64_t i = 0; i < *steps; ++i) losses[i] = (double)ls[i] / (double)(*batch * mlp::NOUT);
-    });
-}
-
-int hit3d_mlp_grads(const int64_t* count, const float* X, const float* T, float* grads,
-                    double* loss) {
-    return guarded([&] {
-        require_ready();
-        Lib& L = lib();
-        auto& M = L.mlp;
-        if (!M.have_params) throw std::runtime_error("mlp: parameters first");
-        const int64_t n = *count;
-        if (n < 32 || n % 32 != 0) throw std::runtime_error("mlp: count must be a multiple of 32");
-        float* dX = reinterpret_cast<float*>(dalloc(sizeof(float) * mlp::NIN * n));
-        float* dT = reinterpret_cast<float*>(dalloc(sizeof(float) * mlp::NOUT * n));
-        check(hipMemcpyAsync(dX, X, sizeof(float) * mlp::NIN * n, hipMemcpyHostToDevice, L.st), "hipMemcpyAsync");
-        check(hipMemcpyAsync(dT, T, sizeof(float) * mlp::NOUT * n, hipMemcpyHostToDevice, L.st), "hipMemcpyAsync");
-        const int64_t tiles = n / 32;
-        mlp_reserve(tiles, 1);
-        launch_mlp_train_tiles(dX, dT, n, 0, (int)tiles, n, M.wb, M.p, M.partial, M.lossp, L.st);
-        launch_mlp_reduce(M.partial, M.lossp, (int)tiles, M.red, M.g, M.loss, L.st);
-        float ls = 0.f;
-        check(hipMemcpyAsync(grads, M.g, sizeof(float) * mlp::NPARAM, hipMemcpyDeviceToHost, L.st), "hipMemcpyAsync");
-        check(hipMemcpyAsync(&ls, M.loss, sizeof(float), hipMemcpyDeviceToHost, L.st), "hipMemcpyAsync");
-        check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
-        check(hipFree(dX), "hipFree");
-        check(hipFree(dT), "hipFree");
-        if (loss) *loss = (double)ls / (double)(n * mlp::NOUT);
-    });
-}
-
-int hit3d_mlp_forward(const int64_t* count, const float* X, float* Y) {
-    return guarded([&] {
-        require_ready();
-        Lib& L = lib();
-        auto& M = L.mlp;
-        if (!M.have_params) throw std::runtime_error("mlp: parameters first");
-        const int64_t n = *count;
-        if (n < 1) return;
-        float* dX = reinterpret_cast<float*>(dalloc(sizeof(float) * mlp::NIN * n));
-        float* dY = reinterpret_cast<float*>(dalloc(sizeof(float) * mlp::NOUT * n));
-        check(hipMemcpyAsync(dX, X, sizeof(float) * mlp::NIN * n, hipMemcpyHostToDevice, L.st), "hipMemcpyAsync");
-        launch_mlp_forward(dX, n, M.wb, M.p, dY, L.st);
-        check(hipMemcpyAsync(Y, dY, sizeof(float) * mlp::NOUT * n, hipMemcpyDeviceToHost, L.st), "hipMemcpyAsync");
-        check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
-        check(hipFree(dX), "hipFree");
-        check(hipFree(dY), "hipFree");
-    });
-}
-
-int hit3d_box_filter(const int64_t* ratio, double* ubar, double* tau) {
-    return guarded([&] {
-        require_ready();
-        Lib& L = lib();
-        const int r = (int)*ratio;
-        if (r < 1 || L.n % r != 0) throw std::runtime_error("box filter: ratio must divide n");
-        if (9 * L.n * (int64_t)sizeof(double) > 65536)
-            throw std::runtime_error("box filter: grids up to 512 points per direction");
-        const int m = L.n / r;
-        const size_t m3 = (size_t)m * m * m;
-        double* d = dalloc(sizeof(double) * 9 * m3);
-        {
-            Timed t(TC_POINTWISE);
-            launch_box_filter(L.g, L.W[1], L.W[2], L.W[3], r, d, d + 3 * m3, L.st);
-        }
-        check(hipMemcpyAsync(ubar, d, sizeof(double) * 3 * m3, hipMemcpyDeviceToHost, L.st), "hipMemcpyAsync");
-        check(hipMemcpyAsync(tau, d + 3 * m3, sizeof(double) * 6 * m3, hipMemcpyDeviceToHost, L.st),
-              "hipMemcpyAsync");
-        check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
-        check(hipFree(d), "hipFree");
-    });
-}
-
-int hit3d_sgs_features(const double* delta, float* X) {
-    return guarded([&] {
-        require_ready();
-        Lib& L = lib();
-        mlp_gradients();
-        const double* gr[9];
-        for (int i = 0; i < 9; ++i) gr[i] = L.mlp.buf[i];
-        const size_t n3 = (size_t)L.n * L.n * L.n;
-        float* d = reinterpret_cast<float*>(dalloc(sizeof(float) * mlp::NIN * n3));
-        {
-            Timed t(TC_POINTWISE);
-            launch_sgs_features(L.g, gr, *delta, d, L.st);
-        }
-        check(hipMemcpyAsync(X, d, sizeof(float) * mlp::NIN * n3, hipMemcpyDeviceToHost, L.st), "hipMemcpyAsync");
-        check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
-        check(hipFree(d), "hipFree");
-    });
-}
-
-int hit3d_mlp_fallbacks(int64_t* count) {
-    return guarded([&] { *count = lib().mlp.fallbacks; }, false);
 }
 
 int hit3d_set_fusion(const int64_t* on) {

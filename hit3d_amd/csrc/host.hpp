@@ -1,0 +1,63 @@
+// Host-side operations shared by the C ABI files: slot bookkeeping, the
+// transform sequences, the reference subroutines as launch sequences, the
+// statistics' host sides and the step orchestration.  All act on lib().
+#pragma once
+#include <cstdint>
+#include <vector>
+
+#include "state.hpp"
+
+namespace hit3d {
+
+// timers, state, slots (host_state.hip)
+extern const char* const kTimerNames;  // comma-separated TimerClass names
+void collect_timers();
+void require_ready();
+size_t slot_bytes();
+double*& slot_ref(int64_t array_id, int64_t s);
+void copy_slot(double* dst, const double* src);
+void zero_slot(double* dst);
+void pmul(const double* a, const double* b, double* out);
+void clear_graphs();
+void free_all();
+double* dalloc(size_t bytes);
+
+// transforms (host_transforms.hip); fft_forward / fft_inverse are in state.hpp
+void pass_y(double* slot, int dir);
+void fft_forward_yz(double*& slot, int ph);
+void fft_forward_product(const double* a, const double* b, double*& dst, int ph);
+void fft_forward_product_acc(const double* a, const double* b, double* work, double* r0, int ax0,
+                             bool mask0, double* r1 = nullptr, int ax1 = 0, bool mask1 = false);
+void fft_inverse_from(const double* src, double*& dst, int ph);
+
+// reference subroutines (host_rhs.hip)
+void les_turb_visc();
+void les_velocity();
+void les_scalar(int n);
+void mlp_alloc_scratch();
+void mlp_gradients();
+void mlp_velocity();
+void do_rhs_scalars(bool int_scalars);
+double read_courant_max();
+void do_rhs_velocity();
+void force_enqueue();
+void do_force(double* energy);
+void do_advance(int which, bool first, double dt, const double* dtp = nullptr);
+void do_pressure();
+
+// statistics (host_stats.hip)
+double key_to_double(uint64_t k);
+void minmax_to_host(const double* a, const double* b, const double* c, double* sum, double* mn,
+                    double* mx);
+void sums_to_host(const double* a, const double* b, const double* c, double out[2]);
+void do_stat_scalars(double* diss, double* mean, double* var, double* mn, double* mx, double* es);
+void do_divergence(double* dmin, double* dmax);
+
+// one step (host_step.hip)
+void step_body(bool fused, bool mixed, bool scal, bool first_v, bool first_s);
+std::vector<void*> slot_state();
+void set_slot_state(const std::vector<void*>& s);
+void restore_work_slots(const std::vector<void*>& s);
+void step_graph(bool fused, bool mixed, bool scal, bool first_v, bool first_s);
+
+}  // namespace hit3d
