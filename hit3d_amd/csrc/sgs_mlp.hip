@@ -3,10 +3,11 @@
 // Matrix products use v_mfma_f32_16x16x16_bf16: one wavefront computes a
 // 16x16 FP32 tile D += A B over K = 16; lane l holds A(i, k) for
 // i = l % 16, k = 4 (l / 16) + j, B(k, n) for n = l % 16 and the same k, and
-// D(i, n) for n = l % 16, i = 4 (l / 16) + j (j = 0..3).  Every product
-// reads bfloat16 operands staged in LDS; accumulation, biases, GELU and its
-// derivative, the loss and all reductions are FP32 in a fixed order, so a
-// training run repeats bit for bit.
+// D(i, n) for n = l % 16, i = 4 (l / 16) + j (j = 0..3).  Training reads
+// both bfloat16 operands from LDS; inference keeps the weight operands in
+// registers.  Accumulation, biases, GELU and its derivative, the loss and all
+// reductions are FP32 in a fixed order, so a training run repeats bit for
+// bit.
 #include "sgs_mlp.hpp"
 
 namespace hit3d {
@@ -87,9 +88,8 @@ __device__ void stage_weights(Lds& s, const uint16_t* wb, const float* p) {
     for (int i = threadIdx.x; i < KOUT; i += NT) s.b3[i] = p[OB3 + i];
 }
 
-// forward through the two hidden layers for the TS samples staged in s.xb;
-// GRAD also keeps gelu'(Z) in s.g1, s.g2 for the backward pass
-template <bool GRAD>
+// forward through the two hidden layers for the TS samples staged in s.xb,
+// keeping gelu'(Z) in s.g1, s.g2 for the backward pass
 __device__ void hidden_layers(Lds& s) {
     const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, r = l & 15, gq = l >> 4;
     const int n0 = 16 * wv;
@@ -99,7 +99,7 @@ __device__ void hidden_layers(Lds& s) {
         for (int j = 0; j < 4; ++j) {
             const int i = 16 * mt + 4 * gq + j, n = n0 + r;
             const float z = acc[j] + s.b1[n];
-            if (GRAD) s.g1[i * NH + n] = gelu_grad(z);
+            s.g1[i * NH + n] = gelu_grad(z);
             s.h1[i * NH + n] = to_bf16(gelu(z));
         }
     }
@@ -110,7 +110,7 @@ __device__ void hidden_layers(Lds& s) {
         for (int j = 0; j < 4; ++j) {
             const int i = 16 * mt + 4 * gq + j, n = n0 + r;
             const float z = acc[j] + s.b2[n];
-            if (GRAD) s.g2[i * NH + n] = gelu_grad(z);
+            s.g2[i * NH + n] = gelu_grad(z);
             s.h2[i * NH + n] = to_bf16(gelu(z));
         }
     }
@@ -131,7 +131,7 @@ __global__ __launch_bounds__(NT) void k_train(const float* __restrict__ X, const
     }
     stage_weights(s, wb, p);
     __syncthreads();
-    hidden_layers<true>(s);
+    hidden_layers(s);
     float* part = partial + (int64_t)blockIdx.x * NPARAM;
     // output layer Y = H2 W3 + b3, loss and dY (waves 0, 1: one row tile each)
     const float inv = 2.0f / (float)(nb_total * NOUT);
@@ -272,26 +272,95 @@ __global__ void k_to_bf16(const float* p, uint16_t* wb) {
     if (k < NPARAM) wb[k] = to_bf16(p[k]);
 }
 
-// output layer for the staged tile: y(i, n) for n < 6 handed to f(i, n, y)
+// ---- inference: weights in registers
+// Each wavefront keeps the B operands of its 16 output columns of every
+// layer in registers for all the tiles of its block (A operands come from
+// LDS as one 8-byte read per lane per product); the products, their order
+// and the FP32 epilogues are those of hidden_layers/output_layer, so the
+// results are identical.
+struct FwdRegs {
+    bf16x4 w1, w2[NH / 16], w3[NH / 16];
+    float b1, b2, b3;
+};
+
+__device__ FwdRegs load_fwd_regs(const uint16_t* __restrict__ wb, const float* __restrict__ p) {
+    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, r = l & 15, gq = l >> 4;
+    const int n0 = 16 * wv;
+    FwdRegs w;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        w.w1[j] = (short)wb[OW1 + (4 * gq + j) * NH + n0 + r];
+#pragma unroll
+        for (int ks = 0; ks < NH / 16; ++ks) {
+            w.w2[ks][j] = (short)wb[OW2 + (16 * ks + 4 * gq + j) * NH + n0 + r];
+            w.w3[ks][j] = (short)wb[OW3 + (16 * ks + 4 * gq + j) * KOUT + r];
+        }
+    }
+    w.b1 = p[OB1 + n0 + r];
+    w.b2 = p[OB2 + n0 + r];
+    w.b3 = p[OB3 + r];
+    return w;
+}
+
+// D(i0.., n) = sum_k A(i, k) B(k, n), A row-major in LDS (lda a multiple of 4,
+// 8-byte aligned rows), B fragments in registers
+template <int KSTEPS>
+__device__ __forceinline__ f32x4 mma_reg(const uint16_t* A, int lda, int i0, const bf16x4* b) {
+    const int l = threadIdx.x & 63, r = l & 15, gq = l >> 4;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < KSTEPS; ++ks) {
+        const bf16x4 a = *reinterpret_cast<const bf16x4*>(A + (i0 + r) * lda + 16 * ks + 4 * gq);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[ks], acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+// LDS of an inference tile
+struct FwdLds {
+    alignas(16) uint16_t xb[TS * KIN];
+    alignas(16) uint16_t h1[TS * NH];
+    alignas(16) uint16_t h2[TS * NH];
+};
+
+__device__ void hidden_layers_reg(FwdLds& s, const FwdRegs& w) {
+    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, r = l & 15, gq = l >> 4;
+    const int n0 = 16 * wv;
+#pragma unroll
+    for (int mt = 0; mt < TS / 16; ++mt) {
+        const f32x4 acc = mma_reg<1>(s.xb, KIN, 16 * mt, &w.w1);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) s.h1[(16 * mt + 4 * gq + j) * NH + n0 + r] = to_bf16(gelu(acc[j] + w.b1));
+    }
+    __syncthreads();
+#pragma unroll
+    for (int mt = 0; mt < TS / 16; ++mt) {
+        const f32x4 acc = mma_reg<NH / 16>(s.h1, NH, 16 * mt, w.w2);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) s.h2[(16 * mt + 4 * gq + j) * NH + n0 + r] = to_bf16(gelu(acc[j] + w.b2));
+    }
+    __syncthreads();
+}
+
 template <typename F>
-__device__ void output_layer(Lds& s, F&& f) {
+__device__ void output_layer_reg(FwdLds& s, const FwdRegs& w, F&& f) {
     const int wv = threadIdx.x >> 6, l = threadIdx.x & 63, r = l & 15, gq = l >> 4;
     if (wv < TS / 16) {
-        const f32x4 acc = mma<false, false>(s.h2, NH, s.w3, KOUT, 16 * wv, 0, NH);
+        const f32x4 acc = mma_reg<NH / 16>(s.h2, NH, 16 * wv, w.w3);
 #pragma unroll
         for (int j = 0; j < 4; ++j)
-            if (r < NOUT) f(16 * wv + 4 * gq + j, r, acc[j] + s.b3[r]);
+            if (r < NOUT) f(16 * wv + 4 * gq + j, r, acc[j] + w.b3);
     }
 }
 
-// inference kernels stage the weights once per block and run this many tiles
-constexpr int FWD_TILES = 8;
+// inference kernels run this many tiles per block
+constexpr int FWD_TILES = 4;
 
 __global__ __launch_bounds__(NT) void k_forward(const float* __restrict__ X, int64_t count,
                                                 const uint16_t* __restrict__ wb, const float* __restrict__ p,
                                                 float* __restrict__ Y) {
-    __shared__ Lds s;
-    stage_weights(s, wb, p);
+    __shared__ FwdLds s;
+    const FwdRegs w = load_fwd_regs(wb, p);
     for (int it = 0; it < FWD_TILES; ++it) {
         const int64_t first = ((int64_t)blockIdx.x * FWD_TILES + it) * TS;
         if (first >= count) break;  // the same for the whole block
@@ -301,8 +370,8 @@ __global__ __launch_bounds__(NT) void k_forward(const float* __restrict__ X, int
             s.xb[i] = (c < NIN && q < count) ? to_bf16(X[q * NIN + c]) : (uint16_t)0;
         }
         __syncthreads();
-        hidden_layers<false>(s);
-        output_layer(s, [&](int i, int n, float y) {
+        hidden_layers_reg(s, w);
+        output_layer_reg(s, w, [&](int i, int n, float y) {
             if (first + i < count) Y[(first + i) * NOUT + n] = y;
         });
         __syncthreads();  // s.xb and s.h2 are rewritten by the next tile
@@ -323,7 +392,7 @@ struct Ptr6 {
 __global__ __launch_bounds__(NT) void k_les(Geo g, Ptr9 gr, float delta, MlpNorm nrm,
                                             const uint16_t* __restrict__ wb, const float* __restrict__ p,
                                             Ptr6 tau, unsigned int* bad) {
-    __shared__ Lds s;
+    __shared__ FwdLds s;
     __shared__ int64_t offs[TS];
     __shared__ float graw[TS][9];
     __shared__ float tl[TS][NOUT];
@@ -344,7 +413,7 @@ __global__ __launch_bounds__(NT) void k_les(Geo g, Ptr9 gr, float delta, MlpNorm
     };
     float cur[2];
     gather((int64_t)blockIdx.x * FWD_TILES * TS, cur);
-    stage_weights(s, wb, p);
+    const FwdRegs w = load_fwd_regs(wb, p);
     for (int it = 0; it < FWD_TILES; ++it) {
         const int64_t first = ((int64_t)blockIdx.x * FWD_TILES + it) * TS;
         if (first >= total) break;  // the same for the whole block
@@ -366,8 +435,8 @@ __global__ __launch_bounds__(NT) void k_les(Geo g, Ptr9 gr, float delta, MlpNorm
         float nxt[2] = {0.f, 0.f};
         if (it + 1 < FWD_TILES && first + TS < total) gather(first + TS, nxt);
         __syncthreads();
-        hidden_layers<false>(s);
-        output_layer(s, [&](int i, int n, float y) { tl[i][n] = y * nrm.ystd[n] + nrm.ymean[n]; });
+        hidden_layers_reg(s, w);
+        output_layer_reg(s, w, [&](int i, int n, float y) { tl[i][n] = y * nrm.ystd[n] + nrm.ymean[n]; });
         __syncthreads();
         if (threadIdx.x < TS) {
             const int i = threadIdx.x;
