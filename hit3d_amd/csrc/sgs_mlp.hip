@@ -8,6 +8,8 @@
 // registers.  Accumulation, biases, GELU and its derivative, the loss and all
 // reductions are FP32 in a fixed order, so a training run repeats bit for
 // bit.
+#include <stdexcept>
+
 #include "sgs_mlp.hpp"
 
 namespace hit3d {
@@ -397,7 +399,11 @@ __global__ __launch_bounds__(NT) void k_les(Geo g, Ptr9 gr, float delta, MlpNorm
     __shared__ float graw[TS][9];
     __shared__ float tl[TS][NOUT];
     const int64_t total = (int64_t)g.n * g.n * g.n;
-    auto offset = [&](int64_t q) { return (q / g.n) * 2 * g.P + q % g.n; };
+    // grid point q -> physical slot offset; 32-bit division (n^3 <= 2^30)
+    auto offset = [&](int64_t q) {
+        const uint32_t qq = (uint32_t)q, nn = (uint32_t)g.n;
+        return (int64_t)(qq / nn) * 2 * g.P + (int64_t)(qq % nn);
+    };
     // Each thread gathers entries e = threadIdx.x + j*NT of a tile, feature
     // major (feature e / TS of sample e % TS): consecutive lanes read
     // consecutive grid points of one gradient slot.  The next tile's values
@@ -570,6 +576,7 @@ void launch_mlp_les(const Geo& g, const double* const grad[9], double delta, con
     Ptr6 tp;
     for (int i = 0; i < 6; ++i) tp.p[i] = tau[i];
     const int64_t total = (int64_t)g.n * g.n * g.n;
+    if (total > ((int64_t)1 << 31)) throw std::runtime_error("closure inference: grid above 1024^3");
     const int64_t per = (int64_t)TS * FWD_TILES;
     hipLaunchKernelGGL(k_les, dim3((unsigned)((total + per - 1) / per)), dim3(NT), 0, s, g, gr,
                        (float)delta, nrm, wb, p, tp, bad);
