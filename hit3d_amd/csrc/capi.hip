@@ -271,7 +271,9 @@ int hit3d_step(const double* dt, int64_t* fov, int64_t* fos, const int64_t* int_
         // it), so the work-slot labels are put back afterwards: every step
         // then starts from the same pointers and one captured graph serves
         // them all.
-        if (L.les_model == 11) mlp_alloc_scratch();  // before the pointer snapshot
+        // scratch the step allocates on first use, before the pointer snapshot
+        if (L.les_model == 11) mlp_alloc_scratch();
+        if (!fused) batch_spares();
         const std::vector<void*> s0 = slot_state();
         // the learned closure checks its output on the host mid-step
         if (L.graph_mode && !L.timing && L.les_model != 11)

@@ -43,15 +43,27 @@ __device__ __forceinline__ cpx<T> phase(cpx<T> a, int kx, int jy, int jz, const 
     else return cpx<T>{a.x * c + a.y * sn, a.y * c - a.x * sn};
 }
 
+// up to three slots of one launch, the slot chosen by blockIdx.z
+struct Slots3 {
+    const void* in[3];
+    void* out[3];
+};
+
 template <int N, int E, int B, int DIR, typename T, int PH = 0, bool ACC = false>
-__global__ __launch_bounds__(B* N / E) void k_cols(const cpx<T>* __restrict__ in, cpx<T>* out,
-                                                   int64_t a_in, int64_t a_out, int64_t s_in,
-                                                   int64_t s_out, int ncols,
+__global__ __launch_bounds__(B* N / E) void k_cols(Slots3 io, int64_t a_in, int64_t a_out,
+                                                   int64_t s_in, int64_t s_out, int ncols,
                                                    const cpx<T>* __restrict__ tw,
                                                    const double* __restrict__ tab, ColAcc acc) {
     constexpr int TPF = N / E;
     constexpr int PS = PadFor<T>::value;
     __shared__ cpx<T> lds[padded_size<PS>(N * B)];
+    // select the slot without indexing the argument array at run time
+    const void* ip = io.in[0];
+    void* op = io.out[0];
+    if (blockIdx.z == 1) ip = io.in[1], op = io.out[1];
+    if (blockIdx.z == 2) ip = io.in[2], op = io.out[2];
+    const cpx<T>* __restrict__ in = static_cast<const cpx<T>*>(ip);
+    cpx<T>* out = static_cast<cpx<T>*>(op);
     const int c = threadIdx.x % B;
     const int t = threadIdx.x / B;
     const int col = blockIdx.x * B + c;
@@ -115,38 +127,45 @@ __global__ __launch_bounds__(B* N / E) void k_cols(const cpx<T>* __restrict__ in
 }
 
 template <int N, typename T>
-static void cols_n(int dir, const cpx<T>* in, cpx<T>* out, int64_t a_in, int64_t a_out,
-                   int64_t s_in, int64_t s_out, const cpx<T>* tw, hipStream_t s, int ph,
-                   const double* tab, ColAcc acc) {
+static void cols_n(int dir, const cpx<T>* const* in, cpx<T>* const* out, int nb, int64_t a_in,
+                   int64_t a_out, int64_t s_in, int64_t s_out, const cpx<T>* tw, hipStream_t s,
+                   int ph, const double* tab, ColAcc acc) {
     constexpr int B = ColCfg<T>::B;
     constexpr int E = ColE<N, T>::E;
     const int ncols = N / 2 + 1;
-    dim3 grid((ncols + B - 1) / B, N);
+    if (nb < 1 || nb > 3 || (nb > 1 && (ph != 0 || acc.r[0])))
+        throw std::runtime_error("column pass: batches of 1..3 plain transforms only");
+    Slots3 io{};
+    for (int i = 0; i < nb; ++i) {
+        io.in[i] = in[i];
+        io.out[i] = out[i];
+    }
+    dim3 grid((ncols + B - 1) / B, N, nb);
     dim3 block(B * N / E);
     if (ph != 0 && !tab) throw std::runtime_error("column pass: phase table missing");
     if constexpr (std::is_same_v<T, double>) {
         if (dir < 0 && ph < 0 && acc.r[0]) {
-            hipLaunchKernelGGL((k_cols<N, E, B, -1, T, -1, true>), grid, block, 0, s, in, out, a_in,
+            hipLaunchKernelGGL((k_cols<N, E, B, -1, T, -1, true>), grid, block, 0, s, io, a_in,
                                a_out, s_in, s_out, ncols, tw, tab, acc);
             return;
         }
         if (dir < 0 && ph < 0) {
-            hipLaunchKernelGGL((k_cols<N, E, B, -1, T, -1>), grid, block, 0, s, in, out, a_in,
+            hipLaunchKernelGGL((k_cols<N, E, B, -1, T, -1>), grid, block, 0, s, io, a_in,
                                a_out, s_in, s_out, ncols, tw, tab, acc);
             return;
         }
         if (dir > 0 && ph > 0) {
-            hipLaunchKernelGGL((k_cols<N, E, B, 1, T, 1>), grid, block, 0, s, in, out, a_in, a_out,
+            hipLaunchKernelGGL((k_cols<N, E, B, 1, T, 1>), grid, block, 0, s, io, a_in, a_out,
                                s_in, s_out, ncols, tw, tab, acc);
             return;
         }
     }
     if (ph != 0 || acc.r[0]) throw std::runtime_error("column pass: unsupported option");
     if (dir < 0)
-        hipLaunchKernelGGL((k_cols<N, E, B, -1, T>), grid, block, 0, s, in, out, a_in, a_out, s_in,
+        hipLaunchKernelGGL((k_cols<N, E, B, -1, T>), grid, block, 0, s, io, a_in, a_out, s_in,
                            s_out, ncols, tw, tab, acc);
     else
-        hipLaunchKernelGGL((k_cols<N, E, B, 1, T>), grid, block, 0, s, in, out, a_in, a_out, s_in,
+        hipLaunchKernelGGL((k_cols<N, E, B, 1, T>), grid, block, 0, s, io, a_in, a_out, s_in,
                            s_out, ncols, tw, tab, acc);
 }
 
@@ -163,10 +182,14 @@ struct RowCfg {
 
 template <int N, typename T>
 __global__ __launch_bounds__((RowCfg<N, T>::ROWS * RowCfg<N, T>::TPF)) void k_r2c(
-    T* data, int64_t P, const cpx<T>* __restrict__ tw) {
+    Slots3 io, int64_t P, const cpx<T>* __restrict__ tw) {
     using C = RowCfg<N, T>;
     constexpr int M = C::M, E = C::E, TPF = C::TPF;
     __shared__ cpx<T> lds[C::ROWS][M + 1];
+    void* dp = io.out[0];
+    if (blockIdx.z == 1) dp = io.out[1];
+    if (blockIdx.z == 2) dp = io.out[2];
+    T* data = static_cast<T*>(dp);
     const int t = threadIdx.x % TPF;
     const int r = threadIdx.x / TPF;
     const int64_t row = (int64_t)blockIdx.x * C::ROWS + r;
@@ -238,10 +261,14 @@ __global__ __launch_bounds__((RowCfg<N, T>::ROWS * RowCfg<N, T>::TPF)) void k_r2
 
 template <int N, typename T>
 __global__ __launch_bounds__((RowCfg<N, T>::ROWS * RowCfg<N, T>::TPF)) void k_c2r(
-    T* data, int64_t P, const cpx<T>* __restrict__ tw, T norm) {
+    Slots3 io, int64_t P, const cpx<T>* __restrict__ tw, T norm) {
     using C = RowCfg<N, T>;
     constexpr int M = C::M, E = C::E, TPF = C::TPF;
     __shared__ cpx<T> lds[C::ROWS][M + 1];
+    void* dp = io.out[0];
+    if (blockIdx.z == 1) dp = io.out[1];
+    if (blockIdx.z == 2) dp = io.out[2];
+    T* data = static_cast<T*>(dp);
     const int t = threadIdx.x % TPF;
     const int r = threadIdx.x / TPF;
     const int64_t row = (int64_t)blockIdx.x * C::ROWS + r;
@@ -270,12 +297,19 @@ __global__ __launch_bounds__((RowCfg<N, T>::ROWS * RowCfg<N, T>::TPF)) void k_c2
     for (int s = 0; s < E; ++s) rowp[t + s * TPF] = scale(v[s], norm);
 }
 
+static Slots3 batch_of(void* const* slots, int nb) {
+    if (nb < 1 || nb > 3) throw std::runtime_error("row pass: batches of 1..3 slots only");
+    Slots3 io{};
+    for (int i = 0; i < nb; ++i) io.out[i] = slots[i];
+    return io;
+}
+
 template <int N, typename T>
-static void r2c_n(T* slot, int64_t P, const cpx<T>* tw, hipStream_t s) {
+static void r2c_n(T* const* slots, int nb, int64_t P, const cpx<T>* tw, hipStream_t s) {
     using C = RowCfg<N, T>;
     const int64_t rows = (int64_t)N * N;
-    hipLaunchKernelGGL((k_r2c<N, T>), dim3(rows / C::ROWS), dim3(C::ROWS * C::TPF), 0, s, slot, P,
-                       tw);
+    hipLaunchKernelGGL((k_r2c<N, T>), dim3(rows / C::ROWS, 1, nb), dim3(C::ROWS * C::TPF), 0, s,
+                       batch_of(reinterpret_cast<void* const*>(slots), nb), P, tw);
 }
 
 template <int N, typename T>
@@ -287,11 +321,11 @@ static void r2c_mul_n(const T* a, const T* b, T* out, int64_t P, const cpx<T>* t
 }
 
 template <int N, typename T>
-static void c2r_n(T* slot, int64_t P, const cpx<T>* tw, T norm, hipStream_t s) {
+static void c2r_n(T* const* slots, int nb, int64_t P, const cpx<T>* tw, T norm, hipStream_t s) {
     using C = RowCfg<N, T>;
     const int64_t rows = (int64_t)N * N;
-    hipLaunchKernelGGL((k_c2r<N, T>), dim3(rows / C::ROWS), dim3(C::ROWS * C::TPF), 0, s, slot, P,
-                       tw, norm);
+    hipLaunchKernelGGL((k_c2r<N, T>), dim3(rows / C::ROWS, 1, nb), dim3(C::ROWS * C::TPF), 0, s,
+                       batch_of(reinterpret_cast<void* const*>(slots), nb), P, tw, norm);
 }
 
 // ---------------------------------------------------------------------------
@@ -314,11 +348,19 @@ static void dispatch(int n, F&& f) {
 
 template <typename T>
 void launch_x_r2c(int n, T* slot, int64_t P, const cpx<T>* tw, hipStream_t s) {
-    dispatch(n, [&](auto NC) { r2c_n<decltype(NC)::value, T>(slot, P, tw, s); });
+    dispatch(n, [&](auto NC) { r2c_n<decltype(NC)::value, T>(&slot, 1, P, tw, s); });
 }
 template <typename T>
 void launch_x_c2r(int n, T* slot, int64_t P, const cpx<T>* tw, T norm, hipStream_t s) {
-    dispatch(n, [&](auto NC) { c2r_n<decltype(NC)::value, T>(slot, P, tw, norm, s); });
+    dispatch(n, [&](auto NC) { c2r_n<decltype(NC)::value, T>(&slot, 1, P, tw, norm, s); });
+}
+void launch_x_r2c_batch(int n, double* const* slots, int nb, int64_t P, const cpx<double>* tw,
+                        hipStream_t s) {
+    dispatch(n, [&](auto NC) { r2c_n<decltype(NC)::value, double>(slots, nb, P, tw, s); });
+}
+void launch_x_c2r_batch(int n, double* const* slots, int nb, int64_t P, const cpx<double>* tw,
+                        double norm, hipStream_t s) {
+    dispatch(n, [&](auto NC) { c2r_n<decltype(NC)::value, double>(slots, nb, P, tw, norm, s); });
 }
 template <typename T>
 void launch_x_r2c_mul(int n, const T* a, const T* b, T* out, int64_t P, const cpx<T>* tw,
@@ -330,7 +372,16 @@ void launch_cols(int n, int dir, const cpx<T>* in, cpx<T>* out, int64_t a_in, in
                  int64_t s_in, int64_t s_out, const cpx<T>* tw, hipStream_t s, int ph,
                  const double* tab, ColAcc acc) {
     dispatch(n, [&](auto NC) {
-        cols_n<decltype(NC)::value, T>(dir, in, out, a_in, a_out, s_in, s_out, tw, s, ph, tab, acc);
+        cols_n<decltype(NC)::value, T>(dir, &in, &out, 1, a_in, a_out, s_in, s_out, tw, s, ph, tab,
+                                       acc);
+    });
+}
+void launch_cols_batch(int n, int dir, const cpx<double>* const* in, cpx<double>* const* out,
+                       int nb, int64_t a_in, int64_t a_out, int64_t s_in, int64_t s_out,
+                       const cpx<double>* tw, hipStream_t s) {
+    dispatch(n, [&](auto NC) {
+        cols_n<decltype(NC)::value, double>(dir, in, out, nb, a_in, a_out, s_in, s_out, tw, s, 0,
+                                            nullptr, ColAcc{});
     });
 }
 

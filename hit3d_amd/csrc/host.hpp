@@ -29,6 +29,12 @@ void fft_forward_product(const double* a, const double* b, double*& dst, int ph)
 void fft_forward_product_acc(const double* a, const double* b, double* work, double* r0, int ax0,
                              bool mask0, double* r1 = nullptr, int ax1 = 0, bool mask1 = false);
 void fft_inverse_from(const double* src, double*& dst, int ph);
+// up to three slots per launch (slots[i] points at the slot pointer, which the
+// out-of-place pass swaps like the single-slot forms do)
+bool batch_spares();
+void fft_forward_batch(double** const* slots, int nb);
+void fft_inverse_batch(double** const* slots, int nb);
+void fft_inverse_from_batch(const double* const* src, double** const* dst, int nb);
 
 // reference subroutines (host_rhs.hip)
 void les_turb_visc();

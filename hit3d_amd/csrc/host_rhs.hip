@@ -27,7 +27,8 @@ void les_turb_visc() {
             Timed t(TC_POINTWISE);
             launch_strain(L.g, U, out, group, L.st);
         }
-        for (int c = 4; c <= 6; ++c) fft_inverse(W[c]);
+        double** b[3] = {&W[4], &W[5], &W[6]};
+        fft_inverse_batch(b, 3);
         Timed t(TC_POINTWISE);
         launch_tv_accum(L.g, W[4], W[5], W[6], L.tv, group, coef, L.st);
     }
@@ -50,14 +51,13 @@ void les_velocity() {
             Timed t(TC_POINTWISE);
             launch_strain(L.g, U, out, group, L.st);
         }
-        for (int c = 4; c <= 6; ++c) {
-            fft_inverse(W[c]);
-            {
-                Timed t(TC_POINTWISE);
-                launch_tau(L.g, W[c], L.tv, L.st);
-            }
-            fft_forward(W[c]);
+        double** b[3] = {&W[4], &W[5], &W[6]};
+        fft_inverse_batch(b, 3);
+        {
+            Timed t(TC_POINTWISE);
+            for (int c = 4; c <= 6; ++c) launch_tau(L.g, W[c], L.tv, L.st);
         }
+        fft_forward_batch(b, 3);
         const double* tau[3] = {W[4], W[5], W[6]};
         double* rhs[3] = {W[1], W[2], W[3]};
         Timed t(TC_POINTWISE);
@@ -106,14 +106,14 @@ void mlp_gradients() {
     Lib& L = lib();
     auto& B = L.mlp.buf;
     mlp_alloc_scratch();
-    for (int c = 0; c < 3; ++c)
+    for (int c = 0; c < 3; ++c) {
         for (int a = 0; a < 3; ++a) {
-            {
-                Timed t(TC_POINTWISE);
-                launch_derivative(L.g, L.F[c + 1], B[3 * c + a], a + 1, L.st);
-            }
-            fft_inverse(B[3 * c + a]);
+            Timed t(TC_POINTWISE);
+            launch_derivative(L.g, L.F[c + 1], B[3 * c + a], a + 1, L.st);
         }
+        double** b[3] = {&B[3 * c], &B[3 * c + 1], &B[3 * c + 2]};
+        fft_inverse_batch(b, 3);
+    }
 }
 
 // The learned closure's stress divergence: tau_ij from the MLP at every point,
@@ -146,7 +146,10 @@ void mlp_velocity() {
         L.les_model = keep;
         return;
     }
-    for (int i = 9; i < 15; ++i) fft_forward(M.buf[i]);
+    for (int i = 9; i < 15; i += 3) {
+        double** b[3] = {&M.buf[i], &M.buf[i + 1], &M.buf[i + 2]};
+        fft_forward_batch(b, 3);
+    }
     double* rhs[3] = {L.W[1], L.W[2], L.W[3]};
     const double* t1[3] = {M.buf[9], M.buf[10], M.buf[11]};   // tau11, tau12, tau13
     const double* t2[3] = {M.buf[12], M.buf[13], M.buf[14]};  // tau22, tau23, tau33
@@ -154,6 +157,13 @@ void mlp_velocity() {
     launch_les_vel_accum(L.g, t1, rhs, 1, L.st);
     launch_les_vel_accum(L.g, t2, rhs, 2, L.st);
     for (int c = 1; c <= 3; ++c) launch_mask_alias(L.g, L.W[c], 0, L.st);
+}
+
+// physical velocity in wrk(1:3) from fields(1:3), one batch
+static void inverse_velocity(std::vector<double*>& F, std::vector<double*>& W) {
+    const double* src[3] = {F[1], F[2], F[3]};
+    double** dst[3] = {&W[1], &W[2], &W[3]};
+    fft_inverse_from_batch(src, dst, 3);
 }
 
 void do_rhs_scalars(bool int_scalars) {
@@ -167,11 +177,11 @@ void do_rhs_scalars(bool int_scalars) {
     // subtracted inside the kernel that completes its RHS
     auto source = [&](int s) -> const double* { return L.stype[s - 1] == 0 ? F[1] : nullptr; };
     if (!int_scalars || L.ns == 0) {  // rhs_scalars.f90:32-39
-        for (int c = 1; c <= 3; ++c) fft_inverse_from(F[c], W[c], 0);
+        inverse_velocity(F, W);
         return;
     }
     if (L.dealias == 0) {  // rhs_scalars.f90:48-134
-        for (int c = 1; c <= 3; ++c) fft_inverse_from(F[c], W[c], 0);
+        inverse_velocity(F, W);
         for (int s = 1; s <= L.ns; ++s) {
             fft_inverse_from(F[3 + s], W[0], 0);
             // products phi u_i straight into the forward x pass
@@ -206,7 +216,7 @@ void do_rhs_scalars(bool int_scalars) {
             Timed t(TC_POINTWISE);
             launch_scalar_rhs_ps(g, Q, F[c], W[c], 0.0, false, nullptr, L.st);
         }
-        for (int c = 1; c <= 3; ++c) fft_inverse_from(F[c], W[c], 0);
+        inverse_velocity(F, W);
         for (int c = 4; c <= 3 + L.ns; ++c) {
             fft_inverse_from(F[c], W[0], 0);
             fft_forward_product(W[0], W[1], W[n1], 0);
@@ -259,7 +269,10 @@ void do_rhs_velocity() {
         Timed t(TC_POINTWISE);
         launch_products(g, W[1], W[2], W[3], out, L.dmax, L.dscal, L.st);
     }
-    for (int c = 1; c <= 6; ++c) fft_forward(W[c]);
+    for (int c = 1; c <= 6; c += 3) {
+        double** b[3] = {&W[c], &W[c + 1], &W[c + 2]};
+        fft_forward_batch(b, 3);
+    }
     {
         const double* P[6] = {W[1], W[2], W[3], W[4], W[5], W[6]};
         const double* U[3] = {F[1], F[2], F[3]};

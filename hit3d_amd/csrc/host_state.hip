@@ -111,6 +111,7 @@ void free_all() {
     for (auto* p : L.R) if (p) (void)hipFree(p);
     for (auto* p : L.W32) if (p) (void)hipFree(p);
     if (L.spare) (void)hipFree(L.spare);
+    for (auto* p : L.spares_b) if (p) (void)hipFree(p);
     if (L.tw64) (void)hipFree(L.tw64);
     if (L.tw32) (void)hipFree(L.tw32);
     if (L.ps_tab) (void)hipFree(L.ps_tab);

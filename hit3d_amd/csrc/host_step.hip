@@ -41,6 +41,7 @@ std::vector<void*> slot_state() {
     for (auto* p : L.W) s.push_back(p);
     for (auto* p : L.W32) s.push_back(p);
     for (auto* p : L.mlp.buf) s.push_back(p);  // closure scratch, transformed in place
+    for (auto* p : L.spares_b) s.push_back(p);
     s.push_back(L.spare);
     return s;
 }
@@ -53,6 +54,7 @@ void set_slot_state(const std::vector<void*>& s) {
     for (auto*& p : L.W) p = static_cast<double*>(s[i++]);
     for (auto*& p : L.W32) p = static_cast<float*>(s[i++]);
     for (auto*& p : L.mlp.buf) p = static_cast<double*>(s[i++]);
+    for (auto*& p : L.spares_b) p = static_cast<double*>(s[i++]);
     L.spare = static_cast<double*>(s[i++]);
 }
 
@@ -70,6 +72,7 @@ void restore_work_slots(const std::vector<void*>& s) {
     for (auto*& p : L.W) p = static_cast<double*>(s[i++]);
     for (auto*& p : L.W32) p = static_cast<float*>(s[i++]);
     for (auto*& p : L.mlp.buf) p = static_cast<double*>(s[i++]);
+    for (auto*& p : L.spares_b) p = static_cast<double*>(s[i++]);
     L.spare = static_cast<double*>(s[i++]);
 }
 

@@ -42,6 +42,16 @@ void launch_cols(int n, int dir, const cpx<T>* in, cpx<T>* out, int64_t a_in, in
                  int64_t s_in, int64_t s_out, const cpx<T>* tw, hipStream_t s, int ph = 0,
                  const double* tab = nullptr, ColAcc acc = ColAcc{});
 
+// The same passes on nb <= 3 FP64 slots in one launch (slot i of the batch
+// in the launch's z dimension); no phase or accumulate options.
+void launch_x_r2c_batch(int n, double* const* slots, int nb, int64_t P, const cpx<double>* tw,
+                        hipStream_t s);
+void launch_x_c2r_batch(int n, double* const* slots, int nb, int64_t P, const cpx<double>* tw,
+                        double norm, hipStream_t s);
+void launch_cols_batch(int n, int dir, const cpx<double>* const* in, cpx<double>* const* out,
+                       int nb, int64_t a_in, int64_t a_out, int64_t s_in, int64_t s_out,
+                       const cpx<double>* tw, hipStream_t s);
+
 // x pass, forward, of the pointwise product of two physical slots:
 // out = r2c(a * b) row by row (out may alias a or b).
 template <typename T>

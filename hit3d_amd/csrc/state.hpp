@@ -50,6 +50,7 @@ struct Lib {
     std::vector<double*> W;  // wrk, index 0..nw
     std::vector<double*> R;  // rhs_old, index 1..nf
     double* spare = nullptr;
+    std::vector<double*> spares_b;  // two more spares for batched transforms (N <= 128)
 
     cpx<double>* tw64 = nullptr;
     cpx<float>* tw32 = nullptr;
