@@ -130,6 +130,49 @@ def run_c4(steps, warmup, train_steps=50, batch=65536):
     print(json.dumps(out), flush=True)
 
 
+def run_fft_micro(sizes=(256, 512, 1024), reps=20, warmup=3):
+    """Transform microbenchmark of the measurement plan: a uniform(-1, 1) FP64
+    field from a fixed seed (12345) in one wrk slot of a library holding all
+    its slots, forward then inverse xFFT3d, median of `reps` after `warmup`.
+    A transform moves 3 passes x (read + write) of one slot.  Prints its own
+    JSON line."""
+    from hit3d_amd import WRK, Hit3D
+    out = {"workload": "fft", "config": "xFFT3d(+1) and xFFT3d(-1) on one FP64 slot, "
+                                        "uniform(-1, 1) field, seed 12345", "reps": reps,
+           "warmup": warmup, "sizes": {}}
+    for n in sizes:
+        h = Hit3D(n)
+        try:
+            a = np.zeros((n, n, n + 2))
+            a[:, :, :n] = np.random.default_rng(12345).uniform(-1.0, 1.0, (n, n, n))
+            h.put(WRK, 1, a)
+            del a
+            fwd, inv = [], []
+            for i in range(warmup + reps):
+                h.synchronize()
+                t0 = time.perf_counter()
+                h.xFFT3d(1, 1)
+                h.synchronize()
+                t1 = time.perf_counter()
+                h.xFFT3d(-1, 1)
+                h.synchronize()
+                t2 = time.perf_counter()
+                if i >= warmup:
+                    fwd.append(t1 - t0)
+                    inv.append(t2 - t1)
+        finally:
+            h.close()
+        byt = 6 * slot_bytes(n)
+        tf, ti = float(np.median(fwd)), float(np.median(inv))
+        out["sizes"][str(n)] = {"forward_ms": tf * 1e3, "inverse_ms": ti * 1e3,
+                                "bytes_per_transform": byt,
+                                "forward_GBs": byt / tf / 1e9, "inverse_GBs": byt / ti / 1e9,
+                                "forward_frac_of_8TBs": byt / tf / (PEAK_HBM_GBS * 1e9),
+                                "inverse_frac_of_8TBs": byt / ti / (PEAK_HBM_GBS * 1e9)}
+        log(f"[bench] fft n={n}: forward {tf * 1e3:.3f} ms, inverse {ti * 1e3:.3f} ms")
+    print(json.dumps(out), flush=True)
+
+
 def run_workload(name, steps, warmup):
     """One of the other configurations: FP64 driver steps (unfused where the
     phase shift or scalars apply), with particles moved every step when the
@@ -334,15 +377,12 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="c3", choices=["c3", "c4"] + sorted(WORKLOADS),
+    ap.add_argument("--workload", default="c3", choices=["c3", "c4", "fft"] + sorted(WORKLOADS),
                     help="c3 (default) prints the benchmark line; c1, c2, c4, c5 time the other "
-                         "configurations")
+                         "configurations; fft is the transform microbenchmark")
     ap.add_argument("--accuracy-steps", type=int, default=100,
                     help="untimed FP64 vs mixed run for time-averaged E(k) and skewness errors "
                          "(0 skips it)")
-    ap.add_argument("--calib", action="store_true",
-                    help="accepted for older command lines; the copy of known size (the "
-                         "reference for PMC byte counters) now always runs")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs the whole box on one GPU; --gpus {a.gpus} is not "
@@ -350,6 +390,9 @@ def main():
         sys.exit(2)
     if a.workload == "c4":
         run_c4(a.steps, a.warmup)
+        return
+    if a.workload == "fft":
+        run_fft_micro()
         return
     if a.workload != "c3":
         run_workload(a.workload, a.steps, a.warmup)
