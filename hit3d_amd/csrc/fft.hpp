@@ -13,9 +13,21 @@
 // DIR = +1 the unnormalised inverse.  Twiddles come from a table
 // tw[j] = exp(-2 pi i j / N), j < N, built on the host in long double.
 #pragma once
+#include <type_traits>
+
 #include "common.hpp"
 
 namespace hit3d {
+
+// number of radix-E stages before span Pspan (Pspan a power of E)
+__host__ __device__ constexpr int stage_index(int Pspan, int E) {
+    int s = 0;
+    while (Pspan > 1) {
+        Pspan /= E;
+        ++s;
+    }
+    return s;
+}
 
 template <typename T>
 struct Consts;
@@ -186,19 +198,33 @@ __device__ __forceinline__ void lds_sync() {
 // seven loads fewer); used where twiddle loads dominate the wait time.
 // TW: precision of the twiddle table and of that recurrence (FP64 for FP32
 // data keeps the twiddles within an ulp of FP32).
+// TC: a TwCache of this thread's twiddles (see below), used instead of
+// computing them when transforms with the same thread mapping repeat.
 template <int N, int E, int DIR, typename T, int LS, int NF, int FS, int PADSH, bool WAVE,
-          int Pspan, bool TWREC = false, typename TW = T>
+          int Pspan, bool TWREC = false, typename TW = T, typename TC = void>
 struct Stages {
     static constexpr int R = (E < N / Pspan) ? E : N / Pspan;
     static constexpr int TPF = N / E;
     static constexpr int NB = E / R;  // butterflies per thread
     __device__ static __forceinline__ void run(cpx<T> (&v)[NF][E], int t, cpx<T>* lds,
-                                               const cpx<TW>* __restrict__ tw) {
+                                               const cpx<TW>* __restrict__ tw,
+                                               const TC* tc = nullptr) {
 #pragma unroll
         for (int q = 0; q < NB; ++q) {
             const int b = t + q * TPF;
             const int k = b & (Pspan - 1);
-            if constexpr (Pspan > 1 && TWREC && (R == 8 || R == 4 || R == 2)) {
+            if constexpr (Pspan > 1 && !std::is_void_v<TC>) {
+                static_assert(NB == 1, "cached twiddles need one butterfly per thread per stage");
+                constexpr int si = stage_index(Pspan, E) - 1;
+#pragma unroll
+                for (int m = 1; m < R; ++m) {
+                    // the cache holds the forward twiddles; the inverse uses their
+                    // conjugates (exactly the TWREC recurrence of the conjugate)
+                    const cpx<T> wm = DIR > 0 ? conj(tc->w[si][m]) : tc->w[si][m];
+#pragma unroll
+                    for (int f = 0; f < NF; ++f) v[f][q + m * NB] = cmul(v[f][q + m * NB], wm);
+                }
+            } else if constexpr (Pspan > 1 && TWREC && (R == 8 || R == 4 || R == 2)) {
                 cpx<TW> w[R];
                 w[1] = ldtw(tw, k * (N / (Pspan * R)));
                 if constexpr (DIR > 0) w[1].y = -w[1].y;
@@ -249,7 +275,8 @@ struct Stages {
             for (int s = 0; s < E; ++s)
 #pragma unroll
                 for (int f = 0; f < NF; ++f) v[f][s] = lds[f * FS + padi<PADSH>((t + s * TPF) * LS)];
-            Stages<N, E, DIR, T, LS, NF, FS, PADSH, WAVE, Pspan * R, TWREC, TW>::run(v, t, lds, tw);
+            Stages<N, E, DIR, T, LS, NF, FS, PADSH, WAVE, Pspan * R, TWREC, TW, TC>::run(v, t, lds, tw,
+                                                                                       tc);
         }
     }
 };
@@ -257,10 +284,45 @@ struct Stages {
 // NF transforms at once.  Every thread taking part in the exchange must call
 // this (it synchronises), also threads whose data is masked out.
 template <int N, int E, int DIR, typename T, int LS, int NF, int FS, int PADSH, bool WAVE,
-          bool TWREC = false, typename TW = T>
+          bool TWREC = false, typename TW = T, typename TC = void>
 __device__ __forceinline__ void multi_fft(cpx<T> (&v)[NF][E], int t, cpx<T>* lds,
-                                          const cpx<TW>* __restrict__ tw) {
-    Stages<N, E, DIR, T, LS, NF, FS, PADSH, WAVE, 1, TWREC, TW>::run(v, t, lds, tw);
+                                          const cpx<TW>* __restrict__ tw,
+                                          const TC* tc = nullptr) {
+    Stages<N, E, DIR, T, LS, NF, FS, PADSH, WAVE, 1, TWREC, TW, TC>::run(v, t, lds, tw, tc);
+}
+
+// Forward twiddles of one thread for every twiddled stage of a length-N
+// transform with E values per thread and radix E throughout (N a power of
+// E): computed once, by the same recurrence as TWREC in TW and rounded to T,
+// then shared by every transform with that thread mapping, either direction
+// (the same values as TWREC up to the compiler's choice of fused
+// multiply-adds).
+template <typename T, int N, int E>
+struct TwCache {
+    static constexpr int NS = stage_index(N, E) - 1;  // twiddled stages
+    cpx<T> w[NS > 0 ? NS : 1][E];
+};
+
+template <int N, int E, typename T, typename TW>
+__device__ __forceinline__ void fill_twiddles(TwCache<T, N, E>& c, int t,
+                                              const cpx<TW>* __restrict__ tw) {
+    static_assert(E == 8, "the cache follows the radix-8 recurrence");
+#pragma unroll
+    for (int si = 0; si < TwCache<T, N, E>::NS; ++si) {
+        int Pspan = E;
+        for (int i = 0; i < si; ++i) Pspan *= E;
+        const int k = t & (Pspan - 1);
+        cpx<TW> w[8];
+        w[1] = ldtw(tw, k * (N / (Pspan * E)));
+        w[2] = cmul(w[1], w[1]);
+        w[3] = cmul(w[2], w[1]);
+        w[4] = cmul(w[2], w[2]);
+        w[5] = cmul(w[4], w[1]);
+        w[6] = cmul(w[3], w[3]);
+        w[7] = cmul(w[6], w[1]);
+#pragma unroll
+        for (int m = 1; m < 8; ++m) c.w[si][m] = cpx<T>{(T)w[m].x, (T)w[m].y};
+    }
 }
 
 // One transform with block-wide synchronisation; element j at lds[padi(j*LS)].
