@@ -220,7 +220,8 @@ struct Stages {
                 for (int m = 1; m < R; ++m) {
                     // the cache holds the forward twiddles; the inverse uses their
                     // conjugates (exactly the TWREC recurrence of the conjugate)
-                    const cpx<T> wm = DIR > 0 ? conj(tc->w[si][m]) : tc->w[si][m];
+                    const cpx<T> wc = tc->get(si, m);
+                    const cpx<T> wm = DIR > 0 ? conj(wc) : wc;
 #pragma unroll
                     for (int f = 0; f < NF; ++f) v[f][q + m * NB] = cmul(v[f][q + m * NB], wm);
                 }
@@ -301,6 +302,20 @@ template <typename T, int N, int E>
 struct TwCache {
     static constexpr int NS = stage_index(N, E) - 1;  // twiddled stages
     cpx<T> w[NS > 0 ? NS : 1][E];
+    __device__ __forceinline__ cpx<T> get(int si, int m) const { return w[si][m]; }
+};
+
+// The same twiddles for all N/E thread positions of a block, in LDS: entry
+// (si, m) of position t at base[(si * E + m) * (N / E) + t].
+template <typename T, int N, int E>
+struct TwLds {
+    static constexpr int NS = TwCache<T, N, E>::NS;
+    static constexpr int SIZE = (NS > 0 ? NS : 1) * E * (N / E);
+    const cpx<T>* base;
+    int t;
+    __device__ __forceinline__ cpx<T> get(int si, int m) const {
+        return base[(si * E + m) * (N / E) + t];
+    }
 };
 
 template <int N, int E, typename T, typename TW>
@@ -327,11 +342,11 @@ __device__ __forceinline__ void fill_twiddles(TwCache<T, N, E>& c, int t,
 
 // One transform with block-wide synchronisation; element j at lds[padi(j*LS)].
 template <int N, int E, int DIR, typename T, int LS, int PADSH = 0, bool TWREC = false,
-          typename TW = T>
+          typename TW = T, typename TC = void>
 __device__ __forceinline__ void block_fft(cpx<T> (&v)[E], int t, cpx<T>* lds,
-                                          const cpx<TW>* __restrict__ tw) {
+                                          const cpx<TW>* __restrict__ tw, const TC* tc = nullptr) {
     cpx<T>(&vv)[1][E] = *reinterpret_cast<cpx<T>(*)[1][E]>(&v);
-    Stages<N, E, DIR, T, LS, 1, 0, PADSH, false, 1, TWREC, TW>::run(vv, t, lds, tw);
+    Stages<N, E, DIR, T, LS, 1, 0, PADSH, false, 1, TWREC, TW, TC>::run(vv, t, lds, tw, tc);
 }
 
 }  // namespace hit3d
