@@ -274,6 +274,7 @@ int hit3d_step(const double* dt, int64_t* fov, int64_t* fos, const int64_t* int_
         // scratch the step allocates on first use, before the pointer snapshot
         if (L.les_model == 11) mlp_alloc_scratch();
         if (!fused) batch_spares();
+        if (scal && L.dealias == 1) keep_slots();
         const std::vector<void*> s0 = slot_state();
         // the learned closure checks its output on the host mid-step
         if (L.graph_mode && !L.timing && L.les_model != 11)

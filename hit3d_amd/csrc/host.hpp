@@ -43,9 +43,12 @@ void les_scalar(int n);
 void mlp_alloc_scratch();
 void mlp_gradients();
 void mlp_velocity();
-void do_rhs_scalars(bool int_scalars);
+// keep_shifted (phase shift with scalars): leave the shifted physical
+// velocity in lib().keep for do_rhs_velocity(true) instead of discarding it
+void do_rhs_scalars(bool int_scalars, bool keep_shifted = false);
 double read_courant_max();
-void do_rhs_velocity();
+void do_rhs_velocity(bool use_kept = false);
+void keep_slots();  // allocates lib().keep
 void force_enqueue();
 void do_force(double* energy);
 void do_advance(int which, bool first, double dt, const double* dtp = nullptr);

@@ -166,7 +166,15 @@ static void inverse_velocity(std::vector<double*>& F, std::vector<double*>& W) {
     fft_inverse_from_batch(src, dst, 3);
 }
 
-void do_rhs_scalars(bool int_scalars) {
+void keep_slots() {
+    Lib& L = lib();
+    if (L.keep.empty()) {
+        L.keep.assign(3, nullptr);
+        for (auto& q : L.keep) q = dalloc(slot_bytes());
+    }
+}
+
+void do_rhs_scalars(bool int_scalars, bool keep_shifted) {
     Lib& L = lib();
     auto& F = L.F;
     auto& W = L.W;
@@ -216,6 +224,8 @@ void do_rhs_scalars(bool int_scalars) {
             Timed t(TC_POINTWISE);
             launch_scalar_rhs_ps(g, Q, F[c], W[c], 0.0, false, nullptr, L.st);
         }
+        if (keep_shifted)  // the shifted velocity moves to lib().keep, free
+            for (int c = 0; c < 3; ++c) std::swap(W[1 + c], L.keep[c]);
         inverse_velocity(F, W);
         for (int c = 4; c <= 3 + L.ns; ++c) {
             fft_inverse_from(F[c], W[0], 0);
@@ -259,7 +269,7 @@ double read_courant_max() {
     return m;
 }
 
-void do_rhs_velocity() {
+void do_rhs_velocity(bool use_kept) {
     Lib& L = lib();
     auto& F = L.F;
     auto& W = L.W;
@@ -284,14 +294,23 @@ void do_rhs_velocity() {
         // shift on the first inverse pass, products in the forward x pass,
         // shift back on the last forward pass
         // and the half-RHS accumulation in the last forward pass's epilogue
-        fft_inverse_from(F[1], W[4], +1);
-        fft_inverse_from(F[2], W[5], +1);
+        // (or the shifted velocity rhs_scalars kept, swapped in)
+        if (use_kept) {
+            std::swap(W[4], L.keep[0]);
+            std::swap(W[5], L.keep[1]);
+        } else {
+            fft_inverse_from(F[1], W[4], +1);
+            fft_inverse_from(F[2], W[5], +1);
+        }
         // (the last accumulation into each component applies the mask of
         // rhs_velocity.f90:400-404)
         fft_forward_product_acc(W[4], W[4], W[6], W[1], 1, false);
         fft_forward_product_acc(W[4], W[5], W[6], W[1], 2, false, W[2], 1, false);
         fft_forward_product_acc(W[5], W[5], W[6], W[2], 2, false);
-        fft_inverse_from(F[3], W[6], +1);
+        if (use_kept)
+            std::swap(W[6], L.keep[2]);
+        else
+            fft_inverse_from(F[3], W[6], +1);
         fft_forward_product_acc(W[4], W[6], W[4], W[1], 3, true, W[3], 1, false);
         fft_forward_product_acc(W[5], W[6], W[5], W[2], 3, true, W[3], 2, false);
         fft_forward_product_acc(W[6], W[6], W[6], W[3], 3, true);

@@ -112,6 +112,7 @@ void free_all() {
     for (auto* p : L.W32) if (p) (void)hipFree(p);
     if (L.spare) (void)hipFree(L.spare);
     for (auto* p : L.spares_b) if (p) (void)hipFree(p);
+    for (auto* p : L.keep) if (p) (void)hipFree(p);
     if (L.tw64) (void)hipFree(L.tw64);
     if (L.tw32) (void)hipFree(L.tw32);
     if (L.ps_tab) (void)hipFree(L.ps_tab);
@@ -143,6 +144,10 @@ double* dalloc(size_t bytes) {
     void* p = nullptr;
     check(hipMalloc(&p, bytes), "hipMalloc");
     check(hipMemset(p, 0, bytes), "hipMemset");
+    // hipMemset runs on the null stream, which the library's non-blocking
+    // stream does not wait for: without this, a buffer allocated on first
+    // use could be cleared after the kernels that write it
+    check(hipDeviceSynchronize(), "hipDeviceSynchronize");
     return static_cast<double*>(p);
 }
 

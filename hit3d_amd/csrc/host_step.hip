@@ -20,9 +20,12 @@ void step_body(bool fused, bool mixed, bool scal, bool first_v, bool first_s) {
         fused_step(dtp, first_v, mixed);
         return;
     }
-    do_rhs_scalars(scal);
+    // with the phase shift and scalars, the shifted velocity rhs_scalars
+    // transforms is kept for rhs_velocity (the fields do not change between)
+    const bool keep = scal && L.dealias == 1 && L.keep.size() == 3;
+    do_rhs_scalars(scal, keep);
     if (scal) do_advance(2, first_s, 0.0, dtp);
-    do_rhs_velocity();
+    do_rhs_velocity(keep);
     if (L.flow_type == 1) force_enqueue();
     {  // main.f90:156-164 and pressure.f90:13-47 in one pass
         Timed t(TC_POINTWISE);
@@ -42,6 +45,7 @@ std::vector<void*> slot_state() {
     for (auto* p : L.W32) s.push_back(p);
     for (auto* p : L.mlp.buf) s.push_back(p);  // closure scratch, transformed in place
     for (auto* p : L.spares_b) s.push_back(p);
+    for (auto* p : L.keep) s.push_back(p);
     s.push_back(L.spare);
     return s;
 }
@@ -55,6 +59,7 @@ void set_slot_state(const std::vector<void*>& s) {
     for (auto*& p : L.W32) p = static_cast<float*>(s[i++]);
     for (auto*& p : L.mlp.buf) p = static_cast<double*>(s[i++]);
     for (auto*& p : L.spares_b) p = static_cast<double*>(s[i++]);
+    for (auto*& p : L.keep) p = static_cast<double*>(s[i++]);
     L.spare = static_cast<double*>(s[i++]);
 }
 
@@ -73,6 +78,7 @@ void restore_work_slots(const std::vector<void*>& s) {
     for (auto*& p : L.W32) p = static_cast<float*>(s[i++]);
     for (auto*& p : L.mlp.buf) p = static_cast<double*>(s[i++]);
     for (auto*& p : L.spares_b) p = static_cast<double*>(s[i++]);
+    for (auto*& p : L.keep) p = static_cast<double*>(s[i++]);
     L.spare = static_cast<double*>(s[i++]);
 }
 

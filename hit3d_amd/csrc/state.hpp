@@ -51,6 +51,7 @@ struct Lib {
     std::vector<double*> R;  // rhs_old, index 1..nf
     double* spare = nullptr;
     std::vector<double*> spares_b;  // two more spares for batched transforms (N <= 128)
+    std::vector<double*> keep;      // shifted physical velocity kept between the two RHS (3 slots)
 
     cpx<double>* tw64 = nullptr;
     cpx<float>* tw32 = nullptr;
