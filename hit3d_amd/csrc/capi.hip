@@ -275,6 +275,8 @@ int hit3d_step(const double* dt, int64_t* fov, int64_t* fos, const int64_t* int_
         if (L.les_model == 11) mlp_alloc_scratch();
         if (!fused) batch_spares();
         if (scal && L.dealias == 1) keep_slots();
+        // the mixed step keeps rhs_old(1:3) in FP32, every other step in FP64
+        rhs_old_format(mixed);
         const std::vector<void*> s0 = slot_state();
         // the learned closure checks its output on the host mid-step
         if (L.graph_mode && !L.timing && L.les_model != 11)

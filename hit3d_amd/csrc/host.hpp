@@ -17,6 +17,10 @@ size_t slot_bytes();
 double*& slot_ref(int64_t array_id, int64_t s);
 void copy_slot(double* dst, const double* src);
 void zero_slot(double* dst);
+// rhs_old(1:3) to FP32 (the mixed step's format) or back to FP64; no-op when
+// already so.  Every path that reads rhs_old(1:3), other than the mixed
+// step, asks for FP64 first.
+void rhs_old_format(bool fp32);
 void pmul(const double* a, const double* b, double* out);
 void clear_graphs();
 void free_all();

@@ -349,6 +349,7 @@ void do_advance(int which, bool first, double dt, const double* dtp) {
     } else if (which != 1) {
         throw std::runtime_error("advance: which must be 1 (velocity) or 2 (scalars)");
     }
+    if (which == 1) rhs_old_format(false);
     Timed t(TC_POINTWISE);
     for (int c = lo; c <= hi; ++c)
         launch_advance(L.g, L.F[c], L.W[c], L.R[c], dt, dtp, first, L.st);

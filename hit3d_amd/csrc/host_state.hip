@@ -71,6 +71,7 @@ double*& slot_ref(int64_t array_id, int64_t s) {
     }
     if (array_id == 3) {
         if (s < 1 || s > L.nf) throw std::runtime_error("rhs_old slot out of range");
+        if (s <= 3) rhs_old_format(false);
         return L.R[s];
     }
     throw std::runtime_error("unknown array id");
@@ -86,6 +87,27 @@ void zero_slot(double* dst) {
     Lib& L = lib();
     Timed t(TC_POINTWISE);
     launch_fill_zero(dst, (int64_t)(slot_bytes() / sizeof(double)), L.st);
+}
+
+void rhs_old_format(bool fp32) {
+    Lib& L = lib();
+    if (L.ro32 == fp32) return;
+    Timed t(TC_POINTWISE);
+    const int64_t count = (int64_t)(slot_bytes() / sizeof(double));
+    // each slot goes through the spare: the two layouts overlap in place
+    for (int c = 1; c <= 3; ++c) {
+        if (fp32) {
+            check(hipMemcpyAsync(L.spare, L.R[c], slot_bytes(), hipMemcpyDeviceToDevice, L.st),
+                  "hipMemcpyAsync");
+            launch_narrow(L.spare, reinterpret_cast<float*>(L.R[c]), count, L.st);
+        } else {
+            check(hipMemcpyAsync(L.spare, L.R[c], slot_bytes() / 2, hipMemcpyDeviceToDevice,
+                                 L.st),
+                  "hipMemcpyAsync");
+            launch_widen(reinterpret_cast<const float*>(L.spare), L.R[c], count, L.st);
+        }
+    }
+    L.ro32 = fp32;
 }
 
 void pmul(const double* a, const double* b, double* out) {

@@ -122,6 +122,12 @@ __global__ void k_advance(Geo g, double* f, const double* r, double* ro, double 
 __global__ void k_fill_zero(double* p, int64_t count) {
     GRID_STRIDE(i, count) p[i] = 0.0;
 }
+__global__ void k_narrow(const double* __restrict__ src, float* __restrict__ dst, int64_t count) {
+    GRID_STRIDE(i, count) dst[i] = (float)src[i];
+}
+__global__ void k_widen(const float* __restrict__ src, double* __restrict__ dst, int64_t count) {
+    GRID_STRIDE(i, count) dst[i] = (double)src[i];
+}
 
 struct Ptr6 {
     double* p[6];
@@ -497,6 +503,12 @@ void launch_advance_project(const Geo& g, double* const f[3], const double* cons
 }
 void launch_fill_zero(double* p, int64_t count, hipStream_t s) {
     hipLaunchKernelGGL(k_fill_zero, dim3(grid_for(count)), dim3(TB), 0, s, p, count);
+}
+void launch_narrow(const double* src, float* dst, int64_t count, hipStream_t s) {
+    hipLaunchKernelGGL(k_narrow, dim3(grid_for(count)), dim3(TB), 0, s, src, dst, count);
+}
+void launch_widen(const float* src, double* dst, int64_t count, hipStream_t s) {
+    hipLaunchKernelGGL(k_widen, dim3(grid_for(count)), dim3(TB), 0, s, src, dst, count);
 }
 void launch_max_fold(const double* part, int64_t count, double* out, hipStream_t s) {
     hipLaunchKernelGGL(k_max_fold, dim3(1), dim3(1024), 0, s, part, count, out);

@@ -30,6 +30,9 @@ void launch_advance_project(const Geo& g, double* const f[3], const double* cons
                             hipStream_t s);
 // p[0..count) = 0 by a kernel.
 void launch_fill_zero(double* p, int64_t count, hipStream_t s);
+// dst[i] = src[i] rounded to FP32 / widened to FP64, i < count (no overlap).
+void launch_narrow(const double* src, float* dst, int64_t count, hipStream_t s);
+void launch_widen(const float* src, double* dst, int64_t count, hipStream_t s);
 // rhs_velocity.f90:31-32,56-72: max(|u|+|v|+|w|) into *maxout (per-block
 // maxima staged in blockmax, >= 65536 doubles) and the six products u_i u_j
 // (written to out[0..5], which may alias u, v, w).

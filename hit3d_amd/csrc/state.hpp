@@ -49,6 +49,10 @@ struct Lib {
     std::vector<double*> F;  // fields, index 1..nf (0 unused)
     std::vector<double*> W;  // wrk, index 0..nw
     std::vector<double*> R;  // rhs_old, index 1..nf
+    // rhs_old(1:3) holds FP32 values (cpx<float> at the FP64 entry index, in
+    // the first half of each slot): the mixed step's format, converted back
+    // before anything else reads rhs_old
+    bool ro32 = false;
     double* spare = nullptr;
     std::vector<double*> spares_b;  // two more spares for batched transforms (N <= 128)
     std::vector<double*> keep;      // shifted physical velocity kept between the two RHS (3 slots)

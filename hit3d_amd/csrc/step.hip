@@ -17,7 +17,10 @@
 //      fields and rhs_old.
 // The work buffers (wrk slots) use a pruned row pitch PA >= kmax+1 and T =
 // double (FP64 form) or float (mixed form: FP32 transforms and nonlinear
-// term, FP64 state, update, forcing energy and Courant reduction).
+// term, FP64 state, update, forcing energy and Courant reduction; rhs_old
+// stored in FP32, cpx<float> at the FP64 entry index, which saves a sixth
+// of kernel E's traffic and costs nothing the FP32 RHS had not already
+// lost).
 #include "fft.hpp"
 #include "passes.hpp"
 #include "state.hpp"
@@ -103,12 +106,12 @@ __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_inv_z(SG g, P3 F, W6<T>
 #pragma unroll
     for (int s = 0; s < E; ++s) {
         const int e = t + s * TPF;
-        v[s] = (valid && kept(e, Kc<N>::K, N)) ? to<T>(src[(int64_t)e * g.P]) : cpx<T>{(T)0, (T)0};
+        v[s] = (valid && kept(e, Kc<N>::K, N)) ? to<T>(ld_nt(&src[(int64_t)e * g.P])) : cpx<T>{(T)0, (T)0};
     }
     block_fft<N, E, 1, T, B, PS>(v, t, lds + c, tw + N);
     if (valid) {
 #pragma unroll
-        for (int s = 0; s < E; ++s) dst[(int64_t)(t + s * TPF) * N * g.PA] = v[s];
+        for (int s = 0; s < E; ++s) st_nt(&dst[(int64_t)(t + s * TPF) * N * g.PA], v[s]);
     }
 }
 
@@ -131,14 +134,14 @@ __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_y(SG g, W6<T> A,
     for (int s = 0; s < E; ++s) {
         const int e = t + s * TPF;
         const bool ld = valid && (DIR < 0 || kept(e, Kc<N>::K, N));
-        v[s] = ld ? base[(int64_t)e * g.PA] : cpx<T>{(T)0, (T)0};
+        v[s] = ld ? ld_nt(&base[(int64_t)e * g.PA]) : cpx<T>{(T)0, (T)0};
     }
     block_fft<N, E, DIR, T, B, PS>(v, t, lds + c, tw + N);
     if (valid) {
 #pragma unroll
         for (int s = 0; s < E; ++s) {
             const int e = t + s * TPF;
-            if (DIR > 0 || kept(e, Kc<N>::K, N)) base[(int64_t)e * g.PA] = v[s];
+            if (DIR > 0 || kept(e, Kc<N>::K, N)) st_nt(&base[(int64_t)e * g.PA], v[s]);
         }
     }
 }
@@ -157,7 +160,9 @@ __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_y(SG g, W6<T> A,
 // (computed in FP64 for the FP32 form as well, then rounded once).
 template <int N, typename T>
 __device__ __forceinline__ cpx<T> herm_load(const cpx<T>* rowp, int k, int nk) {
-    // entry k of the Hermitian-extended half spectrum (zero outside kx < nk)
+    // entry k of the Hermitian-extended half spectrum (zero outside kx < nk);
+    // cached loads: the row is read twice (k and N-k), and streaming loads
+    // made the pass slower (1.60 -> 1.72 ms at 512^3 FP64)
     if (2 * k <= N) {
         cpx<T> a = (k < nk) ? rowp[k] : cpx<T>{(T)0, (T)0};
         if (k == 0) a.y = (T)0;
@@ -257,8 +262,8 @@ __global__ __launch_bounds__((XP<N, T>::THREADS)) void k_xp(SG g, W6<T> Wb,
                 const int k = t + s * TPF;
                 if (k < NK) {
                     const cpx<T> zk = z[0][s];
-                    outp[k] = scale(zk + zm, (T)0.5);
-                    outq[k] = scale(mul_mi(zk - zm), (T)0.5);
+                    st_nt(&outp[k], scale(zk + zm, (T)0.5));
+                    st_nt(&outq[k], scale(mul_mi(zk - zm), (T)0.5));
                 }
             }
         }
@@ -282,7 +287,7 @@ __global__ __launch_bounds__((XP<N, T>::THREADS)) void k_xp(SG g, W6<T> Wb,
 // ---------------------------------------------------------------- E
 struct Upd {
     double* F[3];
-    double* R[3];
+    double* R[3];  // rhs_old: FP64, or FP32 values for the mixed form
     const double* eshell;  // band energy per shell 1..kfmax (device)
     double* energy_out;
     const double* dtp;  // the step size (device value)
@@ -341,7 +346,7 @@ __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_fwd_z_update(
         const cpx<T>* src = reinterpret_cast<const cpx<T>*>(qp) + (int64_t)jy * g.PA + kx;
 #pragma unroll
         for (int s = 0; s < E; ++s)
-            dst[s] = valid ? src[(int64_t)(t + s * TPF) * N * g.PA] : cpx<T>{(T)0, (T)0};
+            dst[s] = valid ? ld_nt(&src[(int64_t)(t + s * TPF) * N * g.PA]) : cpx<T>{(T)0, (T)0};
     };
     cpx<T> nxt[E];
     if (PREFETCH) load(0, nxt);
@@ -412,8 +417,15 @@ __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_fwd_z_update(
             const int64_t off = ((int64_t)jy * N + e) * g.P + kx;
 #pragma unroll
             for (int cc = 0; cc < 3; ++cc) {
-                f[cc][q] = reinterpret_cast<const cpx<double>*>(u.F[cc])[off];
-                if (!u.first) o[cc][q] = reinterpret_cast<const cpx<double>*>(u.R[cc])[off];
+                f[cc][q] = ld_nt(&reinterpret_cast<const cpx<double>*>(u.F[cc])[off]);
+                if (!u.first) {
+                    if constexpr (sizeof(T) == 8) {
+                        o[cc][q] = ld_nt(&reinterpret_cast<const cpx<double>*>(u.R[cc])[off]);
+                    } else {
+                        const cpx<float> r32 = ld_nt(&reinterpret_cast<const cpx<float>*>(u.R[cc])[off]);
+                        o[cc][q] = cpx<double>{(double)r32.x, (double)r32.y};
+                    }
+                }
             }
         }
 #pragma unroll
@@ -444,16 +456,16 @@ __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_fwd_z_update(
                     fn[cc] = {fc.x + dt * (1.5 * r.x - 0.5 * oc.x),
                               fc.y + dt * (1.5 * r.y - 0.5 * oc.y)};
                 }
-                reinterpret_cast<cpx<double>*>(u.R[cc])[off] = r;
+                st_nt(&reinterpret_cast<cpx<T>*>(u.R[cc])[off], to<T>(r));
             }
             // projection, pressure.f90:21-43
             const double dre = kxd * fn[0].x + kyd * fn[1].x + kzd * fn[2].x;
             const double dim = kxd * fn[0].y + kyd * fn[1].y + kzd * fn[2].y;
             const double lapl = k2 == 0.0 ? 9e20 : k2;
             const double pr = dre / lapl, pi = dim / lapl;
-            reinterpret_cast<cpx<double>*>(u.F[0])[off] = {fn[0].x - pr * kxd, fn[0].y - pi * kxd};
-            reinterpret_cast<cpx<double>*>(u.F[1])[off] = {fn[1].x - pr * kyd, fn[1].y - pi * kyd};
-            reinterpret_cast<cpx<double>*>(u.F[2])[off] = {fn[2].x - pr * kzd, fn[2].y - pi * kzd};
+            st_nt(&reinterpret_cast<cpx<double>*>(u.F[0])[off], cpx<double>{fn[0].x - pr * kxd, fn[0].y - pi * kxd});
+            st_nt(&reinterpret_cast<cpx<double>*>(u.F[1])[off], cpx<double>{fn[1].x - pr * kyd, fn[1].y - pi * kyd});
+            st_nt(&reinterpret_cast<cpx<double>*>(u.F[2])[off], cpx<double>{fn[2].x - pr * kzd, fn[2].y - pi * kzd});
         }
     }
 }

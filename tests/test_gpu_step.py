@@ -132,3 +132,47 @@ def test_step_rejects_bad_precision():
     with pytest.raises(hit3d_amd.Hit3dError, match="precision"):
         h.step(0.001, True, True, precision=7)
     h.close()
+
+
+def test_mixed_rhs_old_storage_and_transitions():
+    """The mixed step keeps rhs_old(1:3) in FP32 between steps.  Read back
+    after a mixed step it holds FP32-representable values and zeros outside
+    the 2/3 cube; after an FP64 step, FP64 values.  Reading it between steps
+    (a conversion to FP64 and, at the next mixed step, back) must leave a
+    trajectory that switches precisions bit-identical to the same trajectory
+    without the reads."""
+    n = 32
+    note("mixed rhs_old storage and precision switches n=32")
+    h = fresh(n)
+    f0 = fields(h)
+    r0 = fields(h, RHS_OLD)
+    seq = [1, 1, 0, 1, 1, 0, 0, 1]
+
+    def run(read_between):
+        for c in range(3):
+            h.put(FIELDS, c + 1, f0[c])
+            h.put(RHS_OLD, c + 1, r0[c])
+        fov = True
+        seen = []
+        for i, p in enumerate(seq):
+            _, _, fov, _ = h.step(0.002, fov, True, precision=p)
+            if read_between:
+                seen.append((p, fields(h, RHS_OLD)))
+        return fields(h), fields(h, RHS_OLD), seen
+
+    fa, ra, _ = run(False)
+    fb, rb, seen = run(True)
+    assert np.array_equal(fa, fb) and np.array_equal(ra, rb)
+    K = n // 3
+    j = np.arange(n)
+    kept_j = (j <= K) | (j >= n - K)
+    kept_x = np.repeat(np.arange(n // 2 + 1) <= K, 2)
+    cube = kept_j[:, None, None] & kept_j[None, :, None] & kept_x[None, None, :]
+    for p, ro in seen:
+        as32 = ro.astype(np.float32).astype(np.float64)
+        if p == 1:
+            assert np.array_equal(ro, as32)
+            assert not np.any(ro[:, ~cube])
+        else:
+            assert not np.array_equal(ro, as32)
+    h.close()
