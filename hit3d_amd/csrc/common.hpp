@@ -49,7 +49,9 @@ __host__ __device__ __forceinline__ cpx<T> mul_mi(cpx<T> a) { return {a.y, -a.x}
 // fields read or written once per pass: a 512^3 slot is 1 GB, four times the
 // last-level cache, so nothing a pass streams is found again by a later one.
 // Marking those accesses streaming took the 512^3 FP64 step from 6.44 to
-// 5.7 ms.
+// 5.7 ms.  From 64^3 up every grid streams (the 64^3 LES step ran 0.271 ->
+// 0.26 ms with it); 32^3 keeps cached accesses (streaming cost its step 4 %:
+// 66 -> 69 us).
 template <typename T>
 __device__ __forceinline__ cpx<T> ld_nt(const cpx<T>* p) {
     typedef T v2 __attribute__((ext_vector_type(2)));
@@ -61,6 +63,20 @@ __device__ __forceinline__ void st_nt(cpx<T>* p, cpx<T> v) {
     typedef T v2 __attribute__((ext_vector_type(2)));
     const v2 x = {v.x, v.y};
     __builtin_nontemporal_store(x, reinterpret_cast<v2*>(p));
+}
+
+template <int N>
+inline constexpr bool kStream = N >= 64;
+__host__ __device__ inline bool streams(int n) { return n >= 64; }
+template <bool NT, typename T>
+__device__ __forceinline__ cpx<T> ld_g(const cpx<T>* p) {
+    if constexpr (NT) return ld_nt(p);
+    else return *p;
+}
+template <bool NT, typename T>
+__device__ __forceinline__ void st_g(cpx<T>* p, cpx<T> v) {
+    if constexpr (NT) st_nt(p, v);
+    else *p = v;
 }
 
 // Pitch (complex entries per row) for grid size n.

@@ -74,7 +74,7 @@ __global__ __launch_bounds__(B* N / E) void k_cols(Slots3 io, int64_t a_in, int6
     cpx<T> v[E];
 #pragma unroll
     for (int s = 0; s < E; ++s) {
-        v[s] = valid ? ld_nt(&src[(int64_t)(t + s * TPF) * s_in]) : cpx<T>{(T)0, (T)0};
+        v[s] = valid ? ld_g<kStream<N>>(&src[(int64_t)(t + s * TPF) * s_in]) : cpx<T>{(T)0, (T)0};
         if constexpr (PH > 0)
             if (valid) v[s] = phase<N, 1>(v[s], col, (int)a, t + s * TPF, tab);
     }
@@ -95,7 +95,7 @@ __global__ __launch_bounds__(B* N / E) void k_cols(Slots3 io, int64_t a_in, int6
                 const int64_t off = a * a_out + col + (int64_t)(t + (h * H + q) * TPF) * s_out;
 #pragma unroll
                 for (int j = 0; j < 2; ++j)
-                    if (acc.r[j]) r[j][q] = ld_nt(&acc.r[j][off]);
+                    if (acc.r[j]) r[j][q] = ld_g<kStream<N>>(&acc.r[j][off]);
             }
 #pragma unroll
             for (int q = 0; q < H; ++q) {
@@ -113,7 +113,7 @@ __global__ __launch_bounds__(B* N / E) void k_cols(Slots3 io, int64_t a_in, int6
                     if (acc.mask[j] &&
                         alias_count(col, wavenum((int)a, N), wavenum(t + s * TPF, N), N) >= 2)
                         val = {0.0, 0.0};
-                    st_nt(&acc.r[j][off], val);
+                    st_g<kStream<N>>(&acc.r[j][off], val);
                 }
             }
         }
@@ -121,7 +121,7 @@ __global__ __launch_bounds__(B* N / E) void k_cols(Slots3 io, int64_t a_in, int6
 #pragma unroll
         for (int s = 0; s < E; ++s) {
             if constexpr (PH < 0) v[s] = phase<N, -1>(v[s], col, (int)a, t + s * TPF, tab);
-            st_nt(&dst[(int64_t)(t + s * TPF) * s_out], v[s]);
+            st_g<kStream<N>>(&dst[(int64_t)(t + s * TPF) * s_out], v[s]);
         }
     }
 }
@@ -196,7 +196,7 @@ __global__ __launch_bounds__((RowCfg<N, T>::ROWS * RowCfg<N, T>::TPF)) void k_r2
     cpx<T>* rowp = reinterpret_cast<cpx<T>*>(data) + row * P;
     cpx<T> v[E];
 #pragma unroll
-    for (int s = 0; s < E; ++s) v[s] = ld_nt(&rowp[t + s * TPF]);
+    for (int s = 0; s < E; ++s) v[s] = ld_g<kStream<N>>(&rowp[t + s * TPF]);
     block_fft<M, E, -1, T, 1>(v, t, lds[r], tw + M);
     __syncthreads();
 #pragma unroll
@@ -210,11 +210,11 @@ __global__ __launch_bounds__((RowCfg<N, T>::ROWS * RowCfg<N, T>::TPF)) void k_r2
         cpx<T> zm = conj(lds[r][(M - k) & (M - 1)]);
         cpx<T> e = scale(zk + zm, (T)0.5);
         cpx<T> o = scale(mul_mi(zk - zm), (T)0.5);
-        st_nt(&rowp[k], e + cmul(twN[k], o));
+        st_g<kStream<N>>(&rowp[k], e + cmul(twN[k], o));
     }
     if (t == 0) {
         cpx<T> z0 = lds[r][0];
-        st_nt(&rowp[M], cpx<T>{z0.x - z0.y, (T)0});
+        st_g<kStream<N>>(&rowp[M], cpx<T>{z0.x - z0.y, (T)0});
     }
 }
 
@@ -235,7 +235,7 @@ __global__ __launch_bounds__((RowCfg<N, T>::ROWS * RowCfg<N, T>::TPF)) void k_r2
     cpx<T> v[E];
 #pragma unroll
     for (int s = 0; s < E; ++s) {
-        const cpx<T> x = ld_nt(&ap[t + s * TPF]), y = ld_nt(&bp[t + s * TPF]);
+        const cpx<T> x = ld_g<kStream<N>>(&ap[t + s * TPF]), y = ld_g<kStream<N>>(&bp[t + s * TPF]);
         v[s] = cpx<T>{x.x * y.x, x.y * y.y};
     }
     block_fft<M, E, -1, T, 1>(v, t, lds[r], tw + M);
@@ -251,11 +251,11 @@ __global__ __launch_bounds__((RowCfg<N, T>::ROWS * RowCfg<N, T>::TPF)) void k_r2
         cpx<T> zm = conj(lds[r][(M - k) & (M - 1)]);
         cpx<T> e = scale(zk + zm, (T)0.5);
         cpx<T> o = scale(mul_mi(zk - zm), (T)0.5);
-        st_nt(&rowp[k], e + cmul(twN[k], o));
+        st_g<kStream<N>>(&rowp[k], e + cmul(twN[k], o));
     }
     if (t == 0) {
         cpx<T> z0 = lds[r][0];
-        st_nt(&rowp[M], cpx<T>{z0.x - z0.y, (T)0});
+        st_g<kStream<N>>(&rowp[M], cpx<T>{z0.x - z0.y, (T)0});
     }
 }
 
@@ -274,8 +274,8 @@ __global__ __launch_bounds__((RowCfg<N, T>::ROWS * RowCfg<N, T>::TPF)) void k_c2
     const int64_t row = (int64_t)blockIdx.x * C::ROWS + r;
     cpx<T>* rowp = reinterpret_cast<cpx<T>*>(data) + row * P;
 #pragma unroll
-    for (int s = 0; s < E; ++s) lds[r][t + s * TPF] = ld_nt(&rowp[t + s * TPF]);
-    if (t == 0) lds[r][M] = ld_nt(&rowp[M]);
+    for (int s = 0; s < E; ++s) lds[r][t + s * TPF] = ld_g<kStream<N>>(&rowp[t + s * TPF]);
+    if (t == 0) lds[r][M] = ld_g<kStream<N>>(&rowp[M]);
     __syncthreads();
     const cpx<T>* twN = tw + N;
     cpx<T> v[E];
@@ -294,7 +294,7 @@ __global__ __launch_bounds__((RowCfg<N, T>::ROWS * RowCfg<N, T>::TPF)) void k_c2
     }
     block_fft<M, E, 1, T, 1>(v, t, lds[r], tw + M);
 #pragma unroll
-    for (int s = 0; s < E; ++s) st_nt(&rowp[t + s * TPF], scale(v[s], norm));
+    for (int s = 0; s < E; ++s) st_g<kStream<N>>(&rowp[t + s * TPF], scale(v[s], norm));
 }
 
 static Slots3 batch_of(void* const* slots, int nb) {

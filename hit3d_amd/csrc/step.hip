@@ -49,6 +49,17 @@ struct CE {
     static constexpr int THREADS = B * TPF;
 };
 
+// y passes: FP32 blocks of 16 columns at N >= 512 take radix 16 (half the
+// threads, one LDS exchange fewer): 0.377 -> 0.350 ms inverse and 0.735 ->
+// 0.704 ms forward at 512^3; the z kernels measured slower that way
+template <int N, typename T>
+struct CY {
+    static constexpr int B = CE<N, T>::B;
+    static constexpr int E = (sizeof(T) == 4 && B * N / 8 > 512) ? 16 : CE<N, T>::E;
+    static constexpr int TPF = N / E;
+    static constexpr int THREADS = B * TPF;
+};
+
 // x pass as full-length complex transforms of two real rows at once:
 // z = a + i b, TPF threads per row, all inside one wavefront
 template <int N, typename T>
@@ -106,12 +117,12 @@ __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_inv_z(SG g, P3 F, W6<T>
 #pragma unroll
     for (int s = 0; s < E; ++s) {
         const int e = t + s * TPF;
-        v[s] = (valid && kept(e, Kc<N>::K, N)) ? to<T>(ld_nt(&src[(int64_t)e * g.P])) : cpx<T>{(T)0, (T)0};
+        v[s] = (valid && kept(e, Kc<N>::K, N)) ? to<T>(ld_g<kStream<N>>(&src[(int64_t)e * g.P])) : cpx<T>{(T)0, (T)0};
     }
     block_fft<N, E, 1, T, B, PS>(v, t, lds + c, tw + N);
     if (valid) {
 #pragma unroll
-        for (int s = 0; s < E; ++s) st_nt(&dst[(int64_t)(t + s * TPF) * N * g.PA], v[s]);
+        for (int s = 0; s < E; ++s) st_g<kStream<N>>(&dst[(int64_t)(t + s * TPF) * N * g.PA], v[s]);
     }
 }
 
@@ -119,9 +130,9 @@ __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_inv_z(SG g, P3 F, W6<T>
 // DIR=+1: inverse y, loads kept rows only, stores all rows.
 // DIR=-1: forward y, loads all rows, stores kept rows only.
 template <int N, int DIR, typename T>
-__global__ __launch_bounds__((CE<N, T>::THREADS)) void k_y(SG g, W6<T> A,
+__global__ __launch_bounds__((CY<N, T>::THREADS)) void k_y(SG g, W6<T> A,
                                                          const cpx<T>* __restrict__ tw) {
-    constexpr int E = CE<N, T>::E, B = CE<N, T>::B, TPF = CE<N, T>::TPF;
+    constexpr int E = CY<N, T>::E, B = CY<N, T>::B, TPF = CY<N, T>::TPF;
     constexpr int PS = 0;
     __shared__ cpx<T> lds[padded_size<PS>(N * B)];
     const int c = threadIdx.x % B, t = threadIdx.x / B;
@@ -134,14 +145,14 @@ __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_y(SG g, W6<T> A,
     for (int s = 0; s < E; ++s) {
         const int e = t + s * TPF;
         const bool ld = valid && (DIR < 0 || kept(e, Kc<N>::K, N));
-        v[s] = ld ? ld_nt(&base[(int64_t)e * g.PA]) : cpx<T>{(T)0, (T)0};
+        v[s] = ld ? ld_g<kStream<N>>(&base[(int64_t)e * g.PA]) : cpx<T>{(T)0, (T)0};
     }
     block_fft<N, E, DIR, T, B, PS>(v, t, lds + c, tw + N);
     if (valid) {
 #pragma unroll
         for (int s = 0; s < E; ++s) {
             const int e = t + s * TPF;
-            if (DIR > 0 || kept(e, Kc<N>::K, N)) st_nt(&base[(int64_t)e * g.PA], v[s]);
+            if (DIR > 0 || kept(e, Kc<N>::K, N)) st_g<kStream<N>>(&base[(int64_t)e * g.PA], v[s]);
         }
     }
 }
@@ -262,8 +273,8 @@ __global__ __launch_bounds__((XP<N, T>::THREADS)) void k_xp(SG g, W6<T> Wb,
                 const int k = t + s * TPF;
                 if (k < NK) {
                     const cpx<T> zk = z[0][s];
-                    st_nt(&outp[k], scale(zk + zm, (T)0.5));
-                    st_nt(&outq[k], scale(mul_mi(zk - zm), (T)0.5));
+                    st_g<kStream<N>>(&outp[k], scale(zk + zm, (T)0.5));
+                    st_g<kStream<N>>(&outq[k], scale(mul_mi(zk - zm), (T)0.5));
                 }
             }
         }
@@ -346,7 +357,7 @@ __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_fwd_z_update(
         const cpx<T>* src = reinterpret_cast<const cpx<T>*>(qp) + (int64_t)jy * g.PA + kx;
 #pragma unroll
         for (int s = 0; s < E; ++s)
-            dst[s] = valid ? ld_nt(&src[(int64_t)(t + s * TPF) * N * g.PA]) : cpx<T>{(T)0, (T)0};
+            dst[s] = valid ? ld_g<kStream<N>>(&src[(int64_t)(t + s * TPF) * N * g.PA]) : cpx<T>{(T)0, (T)0};
     };
     cpx<T> nxt[E];
     if (PREFETCH) load(0, nxt);
@@ -417,12 +428,12 @@ __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_fwd_z_update(
             const int64_t off = ((int64_t)jy * N + e) * g.P + kx;
 #pragma unroll
             for (int cc = 0; cc < 3; ++cc) {
-                f[cc][q] = ld_nt(&reinterpret_cast<const cpx<double>*>(u.F[cc])[off]);
+                f[cc][q] = ld_g<kStream<N>>(&reinterpret_cast<const cpx<double>*>(u.F[cc])[off]);
                 if (!u.first) {
                     if constexpr (sizeof(T) == 8) {
-                        o[cc][q] = ld_nt(&reinterpret_cast<const cpx<double>*>(u.R[cc])[off]);
+                        o[cc][q] = ld_g<kStream<N>>(&reinterpret_cast<const cpx<double>*>(u.R[cc])[off]);
                     } else {
-                        const cpx<float> r32 = ld_nt(&reinterpret_cast<const cpx<float>*>(u.R[cc])[off]);
+                        const cpx<float> r32 = ld_g<kStream<N>>(&reinterpret_cast<const cpx<float>*>(u.R[cc])[off]);
                         o[cc][q] = cpx<double>{(double)r32.x, (double)r32.y};
                     }
                 }
@@ -456,16 +467,16 @@ __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_fwd_z_update(
                     fn[cc] = {fc.x + dt * (1.5 * r.x - 0.5 * oc.x),
                               fc.y + dt * (1.5 * r.y - 0.5 * oc.y)};
                 }
-                st_nt(&reinterpret_cast<cpx<T>*>(u.R[cc])[off], to<T>(r));
+                st_g<kStream<N>>(&reinterpret_cast<cpx<T>*>(u.R[cc])[off], to<T>(r));
             }
             // projection, pressure.f90:21-43
             const double dre = kxd * fn[0].x + kyd * fn[1].x + kzd * fn[2].x;
             const double dim = kxd * fn[0].y + kyd * fn[1].y + kzd * fn[2].y;
             const double lapl = k2 == 0.0 ? 9e20 : k2;
             const double pr = dre / lapl, pi = dim / lapl;
-            st_nt(&reinterpret_cast<cpx<double>*>(u.F[0])[off], cpx<double>{fn[0].x - pr * kxd, fn[0].y - pi * kxd});
-            st_nt(&reinterpret_cast<cpx<double>*>(u.F[1])[off], cpx<double>{fn[1].x - pr * kyd, fn[1].y - pi * kyd});
-            st_nt(&reinterpret_cast<cpx<double>*>(u.F[2])[off], cpx<double>{fn[2].x - pr * kzd, fn[2].y - pi * kzd});
+            st_g<kStream<N>>(&reinterpret_cast<cpx<double>*>(u.F[0])[off], cpx<double>{fn[0].x - pr * kxd, fn[0].y - pi * kxd});
+            st_g<kStream<N>>(&reinterpret_cast<cpx<double>*>(u.F[1])[off], cpx<double>{fn[1].x - pr * kyd, fn[1].y - pi * kyd});
+            st_g<kStream<N>>(&reinterpret_cast<cpx<double>*>(u.F[2])[off], cpx<double>{fn[2].x - pr * kzd, fn[2].y - pi * kzd});
         }
     }
 }
@@ -511,7 +522,7 @@ static void run(Lib& L, const double* dtp, bool first, const cpx<T>* tw) {
     }
     {
         Timed tm(TC_STEP_INV_Y);
-        hipLaunchKernelGGL((k_y<N, 1, T>), dim3(ncb, N, 3), dim3(CE<N, T>::THREADS), 0, L.st, g, Wb,
+        hipLaunchKernelGGL((k_y<N, 1, T>), dim3(ncb, N, 3), dim3(CY<N, T>::THREADS), 0, L.st, g, Wb,
                            tw);
     }
     {
@@ -525,7 +536,7 @@ static void run(Lib& L, const double* dtp, bool first, const cpx<T>* tw) {
     }
     {
         Timed tm(TC_STEP_FWD_Y);
-        hipLaunchKernelGGL((k_y<N, -1, T>), dim3(ncb, N, 6), dim3(CE<N, T>::THREADS), 0, L.st, g,
+        hipLaunchKernelGGL((k_y<N, -1, T>), dim3(ncb, N, 6), dim3(CY<N, T>::THREADS), 0, L.st, g,
                            Wb, tw);
     }
     {

@@ -106,8 +106,11 @@ int hit3d_pressure(void);
  * rhs_scalars, scalar update, rhs_velocity, force_velocity (flow_type 1),
  * velocity update, pressure.  fov/fos as in main.f90:71-72 (updated).
  * precision: 0 = FP64 throughout; 1 = FP32 transforms and nonlinear term
- * with FP64 state, update and reductions.  Returns courant and, when
- * forced, the band energy.  Leaves wrk undefined. */
+ * with FP64 state, update and reductions; the mixed form stores
+ * rhs_old(1:3) in FP32, so after it hit3d_get of rhs_old returns
+ * FP32-rounded values (converted to FP64 on that read and before any other
+ * step).  Returns courant and, when forced, the band energy.  Leaves wrk
+ * undefined. */
 int hit3d_step(const double *dt, int64_t *fov, int64_t *fos, const int64_t *int_scalars,
                const int64_t *precision, double *courant, double *energy);
 
