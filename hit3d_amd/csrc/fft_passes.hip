@@ -73,13 +73,20 @@ __global__ __launch_bounds__(B* N / E) void k_cols(Slots3 io, int64_t a_in, int6
     cpx<T>* dst = out + a * a_out + col;
     cpx<T> v[E];
 #pragma unroll
-    for (int s = 0; s < E; ++s) {
+    for (int s = 0; s < E; ++s)
         v[s] = valid ? ld_g<kStream<N>>(&src[(int64_t)(t + s * TPF) * s_in]) : cpx<T>{(T)0, (T)0};
-        if constexpr (PH > 0)
-            if (valid) v[s] = phase<N, 1>(v[s], col, (int)a, t + s * TPF, tab);
-    }
+    // The phase factors in loops of their own: interleaved with the loads
+    // or stores, each table load was issued at its use and waited for on
+    // its own, one memory round trip per element.
+    if constexpr (PH > 0)
+        if (valid)
+#pragma unroll
+            for (int s = 0; s < E; ++s) v[s] = phase<N, 1>(v[s], col, (int)a, t + s * TPF, tab);
     block_fft<N, E, DIR, T, B, PS>(v, t, lds + c, tw + N);
     if (!valid) return;
+    if constexpr (PH < 0)
+#pragma unroll
+        for (int s = 0; s < E; ++s) v[s] = phase<N, -1>(v[s], col, (int)a, t + s * TPF, tab);
     if constexpr (ACC) {
         // rhs += -1/2 i k_axis v for each target (rhs_velocity.f90:282-398).
         // Every target load is issued before the first store: the compiler
@@ -100,7 +107,6 @@ __global__ __launch_bounds__(B* N / E) void k_cols(Slots3 io, int64_t a_in, int6
 #pragma unroll
             for (int q = 0; q < H; ++q) {
                 const int s = h * H + q;
-                if constexpr (PH < 0) v[s] = phase<N, -1>(v[s], col, (int)a, t + s * TPF, tab);
                 const int64_t off = a * a_out + col + (int64_t)(t + s * TPF) * s_out;
 #pragma unroll
                 for (int j = 0; j < 2; ++j) {
@@ -119,10 +125,7 @@ __global__ __launch_bounds__(B* N / E) void k_cols(Slots3 io, int64_t a_in, int6
         }
     } else {
 #pragma unroll
-        for (int s = 0; s < E; ++s) {
-            if constexpr (PH < 0) v[s] = phase<N, -1>(v[s], col, (int)a, t + s * TPF, tab);
-            st_g<kStream<N>>(&dst[(int64_t)(t + s * TPF) * s_out], v[s]);
-        }
+        for (int s = 0; s < E; ++s) st_g<kStream<N>>(&dst[(int64_t)(t + s * TPF) * s_out], v[s]);
     }
 }
 

@@ -114,10 +114,21 @@ __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_inv_z(SG g, P3 F, W6<T>
     const cpx<double>* src = reinterpret_cast<const cpx<double>*>(F.p[comp]) + (int64_t)jy * N * g.P + kx;
     cpx<T>* dst = reinterpret_cast<cpx<T>*>(A.p[comp]) + (int64_t)jy * g.PA + kx;
     cpx<T> v[E];
+    // Branch-free loads (an in-bounds address for dead lanes, conversion
+    // after the load): as a branch, the FP32 form's conversion waited for
+    // every load in flight, one round trip per element.  Slots with no kept
+    // row in any lane (constant once unrolled) load nothing.
+    const cpx<double>* base = reinterpret_cast<const cpx<double>*>(F.p[comp]);
 #pragma unroll
     for (int s = 0; s < E; ++s) {
+        if (!slot_live<N, TPF>(s)) {
+            v[s] = cpx<T>{(T)0, (T)0};
+            continue;
+        }
         const int e = t + s * TPF;
-        v[s] = (valid && kept(e, Kc<N>::K, N)) ? to<T>(ld_g<kStream<N>>(&src[(int64_t)e * g.P])) : cpx<T>{(T)0, (T)0};
+        const bool ok = valid && kept(e, Kc<N>::K, N);
+        const cpx<double> x = ld_g<kStream<N>>(ok ? &src[(int64_t)e * g.P] : base);
+        v[s] = ok ? to<T>(x) : cpx<T>{(T)0, (T)0};
     }
     block_fft<N, E, 1, T, B, PS>(v, t, lds + c, tw + N);
     if (valid) {
@@ -169,18 +180,25 @@ __global__ __launch_bounds__((CY<N, T>::THREADS)) void k_y(SG g, W6<T> A,
 // LDS row of N entries per row for the transform exchanges, synchronised per
 // wavefront; twiddles by recurrence from one FP64 table load per butterfly
 // (computed in FP64 for the FP32 form as well, then rounded once).
-template <int N, typename T>
-__device__ __forceinline__ cpx<T> herm_load(const cpx<T>* rowp, int k, int nk) {
-    // entry k of the Hermitian-extended half spectrum (zero outside kx < nk);
-    // cached loads: the row is read twice (k and N-k), and streaming loads
-    // made the pass slower (1.60 -> 1.72 ms at 512^3 FP64)
-    if (2 * k <= N) {
-        cpx<T> a = (k < nk) ? rowp[k] : cpx<T>{(T)0, (T)0};
-        if (k == 0) a.y = (T)0;
-        return a;
-    }
-    const int kk = N - k;
-    return (kk < nk) ? conj(rowp[kk]) : cpx<T>{(T)0, (T)0};
+template <int N, int TPF, int NK, typename T>
+__device__ __forceinline__ cpx<T> herm_load(const cpx<T>* rowp, int t, int s) {
+    // entry k = t + s*TPF of the Hermitian-extended half spectrum (zero
+    // outside kx < NK).  Cached loads: the row is read twice (k and N-k), and
+    // streaming loads made the pass slower (1.60 -> 1.72 ms at 512^3 FP64).
+    // Branch-free: every live slot loads from an in-bounds address and the
+    // selection and conjugation follow the load; as branches, each
+    // conjugation waited for all loads in flight, one round trip per slot.
+    // A slot with no live entry in any lane (constant once unrolled) loads
+    // nothing.
+    if (s * TPF >= NK && s * TPF + TPF - 1 <= N - NK) return cpx<T>{(T)0, (T)0};
+    const int k = t + s * TPF;
+    const bool mirror = 2 * k > N;
+    const int kk = mirror ? N - k : k;
+    const bool live = kk < NK;
+    const cpx<T> a = rowp[live ? kk : 0];
+    const T re = live ? a.x : (T)0;
+    const T im = !live ? (T)0 : mirror ? -a.y : (k != 0 ? a.y : (T)0);
+    return cpx<T>{re, im};
 }
 
 template <int N, typename T>
@@ -223,7 +241,7 @@ __global__ __launch_bounds__((XP<N, T>::THREADS)) void k_xp(SG g, W6<T> Wb,
 #pragma unroll
         for (int s = 0; s < E; ++s) {
             const int k = t + s * TPF;
-            const cpx<T> a = herm_load<N>(U, k, NK), b = herm_load<N>(V, k, NK);
+            const cpx<T> a = herm_load<N, TPF, NK>(U, t, s), b = herm_load<N, TPF, NK>(V, t, s);
             z[0][s] = cpx<T>{a.x - b.y, a.y + b.x};
         }
         multi_fft<N, E, 1, T, 1, 1, FS, PS, WAVE, true, double, TC>(z, t, L, twd + N, tcp);
@@ -237,7 +255,7 @@ __global__ __launch_bounds__((XP<N, T>::THREADS)) void k_xp(SG g, W6<T> Wb,
         const cpx<T>* Wr = reinterpret_cast<const cpx<T>*>(Wb.p[2]) + row * g.PA;
         cpx<T> z[1][E];
 #pragma unroll
-        for (int s = 0; s < E; ++s) z[0][s] = herm_load<N>(Wr, t + s * TPF, NK);
+        for (int s = 0; s < E; ++s) z[0][s] = herm_load<N, TPF, NK>(Wr, t, s);
         lds_sync<WAVE>();  // the previous transform's readers are done
         multi_fft<N, E, 1, T, 1, 1, FS, PS, WAVE, true, double, TC>(z, t, L, twd + N, tcp);
 #pragma unroll
