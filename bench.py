@@ -267,6 +267,7 @@ def kernel_bytes(n, cls, precision):
     nk, nky = k + 1, 2 * k + 1
     w = 16 if precision == 0 else 8          # work complex word
     cube = nk * nky * nky * 16               # one FP64 field inside the cube
+    cube_ro = nk * nky * nky * w             # rhs_old inside the cube (FP32 in the mixed form)
     zy = n * nky * nk * w                     # one work buffer with kept ky rows, all z
     full = n * n * nk * w                     # one work buffer, all y and z
     table = {
@@ -275,7 +276,7 @@ def kernel_bytes(n, cls, precision):
         "step_inv_y": 3 * zy + 3 * full,
         "step_x": 3 * full + 6 * full,
         "step_fwd_y": 6 * full + 6 * zy,
-        "step_fwd_z": 6 * zy + 6 * cube + 6 * cube,   # products in; fields, rhs_old in/out
+        "step_fwd_z": 6 * zy + 6 * cube + 6 * cube_ro,   # products in; fields, rhs_old in/out
     }
     return table.get(cls)
 

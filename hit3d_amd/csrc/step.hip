@@ -234,6 +234,14 @@ __global__ __launch_bounds__((XP<N, T>::THREADS)) void k_xp(SG g, W6<T> Wb,
         tcp = &tcache;
     }
     T u[E], v[E], w[E];
+    // w's row is loaded with u's and v's, ahead of the first transform
+    // (registers allow it at the same occupancy)
+    cpx<T> zw[1][E];
+    {
+        const cpx<T>* Wr = reinterpret_cast<const cpx<T>*>(Wb.p[2]) + row * g.PA;
+#pragma unroll
+        for (int s = 0; s < E; ++s) zw[0][s] = herm_load<N, TPF, NK>(Wr, t, s);
+    }
     {
         const cpx<T>* U = reinterpret_cast<const cpx<T>*>(Wb.p[0]) + row * g.PA;
         const cpx<T>* V = reinterpret_cast<const cpx<T>*>(Wb.p[1]) + row * g.PA;
@@ -252,10 +260,7 @@ __global__ __launch_bounds__((XP<N, T>::THREADS)) void k_xp(SG g, W6<T> Wb,
         }
     }
     {
-        const cpx<T>* Wr = reinterpret_cast<const cpx<T>*>(Wb.p[2]) + row * g.PA;
-        cpx<T> z[1][E];
-#pragma unroll
-        for (int s = 0; s < E; ++s) z[0][s] = herm_load<N, TPF, NK>(Wr, t, s);
+        cpx<T>(&z)[1][E] = zw;
         lds_sync<WAVE>();  // the previous transform's readers are done
         multi_fft<N, E, 1, T, 1, 1, FS, PS, WAVE, true, double, TC>(z, t, L, twd + N, tcp);
 #pragma unroll
