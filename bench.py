@@ -432,6 +432,16 @@ def main():
                 "alg_bytes_per_launch": byt, "avg_launch_ms": avg_s * 1e3,
                 "launches_per_step": cnt / a.steps, "copy_kernel_GBs": copy_gbs,
                 "frac_of_copy": ach / copy_gbs}
+    # every kernel class of the FP64 step against the same peak: the dominant
+    # one above is the x pass, bound by arithmetic and LDS exchanges; the
+    # others are HBM-bound
+    by_kernel = {}
+    for cls, (ms, cnt) in tim64.items():
+        byt = kernel_bytes(n, cls, 0)
+        if cnt > 0 and byt is not None:
+            ach = byt / (ms / cnt / 1e3) / 1e9
+            by_kernel[cls] = {"achieved_GBs": ach, "frac": ach / PEAK_HBM_GBS,
+                              "frac_of_copy": ach / copy_gbs}
     s = slot_bytes(n)
     b_alg = 6 * 9 * s + 4 * 3 * s  # B_alg of the measurement plan, n_fft = 9, no scalars
     out = {
@@ -456,6 +466,7 @@ def main():
         "mixed": mixed,
         "kernel_ms_per_step": {k: v[0] / a.steps for k, v in tim64.items() if v[1]},
         "roofline": roof,
+        "roofline_by_kernel": by_kernel,
         "cpu_baseline": None if a.no_cpu else cpu_baseline(n),
     }
     print(json.dumps(out), flush=True)
