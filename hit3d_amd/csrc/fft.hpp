@@ -157,11 +157,22 @@ __device__ __forceinline__ cpx<T> ldtw(const cpx<T>* tw, int j) {
     return tw[j];
 }
 
-// LDS entry padding: one spare entry every 2^PADSH entries spreads the
-// strided Stockham writes over all banks (PADSH = 0: no padding).
+// LDS entry map of the exchanges.  PADSH > 0: one spare entry every
+// 2^PADSH entries; PADSH = 0: none; PADSH = kSwizzle: entry i goes to
+// i ^ ((i >> 3) & 15), a permutation inside each aligned group of 16 (the
+// identity below 8).  A thread's radix-8 outputs are 8 consecutive entries,
+// so in a transform whose threads are consecutive lanes (stride 1 between
+// entries) the exchange writes land 8 entries apart: 128 bytes in FP64, the
+// width of the 32 banks, so unpadded every lane of a group hit the same
+// banks.  The swizzle spreads those writes and keeps the unit-stride reads
+// conflict-free for 8- and 16-byte entries.  The row transforms lost their
+// write conflicts (4.5-way unpadded; one pad per 16 or 32 entries still left
+// 1.5- to 2-way, measured as 35 % of LDS cycles in the x pass).
+inline constexpr int kSwizzle = -1;
 template <int PADSH>
 __device__ __forceinline__ int padi(int i) {
-    if constexpr (PADSH > 0) return i + (i >> PADSH);
+    if constexpr (PADSH == kSwizzle) return i ^ ((i >> 3) & 15);
+    else if constexpr (PADSH > 0) return i + (i >> PADSH);
     else return i;
 }
 template <int PADSH>

@@ -200,7 +200,7 @@ __global__ __launch_bounds__((RowCfg<N, T>::ROWS * RowCfg<N, T>::TPF)) void k_r2
     cpx<T> v[E];
 #pragma unroll
     for (int s = 0; s < E; ++s) v[s] = ld_g<kStream<N>>(&rowp[t + s * TPF]);
-    block_fft<M, E, -1, T, 1>(v, t, lds[r], tw + M);
+    block_fft<M, E, -1, T, 1, kSwizzle>(v, t, lds[r], tw + M);
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < E; ++s) lds[r][t + s * TPF] = v[s];
@@ -241,7 +241,7 @@ __global__ __launch_bounds__((RowCfg<N, T>::ROWS * RowCfg<N, T>::TPF)) void k_r2
         const cpx<T> x = ld_g<kStream<N>>(&ap[t + s * TPF]), y = ld_g<kStream<N>>(&bp[t + s * TPF]);
         v[s] = cpx<T>{x.x * y.x, x.y * y.y};
     }
-    block_fft<M, E, -1, T, 1>(v, t, lds[r], tw + M);
+    block_fft<M, E, -1, T, 1, kSwizzle>(v, t, lds[r], tw + M);
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < E; ++s) lds[r][t + s * TPF] = v[s];
@@ -295,7 +295,7 @@ __global__ __launch_bounds__((RowCfg<N, T>::ROWS * RowCfg<N, T>::TPF)) void k_c2
         // Z_k = (X_k + conj X_{M-k}) + i W_N^{-k} (X_k - conj X_{M-k})
         v[s] = (xk + xm) + mul_i(cmulc(xk - xm, twN[k]));
     }
-    block_fft<M, E, 1, T, 1>(v, t, lds[r], tw + M);
+    block_fft<M, E, 1, T, 1, kSwizzle>(v, t, lds[r], tw + M);
 #pragma unroll
     for (int s = 0; s < E; ++s) st_g<kStream<N>>(&rowp[t + s * TPF], scale(v[s], norm));
 }

@@ -207,7 +207,7 @@ __global__ __launch_bounds__((XP<N, T>::THREADS)) void k_xp(SG g, W6<T> Wb,
                                                           T norm, double* __restrict__ blockmax) {
     using C = XP<N, T>;
     constexpr int E = C::E, TPF = C::TPF;
-    constexpr int PS = PadFor<T>::value;
+    constexpr int PS = kSwizzle;
     constexpr int FS = padded_size<PS>(N);
     constexpr int NK = Kc<N>::NK;
     static_assert(64 % TPF == 0 || TPF % 64 == 0, "row transforms must not straddle waves");
