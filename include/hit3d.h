@@ -6,7 +6,7 @@
  *     a double passed by reference, as a Fortran caller built with
  *     -fdefault-integer-8 -fdefault-real-8 passes its default kinds
  *     (reference Makefile:11,29,53); Fortran binds them with bind(C) and
- *     VALUE-less dummies;
+ *     VALUE-less dummies (the binding module is hit3d_amd/fortran/hit3d_hip.f90);
  *   - every entry point returns 0 on success and a non-zero status on error;
  *     hit3d_last_error() copies the message.  The reference has no error
  *     codes: its fatal paths call my_exit(-1) (my_exit.f90:23-62), which the
