@@ -40,19 +40,18 @@ __device__ __forceinline__ const cpx<double>* C(const double* p) {
 }
 __device__ __forceinline__ cpx<double>* C(double* p) { return reinterpret_cast<cpx<double>*>(p); }
 // access to entry off of a slot in the per-step passes that touch each
-// entry once: streaming from 64^3 up (needs the kernel's Geo g in scope)
-#define ldc(p, off) (streams(g.n) ? ld_nt(&C(p)[off]) : C(p)[off])
-#define stc(p, off, ...)                                   \
-    do {                                                   \
-        const cpx<double> v_ = __VA_ARGS__;                \
-        if (streams(g.n)) st_nt(&C(p)[off], v_);           \
-        else C(p)[off] = v_;                               \
-    } while (0)
-#define std_nt(p, v)                                                \
-    do {                                                            \
-        if (streams(g.n)) __builtin_nontemporal_store((double)(v), (p)); \
-        else *(p) = (v);                                            \
-    } while (0)
+// entry once: streaming from 64^3 up
+__device__ __forceinline__ cpx<double> ldc(const Geo& g, const double* p, int64_t off) {
+    return streams(g.n) ? ld_nt(&C(p)[off]) : C(p)[off];
+}
+__device__ __forceinline__ void stc(const Geo& g, double* p, int64_t off, cpx<double> v) {
+    if (streams(g.n)) st_nt(&C(p)[off], v);
+    else C(p)[off] = v;
+}
+__device__ __forceinline__ void std_nt(const Geo& g, double* p, double v) {
+    if (streams(g.n)) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
 
 #define GRID_STRIDE(i, total) \
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (total); i += (int64_t)gridDim.x * blockDim.x)
@@ -119,14 +118,14 @@ __global__ void k_advance(Geo g, double* f, const double* r, double* ro, double 
     if (dtp) dt = *dtp;
     GRID_STRIDE(i, nspec(g)) {
         SIdx s = sdecode(g, i);
-        cpx<double> a = ldc(f, s.off), b = ldc(r, s.off);
+        cpx<double> a = ldc(g, f, s.off), b = ldc(g, r, s.off);
         if (first) {
-            stc(ro, s.off, b);
-            stc(f, s.off, {a.x + dt * b.x, a.y + dt * b.y});
+            stc(g, ro, s.off, b);
+            stc(g, f, s.off, {a.x + dt * b.x, a.y + dt * b.y});
         } else {
-            cpx<double> o = ldc(ro, s.off);
-            stc(f, s.off, {a.x + dt * (1.5 * b.x - 0.5 * o.x), a.y + dt * (1.5 * b.y - 0.5 * o.y)});
-            stc(ro, s.off, b);
+            cpx<double> o = ldc(g, ro, s.off);
+            stc(g, f, s.off, {a.x + dt * (1.5 * b.x - 0.5 * o.x), a.y + dt * (1.5 * b.y - 0.5 * o.y)});
+            stc(g, ro, s.off, b);
         }
     }
 }
@@ -168,9 +167,9 @@ __global__ void k_advance_project(Geo g, Ptr3 f, CPtr3 r, Ptr3 ro, double dt, co
         cpx<double> u[3], b[3], o[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            u[c] = ldc(f.p[c], s.off);
-            b[c] = ldc(r.p[c], s.off);
-            if (!first) o[c] = ldc(ro.p[c], s.off);
+            u[c] = ldc(g, f.p[c], s.off);
+            b[c] = ldc(g, r.p[c], s.off);
+            if (!first) o[c] = ldc(g, ro.p[c], s.off);
         }
         cpx<double> a[3];
 #pragma unroll
@@ -180,7 +179,7 @@ __global__ void k_advance_project(Geo g, Ptr3 f, CPtr3 r, Ptr3 ro, double dt, co
             else
                 a[c] = {u[c].x + dt * (1.5 * b[c].x - 0.5 * o[c].x),
                         u[c].y + dt * (1.5 * b[c].y - 0.5 * o[c].y)};
-            stc(ro.p[c], s.off, b[c]);
+            stc(g, ro.p[c], s.off, b[c]);
         }
         const double kx = s.kx, ky = s.kyv, kz = s.kzv;
         const double dre = kx * a[0].x + ky * a[1].x + kz * a[2].x;
@@ -188,9 +187,9 @@ __global__ void k_advance_project(Geo g, Ptr3 f, CPtr3 r, Ptr3 ro, double dt, co
         double lap = kx * kx + ky * ky + kz * kz;
         if (lap == 0.0) lap = 9e20;
         const double pr = dre / lap, pi = dim / lap;
-        stc(f.p[0], s.off, {a[0].x - pr * kx, a[0].y - pi * kx});
-        stc(f.p[1], s.off, {a[1].x - pr * ky, a[1].y - pi * ky});
-        stc(f.p[2], s.off, {a[2].x - pr * kz, a[2].y - pi * kz});
+        stc(g, f.p[0], s.off, {a[0].x - pr * kx, a[0].y - pi * kx});
+        stc(g, f.p[1], s.off, {a[1].x - pr * ky, a[1].y - pi * ky});
+        stc(g, f.p[2], s.off, {a[2].x - pr * kz, a[2].y - pi * kz});
     }
 }
 
@@ -207,12 +206,12 @@ __global__ void k_products(Geo g, const double* u, const double* v, const double
         const double t = (fabs(a) + fabs(b)) + fabs(c);
         m = t > m ? t : m;
         const double p0 = a * a, p1 = a * b, p2 = a * c, p3 = b * b, p4 = b * c, p5 = c * c;
-        std_nt(&out.p[0][o], p0);
-        std_nt(&out.p[1][o], p1);
-        std_nt(&out.p[2][o], p2);
-        std_nt(&out.p[3][o], p3);
-        std_nt(&out.p[4][o], p4);
-        std_nt(&out.p[5][o], p5);
+        std_nt(g, &out.p[0][o], p0);
+        std_nt(g, &out.p[1][o], p1);
+        std_nt(g, &out.p[2][o], p2);
+        std_nt(g, &out.p[3][o], p3);
+        std_nt(g, &out.p[4][o], p4);
+        std_nt(g, &out.p[5][o], p5);
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -254,15 +253,15 @@ __global__ void k_rhs_assemble(Geo g, CPtr6 P, CPtr3 U, Ptr3 out, double nu, int
         const int ia = alias_count(s.kx, s.kyv, s.kzv, g.n);
         if (ia > (phase_shift ? 1 : 0)) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) stc(out.p[c], s.off, {0.0, 0.0});
+            for (int c = 0; c < 3; ++c) stc(g, out.p[c], s.off, {0.0, 0.0});
             continue;
         }
         cpx<double> p[6];
 #pragma unroll
-        for (int q = 0; q < 6; ++q) p[q] = ldc(P.p[q], s.off);
+        for (int q = 0; q < 6; ++q) p[q] = ldc(g, P.p[q], s.off);
         cpx<double> u[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) u[c] = ldc(U.p[c], s.off);
+        for (int c = 0; c < 3; ++c) u[c] = ldc(g, U.p[c], s.off);
         const double lap = -nu * (kx * kx + ky * ky + kz * kz);
         const int idx[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
         const double h = phase_shift ? 0.5 : 1.0;
@@ -271,7 +270,7 @@ __global__ void k_rhs_assemble(Geo g, CPtr6 P, CPtr3 U, Ptr3 out, double nu, int
             const cpx<double> a = p[idx[c][0]], b = p[idx[c][1]], d = p[idx[c][2]];
             const double tre = -(kx * a.x + ky * b.x + kz * d.x);
             const double tim = -(kx * a.y + ky * b.y + kz * d.y);
-            stc(out.p[c], s.off, {-(h * tim) + lap * u[c].x, h * tre + lap * u[c].y});
+            stc(g, out.p[c], s.off, {-(h * tim) + lap * u[c].x, h * tre + lap * u[c].y});
         }
     }
 }
@@ -294,18 +293,18 @@ __global__ void k_scalar_rhs(Geo g, CPtr3 Q, const double* F, double* out, doubl
         cpx<double> r = {0.0, 0.0};
         if (alias_count(s.kx, s.kyv, s.kzv, g.n) == 0) {
             const double kx = s.kx, ky = s.kyv, kz = s.kzv;
-            const cpx<double> a = ldc(Q.p[0], s.off), b = ldc(Q.p[1], s.off), c = ldc(Q.p[2], s.off);
-            const cpx<double> f = ldc(F, s.off);
+            const cpx<double> a = ldc(g, Q.p[0], s.off), b = ldc(g, Q.p[1], s.off), c = ldc(g, Q.p[2], s.off);
+            const cpx<double> f = ldc(g, F, s.off);
             const double r1 = kx * a.y + ky * b.y + kz * c.y;
             const double r2 = kx * a.x + ky * b.x + kz * c.x;
             const double k2 = kx * kx + ky * ky + kz * kz;
             r = {r1 - pe * k2 * f.x, -r2 - pe * k2 * f.y};
         }
         if (sub) {
-            const cpx<double> u = ldc(sub, s.off);
+            const cpx<double> u = ldc(g, sub, s.off);
             r = {r.x - u.x, r.y - u.y};
         }
-        stc(out, s.off, r);
+        stc(g, out, s.off, r);
     }
 }
 
@@ -315,23 +314,23 @@ __global__ void k_scalar_rhs_ps(Geo g, CPtr3 Q, const double* F, double* out, do
         SIdx s = sdecode(g, i);
         const bool keep = alias_count(s.kx, s.kyv, s.kzv, g.n) <= 1;
         const double kx = s.kx, ky = s.kyv, kz = s.kzv;
-        const cpx<double> a = ldc(Q.p[0], s.off), b = ldc(Q.p[1], s.off), c = ldc(Q.p[2], s.off);
+        const cpx<double> a = ldc(g, Q.p[0], s.off), b = ldc(g, Q.p[1], s.off), c = ldc(g, Q.p[2], s.off);
         const double r1 = kx * a.y + ky * b.y + kz * c.y;
         const double r2 = kx * a.x + ky * b.x + kz * c.x;
         if (!acc) {
-            stc(out, s.off, keep ? cpx<double>{0.5 * r1, -0.5 * r2} : cpx<double>{0.0, 0.0});
+            stc(g, out, s.off, keep ? cpx<double>{0.5 * r1, -0.5 * r2} : cpx<double>{0.0, 0.0});
         } else {
-            cpx<double> o = ldc(out, s.off);
+            cpx<double> o = ldc(g, out, s.off);
             if (keep) {
-                const cpx<double> f = ldc(F, s.off);
+                const cpx<double> f = ldc(g, F, s.off);
                 const double k2 = kx * kx + ky * ky + kz * kz;
                 o = {o.x + 0.5 * r1 - pe * k2 * f.x, o.y - 0.5 * r2 - pe * k2 * f.y};
             }
             if (sub) {
-                const cpx<double> u = ldc(sub, s.off);
+                const cpx<double> u = ldc(g, sub, s.off);
                 o = {o.x - u.x, o.y - u.y};
             }
-            stc(out, s.off, o);
+            stc(g, out, s.off, o);
         }
     }
 }
