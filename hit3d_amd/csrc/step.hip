@@ -60,11 +60,24 @@ struct CY {
     static constexpr int THREADS = B * TPF;
 };
 
+// forward z + update: FP64 blocks of 4 columns at N >= 1024, where a block
+// of 8 columns is 1024 threads with 128 registers each (it spilled 412 bytes
+// per lane): 20.1 ms -> 17.3-20.5 ms at 1024^3, no spills
+template <int N, typename T>
+struct CZ {
+    static constexpr int B = (N >= 1024 && sizeof(T) == 8) ? 4 : CE<N, T>::B;
+    static constexpr int E = CE<N, T>::E;
+    static constexpr int TPF = N / E;
+    static constexpr int THREADS = B * TPF;
+};
+
 // x pass as full-length complex transforms of two real rows at once:
-// z = a + i b, TPF threads per row, all inside one wavefront
+// z = a + i b, TPF threads per row: inside one wavefront up to N = 512, two
+// wavefronts at N = 1024 (radix 16 in one wavefront needed 256 VGPRs + 81
+// AGPRs, one wavefront per SIMD: 28.7 ms at 1024^3)
 template <int N, typename T>
 struct XP {
-    static constexpr int E = (N >= 1024) ? 16 : (N >= 8 ? 8 : N);
+    static constexpr int E = N >= 8 ? 8 : N;
     static constexpr int TPF = N / E;
     static constexpr int ROWS = (256 / TPF < N * N) ? 256 / TPF : N * N;
     static constexpr int THREADS = ROWS * TPF;
@@ -210,9 +223,10 @@ __global__ __launch_bounds__((XP<N, T>::THREADS)) void k_xp(SG g, W6<T> Wb,
     constexpr int PS = kSwizzle;
     constexpr int FS = padded_size<PS>(N);
     constexpr int NK = Kc<N>::NK;
-    static_assert(64 % TPF == 0 || TPF % 64 == 0, "row transforms must not straddle waves");
-    static_assert(TPF <= 64, "the output split shuffles within a wavefront");
-    constexpr bool WAVE = true;
+    static_assert(64 % TPF == 0 || TPF % 64 == 0, "a row is part of a wavefront or whole ones");
+    // a row inside one wavefront synchronises per wavefront and splits the
+    // outputs by lane shuffles; a row over several, per block and through LDS
+    constexpr bool WAVE = TPF <= 64;
     __shared__ cpx<T> lds[C::ROWS][FS];
     __shared__ double redm[C::THREADS / 64 + 1];
     // With radix 8 throughout (N a power of 8) the five transforms of a row
@@ -287,14 +301,34 @@ __global__ __launch_bounds__((XP<N, T>::THREADS)) void k_xp(SG g, W6<T> Wb,
         multi_fft<N, E, -1, T, 1, 1, FS, PS, WAVE, true, double, TC>(z, t, L, twd + N, tcp);
         cpx<T>* outp = reinterpret_cast<cpx<T>*>(Wb.p[2 * pr]) + row * g.PA;
         cpx<T>* outq = reinterpret_cast<cpx<T>*>(Wb.p[2 * pr + 1]) + row * g.PA;
+        if constexpr (WAVE) {
 #pragma unroll
-        for (int s = 0; s < E; ++s) {
-            if (s * TPF < NK) {  // the same for every lane
-                const cpx<T> mine = (t == 0) ? z[0][(E - s) & (E - 1)] : z[0][E - 1 - s];
-                const cpx<T> zm = conj(cpx<T>{__shfl(mine.x, partner, TPF),
-                                              __shfl(mine.y, partner, TPF)});
+            for (int s = 0; s < E; ++s) {
+                if (s * TPF < NK) {  // the same for every lane
+                    const cpx<T> mine = (t == 0) ? z[0][(E - s) & (E - 1)] : z[0][E - 1 - s];
+                    const cpx<T> zm = conj(cpx<T>{__shfl(mine.x, partner, TPF),
+                                                  __shfl(mine.y, partner, TPF)});
+                    const int k = t + s * TPF;
+                    if (k < NK) {
+                        const cpx<T> zk = z[0][s];
+                        st_g<kStream<N>>(&outp[k], scale(zk + zm, (T)0.5));
+                        st_g<kStream<N>>(&outq[k], scale(mul_mi(zk - zm), (T)0.5));
+                    }
+                }
+            }
+        } else {
+            // Z_{N-k} comes through LDS (after the transform's last exchange
+            // has been read); the next transform synchronises before it
+            // writes the row buffer again
+            lds_sync<false>();
+#pragma unroll
+            for (int s = 0; s < E; ++s) L[padi<PS>(t + s * TPF)] = z[0][s];
+            lds_sync<false>();
+#pragma unroll
+            for (int s = 0; s < E; ++s) {
                 const int k = t + s * TPF;
-                if (k < NK) {
+                if (s * TPF < NK && k < NK) {
+                    const cpx<T> zm = conj(L[padi<PS>((N - k) & (N - 1))]);
                     const cpx<T> zk = z[0][s];
                     st_g<kStream<N>>(&outp[k], scale(zk + zm, (T)0.5));
                     st_g<kStream<N>>(&outq[k], scale(mul_mi(zk - zm), (T)0.5));
@@ -330,9 +364,9 @@ struct Upd {
 };
 
 template <int N, typename T>
-__global__ __launch_bounds__((CE<N, T>::THREADS)) void k_fwd_z_update(
+__global__ __launch_bounds__((CZ<N, T>::THREADS)) void k_fwd_z_update(
     SG g, W6<T> Q, Upd u, const cpx<double>* __restrict__ twd) {
-    constexpr int E = CE<N, T>::E, B = CE<N, T>::B, TPF = CE<N, T>::TPF;
+    constexpr int E = CZ<N, T>::E, B = CZ<N, T>::B, TPF = CZ<N, T>::TPF;
     constexpr int PS = 0;
     __shared__ cpx<T> lds[padded_size<PS>(N * B)];
     const int c = threadIdx.x % B, t = threadIdx.x / B;
@@ -577,7 +611,8 @@ static void run(Lib& L, const double* dtp, bool first, const cpx<T>* tw) {
         u.kfmax = L.kfmax;
         u.forced = forced ? 1 : 0;
         u.first = first ? 1 : 0;
-        hipLaunchKernelGGL((k_fwd_z_update<N, T>), dim3(ncb, nky), dim3(CE<N, T>::THREADS), 0,
+        const int ncbz = (g.NK + CZ<N, T>::B - 1) / CZ<N, T>::B;
+        hipLaunchKernelGGL((k_fwd_z_update<N, T>), dim3(ncbz, nky), dim3(CZ<N, T>::THREADS), 0,
                            L.st, g, Wb, u, L.tw64);
     }
 }
