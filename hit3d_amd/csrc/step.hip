@@ -60,13 +60,15 @@ struct CY {
     static constexpr int THREADS = B * TPF;
 };
 
-// forward z + update: FP64 blocks of 4 columns at N >= 1024, where a block
-// of 8 columns is 1024 threads with 128 registers each (it spilled 412 bytes
-// per lane): 20.1 ms -> 17.3-20.5 ms at 1024^3, no spills
+// forward z + update at N >= 1024, in 512-thread blocks: the column blocks
+// of the other passes are 1024 threads there, with 128 registers each, and
+// spilled 412 (FP64) / 452 (FP32) bytes per lane.  FP64: 4 columns, radix 8
+// (20.1 -> 17.3-20.5 ms at 1024^3); FP32: 8 columns, radix 16 (13.8 ->
+// 9.6 ms)
 template <int N, typename T>
 struct CZ {
-    static constexpr int B = (N >= 1024 && sizeof(T) == 8) ? 4 : CE<N, T>::B;
-    static constexpr int E = CE<N, T>::E;
+    static constexpr int B = N >= 1024 ? (sizeof(T) == 8 ? 4 : 8) : CE<N, T>::B;
+    static constexpr int E = (N >= 1024 && sizeof(T) == 4) ? 16 : CE<N, T>::E;
     static constexpr int TPF = N / E;
     static constexpr int THREADS = B * TPF;
 };
