@@ -20,8 +20,9 @@ extern "C" {
 int hit3d_init(const int64_t* nx, const int64_t* ny, const int64_t* nz_all,
                const int64_t* n_scalars, const int64_t* n_les, const int64_t* dealias) {
     return guarded([&] {
+        // a previous init, complete or not, leaves nothing held
+        free_all();
         Lib& L = lib();
-        if (L.ready) free_all();
         if (*nx != *ny || *nx != *nz_all)
             throw std::runtime_error("only cubic grids nx = ny = nz_all are supported");
         if (!supported_n((int)*nx))
@@ -100,7 +101,8 @@ int hit3d_set_physics(const double* nu, const int64_t* flow_type, const int64_t*
         L.flow_type = (int)*flow_type;
         L.kfmax = (int)*kfmax;
         L.famp = *famp;
-        if (L.kfmax > L.kmax) throw std::runtime_error("kfmax exceeds kmax");
+        // m_force_init (m_force.f90:13-97) reads KFMAX for forced runs only
+        if (L.flow_type == 1 && L.kfmax > L.kmax) throw std::runtime_error("kfmax exceeds kmax");
         L.pe.assign(L.ns, 0.0);
         L.stype.assign(L.ns, 0);
         L.reac.assign(L.ns, 0.0);
@@ -585,6 +587,24 @@ int hit3d_snapshot(const int64_t* n_field, float* out) {
     });
 }
 
+int hit3d_turb_visc(float* out) {
+    return guarded([&] {
+        require_ready();
+        Lib& L = lib();
+        if (L.les_model != 1 || !L.tv)
+            throw std::runtime_error("turb_visc: only les_model 1 holds a turbulent viscosity");
+        float* stage = reinterpret_cast<float*>(L.spare);  // free between transforms
+        {
+            Timed t(TC_POINTWISE);
+            launch_pack_f32(L.g, L.tv, stage, L.st);
+        }
+        check(hipMemcpyAsync(out, stage, sizeof(float) * (size_t)L.n * L.n * L.n,
+                             hipMemcpyDeviceToHost, L.st),
+              "hipMemcpyAsync");
+        check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
+    });
+}
+
 int hit3d_rng_state(int64_t* iy) {
     return guarded([&] { *iy = lib().rng_iy; }, false);
 }
@@ -626,12 +646,10 @@ int hit3d_get_ialias(int8_t* out) {
         require_ready();
         Lib& L = lib();
         const size_t bytes = (size_t)L.n * L.n * (L.n + 2);
-        void* d = nullptr;
-        check(hipMalloc(&d, bytes), "hipMalloc");
-        launch_ialias(L.g, static_cast<int8_t*>(d), L.st);
-        check(hipMemcpyAsync(out, d, bytes, hipMemcpyDeviceToHost, L.st), "hipMemcpyAsync");
+        DevBuf d(bytes);
+        launch_ialias(L.g, d.as<int8_t>(), L.st);
+        check(hipMemcpyAsync(out, d.p, bytes, hipMemcpyDeviceToHost, L.st), "hipMemcpyAsync");
         check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
-        check(hipFree(d), "hipFree");
     });
 }
 

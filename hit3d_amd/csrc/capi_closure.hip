@@ -149,8 +149,9 @@ int hit3d_mlp_grads(const int64_t* count, const float* X, const float* T, float*
         if (!M.have_params) throw std::runtime_error("mlp: parameters first");
         const int64_t n = *count;
         if (n < 32 || n % 32 != 0) throw std::runtime_error("mlp: count must be a multiple of 32");
-        float* dX = reinterpret_cast<float*>(dalloc(sizeof(float) * mlp::NIN * n));
-        float* dT = reinterpret_cast<float*>(dalloc(sizeof(float) * mlp::NOUT * n));
+        DevBuf bX(sizeof(float) * mlp::NIN * n), bT(sizeof(float) * mlp::NOUT * n);
+        float* dX = bX.as<float>();
+        float* dT = bT.as<float>();
         check(hipMemcpyAsync(dX, X, sizeof(float) * mlp::NIN * n, hipMemcpyHostToDevice, L.st), "hipMemcpyAsync");
         check(hipMemcpyAsync(dT, T, sizeof(float) * mlp::NOUT * n, hipMemcpyHostToDevice, L.st), "hipMemcpyAsync");
         const int64_t tiles = n / 32;
@@ -161,8 +162,6 @@ int hit3d_mlp_grads(const int64_t* count, const float* X, const float* T, float*
         check(hipMemcpyAsync(grads, M.g, sizeof(float) * mlp::NPARAM, hipMemcpyDeviceToHost, L.st), "hipMemcpyAsync");
         check(hipMemcpyAsync(&ls, M.loss, sizeof(float), hipMemcpyDeviceToHost, L.st), "hipMemcpyAsync");
         check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
-        check(hipFree(dX), "hipFree");
-        check(hipFree(dT), "hipFree");
         if (loss) *loss = (double)ls / (double)(n * mlp::NOUT);
     });
 }
@@ -175,14 +174,13 @@ int hit3d_mlp_forward(const int64_t* count, const float* X, float* Y) {
         if (!M.have_params) throw std::runtime_error("mlp: parameters first");
         const int64_t n = *count;
         if (n < 1) return;
-        float* dX = reinterpret_cast<float*>(dalloc(sizeof(float) * mlp::NIN * n));
-        float* dY = reinterpret_cast<float*>(dalloc(sizeof(float) * mlp::NOUT * n));
+        DevBuf bX(sizeof(float) * mlp::NIN * n), bY(sizeof(float) * mlp::NOUT * n);
+        float* dX = bX.as<float>();
+        float* dY = bY.as<float>();
         check(hipMemcpyAsync(dX, X, sizeof(float) * mlp::NIN * n, hipMemcpyHostToDevice, L.st), "hipMemcpyAsync");
         launch_mlp_forward(dX, n, M.wb, M.p, dY, L.st);
         check(hipMemcpyAsync(Y, dY, sizeof(float) * mlp::NOUT * n, hipMemcpyDeviceToHost, L.st), "hipMemcpyAsync");
         check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
-        check(hipFree(dX), "hipFree");
-        check(hipFree(dY), "hipFree");
     });
 }
 
@@ -196,7 +194,8 @@ int hit3d_box_filter(const int64_t* ratio, double* ubar, double* tau) {
             throw std::runtime_error("box filter: grids up to 512 points per direction");
         const int m = L.n / r;
         const size_t m3 = (size_t)m * m * m;
-        double* d = dalloc(sizeof(double) * 9 * m3);
+        DevBuf buf(sizeof(double) * 9 * m3);
+        double* d = buf.as<double>();
         {
             Timed t(TC_POINTWISE);
             launch_box_filter(L.g, L.W[1], L.W[2], L.W[3], r, d, d + 3 * m3, L.st);
@@ -205,7 +204,6 @@ int hit3d_box_filter(const int64_t* ratio, double* ubar, double* tau) {
         check(hipMemcpyAsync(tau, d + 3 * m3, sizeof(double) * 6 * m3, hipMemcpyDeviceToHost, L.st),
               "hipMemcpyAsync");
         check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
-        check(hipFree(d), "hipFree");
     });
 }
 
@@ -217,14 +215,14 @@ int hit3d_sgs_features(const double* delta, float* X) {
         const double* gr[9];
         for (int i = 0; i < 9; ++i) gr[i] = L.mlp.buf[i];
         const size_t n3 = (size_t)L.n * L.n * L.n;
-        float* d = reinterpret_cast<float*>(dalloc(sizeof(float) * mlp::NIN * n3));
+        DevBuf buf(sizeof(float) * mlp::NIN * n3);
+        float* d = buf.as<float>();
         {
             Timed t(TC_POINTWISE);
             launch_sgs_features(L.g, gr, *delta, d, L.st);
         }
         check(hipMemcpyAsync(X, d, sizeof(float) * mlp::NIN * n3, hipMemcpyDeviceToHost, L.st), "hipMemcpyAsync");
         check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
-        check(hipFree(d), "hipFree");
     });
 }
 

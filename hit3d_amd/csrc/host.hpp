@@ -26,6 +26,22 @@ void clear_graphs();
 void free_all();
 double* dalloc(size_t bytes);
 
+// A temporary device buffer of one call, freed when the call's scope ends,
+// also when a later check throws.
+struct DevBuf {
+    void* p = nullptr;
+    explicit DevBuf(size_t bytes) : p(dalloc(bytes)) {}
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    template <typename T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
 // transforms (host_transforms.hip); fft_forward / fft_inverse are in state.hpp
 void pass_y(double* slot, int dir);
 void fft_forward_yz(double*& slot, int ph);

@@ -322,6 +322,7 @@ void do_rhs_velocity(bool use_kept) {
 void force_enqueue() {
     Lib& L = lib();
     if (L.kfmax < 1) throw std::runtime_error("force_velocity needs kfmax >= 1");
+    if (L.kfmax > L.kmax) throw std::runtime_error("force_velocity: kfmax exceeds kmax");
     Timed t(TC_REDUCE);
     launch_shell_sums(L.g, L.F[1], L.F[2], L.F[3], 1, L.kfmax, L.dshell, L.st);
     const double* f[3] = {L.F[1], L.F[2], L.F[3]};

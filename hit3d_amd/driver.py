@@ -444,16 +444,25 @@ class Run:
             self.dt = 0.5 * self.dt
         self.int_scalars = p.n_scalars > 0 and self.time > p.tscalar
         self.fov = self.fos = True
+        # main.f90:73-76: dealias_all after begin_restart too -- truncates
+        # what the file or init_scalar put outside the kmax cube
+        self.h.dealias_all()
 
     def write_snapshot(self, directory: str = "."):
         """io_write_4 (io_write_4.f90:1-85): u., v., w.<itime> and scNN.<itime>
-        as float32 with the k > kmax modes removed."""
+        as float32 with the k > kmax modes removed, and nu_t.<itime> (the
+        turbulent viscosity of the last right-hand side, :73-79) when the
+        Smagorinsky model is on.  The turbulent viscosity exists once a
+        step has run; before that, as the reference's zero-initialised
+        array, it is written as zeros."""
         ext = f"{self.itime:06d}"
         for c, name in ((1, "u"), (2, "v"), (3, "w")):
             write_tmp4(os.path.join(directory, f"{name}.{ext}"), self.h.snapshot(c))
         if self.int_scalars:
             for s in range(1, self.p.n_scalars + 1):
                 write_tmp4(os.path.join(directory, f"sc{s:02d}.{ext}"), self.h.snapshot(3 + s))
+        if self.p.les_model == 1:
+            write_tmp4(os.path.join(directory, f"nu_t.{ext}"), self.h.turb_visc())
 
     def init_particles(self, run_name: str | None = None, directory: str = "."):
         """particles_init (m_particles.f90:157-337): pt.<itmin> binary restart

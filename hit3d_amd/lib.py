@@ -72,6 +72,7 @@ _SIGNATURES = {
     "hit3d_init_scalar": [_I64P, _F64P, _F64P],
     "hit3d_rng_state": [_I64P],
     "hit3d_snapshot": [_I64P, ctypes.POINTER(ctypes.c_float)],
+    "hit3d_turb_visc": [ctypes.POINTER(ctypes.c_float)],
     "hit3d_scalar_gaussian": [_I64P, _F64P, _I64P, _F64P, _I64P],
     "hit3d_velocity_rng_state": [_I64P, _F64P, _I64P],
     "hit3d_get_ialias": [ctypes.POINTER(ctypes.c_int8)],
@@ -358,6 +359,13 @@ class Hit3D:
         """io_write_4 for one field: float32 physical array [z, y, x]."""
         out = np.zeros((self.n, self.n, self.n), np.float32)
         _check(self.lib.hit3d_snapshot(_i(n_field), out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out
+
+    def turb_visc(self) -> np.ndarray:
+        """The Smagorinsky turbulent viscosity of the last LES right-hand
+        side as io_write_4 writes it (nu_t.<ext>): float32 [z, y, x]."""
+        out = np.zeros((self.n, self.n, self.n), np.float32)
+        _check(self.lib.hit3d_turb_visc(out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
         return out
 
     # -- learned closure --------------------------------------------------------

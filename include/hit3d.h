@@ -256,6 +256,12 @@ int hit3d_init_scalar(const int64_t *n_scalar, const double *rn2, const double *
  * (write_tmp4.f90:14-30).  Uses wrk(0) as work space. */
 int hit3d_snapshot(const int64_t *n_field, float *out);
 
+/* The turbulent viscosity of les_model 1 as the last LES right-hand side
+ * left it, rounded to float32 into out(nx, ny, nz): the tmp4 = turb_visc
+ * that io_write_4 writes as nu_t.<ext> when LES is on (io_write_4.f90:73-79).
+ * Other models hold none (an error). */
+int hit3d_turb_visc(float *out);
+
 /* The generator's saved iy (RANDu.f:22-24) as the library holds it after its
  * last initialisation (0 before any). */
 int hit3d_rng_state(int64_t *iy);

@@ -750,6 +750,17 @@ def init_velocity(n: int, rn1: float, kmax: int, isp_type: int, peak: float, ir_
     physical, e_spec_before_rescale)."""
     uni, _ = init_velocity_uniforms(n, rn1)
     g = box_muller(uni[0:3], uni[3:6])
+    f, e = init_velocity_from_gaussian(g, n, kmax, isp_type, peak, ir_exp)
+    return f, g, e
+
+
+def init_velocity_from_gaussian(g: np.ndarray, n: int, kmax: int, isp_type: int, peak: float,
+                                ir_exp: float):
+    """The post-RNG stage of init_velocity (init_velocity.f90:116-237): forward
+    transforms of the three physical slots g, the curl i k x W on shells
+    1..kmax (FP64 shell index, :123-147), then each shell rescaled to the
+    target spectrum (:153-237).  Returns (fields[3] spectral,
+    e_spec_before_rescale)."""
     W = [to_complex(fft_forward(g[c])) for c in range(3)]
     kx, ky, kz = _kgrid(n)
     sh = shell_index_f64((kx ** 2 + ky ** 2) + kz ** 2)
@@ -767,7 +778,7 @@ def init_velocity(n: int, rn1: float, kmax: int, isp_type: int, peak: float, ir_
     ok &= e_at > 0.0
     scale = np.where(ok, np.sqrt(np.where(ok, t_at / np.where(ok, e_at, 1.0), 0.0)), 0.0)
     f = np.where(ok[None], f * scale[None], 0.0)
-    return f, g, e
+    return f, e
 
 
 def scalar_target_spectrum(kmax: int, sc_type: int, peak: float) -> np.ndarray:

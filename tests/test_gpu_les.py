@@ -50,6 +50,10 @@ def test_les_rhs_velocity_and_scalars(oracle):
     rv = oracle.les_rhs_velocity(f, rv, tv, n)
     for c in range(3):
         assert rel(h.get(WRK, c + 1), rv[c]) < 1e-12
+    # nu_t as io_write_4 writes it (io_write_4.f90:73-79): float32 of turb_visc
+    nut = h.turb_visc()
+    assert nut.dtype == np.float32 and nut.shape == (n, n, n)
+    assert np.array_equal(nut, tv.astype(np.float32)) or rel(nut, tv) < 1e-7
     h.close()
 
 
@@ -79,4 +83,7 @@ def test_les_rejects_phase_shift():
     h = hit3d_amd.Hit3D(16, dealias=1)
     with pytest.raises(hit3d_amd.Hit3dError, match="2/3"):
         h.set_les(1)
+    # no LES: no turbulent viscosity to write
+    with pytest.raises(hit3d_amd.Hit3dError, match="turb_visc"):
+        h.turb_visc()
     h.close()

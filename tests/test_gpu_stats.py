@@ -181,6 +181,46 @@ def test_restart_round_trip_through_run(tmp_path):
     assert path.endswith("abcdefghij.64.000003")
 
 
+def test_restart_adds_scalar_then_dealias_all(tmp_path, oracle):
+    """Restarting with one more scalar than the file holds: init_scalar
+    makes the missing one (restart_io.f90:420-431) and main.f90:73-76 then
+    runs dealias_all, so a type-11 tanh slab, which has content in every
+    kz, is truncated to the kmax cube and equals the oracle's
+    dealias_all of its transform."""
+    from hit3d_amd.driver import Params, Run
+    n = 16
+    p = Params(nx=n, ny=n, nz_all=n, flow_type=1, re=200.0, dt=0.002, variable_dt=False,
+               isp_type=1, peak_wavenum=3.0, n_scalars=1, scalar_type=[2], sc=[0.7],
+               ir_exp_sc=[0.0], peak_wavenum_sc=[2.0], reac_sc=[0.0], tscalar=-1.0)
+    r = Run(p)
+    try:
+        r.begin_new()
+        r.step()
+        r.restart_write("abcdefghij", str(tmp_path))
+    finally:
+        r.h.close()
+    p2 = Params(nx=n, ny=n, nz_all=n, flow_type=1, re=200.0, dt=0.002, variable_dt=False,
+                isp_type=1, peak_wavenum=3.0, n_scalars=2, scalar_type=[2, 11], sc=[0.7, 1.0],
+                ir_exp_sc=[0.0, 0.0], peak_wavenum_sc=[2.0, 4.0], reac_sc=[0.0, 0.0],
+                tscalar=-1.0)
+    p2.itmin = 1
+    r2 = Run(p2)
+    try:
+        r2.begin_restart("abcdefghij", str(tmp_path))
+        got = r2.h.get(FIELDS, 5)
+        f = oracle.fft_forward(oracle.scalar_space(n, 11, 4.0))
+        f[0, 0, 0] = 0.0
+        kmax = oracle.kmax_of(n, 0)
+        raw_outside = np.abs(f - oracle.dealias_all(f, kmax, n)).max()
+        assert raw_outside > 1e-3 * np.abs(f).max()  # the slab has modes to cut
+        want = oracle.dealias_all(f, kmax, n)
+        assert np.abs(got - want).max() <= 1e-13 * np.abs(want).max()
+        outside = oracle.dealias_all(np.ones_like(got), kmax, n) == 0.0
+        assert np.all(got[outside] == 0.0)
+    finally:
+        r2.h.close()
+
+
 def test_skewness_matches_oracle(oracle):
     n = 32
     f = random_spec(oracle, n, 21, 3)

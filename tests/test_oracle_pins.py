@@ -1,0 +1,277 @@
+"""Pins of the oracle's phase-shift de-aliasing, the seeded initial field's
+orientation and the driver-step composition, each against something fixed
+outside the oracle's own code: exact products on a zero-padded grid, known
+values of the Box-Muller formula, the analytic curl of single modes, and the
+closed-form evolution of a forced shear wave.
+
+Each test names the mistake it would catch.
+"""
+import numpy as np
+import pytest
+
+
+def _modes_field(n, modes, m=None):
+    """Physical real field sum_k a cos(k.x) + b sin(k.x) sampled on an m^3
+    grid (m defaults to n), returned as [z, y, x] of shape (m, m, m)."""
+    m = m or n
+    x = np.arange(m) * (2 * np.pi / m)
+    zz, yy, xx = np.meshgrid(x, x, x, indexing="ij")
+    out = np.zeros((m, m, m))
+    for (kx, ky, kz), a, b in modes:
+        ph = kx * xx + ky * yy + kz * zz
+        out += a * np.cos(ph) + b * np.sin(ph)
+    return out
+
+
+def _pad(p, n):
+    out = np.zeros((n, n, n + 2))
+    out[:, :, :n] = p
+    return out
+
+
+def _random_modes(rng, kxs, kys, kzs, count):
+    out = []
+    for _ in range(count):
+        k = (int(rng.choice(kxs)), int(rng.choice(kys)), int(rng.choice(kzs)))
+        out.append((k, float(rng.standard_normal()), float(rng.standard_normal())))
+    return out
+
+
+def _exact_coefficients(n, prod_2n):
+    """Fourier coefficients of a product sampled on the (2n)^3 grid (exact
+    for band-limited factors), scaled to the unnormalised n^3 convention and
+    laid out as the n-grid half spectrum [ky, kz, kx] (complex).  Modes with
+    |k_i| >= n/2 are dropped: the test fields keep products off them."""
+    m = 2 * n
+    c = np.fft.rfftn(prod_2n, axes=(0, 1, 2)) * (float(n) ** 3 / float(m) ** 3)  # [kz, ky, kx]
+    out = np.zeros((n, n, n // 2 + 1), complex)
+    ks = [k if k <= n // 2 else k - n for k in range(n)]
+    for iy, ky in enumerate(ks):
+        for iz, kz in enumerate(ks):
+            if abs(ky) >= n // 2 or abs(kz) >= n // 2:
+                continue
+            out[iy, iz, :n // 2] = c[kz % m, ky % m, :n // 2]
+    return out
+
+
+# single aliasing in x only: |kx| in {6, 7} at n = 16 (n/3 = 5.33), sums
+# 12..14 wrap to -4..-2, never reach the Nyquist column; |ky|, |kz| <= 1
+N = 16
+KXS = [6, 7, -6, -7, 0]
+KYS = [-1, 0, 1]
+KZS = [-1, 0, 1]
+
+
+def _velocity_modes(seed):
+    rng = np.random.default_rng(seed)
+    return [_random_modes(rng, KXS, KYS, KZS, 5) for _ in range(3)]
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_phase_shift_velocity_rhs_equals_exact_products(oracle, seed):
+    """rhs_velocity.f90:150-410 with fields whose products alias once (in
+    x): the half-cell shift of :248-249 cancels every such alias, so the
+    phase-shift RHS equals -i k_j (u_i u_j)^ - nu k^2 u_i with the products
+    taken exactly (zero-padded 2n grid) on every entry with ialias < 2, and
+    is zero where ialias >= 2.  The 2/3-rule RHS keeps the aliases.
+    Catches a shift of dx instead of dx/2 (no cancellation), a wrong mask
+    threshold (ialias > 0 / >= 1 zeroes kept modes), a missing back-shift
+    or a wrong sign of the shift."""
+    n, nu = N, 0.05
+    modes = _velocity_modes(seed)
+    U = np.stack([oracle.fft_forward(_pad(_modes_field(n, md), n)) for md in modes])
+    rhs, _ = oracle.rhs_velocity(U, nu, 1, n)
+    u2 = [_modes_field(n, md, 2 * n) for md in modes]
+    P = {}
+    for i in range(3):
+        for j in range(i, 3):
+            P[(i, j)] = P[(j, i)] = _exact_coefficients(n, u2[i] * u2[j])
+    kx, ky, kz = oracle._kgrid(n)
+    k = (kx, ky, kz)
+    k2 = (kx ** 2 + ky ** 2) + kz ** 2
+    ia = oracle.ialias(n)[:, :, 0::2]
+    assert (ia == 1).sum() > 0
+    worst = 0.0
+    for i in range(3):
+        Ui = oracle.to_complex(U[i])
+        ex = -1j * sum(k[j] * P[(i, j)] for j in range(3)) - nu * k2 * Ui
+        ex = np.where(ia >= 2, 0.0, ex)
+        got = oracle.to_complex(rhs[i])
+        scale = np.abs(ex).max()
+        worst = max(worst, np.abs(got - ex).max() / scale)
+        # entries with one component above n/3 carry content
+        assert np.abs(np.where(ia == 1, ex, 0.0)).max() > 0.1 * scale
+        assert np.all(got[ia >= 2] == 0.0)
+    assert worst < 1e-13
+    # the 2/3 rule (rhs_velocity.f90:53-144) keeps the aliases on ialias == 0
+    rhs23, _ = oracle.rhs_velocity(U, nu, 0, n)
+    diff = 0.0
+    for i in range(3):
+        Ui = oracle.to_complex(U[i])
+        ex = -1j * sum(k[j] * P[(i, j)] for j in range(3)) - nu * k2 * Ui
+        got = oracle.to_complex(rhs23[i])
+        diff = max(diff, np.abs(np.where(ia == 0, got - ex, 0.0)).max() / np.abs(ex).max())
+    assert diff > 1e-2
+
+
+def test_phase_shift_velocity_masks_double_alias_entries(oracle):
+    """Fields carrying modes with two components above n/3: every entry
+    with ialias >= 2 of the phase-shift RHS is exactly zero (the viscous
+    term too), and entries with ialias == 1 are not (rhs_velocity.f90:215,
+    :387).  Catches a final mask of ialias >= 3 or ialias >= 1."""
+    n = N
+    rng = np.random.default_rng(7)
+    modes = [_random_modes(rng, [6, 7, 0], [6, -6, 1], [0, 1, 7], 6) for _ in range(3)]
+    U = np.stack([oracle.fft_forward(_pad(_modes_field(n, md), n)) for md in modes])
+    rhs, _ = oracle.rhs_velocity(U, 0.1, 1, n)
+    ia = oracle.ialias(n)
+    for i in range(3):
+        assert np.all(rhs[i][ia >= 2] == 0.0)
+        assert np.abs(rhs[i][ia == 1]).max() > 0.0
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_phase_shift_scalar_rhs_equals_exact_products(oracle, seed):
+    """rhs_scalars.f90:144-271 and the gradient source :387-389 with a
+    velocity and a scalar whose products alias once (in x): the RHS equals
+    -i k_j (phi u_j)^ - pe k^2 phi - u_1 (type 0) with exact products on
+    every entry with ialias < 2, and -u_1 where ialias >= 2.  The 2/3-rule
+    RHS keeps the aliases.  Catches the scalar path's shift (:152-153),
+    its two masks (:200, :257) and a sign of the back-shift."""
+    n, pe = N, 0.07
+    modes = _velocity_modes(seed)
+    rng = np.random.default_rng(seed + 100)
+    smodes = _random_modes(rng, KXS, KYS, KZS, 5)
+    U = [oracle.fft_forward(_pad(_modes_field(n, md), n)) for md in modes]
+    S = oracle.fft_forward(_pad(_modes_field(n, smodes), n))
+    f = np.stack(U + [S])
+    r = oracle.rhs_scalars(f, 1, (pe,), (0,), 1, n)[0]
+    phi2 = _modes_field(n, smodes, 2 * n)
+    Q = [_exact_coefficients(n, phi2 * _modes_field(n, md, 2 * n)) for md in modes]
+    kx, ky, kz = oracle._kgrid(n)
+    k2 = (kx ** 2 + ky ** 2) + kz ** 2
+    ia = oracle.ialias(n)[:, :, 0::2]
+    Sc = oracle.to_complex(S)
+    U1 = oracle.to_complex(U[0])
+    ex = -1j * ((kx * Q[0] + ky * Q[1]) + kz * Q[2]) - pe * k2 * Sc
+    ex = np.where(ia >= 2, 0.0, ex) - U1
+    got = oracle.to_complex(r)
+    assert np.abs(got - ex).max() < 1e-13 * np.abs(ex).max()
+    assert np.abs(np.where(ia == 1, ex + U1, 0.0)).max() > 0.1 * np.abs(ex).max()
+    r23 = oracle.rhs_scalars(f, 1, (pe,), (0,), 0, n)[0]
+    diff = np.abs(np.where(ia == 0, oracle.to_complex(r23) - ex, 0.0)).max()
+    assert diff > 1e-2 * np.abs(ex).max()
+
+
+# ------------------------------------------------------------ Box-Muller, curl
+
+def test_box_muller_known_values(oracle):
+    """init_velocity.f90:113, g = sqrt(-2 log a) sin(2 pi b): a = e^(-1/2)
+    gives the factor 1, so b = 1/4 -> 1, b = 1/2 -> 0, b = 3/4 -> -1,
+    b = 1/12 -> 1/2.  Catches sin/cos swapped, a missing factor 2 in either
+    place, or the two slots swapped (a and b exchanged)."""
+    a = np.full(4, np.exp(-0.5))
+    b = np.array([0.25, 0.5, 0.75, 1.0 / 12.0])
+    g = oracle.box_muller(a, b)
+    assert abs(g[0] - 1.0) < 1e-15
+    assert abs(g[1]) < 1e-15
+    assert abs(g[2] + 1.0) < 1e-15
+    assert abs(g[3] - 0.5) < 1e-15
+    # a = e^(-2) gives the factor 2
+    g2 = oracle.box_muller(np.array([np.exp(-2.0)]), np.array([0.25]))
+    assert abs(g2[0] - 2.0) < 1e-15
+
+
+def test_init_velocity_curl_single_modes(oracle):
+    """The post-RNG stage of init_velocity.f90:116-237 on Gaussian slots
+    g1 = sum_m cos mz, g2 = sum_m cos mx, g3 = sum_m cos my (m = 1..kmax):
+    the curl i k x W (:123-147) is (dg3/dy - dg2/dz, dg1/dz - dg3/dx,
+    dg2/dx - dg1/dy) = -sum_m m (sin my, sin mz, sin mx), energy 3 m^2/4 in
+    shell m; the rescale (:153-237) brings shell m to the normalised
+    target, amplitude sqrt(4 target_m / 3).  Catches a curl of the wrong
+    sign, components permuted, or a rescale by E/target instead of
+    target/E."""
+    n, kmax = 16, 5
+    ms = range(1, kmax + 1)
+    g = np.stack([_pad(_modes_field(n, [((0, 0, m), 1.0, 0.0) for m in ms]), n),
+                  _pad(_modes_field(n, [((m, 0, 0), 1.0, 0.0) for m in ms]), n),
+                  _pad(_modes_field(n, [((0, m, 0), 1.0, 0.0) for m in ms]), n)])
+    f, e0 = oracle.init_velocity_from_gaussian(g, n, kmax, 2, 3.0, 4.0)
+    tgt = oracle.target_spectrum(kmax, 2, 3.0, 4.0)
+    assert np.allclose(e0, [3.0 * m * m / 4.0 for m in ms], rtol=1e-13, atol=0)
+    amp = [np.sqrt(4.0 * tgt[m - 1] / 3.0) for m in ms]
+    ex = [_modes_field(n, [((0, m, 0), 0.0, -a) for m, a in zip(ms, amp)]),
+          _modes_field(n, [((0, 0, m), 0.0, -a) for m, a in zip(ms, amp)]),
+          _modes_field(n, [((m, 0, 0), 0.0, -a) for m, a in zip(ms, amp)])]
+    for c in range(3):
+        got = oracle.fft_inverse(f[c])[:, :, :n]
+        assert np.abs(got - ex[c]).max() < 1e-13
+
+
+# ------------------------------------------------------------ step composition
+
+def _shear_wave(oracle, n, a):
+    """w = a sin x (u = v = 0): divergence-free, its nonlinear term is zero
+    in divergence form (d_x(u w) = 0, d_z(w w) = 0) and its only entry is
+    kx = 1, in forcing shell 1."""
+    z = np.zeros((n, n, n + 2))
+    return np.stack([z, z.copy(),
+                     oracle.fft_forward(_pad(_modes_field(n, [((1, 0, 0), 0.0, a)]), n))])
+
+
+def test_step_forced_shear_wave_closed_form(oracle):
+    """main.f90:124-176 on a forced shear wave w = A sin x, KFMAX = 2: the
+    RHS is (FAMP/E - nu) w-hat with E = A^2/4 the band energy, so Euler
+    gives A1 = A0 (1 + dt (FAMP/E0 - nu)) and AB2 then gives
+    A2 = A1 + dt (1.5 r1 - 0.5 r0).  Catches the forcing sign or factor
+    (m_force.f90:163-173), AB2 weights or order, the Euler first step,
+    and forcing applied after the update instead of to the RHS."""
+    n, nu, famp, dt, a0 = 16, 0.02, 0.5, 0.01, 0.8
+    cfg = oracle.Config(n=n, nu=nu, dt=dt, dealias=0, flow_type=1, kfmax=2, famp=famp)
+    f0 = _shear_wave(oracle, n, a0)
+    st = oracle.State(fields=f0, rhs_old=np.zeros_like(f0), dt=dt)
+    st1 = oracle.step(cfg, st)
+    e0 = a0 * a0 / 4.0
+    r0 = (famp / e0 - nu) * a0
+    a1 = a0 + dt * r0
+    assert abs(st1.energy - e0) < 1e-15
+    e1 = a1 * a1 / 4.0
+    r1 = (famp / e1 - nu) * a1
+    a2 = a1 + dt * (1.5 * r1 - 0.5 * r0)
+    st2 = oracle.step(cfg, st1)
+    assert abs(st2.energy - e1) < 1e-14
+    for s, a in ((st1, a1), (st2, a2)):
+        w = oracle.fft_inverse(s.fields[2])[:, :, :n]
+        assert np.abs(w - _modes_field(n, [((1, 0, 0), 0.0, a)])).max() < 1e-13
+        assert np.abs(s.fields[:2]).max() < 1e-13
+    # Courant number of w = A sin x: max |w| dt / dx (rhs_velocity.f90:31-38)
+    assert abs(st1.courant - a0 * dt / (2 * np.pi / n)) < 1e-14
+    # decaying (flow_type 0): pure viscous decay, A1 = A0 (1 - nu dt)
+    cfg0 = oracle.Config(n=n, nu=nu, dt=dt, dealias=0, flow_type=0)
+    s0 = oracle.step(cfg0, oracle.State(fields=f0, rhs_old=np.zeros_like(f0), dt=dt))
+    w = oracle.fft_inverse(s0.fields[2])[:, :, :n]
+    assert np.abs(w - _modes_field(n, [((1, 0, 0), 0.0, a0 * (1 - nu * dt))])).max() < 1e-14
+
+
+def test_step_projects_after_the_update(oracle):
+    """main.f90:150-167 projects the updated velocity, not the RHS alone or
+    the old field: from a field whose RHS has a gradient part, the stepped
+    field is solenoidal to roundoff, and equals the projection of the
+    unprojected update.  A projection before the update leaves dt times
+    the RHS's gradient part in the field."""
+    n, dt = 16, 0.01
+    f = np.stack([oracle.fft_forward(_pad(_modes_field(n, [((1, 2, 0), 0.0, 1.0)]), n)),
+                  oracle.fft_forward(_pad(_modes_field(n, [((0, 0, 3), 1.0, 0.0)]), n)),
+                  oracle.fft_forward(_pad(_modes_field(n, [((2, 0, 0), 0.0, 1.0)]), n))])
+    cfg = oracle.Config(n=n, nu=0.0, dt=dt, dealias=0, flow_type=0)
+    st = oracle.step(cfg, oracle.State(fields=f, rhs_old=np.zeros_like(f), dt=dt))
+    kx, ky, kz = oracle._kgrid(n)
+    C = [oracle.to_complex(st.fields[c]) for c in range(3)]
+    assert np.abs((kx * C[0] + ky * C[1]) + kz * C[2]).max() < 1e-10
+    rhs, _ = oracle.rhs_velocity(f, 0.0, 0, n)
+    D = [oracle.to_complex(rhs[c]) for c in range(3)]
+    grad = np.abs((kx * D[0] + ky * D[1]) + kz * D[2]).max()
+    assert grad > 1.0   # the RHS is not solenoidal
+    # the step result differs from "project the old field, then add dt*rhs"
+    wrong = oracle.pressure(f, n) + dt * rhs
+    assert np.abs(st.fields - wrong).max() > 1e-3 * dt * grad
