@@ -232,6 +232,30 @@ def shell_spectrum(fields: np.ndarray, n: int, kshells: int) -> np.ndarray:
     return e[:kshells + 1]
 
 
+def shell_spectrum_chunked(fields, n: int, kshells: int, planes: int = 32) -> np.ndarray:
+    """shell_spectrum for grids too large for whole-array temporaries: the
+    same terms, summed ``planes`` ky-planes at a time (the summation order
+    differs, so results agree to rounding).  ``fields`` is a sequence of
+    three spectral slots [ky, kz, i]."""
+    akx, aky, akz = wavenumbers(n)
+    fac = 1.0 / float(n * n * n) ** 2
+    half = (akx[:n] == 0.0)[None, None, :]
+    e = np.zeros(kshells + 1)
+    for k0 in range(0, n, planes):
+        k1 = min(n, k0 + planes)
+        k2 = (akx[None, None, :n] ** 2 + aky[k0:k1, None, None] ** 2) + akz[None, :, None] ** 2
+        sh = shell_index_f32(k2)
+        u = fields[0][k0:k1, :, :n]
+        v = fields[1][k0:k1, :, :n]
+        w = fields[2][k0:k1, :, :n]
+        f2 = fac * ((u * u + v * v) + w * w)
+        f2 = np.where(half, f2 * 0.5, f2)
+        sel = (sh > 0) & (sh <= kshells)
+        e += np.bincount(sh[sel].ravel(), weights=f2[sel].ravel(), minlength=kshells + 1)[
+            :kshells + 1]
+    return e
+
+
 # ----------------------------------------------------------------------------
 # right-hand sides
 # ----------------------------------------------------------------------------

@@ -372,6 +372,18 @@ def test_stats_single_modes(oracle):
     assert abs(st["eps_v"] - 2 * 0.01 * 0.25) < 1e-15
 
 
+def test_shell_spectrum_chunked_equals_whole(oracle):
+    """The plane-chunked shell sums (used on 1024^3 fields) equal the
+    whole-array ones to rounding, for chunks that do and do not divide n."""
+    n = 16
+    rng = np.random.default_rng(9)
+    f = rng.standard_normal((3, n, n, n + 2))
+    whole = oracle.shell_spectrum(f, n, 12)
+    for planes in (1, 5, 16):
+        assert np.allclose(oracle.shell_spectrum_chunked(f, n, 12, planes), whole,
+                           rtol=1e-13, atol=0)
+
+
 def test_stats_parseval(oracle):
     """Sum of E(k) over all shells equals 1/2 <|u|^2> for a field truncated to
     a sphere inside the shells counted."""
