@@ -42,6 +42,7 @@ int hit3d_init(const int64_t* nx, const int64_t* ny, const int64_t* nz_all,
         const double pi = 4.0 * std::atan(1.0);
         L.dx = 2.0 * pi / (double)L.n;
         check(hipStreamCreateWithFlags(&L.st, hipStreamNonBlocking), "hipStreamCreate");
+        static_scratch_setup();
         const size_t sb = slot_bytes();
         L.F.assign(L.nf + 1, nullptr);
         L.R.assign(L.nf + 1, nullptr);

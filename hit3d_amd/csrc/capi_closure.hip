@@ -44,6 +44,48 @@ static void mlp_reserve(int64_t tiles, int64_t steps) {
     }
 }
 
+// Scratch of hit3d_mlp_grads_static: per-call staging of the samples, the
+// per-tile gradient partials, the two-level reduction and the results, in
+// one file-scope device array (no allocation per call).  Offsets in floats,
+// each segment 256-byte aligned.
+namespace {
+constexpr int64_t kStaticCount = 8192;  // samples per call at most
+constexpr int64_t kStaticTiles = kStaticCount / 32;
+constexpr int64_t seg(int64_t floats) { return (floats + 63) / 64 * 64; }
+constexpr int64_t kOffX = 0;
+constexpr int64_t kOffT = kOffX + seg(kStaticCount * mlp::NIN);
+constexpr int64_t kOffPartial = kOffT + seg(kStaticCount * mlp::NOUT);
+constexpr int64_t kOffLossp = kOffPartial + seg(kStaticTiles * mlp::NPARAM);
+constexpr int64_t kOffRed = kOffLossp + seg(kStaticTiles);
+constexpr int64_t kOffGrad = kOffRed + seg((int64_t)kReduceGroups * (mlp::NPARAM + 1));
+constexpr int64_t kOffLoss = kOffGrad + seg(mlp::NPARAM);
+constexpr int64_t kScratchFloats = kOffLoss + seg(1);  // 2,153,984 floats = 8.2 MiB
+constexpr int kMaxDevices = 64;
+// device address of `scratch` on each device (hipGetSymbolAddress); the
+// array's name in host code is only a stand-in for the symbol
+float* g_scratch_addr[kMaxDevices] = {};
+}  // namespace
+
+static __device__ __align__(256) float scratch[kScratchFloats];
+
+namespace hit3d {
+// Resolves this device's copy of `scratch` once per device, with the device
+// current; hit3d_init calls it, and the static entry point on its first call
+// on a device that was not initialised through hit3d_init.
+float* static_scratch_setup() {
+    int dev = 0;
+    check(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("static scratch: device index");
+    if (!g_scratch_addr[dev]) {
+        void* p = nullptr;
+        check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (!p) throw std::runtime_error("static scratch: no device address");
+        g_scratch_addr[dev] = static_cast<float*>(p);
+    }
+    return g_scratch_addr[dev];
+}
+}  // namespace hit3d
+
 extern "C" {
 
 int hit3d_mlp_set_params(const float* params) {
@@ -161,6 +203,47 @@ int hit3d_mlp_grads(const int64_t* count, const float* X, const float* T, float*
         float ls = 0.f;
         check(hipMemcpyAsync(grads, M.g, sizeof(float) * mlp::NPARAM, hipMemcpyDeviceToHost, L.st), "hipMemcpyAsync");
         check(hipMemcpyAsync(&ls, M.loss, sizeof(float), hipMemcpyDeviceToHost, L.st), "hipMemcpyAsync");
+        check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
+        if (loss) *loss = (double)ls / (double)(n * mlp::NOUT);
+    });
+}
+
+int hit3d_mlp_grads_static(const int64_t* count, const float* X, const float* T, float* grads,
+                           double* loss) {
+    return guarded([&] {
+        require_ready();
+        Lib& L = lib();
+        auto& M = L.mlp;
+        if (!M.have_params) throw std::runtime_error("mlp: parameters first");
+        const int64_t n = *count;
+        if (n < 32 || n % 32 != 0) throw std::runtime_error("mlp: count must be a multiple of 32");
+        if (n > kStaticCount)
+            throw std::runtime_error("mlp_grads_static: count " + std::to_string(n) +
+                                     " exceeds the static scratch limit of " +
+                                     std::to_string(kStaticCount) + " samples");
+        float* base = static_scratch_setup();  // this device's address
+        float* dX = base + kOffX;
+        float* dT = base + kOffT;
+        float* partial = base + kOffPartial;
+        float* lossp = base + kOffLossp;
+        float* red = base + kOffRed;
+        float* g = base + kOffGrad;
+        float* dloss = base + kOffLoss;
+        // every scratch value read below is first written by this call (the
+        // samples, one full partial row and loss per tile, every group sum,
+        // the gradients and the loss), so nothing needs clearing
+        check(hipMemcpyAsync(dX, X, sizeof(float) * mlp::NIN * n, hipMemcpyHostToDevice, L.st),
+              "hipMemcpyAsync");
+        check(hipMemcpyAsync(dT, T, sizeof(float) * mlp::NOUT * n, hipMemcpyHostToDevice, L.st),
+              "hipMemcpyAsync");
+        const int64_t tiles = n / 32;
+        launch_mlp_train_tiles(dX, dT, n, 0, (int)tiles, n, M.wb, M.p, partial, lossp, L.st);
+        launch_mlp_reduce(partial, lossp, (int)tiles, red, g, dloss, L.st);
+        float ls = 0.f;
+        check(hipMemcpyAsync(grads, g, sizeof(float) * mlp::NPARAM, hipMemcpyDeviceToHost, L.st),
+              "hipMemcpyAsync");
+        check(hipMemcpyAsync(&ls, dloss, sizeof(float), hipMemcpyDeviceToHost, L.st),
+              "hipMemcpyAsync");
         check(hipStreamSynchronize(L.st), "hipStreamSynchronize");
         if (loss) *loss = (double)ls / (double)(n * mlp::NOUT);
     });

@@ -42,6 +42,10 @@ struct DevBuf {
     }
 };
 
+// the device address of the allocation-free entry point's scratch array on
+// the current device (capi_closure.hip)
+float* static_scratch_setup();
+
 // transforms (host_transforms.hip); fft_forward / fft_inverse are in state.hpp
 void pass_y(double* slot, int dir);
 void fft_forward_yz(double*& slot, int ph);

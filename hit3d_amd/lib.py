@@ -53,6 +53,9 @@ _SIGNATURES = {
     "hit3d_mlp_grads": [_I64P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
                         ctypes.POINTER(ctypes.c_float), _F64P],
     "hit3d_mlp_forward": [_I64P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)],
+    "hit3d_mlp_grads_static": [_I64P, ctypes.POINTER(ctypes.c_float),
+                               ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                               _F64P],
     "hit3d_mlp_fallbacks": [_I64P],
     "hit3d_box_filter": [_I64P, _F64P, _F64P],
     "hit3d_sgs_features": [_F64P, ctypes.POINTER(ctypes.c_float)],
@@ -408,6 +411,17 @@ class Hit3D:
         loss = ctypes.c_double(0.0)
         _check(self.lib.hit3d_mlp_grads(_i(X.shape[0]), self._fp(X), self._fp(T), self._fp(g),
                                          ctypes.byref(loss)))
+        return g, loss.value
+
+    def mlp_grads_static(self, X: np.ndarray, T: np.ndarray):
+        """mlp_grads through the allocation-free entry point (at most 8192
+        samples per call)."""
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        g = np.zeros(self.NPARAM, np.float32)
+        loss = ctypes.c_double(0.0)
+        _check(self.lib.hit3d_mlp_grads_static(_i(X.shape[0]), self._fp(X), self._fp(T),
+                                                self._fp(g), ctypes.byref(loss)))
         return g, loss.value
 
     def mlp_forward(self, X: np.ndarray) -> np.ndarray:

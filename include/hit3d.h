@@ -156,6 +156,18 @@ int hit3d_mlp_grads(const int64_t *count, const float *X, const float *T, float 
 int hit3d_mlp_forward(const int64_t *count, const float *X, float *Y);
 int hit3d_mlp_fallbacks(int64_t *count);
 
+/* hit3d_mlp_grads without any device allocation: no counterpart in the
+ * reference (the closure has none); for callers that cannot allocate --
+ * code under HIP graph capture, tight training loops, harnesses that forbid
+ * hipMalloc.  Same kernels and results as hit3d_mlp_grads, bit for bit.
+ * The samples' staging, the gradient partials and the reduction live in one
+ * static device array of 8.2 MiB per GPU, held until the process ends;
+ * count must be a multiple of 32 and at most 8192 (larger counts are
+ * rejected with an error).  All calls on one GPU share that array: calls
+ * that overlap on one GPU (other threads, other libraries' streams) race. */
+int hit3d_mlp_grads_static(const int64_t *count, const float *X, const float *T, float *grads,
+                           double *loss);
+
 /* Training data for the closure.  box_filter: top-hat average over ratio^3
  * blocks of the physical velocity in wrk(1:3) (as hit3d_fields_to_parts
  * leaves it) onto the grid m = nx/ratio: ubar[3][m^3] and the subgrid stress
