@@ -216,7 +216,8 @@ def run_workload(name, steps, warmup):
     s = slot_bytes(n)
     b_alg = 6 * w["n_fft"] * s + 4 * (3 + w["ns"]) * s
     out = {"workload": name, "config": w["desc"], "s_per_step": t, "ns_per_point_step": t / n ** 3 * 1e9,
-           "step_alg_bytes": b_alg, "step_hbm_roofline_frac": b_alg / t / (PEAK_HBM_GBS * 1e9),
+           "plan_alg_bytes": b_alg,
+           "plan_bytes_over_step_time_vs_peak": b_alg / t / (PEAK_HBM_GBS * 1e9),
            "particles": nptot, "steps": steps, "warmup": warmup,
            "graph_replays": replays,
            "kernel_ms_per_step": {k: v[0] / steps for k, v in timers.items() if v[1]}}
@@ -309,11 +310,14 @@ def accuracy_run(n, steps, every):
 
 
 def copy_bandwidth(n, reps=10):
-    """Device-to-device copies of one slot (wrk(2) = wrk(1)) with the runtime's
-    copy kernel: the HBM rate a plain copy reaches (bytes read + written per
-    second, GB/s), and a kernel of known size for scaling the FETCH_SIZE /
-    WRITE_SIZE counters."""
+    """Two copies of one slot (wrk(2) = wrk(1), read + write of every byte):
+    the runtime's copy kernel (hipMemcpyAsync device to device), which also
+    scales the FETCH_SIZE / WRITE_SIZE counters, and the library's own
+    streaming copy (nontemporal dwordx4 loads and stores, several launch
+    shapes, best kept): the achievable rate of a pass that reads and writes
+    each byte once.  GB/s of bytes read + written."""
     import hit3d_amd
+    from hit3d_amd import measure
     h = hit3d_amd.Hit3D(n)
     try:
         h.copy(hit3d_amd.WRK, 2, hit3d_amd.WRK, 1)
@@ -323,12 +327,21 @@ def copy_bandwidth(n, reps=10):
             h.copy(hit3d_amd.WRK, 2, hit3d_amd.WRK, 1)
         h.synchronize()
         t = (time.perf_counter() - t0) / reps
+        best = None
+        for blocks in (1024, 2048, 4096, 8192):
+            ms, byt = measure.stream_copy(h, reps, blocks)
+            gbs = byt / (ms * 1e-3) / 1e9
+            if best is None or gbs > best[0]:
+                best = (gbs, blocks, ms, byt)
     finally:
         h.close()
     p = ((n // 2 + 1 + 15) // 16) * 16
     byt = n * n * p * 16
-    log(f"[bench] copy: {reps + 1} x {byt} bytes, {t * 1e3:.3f} ms each")
-    return 2 * byt / t / 1e9
+    log(f"[bench] runtime copy: {reps + 1} x {byt} bytes, {t * 1e3:.3f} ms each; streaming "
+        f"copy {best[0]:.0f} GB/s at {best[1]} blocks")
+    return {"runtime_copy_GBs": 2 * byt / t / 1e9, "stream_copy_GBs": best[0],
+            "stream_copy_blocks": best[1], "stream_copy_ms": best[2],
+            "stream_copy_bytes": best[3]}
 
 
 def measured_traffic(n, cls):
@@ -347,11 +360,25 @@ def measured_traffic(n, cls):
     return c["fetch_bytes"] + c["write_bytes"]
 
 
-def cpu_baseline(n_target):
-    """The oracle (numpy/C port of the reference step) on the host, on a
-    bounded sample: one step at 128^3 or 256^3, scaled by N^3 log2 N."""
+def host_cpu():
+    """CPU model and logical core count of the host the baseline ran on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def cpu_baseline(n):
+    """The oracle (numpy/C restatement of the reference step: numpy's
+    pocketfft and elementwise numpy, one thread) timed for one driver step at
+    the benchmark's own grid, from the same seeded field: no scaling."""
     from oracle import hit3d_oracle as o
-    n = 256 if n_target >= 256 else n_target
     import hit3d_amd
     h = hit3d_amd.Hit3D(n)
     h.init_velocity(67748937.0, 2, 4, 3.0)
@@ -361,13 +388,15 @@ def cpu_baseline(n_target):
     cfg = o.Config(n=n, nu=1 / 800.0, dt=0.001, dealias=0, flow_type=1, kfmax=2, famp=0.5,
                    variable_dt=True)
     st = o.State(fields=f0, rhs_old=np.zeros_like(f0), dt=cfg.dt)
+    log(f"[bench] cpu baseline: one oracle step at {n}^3")
     t0 = time.perf_counter()
     st = o.step(cfg, st)
     dt_n = time.perf_counter() - t0
-    scale = (n_target / n) ** 3 * (math.log2(n_target) / math.log2(n))
-    return {"value": dt_n * scale, "unit": "s/step", "cores": 1, "kind": "port",
-            "sample": f"one oracle step (numpy pocketfft + C) at {n}^3, {dt_n:.2f} s, "
-                      f"scaled by N^3 log2 N to {n_target}^3"}
+    model, ncpu = host_cpu()
+    return {"value": dt_n, "unit": "s/step", "cores": 1, "kind": "port",
+            "sample": f"one driver step of the oracle (numpy pocketfft + elementwise numpy, "
+                      f"single thread) at {n}^3 from the seeded field, measured directly; "
+                      f"host {model}, {ncpu} logical cores, 1 used"}
 
 
 def main():
@@ -415,7 +444,8 @@ def main():
                 mixed["accuracy"] = accuracy_run(n, a.accuracy_steps, 10)
         except Exception as e:  # reported, never silently replaced
             mixed = {"error": str(e)}
-    copy_gbs = copy_bandwidth(n)
+    copy = copy_bandwidth(n)
+    copy_gbs = copy["stream_copy_GBs"]
     # dominant kernel class of the FP64 step
     best, best_ms = None, -1.0
     for cls, (ms, cnt) in tim64.items():
@@ -430,8 +460,8 @@ def main():
         roof = {"bound": "hbm", "kernel_class": best, "achieved": ach, "peak": PEAK_HBM_GBS,
                 "unit": "GB/s", "frac": ach / PEAK_HBM_GBS, "traffic": measured_traffic(n, best),
                 "alg_bytes_per_launch": byt, "avg_launch_ms": avg_s * 1e3,
-                "launches_per_step": cnt / a.steps, "copy_kernel_GBs": copy_gbs,
-                "frac_of_copy": ach / copy_gbs}
+                "launches_per_step": cnt / a.steps, "stream_copy_GBs": copy_gbs,
+                "frac_of_stream_copy": ach / copy_gbs}
     # every kernel class of the FP64 step against the same peak: the dominant
     # one above is the x pass, bound by arithmetic and LDS exchanges; the
     # others are HBM-bound
@@ -441,9 +471,17 @@ def main():
         if cnt > 0 and byt is not None:
             ach = byt / (ms / cnt / 1e3) / 1e9
             by_kernel[cls] = {"achieved_GBs": ach, "frac": ach / PEAK_HBM_GBS,
-                              "frac_of_copy": ach / copy_gbs}
+                              "frac_of_stream_copy": ach / copy_gbs,
+                              "alg_bytes_per_launch": byt, "avg_launch_ms": ms / cnt,
+                              "launches_per_step": cnt / a.steps}
+    # the step's own algorithmic bytes: what its five kernels must move (the
+    # pruned transforms), per step
+    step_bytes = sum(v["alg_bytes_per_launch"] * v["launches_per_step"]
+                     for v in by_kernel.values())
     s = slot_bytes(n)
-    b_alg = 6 * 9 * s + 4 * 3 * s  # B_alg of the measurement plan, n_fft = 9, no scalars
+    # the measurement plan's byte model of an unpruned step (9 full transforms
+    # of 6 slot passes each + the state): bytes this step never moves
+    model_bytes = 6 * 9 * s + 4 * 3 * s
     out = {
         "metric": METRIC,
         "value": t64,
@@ -461,8 +499,12 @@ def main():
                                "variable DT, no scalars", "grid": n, "precision": "fp64",
                    "parallelism": "single GPU"},
         "ns_per_point_step": t64 / n ** 3 * 1e9,
-        "step_hbm_roofline_frac": b_alg / t64 / (PEAK_HBM_GBS * 1e9),
-        "step_alg_bytes": b_alg,
+        "step_alg_bytes": step_bytes,
+        "step_hbm_roofline_frac": step_bytes / t64 / (PEAK_HBM_GBS * 1e9),
+        "step_frac_of_stream_copy": step_bytes / t64 / (copy_gbs * 1e9),
+        "model_alg_bytes_unpruned": model_bytes,
+        "model_bytes_over_step_time_vs_peak": model_bytes / t64 / (PEAK_HBM_GBS * 1e9),
+        "copy": copy,
         "mixed": mixed,
         "kernel_ms_per_step": {k: v[0] / a.steps for k, v in tim64.items() if v[1]},
         "roofline": roof,

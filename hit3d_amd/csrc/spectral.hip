@@ -1,5 +1,7 @@
 // Pointwise kernels and deterministic FP64 reductions of the time step
 // (unfused forms, one kernel per reference loop nest).
+#include <stdexcept>
+
 #include "spectral.hpp"
 
 namespace hit3d {
@@ -135,6 +137,28 @@ __global__ void k_advance(Geo g, double* f, const double* r, double* ro, double 
 __global__ void k_fill_zero(double* p, int64_t count) {
     GRID_STRIDE(i, count) p[i] = 0.0;
 }
+// Plain streaming copy of n16 16-byte words (nontemporal dwordx4 loads and
+// stores, four words in flight per thread): the HBM rate an ideal pass that
+// reads and writes each byte once reaches on this part, the yardstick the
+// step's kernels are set beside.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_stream_copy(const u32x4* __restrict__ src,
+                                                     u32x4* __restrict__ dst, int64_t n16) {
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    for (; i + 3 * stride < n16; i += 4 * stride) {
+        const u32x4 a = __builtin_nontemporal_load(src + i);
+        const u32x4 b = __builtin_nontemporal_load(src + i + stride);
+        const u32x4 c = __builtin_nontemporal_load(src + i + 2 * stride);
+        const u32x4 d = __builtin_nontemporal_load(src + i + 3 * stride);
+        __builtin_nontemporal_store(a, dst + i);
+        __builtin_nontemporal_store(b, dst + i + stride);
+        __builtin_nontemporal_store(c, dst + i + 2 * stride);
+        __builtin_nontemporal_store(d, dst + i + 3 * stride);
+    }
+    for (; i < n16; i += stride) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+}
+
 __global__ void k_narrow(const double* __restrict__ src, float* __restrict__ dst, int64_t count) {
     GRID_STRIDE(i, count) dst[i] = (float)src[i];
 }
@@ -516,6 +540,11 @@ void launch_advance_project(const Geo& g, double* const f[3], const double* cons
 }
 void launch_fill_zero(double* p, int64_t count, hipStream_t s) {
     hipLaunchKernelGGL(k_fill_zero, dim3(grid_for(count)), dim3(TB), 0, s, p, count);
+}
+void launch_stream_copy(const void* src, void* dst, int64_t bytes, int blocks, hipStream_t s) {
+    if (bytes % 16 != 0 || blocks < 1) throw std::runtime_error("stream copy: 16-byte words");
+    hipLaunchKernelGGL(k_stream_copy, dim3(blocks), dim3(256), 0, s,
+                       static_cast<const u32x4*>(src), static_cast<u32x4*>(dst), bytes / 16);
 }
 void launch_narrow(const double* src, float* dst, int64_t count, hipStream_t s) {
     hipLaunchKernelGGL(k_narrow, dim3(grid_for(count)), dim3(TB), 0, s, src, dst, count);

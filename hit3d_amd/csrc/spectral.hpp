@@ -30,6 +30,8 @@ void launch_advance_project(const Geo& g, double* const f[3], const double* cons
                             hipStream_t s);
 // p[0..count) = 0 by a kernel.
 void launch_fill_zero(double* p, int64_t count, hipStream_t s);
+// streaming copy of `bytes` (a multiple of 16) with `blocks` workgroups of 256
+void launch_stream_copy(const void* src, void* dst, int64_t bytes, int blocks, hipStream_t s);
 // dst[i] = src[i] rounded to FP32 / widened to FP64, i < count (no overlap).
 void launch_narrow(const double* src, float* dst, int64_t count, hipStream_t s);
 void launch_widen(const float* src, double* dst, int64_t count, hipStream_t s);

@@ -28,6 +28,17 @@ def test_library_exports_header_symbols():
     assert sorted(hl.exported_symbols()) == names
 
 
+def test_library_exports_measurement_symbols():
+    """include/hit3d_measure.h: declared, exported and bound."""
+    from hit3d_amd import measure
+    src = open(os.path.join(ROOT, "include", "hit3d_measure.h")).read()
+    names = sorted(set(re.findall(r"\bint\s+(hit3d_\w+)\s*\(", src)))
+    assert names == sorted(measure.exported_symbols()) and names
+    lib = hl.load()
+    for name in names:
+        assert hasattr(lib, name), name
+
+
 def test_uninitialised_call_fails_loudly():
     with pytest.raises(hit3d_amd.Hit3dError, match="hit3d_init"):
         hl.load()
