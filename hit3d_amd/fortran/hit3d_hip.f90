@@ -115,6 +115,167 @@ module hit3d_hip
        real(c_double), intent(in) :: rn1, peak
        integer(c_int64_t), intent(in) :: isp_type, ir_exp
      end function c_init_velocity
+     integer(c_int) function c_init_scalar(n_scalar, rn2, peak) bind(C, name="hit3d_init_scalar")
+       import :: c_int, c_int64_t, c_double
+       integer(c_int64_t), intent(in) :: n_scalar
+       real(c_double), intent(in) :: rn2, peak
+     end function c_init_scalar
+     integer(c_int) function c_set_les(les_model, c_smag) bind(C, name="hit3d_set_les")
+       import :: c_int, c_int64_t, c_double
+       integer(c_int64_t), intent(in) :: les_model
+       real(c_double), intent(in) :: c_smag
+     end function c_set_les
+     integer(c_int) function c_turb_visc(out) bind(C, name="hit3d_turb_visc")
+       import :: c_int, c_float
+       real(c_float), intent(out) :: out(*)
+     end function c_turb_visc
+     integer(c_int) function c_snapshot(n_field, out) bind(C, name="hit3d_snapshot")
+       import :: c_int, c_int64_t, c_float
+       integer(c_int64_t), intent(in) :: n_field
+       real(c_float), intent(out) :: out(*)
+     end function c_snapshot
+     integer(c_int) function c_skewness(skew) bind(C, name="hit3d_skewness")
+       import :: c_int, c_double
+       real(c_double), intent(out) :: skew
+     end function c_skewness
+     ! particles: uvwp may be c_null_ptr (set: -999 everywhere; get: not copied)
+     integer(c_int) function c_particles_set(np, xyzp, uvwp) bind(C, name="hit3d_particles_set")
+       import :: c_int, c_int64_t, c_double, c_ptr
+       integer(c_int64_t), intent(in) :: np
+       real(c_double), intent(in) :: xyzp(*)
+       type(c_ptr), value :: uvwp
+     end function c_particles_set
+     integer(c_int) function c_particles_get(xyzp, uvwp) bind(C, name="hit3d_particles_get")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: xyzp, uvwp
+     end function c_particles_get
+     integer(c_int) function c_fields_to_parts() bind(C, name="hit3d_fields_to_parts")
+       import :: c_int
+     end function c_fields_to_parts
+     integer(c_int) function c_particles_move(dt, scheme) bind(C, name="hit3d_particles_move")
+       import :: c_int, c_int64_t, c_double
+       real(c_double), intent(in) :: dt
+       integer(c_int64_t), intent(in) :: scheme
+     end function c_particles_move
+     integer(c_int) function c_set_fusion(on) bind(C, name="hit3d_set_fusion")
+       import :: c_int, c_int64_t
+       integer(c_int64_t), intent(in) :: on
+     end function c_set_fusion
+     integer(c_int) function c_set_graphs(on) bind(C, name="hit3d_set_graphs")
+       import :: c_int, c_int64_t
+       integer(c_int64_t), intent(in) :: on
+     end function c_set_graphs
+     integer(c_int) function c_graph_stats(captured, replays) bind(C, name="hit3d_graph_stats")
+       import :: c_int, c_int64_t
+       integer(c_int64_t), intent(out) :: captured, replays
+     end function c_graph_stats
+     integer(c_int) function c_get_ialias(out) bind(C, name="hit3d_get_ialias")
+       import :: c_int, c_int8_t
+       integer(c_int8_t), intent(out) :: out(*)
+     end function c_get_ialias
+     integer(c_int) function c_rng_state(iy) bind(C, name="hit3d_rng_state")
+       import :: c_int, c_int64_t
+       integer(c_int64_t), intent(out) :: iy
+     end function c_rng_state
+     integer(c_int) function c_velocity_rng_state(nx, rn1, iy) &
+          bind(C, name="hit3d_velocity_rng_state")
+       import :: c_int, c_int64_t, c_double
+       integer(c_int64_t), intent(in) :: nx
+       real(c_double), intent(in) :: rn1
+       integer(c_int64_t), intent(out) :: iy
+     end function c_velocity_rng_state
+     integer(c_int) function c_scalar_gaussian(nx, rn2, iy_prev, out, iy_next) &
+          bind(C, name="hit3d_scalar_gaussian")
+       import :: c_int, c_int64_t, c_double
+       integer(c_int64_t), intent(in) :: nx, iy_prev
+       real(c_double), intent(in) :: rn2
+       real(c_double), intent(out) :: out(*)
+       integer(c_int64_t), intent(out) :: iy_next
+     end function c_scalar_gaussian
+     integer(c_int) function c_random_draws(seed, skip, count, out) &
+          bind(C, name="hit3d_random_draws")
+       import :: c_int, c_int64_t, c_double
+       real(c_double), intent(in) :: seed
+       integer(c_int64_t), intent(in) :: skip, count
+       real(c_double), intent(out) :: out(*)
+     end function c_random_draws
+     integer(c_int) function c_init_gaussian(nx, rn1, c, out) bind(C, name="hit3d_init_gaussian")
+       import :: c_int, c_int64_t, c_double
+       integer(c_int64_t), intent(in) :: nx, c
+       real(c_double), intent(in) :: rn1
+       real(c_double), intent(out) :: out(*)
+     end function c_init_gaussian
+     ! learned closure (les_model 11)
+     integer(c_int) function c_mlp_set_params(params) bind(C, name="hit3d_mlp_set_params")
+       import :: c_int, c_float
+       real(c_float), intent(in) :: params(*)
+     end function c_mlp_set_params
+     integer(c_int) function c_mlp_get_params(params) bind(C, name="hit3d_mlp_get_params")
+       import :: c_int, c_float
+       real(c_float), intent(out) :: params(*)
+     end function c_mlp_get_params
+     integer(c_int) function c_mlp_set_norm(xmean, xstd, ymean, ystd) &
+          bind(C, name="hit3d_mlp_set_norm")
+       import :: c_int, c_double
+       real(c_double), intent(in) :: xmean(*), xstd(*), ymean(*), ystd(*)
+     end function c_mlp_set_norm
+     integer(c_int) function c_mlp_set_data(count, x, t) bind(C, name="hit3d_mlp_set_data")
+       import :: c_int, c_int64_t, c_float
+       integer(c_int64_t), intent(in) :: count
+       real(c_float), intent(in) :: x(*), t(*)
+     end function c_mlp_set_data
+     integer(c_int) function c_mlp_train(steps, batch, lr, losses) bind(C, name="hit3d_mlp_train")
+       import :: c_int, c_int64_t, c_double
+       integer(c_int64_t), intent(in) :: steps, batch
+       real(c_double), intent(in) :: lr
+       real(c_double), intent(out) :: losses(*)
+     end function c_mlp_train
+     integer(c_int) function c_mlp_grads(count, x, t, grads, loss) bind(C, name="hit3d_mlp_grads")
+       import :: c_int, c_int64_t, c_float, c_double
+       integer(c_int64_t), intent(in) :: count
+       real(c_float), intent(in) :: x(*), t(*)
+       real(c_float), intent(out) :: grads(*)
+       real(c_double), intent(out) :: loss
+     end function c_mlp_grads
+     integer(c_int) function c_mlp_forward(count, x, y) bind(C, name="hit3d_mlp_forward")
+       import :: c_int, c_int64_t, c_float
+       integer(c_int64_t), intent(in) :: count
+       real(c_float), intent(in) :: x(*)
+       real(c_float), intent(out) :: y(*)
+     end function c_mlp_forward
+     integer(c_int) function c_mlp_fallbacks(count) bind(C, name="hit3d_mlp_fallbacks")
+       import :: c_int, c_int64_t
+       integer(c_int64_t), intent(out) :: count
+     end function c_mlp_fallbacks
+     integer(c_int) function c_box_filter(ratio, ubar, tau) bind(C, name="hit3d_box_filter")
+       import :: c_int, c_int64_t, c_double
+       integer(c_int64_t), intent(in) :: ratio
+       real(c_double), intent(out) :: ubar(*), tau(*)
+     end function c_box_filter
+     integer(c_int) function c_sgs_features(delta, x) bind(C, name="hit3d_sgs_features")
+       import :: c_int, c_double, c_float
+       real(c_double), intent(in) :: delta
+       real(c_float), intent(out) :: x(*)
+     end function c_sgs_features
+     ! timing and synchronisation
+     integer(c_int) function c_timing_enable(on) bind(C, name="hit3d_timing_enable")
+       import :: c_int, c_int64_t
+       integer(c_int64_t), intent(in) :: on
+     end function c_timing_enable
+     integer(c_int) function c_timing_read(cap, ms, count) bind(C, name="hit3d_timing_read")
+       import :: c_int, c_int64_t, c_double
+       integer(c_int64_t), intent(in) :: cap
+       real(c_double), intent(out) :: ms(*)
+       integer(c_int64_t), intent(out) :: count(*)
+     end function c_timing_read
+     integer(c_int) function c_timing_names(buf, len) bind(C, name="hit3d_timing_names")
+       import :: c_int, c_int64_t, c_char
+       character(kind=c_char), intent(out) :: buf(*)
+       integer(c_int64_t), intent(in) :: len
+     end function c_timing_names
+     integer(c_int) function c_synchronize() bind(C, name="hit3d_synchronize")
+       import :: c_int
+     end function c_synchronize
      integer(c_int) function c_last_error(buf, len) bind(C, name="hit3d_last_error")
        import :: c_int, c_int64_t, c_char
        character(kind=c_char), intent(out) :: buf(*)

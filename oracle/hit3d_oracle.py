@@ -319,6 +319,30 @@ def _pad(p, n):
     return out
 
 
+def rhs_velocity_rotational(fields: np.ndarray, nu: float, n: int):
+    """The rotational form of the velocity RHS with the 2/3 rule, an option
+    of this build replacing the divergence form of rhs_velocity.f90:56-102:
+    omega = i k x u-hat, the products u x omega in physical space, then
+    r-hat = (u x omega)^ - nu k^2 u-hat on ialias == 0, zero elsewhere
+    (the mask of :104-144).  It differs from the divergence form by the
+    gradient i k (|u|^2/2)^, which the projection removes.  Returns (rhs[3]
+    interleaved, physical velocity[3])."""
+    kx, ky, kz = _kgrid(n)
+    ia = ialias(n)[:, :, 0::2]
+    U = [to_complex(fields[c]) for c in range(3)]
+    om = [1j * (ky * U[2] - kz * U[1]),
+          1j * (kz * U[0] - kx * U[2]),
+          1j * (kx * U[1] - ky * U[0])]
+    phys = [fft_inverse(fields[c]) for c in range(3)]
+    u, v, w = (p[:, :, :n] for p in phys)
+    o1, o2, o3 = (fft_inverse(to_interleaved(o))[:, :, :n] for o in om)
+    cr = [v * o3 - w * o2, w * o1 - u * o3, u * o2 - v * o1]
+    C = [to_complex(fft_forward(_pad(c, n))) for c in cr]
+    lap = -nu * ((kx ** 2 + ky ** 2) + kz ** 2)
+    rhs = [np.where(ia > 0, 0.0, C[c] + lap * U[c]) for c in range(3)]
+    return np.stack([to_interleaved(r) for r in rhs]), np.stack(phys)
+
+
 def reaction_rate(phi: np.ndarray, rtype: int, rrate: float, mean: float) -> np.ndarray:
     """add_reaction (rhs_scalars.f90:405-456) in physical space: rtype 1 KPP
     r(1-phi^2), 2 symmetric bistable r(1-phi^2)phi, 3 self-adjusting
@@ -914,6 +938,7 @@ class Config:
     re: float = 0.0
     les_model: int = 0
     c_smag: float = 0.18
+    rotational: bool = False   # u x omega form of the velocity RHS (2/3 rule only)
 
     @property
     def kmax(self):
@@ -952,7 +977,12 @@ def step(cfg: Config, st: State) -> State:
         new, old = advance(fields[3:3 + ns], rs, rhs_old[3:3 + ns], st.dt, st.fos)
         fields[3:3 + ns] = new
         rhs_old[3:3 + ns] = old
-    rhs, phys = rhs_velocity(fields, cfg.nu, cfg.dealias, n)
+    if cfg.rotational:
+        if cfg.dealias != 0:
+            raise ValueError("the rotational form is built for the 2/3 rule")
+        rhs, phys = rhs_velocity_rotational(fields, cfg.nu, n)
+    else:
+        rhs, phys = rhs_velocity(fields, cfg.nu, cfg.dealias, n)
     if tv is not None:
         rhs = les_rhs_velocity(fields, rhs, tv, n)
     courant = courant_number(phys, n, st.dt)

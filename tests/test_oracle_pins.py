@@ -275,3 +275,70 @@ def test_step_projects_after_the_update(oracle):
     # the step result differs from "project the old field, then add dt*rhs"
     wrong = oracle.pressure(f, n) + dt * rhs
     assert np.abs(st.fields - wrong).max() > 1e-3 * dt * grad
+
+
+# ------------------------------------------------------------ rotational form
+
+def test_rotational_rhs_shear_closed_form(oracle):
+    """u = (sin y, 0, 0): omega = (0, 0, -cos y), u x omega = (0, sin y cos y,
+    0) = grad(|u|^2/2) (u . grad u = 0), so the rotational RHS is that
+    gradient minus nu k^2 u, while the divergence form has no nonlinear
+    term.  Catches a sign of omega, of the cross product, or a permuted
+    component."""
+    n, nu = 16, 0.1
+    z = np.zeros((n, n, n + 2))
+    f = np.stack([oracle.fft_forward(_pad(_modes_field(n, [((0, 1, 0), 0.0, 1.0)]), n)), z, z])
+    rhs, _ = oracle.rhs_velocity_rotational(f, nu, n)
+    got = [oracle.fft_inverse(r)[:, :, :n] for r in rhs]
+    ex = [_modes_field(n, [((0, 1, 0), 0.0, -nu)]),
+          _modes_field(n, [((0, 2, 0), 0.0, 0.5)]),     # sin y cos y = sin(2y)/2
+          np.zeros((n, n, n))]
+    for g, e in zip(got, ex):
+        assert np.abs(g - e).max() < 1e-13
+
+
+def test_rotational_rhs_projects_to_divergence_form(oracle):
+    """For band-limited fields the 2/3 rule is exact for both forms, and
+    u x omega = -d_j(u_i u_j) + grad(|u|^2/2) for solenoidal u: the
+    projected RHS of the two forms agree to rounding, and the unprojected
+    ones differ by a gradient (k x (r_rot - r_div) = 0)."""
+    n, nu = 32, 0.02
+    rng = np.random.default_rng(5)
+    modes = [_random_modes(rng, [0, 1, 2, 3, -2], [0, 1, -1, 3], [0, 2, -1, -3], 6)
+             for _ in range(3)]
+    U = np.stack([oracle.fft_forward(_pad(_modes_field(n, md), n)) for md in modes])
+    U = oracle.pressure(U, n)
+    r_div, _ = oracle.rhs_velocity(U, nu, 0, n)
+    r_rot, _ = oracle.rhs_velocity_rotational(U, nu, n)
+    pd, pr = oracle.pressure(r_div, n), oracle.pressure(r_rot, n)
+    assert np.abs(pr - pd).max() < 1e-12 * np.abs(pd).max()
+    d = [oracle.to_complex(r_rot[c] - r_div[c]) for c in range(3)]
+    assert np.abs(d[0]).max() > 1e-3 * np.abs(pd).max()   # a gradient part exists
+    kx, ky, kz = oracle._kgrid(n)
+    curl = [ky * d[2] - kz * d[1], kz * d[0] - kx * d[2], kx * d[1] - ky * d[0]]
+    assert max(np.abs(c).max() for c in curl) < 1e-11 * np.abs(pd).max()
+
+
+def test_rotational_step_equals_divergence_step(oracle):
+    """Driver steps (Euler then AB2, forcing, projection) with the two forms
+    from one solenoidal band-limited field give the same fields to rounding:
+    the gradient part of the rotational RHS, kept in rhs_old, is removed by
+    the projection after every update."""
+    n = 32
+    rng = np.random.default_rng(8)
+    modes = [_random_modes(rng, [0, 1, 2, -1], [0, 1, -2, 2], [0, 1, 2, -1], 6)
+             for _ in range(3)]
+    U = np.stack([oracle.fft_forward(_pad(_modes_field(n, md), n)) for md in modes])
+    U = oracle.dealias_all(oracle.pressure(U, n), n // 3, n)
+    out = {}
+    for rot in (False, True):
+        cfg = oracle.Config(n=n, nu=0.01, dt=0.002, dealias=0, flow_type=1, kfmax=2, famp=0.5,
+                            rotational=rot)
+        st = oracle.State(fields=U.copy(), rhs_old=np.zeros_like(U), dt=cfg.dt)
+        for _ in range(3):
+            st = oracle.step(cfg, st)
+        out[rot] = st
+    a, b = out[False], out[True]
+    assert np.abs(b.fields - a.fields).max() < 1e-12 * np.abs(a.fields).max()
+    assert abs(b.courant - a.courant) < 1e-12 * a.courant
+    assert abs(b.energy - a.energy) < 1e-12 * a.energy
