@@ -224,10 +224,12 @@ def run_workload(name, steps, warmup):
     print(json.dumps(out), flush=True)
 
 
-def run_form(n, precision, steps, warmup):
+def run_form(n, precision, steps, warmup, form=0):
     from hit3d_amd.driver import Run
     p = params(n)
     r = Run(p, precision=precision)
+    if form:
+        r.h.set_nonlinear(form)
     t0 = time.time()
     r.begin_new()
     r.h.synchronize()
@@ -407,6 +409,8 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-rotational", action="store_true",
+                    help="skip the timing of the rotational (u x omega) form")
     ap.add_argument("--workload", default="c3", choices=["c3", "c4", "fft"] + sorted(WORKLOADS),
                     help="c3 (default) prints the benchmark line; c1, c2, c4, c5 time the other "
                          "configurations; fft is the transform microbenchmark")
@@ -444,6 +448,13 @@ def main():
                 mixed["accuracy"] = accuracy_run(n, a.accuracy_steps, 10)
         except Exception as e:  # reported, never silently replaced
             mixed = {"error": str(e)}
+    # the rotational (u x omega) form of the same step, FP64
+    rotational = None
+    if not a.no_rotational:
+        tr, timr, str_, _ = run_form(n, 0, a.steps, a.warmup, form=1)
+        rotational = {"s_per_step": tr, "vs_divergence_form": tr / t64,
+                      "energy_rel_diff_vs_divergence": abs(str_.energy - st64.energy) / st64.energy,
+                      "kernel_ms_per_step": {k: v[0] / a.steps for k, v in timr.items() if v[1]}}
     copy = copy_bandwidth(n)
     copy_gbs = copy["stream_copy_GBs"]
     # dominant kernel class of the FP64 step
@@ -506,6 +517,7 @@ def main():
         "model_bytes_over_step_time_vs_peak": model_bytes / t64 / (PEAK_HBM_GBS * 1e9),
         "copy": copy,
         "mixed": mixed,
+        "rotational_fp64": rotational,
         "kernel_ms_per_step": {k: v[0] / a.steps for k, v in tim64.items() if v[1]},
         "roofline": roof,
         "roofline_by_kernel": by_kernel,

@@ -72,6 +72,7 @@ struct Lib {
     std::vector<float*> W32;
 
     bool fuse = true;  // hit3d_step uses the fused pipeline where it applies
+    bool rot = false;  // rotational (u x omega) form of the fused step's nonlinear term
 
     // hit3d_step replayed from captured HIP graphs (graph mode).  A capture is
     // valid for the slot pointers it started from (the out-of-place passes

@@ -152,6 +152,87 @@ __global__ __launch_bounds__((CE<N, T>::THREADS)) void k_inv_z(SG g, P3 F, W6<T>
     }
 }
 
+// A for the rotational form: each block loads the three velocity columns
+// (kx, ky) once and transforms six: u, v, w and omega = i k x u, formed in
+// FP64 from the loaded entries (omega_c = i (k_a u_b - k_b u_a), (a, b) =
+// (c+1, c+2) mod 3).  Loading each column once instead of once per output
+// that needs it reads a third of the cube traffic of six independent
+// blocks.
+// Blocks of at most 512 threads (half the columns where the other column
+// passes use 1024): the three loaded columns stay in registers across the
+// six transforms, which 1024-thread blocks (128 VGPRs) would spill.
+template <int N, typename T>
+struct CR {
+    static constexpr int E = CE<N, T>::E;
+    static constexpr int TPF = N / E;
+    static constexpr int B = CE<N, T>::B * TPF > 512 ? CE<N, T>::B / 2 : CE<N, T>::B;
+    static constexpr int THREADS = B * TPF;
+};
+
+template <int N, typename T>
+__global__ __launch_bounds__((CR<N, T>::THREADS)) void k_inv_z_rot(SG g, P3 F, W6<T> A,
+                                                                 const cpx<T>* __restrict__ tw) {
+    constexpr int E = CR<N, T>::E, B = CR<N, T>::B, TPF = CR<N, T>::TPF;
+    constexpr int PS = 0;
+    __shared__ cpx<T> lds[padded_size<PS>(N * B)];
+    const int c = threadIdx.x % B, t = threadIdx.x / B;
+    const int kx = blockIdx.x * B + c;
+    const bool valid = kx < Kc<N>::NK;
+    const int jy = kept_row(blockIdx.y, Kc<N>::K, N);
+    const int kyv = wavenum(jy, N);
+    // the velocity in the work precision (the mixed form's vorticity is
+    // formed from the FP32-rounded velocity, in FP64, then rounded)
+    cpx<T> uv[3][E];
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) {
+        const cpx<double>* base = reinterpret_cast<const cpx<double>*>(F.p[cc]);
+        const cpx<double>* src = base + (int64_t)jy * N * g.P + kx;
+#pragma unroll
+        for (int s = 0; s < E; ++s) {
+            if (!slot_live<N, TPF>(s)) {
+                uv[cc][s] = cpx<T>{(T)0, (T)0};
+                continue;
+            }
+            const int e = t + s * TPF;
+            const bool ok = valid && kept(e, Kc<N>::K, N);
+            const cpx<double> x = ld_g<kStream<N>>(ok ? &src[(int64_t)e * g.P] : base);
+            uv[cc][s] = ok ? to<T>(x) : cpx<T>{(T)0, (T)0};
+        }
+    }
+#pragma unroll
+    for (int comp = 0; comp < 6; ++comp) {
+        cpx<T> v[E];
+#pragma unroll
+        for (int s = 0; s < E; ++s) {
+            if (comp < 3) {
+                v[s] = comp == 0 ? uv[0][s] : comp == 1 ? uv[1][s] : uv[2][s];
+            } else {
+                const int c3 = comp - 3;
+                const int kzv = wavenum(t + s * TPF, N);
+                const double ka = c3 == 0 ? (double)kyv : c3 == 1 ? (double)kzv : (double)kx;
+                const double kb = c3 == 0 ? (double)kzv : c3 == 1 ? (double)kx : (double)kyv;
+                const cpx<T> ub = c3 == 0 ? uv[2][s] : c3 == 1 ? uv[0][s] : uv[1][s];
+                const cpx<T> ua = c3 == 0 ? uv[1][s] : c3 == 1 ? uv[2][s] : uv[0][s];
+                const cpx<double> d{ka * (double)ub.x - kb * (double)ua.x,
+                                    ka * (double)ub.y - kb * (double)ua.y};
+                v[s] = to<T>(mul_i(d));
+            }
+        }
+        block_fft<N, E, 1, T, B, PS>(v, t, lds + c, tw + N);
+        T* ap = A.p[0];
+        if (comp == 1) ap = A.p[1];
+        if (comp == 2) ap = A.p[2];
+        if (comp == 3) ap = A.p[3];
+        if (comp == 4) ap = A.p[4];
+        if (comp == 5) ap = A.p[5];
+        cpx<T>* dst = reinterpret_cast<cpx<T>*>(ap) + (int64_t)jy * g.PA + kx;
+        if (valid) {
+#pragma unroll
+            for (int s = 0; s < E; ++s) st_g<kStream<N>>(&dst[(int64_t)(t + s * TPF) * N * g.PA], v[s]);
+        }
+    }
+}
+
 // ---------------------------------------------------------------- B / D
 // DIR=+1: inverse y, loads kept rows only, stores all rows.
 // DIR=-1: forward y, loads all rows, stores kept rows only.
@@ -354,6 +435,150 @@ __global__ __launch_bounds__((XP<N, T>::THREADS)) void k_xp(SG g, W6<T> Wb,
     }
 }
 
+// ---------------------------------------------------------------- C (rotational)
+// The x pass of the rotational form: three inverse transforms of row pairs
+// (u + i v, w + i omega_1, omega_2 + i omega_3), the Courant max, the cross
+// product c = u x omega (c_1 = v w3 - w w2, c_2 = w w1 - u w3,
+// c_3 = u w2 - v w1), then two forward transforms (c_1 + i c_2 split by the
+// Hermitian symmetry, and c_3 alone) into buffers 0, 1, 2: five transforms
+// per row, as in the divergence form.
+template <int N, typename T>
+__global__ __launch_bounds__((XP<N, T>::THREADS)) void k_xp_rot(SG g, W6<T> Wb,
+                                                              const cpx<double>* __restrict__ twd,
+                                                              T norm, double* __restrict__ blockmax) {
+    using C = XP<N, T>;
+    constexpr int E = C::E, TPF = C::TPF;
+    constexpr int PS = kSwizzle;
+    constexpr int FS = padded_size<PS>(N);
+    constexpr int NK = Kc<N>::NK;
+    static_assert(64 % TPF == 0 || TPF % 64 == 0, "a row is part of a wavefront or whole ones");
+    constexpr bool WAVE = TPF <= 64;
+    __shared__ cpx<T> lds[C::ROWS][FS];
+    __shared__ double redm[C::THREADS / 64 + 1];
+    constexpr bool CACHE = E == 8 && N >= 64 && N == (1 << (3 * stage_index(N, 8)));
+    using TC = std::conditional_t<CACHE, TwCache<T, N, E>, void>;
+    TwCache<T, N, E> tcache;
+    const TC* tcp = nullptr;
+    const int t = threadIdx.x % TPF;
+    const int rr = threadIdx.x / TPF;
+    const int64_t row = (int64_t)blockIdx.x * C::ROWS + rr;
+    const int partner = (TPF - t) & (TPF - 1);
+    cpx<T>* L = lds[rr];
+    if constexpr (CACHE) {
+        fill_twiddles<N, E>(tcache, t, twd + N);
+        tcp = &tcache;
+    }
+    auto rowp = [&](int b) { return reinterpret_cast<const cpx<T>*>(Wb.p[b]) + row * g.PA; };
+    // z = a + i b of two Hermitian-extended half spectra
+    auto pair_load = [&](int a, int b, cpx<T>(&z)[1][E]) {
+        const cpx<T>* A = rowp(a);
+        const cpx<T>* Bp = rowp(b);
+#pragma unroll
+        for (int s = 0; s < E; ++s) {
+            const cpx<T> x = herm_load<N, TPF, NK>(A, t, s), y = herm_load<N, TPF, NK>(Bp, t, s);
+            z[0][s] = cpx<T>{x.x - y.y, x.y + y.x};
+        }
+    };
+    T u[E], v[E], w[E], o1[E];
+    cpx<T> z1[1][E];
+    pair_load(2, 3, z1);  // w + i omega_1, loaded ahead of the first transform
+    {
+        cpx<T> z[1][E];
+        pair_load(0, 1, z);
+        multi_fft<N, E, 1, T, 1, 1, FS, PS, WAVE, true, double, TC>(z, t, L, twd + N, tcp);
+#pragma unroll
+        for (int s = 0; s < E; ++s) {
+            u[s] = z[0][s].x * norm;
+            v[s] = z[0][s].y * norm;
+        }
+    }
+    cpx<T> z2[1][E];
+    pair_load(4, 5, z2);  // omega_2 + i omega_3, in flight during the next transform
+    lds_sync<WAVE>();
+    multi_fft<N, E, 1, T, 1, 1, FS, PS, WAVE, true, double, TC>(z1, t, L, twd + N, tcp);
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+        w[s] = z1[0][s].x * norm;
+        o1[s] = z1[0][s].y * norm;
+    }
+    lds_sync<WAVE>();
+    multi_fft<N, E, 1, T, 1, 1, FS, PS, WAVE, true, double, TC>(z2, t, L, twd + N, tcp);
+    // Courant number, rhs_velocity.f90:31-32 (FP64 max)
+    double m = 0.0;
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+        const double a = ((double)fabs(u[s]) + (double)fabs(v[s])) + (double)fabs(w[s]);
+        m = a > m ? a : m;
+    }
+    // u x omega; c_1 + i c_2 in z1, c_3 in z2 (real)
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+        const T o2 = z2[0][s].x * norm, o3 = z2[0][s].y * norm;
+        z1[0][s] = cpx<T>{v[s] * o3 - w[s] * o2, w[s] * o1[s] - u[s] * o3};
+        z2[0][s] = cpx<T>{u[s] * o2 - v[s] * o1[s], (T)0};
+    }
+    lds_sync<WAVE>();
+    multi_fft<N, E, -1, T, 1, 1, FS, PS, WAVE, true, double, TC>(z1, t, L, twd + N, tcp);
+    {
+        cpx<T>* outp = reinterpret_cast<cpx<T>*>(Wb.p[0]) + row * g.PA;
+        cpx<T>* outq = reinterpret_cast<cpx<T>*>(Wb.p[1]) + row * g.PA;
+        if constexpr (WAVE) {
+#pragma unroll
+            for (int s = 0; s < E; ++s) {
+                if (s * TPF < NK) {
+                    const cpx<T> mine = (t == 0) ? z1[0][(E - s) & (E - 1)] : z1[0][E - 1 - s];
+                    const cpx<T> zm = conj(cpx<T>{__shfl(mine.x, partner, TPF),
+                                                  __shfl(mine.y, partner, TPF)});
+                    const int k = t + s * TPF;
+                    if (k < NK) {
+                        const cpx<T> zk = z1[0][s];
+                        st_g<kStream<N>>(&outp[k], scale(zk + zm, (T)0.5));
+                        st_g<kStream<N>>(&outq[k], scale(mul_mi(zk - zm), (T)0.5));
+                    }
+                }
+            }
+        } else {
+            lds_sync<false>();
+#pragma unroll
+            for (int s = 0; s < E; ++s) L[padi<PS>(t + s * TPF)] = z1[0][s];
+            lds_sync<false>();
+#pragma unroll
+            for (int s = 0; s < E; ++s) {
+                const int k = t + s * TPF;
+                if (s * TPF < NK && k < NK) {
+                    const cpx<T> zm = conj(L[padi<PS>((N - k) & (N - 1))]);
+                    const cpx<T> zk = z1[0][s];
+                    st_g<kStream<N>>(&outp[k], scale(zk + zm, (T)0.5));
+                    st_g<kStream<N>>(&outq[k], scale(mul_mi(zk - zm), (T)0.5));
+                }
+            }
+        }
+    }
+    lds_sync<WAVE>();
+    multi_fft<N, E, -1, T, 1, 1, FS, PS, WAVE, true, double, TC>(z2, t, L, twd + N, tcp);
+    {
+        // a real row's transform is its own half spectrum
+        cpx<T>* outr = reinterpret_cast<cpx<T>*>(Wb.p[2]) + row * g.PA;
+#pragma unroll
+        for (int s = 0; s < E; ++s) {
+            const int k = t + s * TPF;
+            if (s * TPF < NK && k < NK) st_g<kStream<N>>(&outr[k], z2[0][s]);
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double q = __shfl_down(m, o, 64);
+        m = q > m ? q : m;
+    }
+    if ((threadIdx.x & 63) == 0) redm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double mm = 0.0;
+        for (int i = 0; i < (C::THREADS + 63) / 64; ++i) mm = redm[i] > mm ? redm[i] : mm;
+        blockmax[blockIdx.x] = mm;
+    }
+}
+
 // ---------------------------------------------------------------- E
 struct Upd {
     double* F[3];
@@ -365,9 +590,13 @@ struct Upd {
     int kfmax, forced, first;
 };
 
-template <int N, typename T>
+// ROT: three buffers hold (u x omega)_i, which is the nonlinear term itself
+// (r_i = (u x omega)_i^ - nu k^2 u_i); otherwise six hold the products
+// u_i u_j and r_i = -i k_j (u_i u_j)^ - nu k^2 u_i.
+template <int N, typename T, bool ROT = false>
 __global__ __launch_bounds__((CZ<N, T>::THREADS)) void k_fwd_z_update(
     SG g, W6<T> Q, Upd u, const cpx<double>* __restrict__ twd) {
+    constexpr int NP = ROT ? 3 : 6;
     constexpr int E = CZ<N, T>::E, B = CZ<N, T>::B, TPF = CZ<N, T>::TPF;
     constexpr int PS = 0;
     __shared__ cpx<T> lds[padded_size<PS>(N * B)];
@@ -421,12 +650,12 @@ __global__ __launch_bounds__((CZ<N, T>::THREADS)) void k_fwd_z_update(
     cpx<T> nxt[E];
     if (PREFETCH) load(0, nxt);
 #pragma unroll 1
-    for (int p = 0; p < 6; ++p) {
+    for (int p = 0; p < NP; ++p) {
         cpx<T> v[E];
         if (PREFETCH) {
 #pragma unroll
             for (int s = 0; s < E; ++s) v[s] = nxt[s];
-            if (p < 5) load(p + 1, nxt);
+            if (p < NP - 1) load(p + 1, nxt);
         } else {
             load(p, v);
         }
@@ -441,6 +670,12 @@ __global__ __launch_bounds__((CZ<N, T>::THREADS)) void k_fwd_z_update(
 #pragma unroll
         for (int s = 0; s < E; ++s) {
             if (!slot_live<N, TPF>(s) || !kept(t + s * TPF, Kc<N>::K, N)) continue;  // never written out
+            if constexpr (ROT) {
+                if (p == 0) acc[0][s] = v[s];
+                if (p == 1) acc[1][s] = v[s];
+                if (p == 2) acc[2][s] = v[s];
+                continue;
+            }
             const T kzT = (T)wavenum(t + s * TPF, N);
             // p: 0 uu, 1 uv, 2 uw, 3 vv, 4 vw, 5 ww
             if (p == 0) acc[0][s] = scale(v[s], kxT);
@@ -515,9 +750,15 @@ __global__ __launch_bounds__((CZ<N, T>::THREADS)) void k_fwd_z_update(
 #pragma unroll
             for (int cc = 0; cc < 3; ++cc) {
                 const cpx<double> fc = f[cc][q];
-                // r = i * (-t) + lap * f, t = k.P  (rhs_velocity.f90:89-136)
-                const double tre = -(double)acc[cc][s].x, tim = -(double)acc[cc][s].y;
-                cpx<double> r = {-tim + lap * fc.x, tre + lap * fc.y};
+                cpx<double> r;
+                if constexpr (ROT) {
+                    // r = (u x omega)^ + lap * f
+                    r = {(double)acc[cc][s].x + lap * fc.x, (double)acc[cc][s].y + lap * fc.y};
+                } else {
+                    // r = i * (-t) + lap * f, t = k.P  (rhs_velocity.f90:89-136)
+                    const double tre = -(double)acc[cc][s].x, tim = -(double)acc[cc][s].y;
+                    r = {-tim + lap * fc.x, tre + lap * fc.y};
+                }
                 if (fnode) r = {r.x + fac * fc.x, r.y + fac * fc.y};
                 if (u.first) {
                     fn[cc] = {fc.x + dt * r.x, fc.y + dt * r.y};
@@ -555,7 +796,7 @@ static void dispatch_step(int n, F&& f) {
     }
 }
 
-template <int N, typename T>
+template <int N, typename T, bool ROT>
 static void run(Lib& L, const double* dtp, bool first, const cpx<T>* tw) {
     if (L.kmax != Kc<N>::K) throw std::runtime_error("the fused step expects kmax = n/3");
     SG g;
@@ -576,27 +817,35 @@ static void run(Lib& L, const double* dtp, bool first, const cpx<T>* tw) {
     }
     {
         Timed tm(TC_STEP_INV_Z);
-        hipLaunchKernelGGL((k_inv_z<N, T>), dim3(ncb, nky, 3), dim3(CE<N, T>::THREADS), 0, L.st,
-                           g, F, Wb, tw);
+        if constexpr (ROT)
+            hipLaunchKernelGGL((k_inv_z_rot<N, T>), dim3((g.NK + CR<N, T>::B - 1) / CR<N, T>::B, nky),
+                               dim3(CR<N, T>::THREADS), 0, L.st, g, F, Wb, tw);
+        else
+            hipLaunchKernelGGL((k_inv_z<N, T>), dim3(ncb, nky, 3), dim3(CE<N, T>::THREADS), 0,
+                               L.st, g, F, Wb, tw);
     }
     {
         Timed tm(TC_STEP_INV_Y);
-        hipLaunchKernelGGL((k_y<N, 1, T>), dim3(ncb, N, 3), dim3(CY<N, T>::THREADS), 0, L.st, g, Wb,
-                           tw);
+        hipLaunchKernelGGL((k_y<N, 1, T>), dim3(ncb, N, ROT ? 6 : 3), dim3(CY<N, T>::THREADS), 0,
+                           L.st, g, Wb, tw);
     }
     {
         Timed tm(TC_STEP_X);
         const T norm = (T)(1.0 / ((double)N * (double)N * (double)N));
         const int64_t nb = (int64_t)N * N / XP<N, T>::ROWS;  // <= n*n, the size of dmax
         // FP64 twiddles in both forms (the FP32 form's recurrence in FP64)
-        hipLaunchKernelGGL((k_xp<N, T>), dim3(nb), dim3(XP<N, T>::THREADS), 0, L.st, g, Wb,
-                           L.tw64, norm, L.dmax);
+        if constexpr (ROT)
+            hipLaunchKernelGGL((k_xp_rot<N, T>), dim3(nb), dim3(XP<N, T>::THREADS), 0, L.st, g,
+                               Wb, L.tw64, norm, L.dmax);
+        else
+            hipLaunchKernelGGL((k_xp<N, T>), dim3(nb), dim3(XP<N, T>::THREADS), 0, L.st, g, Wb,
+                               L.tw64, norm, L.dmax);
         launch_max_fold(L.dmax, nb, L.dscal, L.st);
     }
     {
         Timed tm(TC_STEP_FWD_Y);
-        hipLaunchKernelGGL((k_y<N, -1, T>), dim3(ncb, N, 6), dim3(CY<N, T>::THREADS), 0, L.st, g,
-                           Wb, tw);
+        hipLaunchKernelGGL((k_y<N, -1, T>), dim3(ncb, N, ROT ? 3 : 6), dim3(CY<N, T>::THREADS), 0,
+                           L.st, g, Wb, tw);
     }
     {
         Timed tm(TC_STEP_FWD_Z);
@@ -614,8 +863,8 @@ static void run(Lib& L, const double* dtp, bool first, const cpx<T>* tw) {
         u.forced = forced ? 1 : 0;
         u.first = first ? 1 : 0;
         const int ncbz = (g.NK + CZ<N, T>::B - 1) / CZ<N, T>::B;
-        hipLaunchKernelGGL((k_fwd_z_update<N, T>), dim3(ncbz, nky), dim3(CZ<N, T>::THREADS), 0,
-                           L.st, g, Wb, u, L.tw64);
+        hipLaunchKernelGGL((k_fwd_z_update<N, T, ROT>), dim3(ncbz, nky), dim3(CZ<N, T>::THREADS),
+                           0, L.st, g, Wb, u, L.tw64);
     }
 }
 
@@ -628,10 +877,16 @@ bool fused_step_supported(bool) {
 
 void fused_step(const double* dtp, bool first, bool mixed) {
     Lib& L = lib();
-    if (mixed)
-        dispatch_step(L.n, [&](auto NC) { run<decltype(NC)::value, float>(L, dtp, first, L.tw32); });
-    else
-        dispatch_step(L.n, [&](auto NC) { run<decltype(NC)::value, double>(L, dtp, first, L.tw64); });
+    dispatch_step(L.n, [&](auto NC) {
+        constexpr int N = decltype(NC)::value;
+        if (mixed) {
+            if (L.rot) run<N, float, true>(L, dtp, first, L.tw32);
+            else run<N, float, false>(L, dtp, first, L.tw32);
+        } else {
+            if (L.rot) run<N, double, true>(L, dtp, first, L.tw64);
+            else run<N, double, false>(L, dtp, first, L.tw64);
+        }
+    });
 }
 
 }  // namespace hit3d

@@ -161,6 +161,10 @@ module hit3d_hip
        import :: c_int, c_int64_t
        integer(c_int64_t), intent(in) :: on
      end function c_set_fusion
+     integer(c_int) function c_set_nonlinear(form) bind(C, name="hit3d_set_nonlinear")
+       import :: c_int, c_int64_t
+       integer(c_int64_t), intent(in) :: form
+     end function c_set_nonlinear
      integer(c_int) function c_set_graphs(on) bind(C, name="hit3d_set_graphs")
        import :: c_int, c_int64_t
        integer(c_int64_t), intent(in) :: on
@@ -237,6 +241,14 @@ module hit3d_hip
        real(c_float), intent(out) :: grads(*)
        real(c_double), intent(out) :: loss
      end function c_mlp_grads
+     integer(c_int) function c_mlp_grads_static(count, x, t, grads, loss) &
+          bind(C, name="hit3d_mlp_grads_static")
+       import :: c_int, c_int64_t, c_float, c_double
+       integer(c_int64_t), intent(in) :: count
+       real(c_float), intent(in) :: x(*), t(*)
+       real(c_float), intent(out) :: grads(*)
+       real(c_double), intent(out) :: loss
+     end function c_mlp_grads_static
      integer(c_int) function c_mlp_forward(count, x, y) bind(C, name="hit3d_mlp_forward")
        import :: c_int, c_int64_t, c_float
        integer(c_int64_t), intent(in) :: count
@@ -273,6 +285,13 @@ module hit3d_hip
        character(kind=c_char), intent(out) :: buf(*)
        integer(c_int64_t), intent(in) :: len
      end function c_timing_names
+     ! include/hit3d_measure.h
+     integer(c_int) function c_stream_copy(reps, blocks, ms, bytes) &
+          bind(C, name="hit3d_stream_copy")
+       import :: c_int, c_int64_t, c_double
+       integer(c_int64_t), intent(in) :: reps, blocks
+       real(c_double), intent(out) :: ms, bytes
+     end function c_stream_copy
      integer(c_int) function c_synchronize() bind(C, name="hit3d_synchronize")
        import :: c_int
      end function c_synchronize

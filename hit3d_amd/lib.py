@@ -43,6 +43,7 @@ _SIGNATURES = {
     "hit3d_pressure": [],
     "hit3d_step": [_F64P, _I64P, _I64P, _I64P, _I64P, _F64P, _F64P],
     "hit3d_set_fusion": [_I64P],
+    "hit3d_set_nonlinear": [_I64P],
     "hit3d_set_graphs": [_I64P],
     "hit3d_graph_stats": [_I64P, _I64P],
     "hit3d_mlp_set_params": [ctypes.POINTER(ctypes.c_float)],
@@ -275,6 +276,11 @@ class Hit3D:
 
     def set_les(self, les_model: int, c_smag: float = 0.18):
         _check(self.lib.hit3d_set_les(_i(les_model), _d(c_smag)))
+
+    def set_nonlinear(self, form: int):
+        """0: divergence form (the reference's); 1: rotational u x omega form
+        of the fused 2/3-rule step."""
+        _check(self.lib.hit3d_set_nonlinear(_i(form)))
 
     def set_fusion(self, on: bool):
         _check(self.lib.hit3d_set_fusion(_i(1 if on else 0)))

@@ -178,6 +178,18 @@ int hit3d_mlp_grads_static(const int64_t *count, const float *X, const float *T,
 int hit3d_box_filter(const int64_t *ratio, double *ubar, double *tau);
 int hit3d_sgs_features(const double *delta, float *X);
 
+/* Form of the nonlinear term in hit3d_step's fused 2/3-rule pipeline:
+ * 0 (default) the divergence form -i k_j (u_i u_j)^ of rhs_velocity.f90:56-102;
+ * 1 the rotational form (u x omega)^, omega = i k x u-hat, an option with no
+ * reference counterpart: it differs from the divergence form by the gradient
+ * i k (|u|^2/2)^, which the projection after the update removes, so the
+ * fields agree to rounding while rhs_old holds the rotational RHS.  Same
+ * number of transforms per step (6 inverse pruned to the cube, 3 forward).
+ * The phase shift rejects form 1; a step that cannot run fused (active
+ * scalars, LES, fusion off) fails while form 1 is selected.  The unfused
+ * entry points (hit3d_rhs_velocity) keep the divergence form. */
+int hit3d_set_nonlinear(const int64_t *form);
+
 /* Selects how hit3d_step runs: *on != 0 (default) the fused pipeline where
  * it applies (2/3 rule without active scalars), 0 the unfused sequence of
  * the reference subroutines above (no reference counterpart; for A/B

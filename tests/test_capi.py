@@ -39,6 +39,18 @@ def test_library_exports_measurement_symbols():
         assert hasattr(lib, name), name
 
 
+def test_fortran_module_binds_every_entry_point():
+    """hit3d_amd/fortran/hit3d_hip.f90 has a bind(C) interface for every
+    entry point of both headers."""
+    f90 = open(os.path.join(ROOT, "hit3d_amd", "fortran", "hit3d_hip.f90")).read()
+    bound = set(re.findall(r'name="(hit3d_\w+)"', f90))
+    declared = set(_declared())
+    src = open(os.path.join(ROOT, "include", "hit3d_measure.h")).read()
+    declared |= set(re.findall(r"\bint\s+(hit3d_\w+)\s*\(", src))
+    assert sorted(declared - bound) == []
+    assert sorted(bound - declared) == []
+
+
 def test_uninitialised_call_fails_loudly():
     with pytest.raises(hit3d_amd.Hit3dError, match="hit3d_init"):
         hl.load()
