@@ -376,14 +376,38 @@ def read_particles_binary(path: str, n: int):
     return rec["ipart"].astype(np.int32), _to_grid_units(rec["xyz"].astype(np.float64), n), t
 
 
+def write_closure_file(path: str, params: np.ndarray, norm):
+    """The learned closure for les_model 11 as the Fortran driver reads it
+    (<run_name>.sgs): 6288 float32 parameters, then float64 xmean(10),
+    xstd(10), ymean(6), ystd(6)."""
+    xm, xs, ym, ys = (np.asarray(v, "<f8") for v in norm)
+    with open(path, "wb") as f:
+        f.write(np.asarray(params, "<f4").tobytes())
+        for v in (xm, xs, ym, ys):
+            f.write(v.tobytes())
+
+
+def read_closure_file(path: str):
+    """Returns (params[6288] float32, (xmean, xstd, ymean, ystd))."""
+    raw = open(path, "rb").read()
+    if len(raw) != 6288 * 4 + 32 * 8:
+        raise ValueError(f"{path}: not a closure file ({len(raw)} bytes)")
+    params = np.frombuffer(raw[:6288 * 4], "<f4").copy()
+    d = np.frombuffer(raw[6288 * 4:], "<f8")
+    return params, (d[0:10].copy(), d[10:20].copy(), d[20:26].copy(), d[26:32].copy())
+
+
 class Run:
     """A run of the driver loop (main.f90:56-288) on the GPU library."""
 
-    def __init__(self, p: Params, precision: int = 0):
+    def __init__(self, p: Params, precision: int = 0, closure: str | None = None):
         if p.nx != p.ny or p.nx != p.nz_all:
             raise ValueError("only cubic grids are supported")
-        if p.les_model not in (0, 1):
-            raise ValueError("only les_model 0 (none) and 1 (Smagorinsky) are built")
+        if p.les_model not in (0, 1, 11):
+            raise ValueError("only les_model 0 (none), 1 (Smagorinsky) and 11 (the learned "
+                             "closure) are built")
+        if p.les_model == 11 and closure is None:
+            raise ValueError("les_model 11 needs the closure file (<run_name>.sgs)")
         if p.nptot > 0 and p.particles_filter_size > 0.0:
             raise ValueError("particles on a locally averaged velocity are not built")
         self.p = p
@@ -395,6 +419,11 @@ class Run:
             self.h.set_reaction(p.reac_sc if len(p.reac_sc) else [0.0] * p.n_scalars)
         if p.les_model == 1:
             self.h.set_les(1, 0.18)   # m_les.f90:1148
+        if p.les_model == 11:
+            params, norm = read_closure_file(closure)
+            self.h.mlp_set_params(params)
+            self.h.mlp_set_norm(*norm)
+            self.h.set_les(11, 0.18)
         self.ipart = None
         self.int_particles = False
         self.time = 0.0
@@ -510,6 +539,53 @@ class Run:
             if p.n_scalars > 0 and not self.int_scalars and self.time > p.tscalar:
                 self.int_scalars = True  # main.f90:199-204
                 self.h.init_scalars(p.rn2, p.peak_wavenum_sc)
+
+    def run(self, run_name: str, directory: str = "."):
+        """The whole driver loop with its output, as main.f90:56-297 (and
+        hit3d_gpu.f90) orders it: start (begin_new with its io_write_4, or
+        begin_restart), particles, then per step the particle move, the
+        hydro step, the restart at IPRINT2 (before my_dt), my_dt,
+        statistics at IPRINT1, snapshots and the particle file at IWRITE4,
+        the TMAX and TSCALAR checks; a final restart file."""
+        p = self.p
+        if p.itmin == 0:
+            self.begin_new()
+            self.itime = 0
+            self.write_snapshot(directory)
+        else:
+            self.begin_restart(run_name, directory)
+        self.divergence()
+        if p.nptot > 0:
+            self.init_particles(run_name, directory)
+        itime = p.itmin
+        for itime in range(p.itmin + 1, p.itmax + 1):
+            self.move_particles()
+            self.itime = itime
+            self.courant, self.energy, self.fov, self.fos = self.h.step(
+                self.dt, self.fov, self.fos, int_scalars=self.int_scalars,
+                precision=self.precision)
+            self.time += self.dt
+            if itime % p.iprint2 == 0:
+                self.restart_write(run_name, directory)
+            if p.variable_dt:
+                self.dt = my_dt(self.dt, self.courant, self.int_scalars, p.sc)
+            if itime % p.iprint1 == 0:
+                write_stats(self.stats(), directory)
+                if self.int_scalars:
+                    write_scalar_stats(self.scalar_stats(), itime, self.time, directory)
+            if itime % p.iwrite4 == 0:
+                self.write_snapshot(directory)
+                if self.int_particles:
+                    self.write_particles(directory)
+            if itime % p.iprint1 == 0 or itime % p.iwrite4 == 0:
+                if self.time > p.tmax:
+                    break
+                if p.n_scalars > 0 and not self.int_scalars and self.time > p.tscalar:
+                    self.int_scalars = True  # main.f90:199-204
+                    self.h.init_scalars(p.rn2, p.peak_wavenum_sc)
+        self.itime = itime
+        if p.itmax > p.itmin and itime % p.iprint2 != 0:
+            self.restart_write(run_name, directory)
 
     def scalar_stats(self) -> dict:
         """stat_scalars (m_stats.f90:295-366); the reference runs it only once

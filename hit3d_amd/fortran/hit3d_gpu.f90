@@ -2,13 +2,29 @@
 ! hit3d_gpu -- Fortran driver of the GPU time step.
 !
 ! Reads <run_name>.in (the reference's positional input file,
-! m_parameters.f90:172-451), creates the seeded initial field
-! (begin_new.f90 / init_velocity.f90), and runs the driver loop of
-! main.f90:87-297 with the hydro step on the GPU (hit3d_step: rhs_scalars,
-! rhs_velocity, force_velocity, AB2 update, pressure).  Statistics every
-! IPRINT1 steps go to stat1.gp, stat2.gp and es.gp in the reference's formats
-! (m_stats.f90:152-185); the final restart file <run_name>.64.NNNNNN has the
-! reference's layout (restart_io.f90:287-314).
+! m_parameters.f90:172-451) and runs the driver loop of main.f90:56-297 with
+! the hydro step on the GPU (hit3d_step: rhs_scalars, rhs_velocity,
+! force_velocity, AB2 update, pressure):
+!   * start: the seeded initial field (begin_new.f90, init_velocity.f90,
+!     init_scalars.f90) and its io_write_4 snapshot, or, for ITMIN > 0, the
+!     restart file <run_name>.64.<ITMIN> (restart_read_parallel,
+!     restart_io.f90:341-506; missing scalars from init_scalar; dt halved for
+!     variable dt, begin_restart.f90:33-38); then divergence and dealias_all
+!     (main.f90:62-76);
+!   * LES: les_model 1 (Smagorinsky, m_les.f90:1142-1215) and 11, the learned
+!     closure, whose parameters come from <run_name>.sgs (6288 float32
+!     parameters, then float64 xmean(10), xstd(10), ymean(6), ystd(6));
+!   * particles (nptot > 0): positions from pt.<ITMIN>, else <run_name>.pt,
+!     else the lattice (m_particles.f90:157-337); moved every step once
+!     TIME >= starttime_particles with the velocity the step starts from
+!     (fields_to_parts + particles_move, main.f90:235-247), and written to
+!     pt.<ITIME> every IWRITE4 steps (particles_restart_write_binary,
+!     m_particles.f90:558-648);
+!   * output: stat1.gp, stat2.gp, es.gp (and scNN.gp, es_NN.gp) every
+!     IPRINT1 steps (m_stats.f90:152-185, :371-437); u, v, w, scNN and, with
+!     Smagorinsky, nu_t snapshots every IWRITE4 steps (io_write_4.f90:1-89,
+!     write_tmp4.f90:14-30); restart files every IPRINT2 steps and at the end
+!     (restart_io.f90:287-314).
 !
 ! Usage:  hit3d_gpu <run_name> [precision]     (precision 0 = FP64, 1 = mixed)
 ! Build with -fdefault-integer-8 -fdefault-real-8, like the reference.
@@ -25,7 +41,11 @@ program hit3d_gpu
   integer :: nptot, ptrack, les_model, n_scalars, n, kmax, itime, precision, ios
   real*8 :: tmax, trescale, tscalar, re, dt, peak, famp, rn1, rn2, rn3, tp, pfs
   real*8 :: nu, time, courant, energy, enstrophy
-  logical :: variable_dt, int_scalars
+  logical :: variable_dt, int_scalars, int_particles, there
+  integer :: np_run, nums_read
+  integer*4, allocatable :: ipart(:)
+  real*8, allocatable, target :: xyzp(:,:)
+  real*8, parameter :: two_pi = 8.d0 * atan(1.d0)
   integer, allocatable :: scalar_type(:)
   real*8, allocatable :: sc(:), pe(:), e_spec(:), ir_sc(:), peak_sc(:), reac_sc(:)
   real*8 :: dummy(3)
@@ -102,8 +122,10 @@ program hit3d_gpu
      end do
   end if
   close(10)
-  if (les_model > 0 .or. nptot > 0) stop 'LES and particles are not part of this driver'
-  if (itmin /= 0) stop 'restart input is not part of this driver'
+  if (les_model /= 0 .and. les_model /= 1 .and. les_model /= 11) &
+       stop 'les_model: 0, 1 (Smagorinsky) and 11 (learned closure) are built'
+  if (les_model > 0 .and. dealias /= 0) stop 'LES needs dealias = 0 (m_parameters.f90:415-420)'
+  if (nptot > 0 .and. pfs > 0.d0) stop 'particles on a locally averaged velocity are not built'
 
   if (dealias == 0) then
      kmax = nx / 3
@@ -117,30 +139,52 @@ program hit3d_gpu
        int(n_scalars, c_int64_t), 0_c_int64_t, int(dealias, c_int64_t)), 'init')
   call h3d_check(c_set_physics(nu, int(flow_type, c_int64_t), int(kfmax, c_int64_t), famp, &
        pe, int(scalar_type, c_int64_t)), 'set_physics')
-  call h3d_check(c_init_velocity(rn1, int(isp_type, c_int64_t), int(ir_exp, c_int64_t), peak), &
-       'init_velocity')
-  if (n_scalars > 0) then                    ! begin_new.f90:14,24
-     call h3d_check(c_set_reaction(reac_sc), 'set_reaction')
-     call h3d_check(c_init_scalars(rn2, peak_sc), 'init_scalars')
+  if (n_scalars > 0) call h3d_check(c_set_reaction(reac_sc), 'set_reaction')
+  if (les_model == 1) call h3d_check(c_set_les(1_c_int64_t, 0.18d0), 'set_les')   ! m_les.f90:1148
+  if (les_model == 11) call closure_in
+  if (itmin == 0) then                       ! begin_new.f90:10-31
+     itime = 0
+     time = 0.d0
+     int_scalars = (tscalar <= 0.d0 .and. n_scalars > 0)
+     call h3d_check(c_init_velocity(rn1, int(isp_type, c_int64_t), int(ir_exp, c_int64_t), &
+          peak), 'init_velocity')
+     if (n_scalars > 0) call h3d_check(c_init_scalars(rn2, peak_sc), 'init_scalars')
+     call io_write_4
+  else                                       ! begin_restart.f90:1-90
+     itime = itmin
+     call restart_in
+     if (variable_dt) dt = 0.5d0 * dt
+     int_scalars = (n_scalars > 0 .and. time > tscalar)
   end if
-  call dealias_all
   call divergence                            ! main.f90:66
-  time = 0.d0
   fov = 1; fos = 1
-  int_scalars = (tscalar <= 0.d0 .and. n_scalars > 0)
+  call dealias_all                           ! main.f90:76
   prec = precision
+  int_particles = .false.
+  if (nptot > 0) call particles_in
 
   ! ---------------------------------------------------------- main loop
   do itime = itmin + 1, itmax
+     ! parts task (main.f90:235-247): the velocity this step starts from
+     if (nptot > 0 .and. time >= tp) then
+        int_particles = .true.
+        call h3d_check(c_fields_to_parts(), 'fields_to_parts')
+        call h3d_check(c_particles_move(dt, int(ptrack, c_int64_t)), 'particles_move')
+     end if
      isc = 0
      if (int_scalars) isc = 1
      call h3d_check(c_step(dt, fov, fos, isc, prec, courant, energy), 'step')
      time = time + dt
+     if (mod(itime, iprint2) == 0) call restart_out           ! main.f90:173
      if (variable_dt) call my_dt_host
      if (mod(itime, iprint1) == 0) then
         write(*, "('ITIME=',i6,3x,'TIME=',f8.4,4x,'DT=',f8.5,3x,'Courn= ',f6.4)") &
              itime, time, dt, courant
         call stats_out
+     end if
+     if (mod(itime, iwrite4) == 0) then
+        call io_write_4
+        if (int_particles) call particles_out
      end if
      if (mod(itime, iprint1) == 0 .or. mod(itime, iwrite4) == 0) then
         if (time > tmax) exit
@@ -152,10 +196,164 @@ program hit3d_gpu
   end do
 
   if (itime > itmax) itime = itmax
-  call restart_out
+  if (itmax > itmin .and. mod(itime, iprint2) /= 0) call restart_out   ! main.f90:288-296
   call h3d_check(c_finalize(), 'finalize')
 
 contains
+
+  function ext6(it) result(e)
+    integer, intent(in) :: it
+    character(len=6) :: e
+    write(e, '(i6.6)') it
+  end function ext6
+
+  subroutine restart_in                      ! restart_io.f90:341-506 layout
+    real*8, allocatable :: slot(:,:,:)
+    integer*4 :: hdr(5)
+    integer :: u, c
+    inquire(file=run_name//'.64.'//ext6(itmin), exist=there)
+    if (.not. there) stop 'restart file <run_name>.64.<ITMIN> not found'
+    open(newunit=u, file=run_name//'.64.'//ext6(itmin), access='stream', form='unformatted', &
+         status='old')
+    read(u) hdr, time, dt
+    if (hdr(1) /= nx .or. hdr(2) /= ny .or. hdr(3) /= nz_all) &
+         stop '*** error: Dimensions are different'
+    nums_read = min(n_scalars, int(hdr(4)))
+    allocate(slot(nx+2, ny, nz_all))
+    slot = 0.d0
+    do c = 1, 3 + nums_read
+       read(u) slot(1:nx, 1:ny, 1:nz_all)
+       call h3d_check(c_put(H3D_FIELDS, int(c, c_int64_t), slot), 'put')
+    end do
+    close(u)
+    deallocate(slot)
+    do c = nums_read + 1, n_scalars          ! restart_io.f90:420-431
+       call h3d_check(c_init_scalar(int(c, c_int64_t), rn2, peak_sc(c)), 'init_scalar')
+    end do
+  end subroutine restart_in
+
+  subroutine closure_in                      ! les_model 11: <run_name>.sgs
+    real(c_float) :: params(6288)
+    real*8 :: xm(10), xs(10), ym(6), ys(6)
+    integer :: u
+    inquire(file=run_name//'.sgs', exist=there)
+    if (.not. there) stop 'les_model 11 needs the closure file <run_name>.sgs'
+    open(newunit=u, file=run_name//'.sgs', access='stream', form='unformatted', status='old')
+    read(u) params, xm, xs, ym, ys
+    close(u)
+    call h3d_check(c_mlp_set_params(params), 'mlp_set_params')
+    call h3d_check(c_mlp_set_norm(xm, xs, ym, ys), 'mlp_set_norm')
+    call h3d_check(c_set_les(11_c_int64_t, 0.18d0), 'set_les')
+  end subroutine closure_in
+
+  subroutine particles_in                    ! m_particles.f90:157-337 on one rank
+    integer :: u, i, j, k, m, npthird, nrec
+    real*8 :: t_file, sctmp
+    integer*4 :: n4, id4
+    inquire(file='pt.'//ext6(itmin), exist=there)
+    if (there) then                          ! particles_restart_read_binary, :655-744
+       open(newunit=u, file='pt.'//ext6(itmin), access='stream', form='unformatted', &
+            status='old')
+       read(u) n4
+       np_run = n4
+       allocate(ipart(np_run), xyzp(3, np_run))
+       do i = 1, np_run
+          read(u) id4, xyzp(:, i)
+          ipart(i) = id4
+       end do
+       read(u, iostat=nrec) t_file
+       close(u)
+    else
+       inquire(file=run_name//'.pt', exist=there)
+       if (there) then                       ! text positions, physical units
+          np_run = nptot
+          allocate(ipart(np_run), xyzp(3, np_run))
+          open(newunit=u, file=run_name//'.pt', form='formatted', status='old')
+          read(u, *) xyzp
+          close(u)
+       else                                  ! the lattice of the nearest cube, :222-262
+          npthird = int(real(nptot, 8)**(1.d0 / 3.d0))
+          if (nptot - npthird**3 >= (npthird + 1)**3 - nptot) npthird = npthird + 1
+          np_run = min(npthird**3, 2**30)
+          allocate(ipart(np_run), xyzp(3, np_run))
+          sctmp = two_pi / real(npthird, 8)
+          m = 0
+          do k = 0, npthird - 1
+             do j = 0, npthird - 1
+                do i = 0, npthird - 1
+                   m = m + 1
+                   xyzp(1, m) = (real(i, 8) + 0.5d0) * sctmp
+                   xyzp(2, m) = (real(j, 8) + 0.5d0) * sctmp
+                   xyzp(3, m) = (real(k, 8) + 0.5d0) * sctmp
+                end do
+             end do
+          end do
+       end if
+       do i = 1, np_run
+          ipart(i) = i
+       end do
+    end if
+    ! wrapped into [0, 2 pi), then grid units x / dx + 1 (:300-324)
+    do i = 1, np_run
+       do j = 1, 3
+          if (xyzp(j, i) >= two_pi .or. xyzp(j, i) < 0.d0) &
+               xyzp(j, i) = xyzp(j, i) - two_pi * floor(xyzp(j, i) / two_pi)
+          xyzp(j, i) = xyzp(j, i) / (two_pi / real(nx, 8)) + 1.d0
+       end do
+    end do
+    call h3d_check(c_particles_set(int(np_run, c_int64_t), xyzp, c_null_ptr), 'particles_set')
+  end subroutine particles_in
+
+  subroutine particles_out                   ! particles_restart_write_binary, :558-648
+    integer :: u, i
+    real*8 :: dx
+    call h3d_check(c_particles_get(c_loc(xyzp), c_null_ptr), 'particles_get')
+    dx = two_pi / real(nx, 8)
+    open(newunit=u, file='pt.'//ext6(itime), access='stream', form='unformatted', &
+         status='replace')
+    write(u) int(np_run, 4)
+    do i = 1, np_run
+       write(u) int(ipart(i), 4), (xyzp(:, i) - 1.d0) * dx
+    end do
+    write(u) time
+    close(u)
+  end subroutine particles_out
+
+  subroutine io_write_4                      ! io_write_4.f90:1-89
+    real(c_float), allocatable :: tmp4(:,:,:)
+    integer :: c
+    character(len=16) :: fname
+    allocate(tmp4(nx, ny, nz_all))
+    do c = 1, 3
+       call h3d_check(c_snapshot(int(c, c_int64_t), tmp4), 'snapshot')
+       fname = 'u.'//ext6(itime)
+       if (c == 2) fname = 'v.'//ext6(itime)
+       if (c == 3) fname = 'w.'//ext6(itime)
+       call write_tmp4(fname, tmp4)
+    end do
+    if (int_scalars) then
+       do c = 1, n_scalars
+          call h3d_check(c_snapshot(int(3 + c, c_int64_t), tmp4), 'snapshot')
+          write(fname, "('sc',i2.2,'.',a6)") c, ext6(itime)
+          call write_tmp4(fname, tmp4)
+       end do
+    end if
+    if (les_model == 1) then
+       call h3d_check(c_turb_visc(tmp4), 'turb_visc')
+       call write_tmp4('nu_t.'//ext6(itime), tmp4)
+    end if
+    deallocate(tmp4)
+  end subroutine io_write_4
+
+  subroutine write_tmp4(fname, a)            ! write_tmp4.f90:14-30
+    character(len=*), intent(in) :: fname
+    real(c_float), intent(in) :: a(:,:,:)
+    integer :: u
+    open(newunit=u, file=trim(fname), access='stream', form='unformatted', status='replace')
+    write(u) int(nx, 4), int(ny, 4), int(nz_all, 4)
+    write(u) a
+    close(u)
+  end subroutine write_tmp4
 
   subroutine my_dt_host                      ! my_dt.f90:8-30
     real*8, parameter :: courant_max = 0.1d0
