@@ -7,7 +7,11 @@ RHS, forcing, AB2 update, projection and the dt controller, with all state
 resident in HBM.  Both precision forms run from the same initial field: FP64
 throughout, and mixed (FP32 transforms and nonlinear term, FP64 state,
 update and reductions); the E(k) difference between them after the timed
-steps is reported.
+steps is reported.  The nonlinear term is formed (--form) as the divergence
+of the trace-free product tensor by default: five forward transforms instead
+of the reference's six, and the same fields to rounding, since the two
+differ by a gradient the projection removes; the reference's own form and
+the rotational form are timed beside it.
 
 Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
 """
@@ -27,6 +31,10 @@ if ROOT not in sys.path:
 import numpy as np  # noqa: E402
 
 PEAK_HBM_GBS = 8000.0
+FORM_NAMES = {0: "divergence -i k_j (u_i u_j)^ (the reference's)",
+              1: "rotational (u x omega)^",
+              2: "divergence of the trace-free tensor u_i u_j - delta_ij |u|^2/3 (differs "
+                 "from the reference's by a gradient the projection removes)"}
 METRIC = "s/step at 512^3 FP64 vs mixed (E(k) err stated) + % of HBM roofline"
 
 
@@ -261,10 +269,12 @@ def run_form(n, precision, steps, warmup, form=0):
 # algorithmic HBM bytes per launch of each timed kernel class (FP64 slot S):
 # every axis pass reads and writes one slot's valid entries; the fused step
 # kernels are charged the slots they must read and write.
-def kernel_bytes(n, cls, precision):
+def kernel_bytes(n, cls, precision, form=0):
     """Bytes each kernel must move: the unfused axis passes read and write one
     slot's valid entries; the fused step kernels move only the 2/3-rule cube
-    (kmax+1 kx columns, 2kmax+1 ky/kz rows) they read or produce."""
+    (kmax+1 kx columns, 2kmax+1 ky/kz rows) they read or produce.  form: the
+    nonlinear term's form (0 divergence, six product buffers; 1 rotational,
+    six inverse and three forward buffers; 2 deviatoric, five products)."""
     s64 = slot_bytes(n)
     k = n // 3
     nk, nky = k + 1, 2 * k + 1
@@ -273,18 +283,20 @@ def kernel_bytes(n, cls, precision):
     cube_ro = nk * nky * nky * w             # rhs_old inside the cube (FP32 in the mixed form)
     zy = n * nky * nk * w                     # one work buffer with kept ky rows, all z
     full = n * n * nk * w                     # one work buffer, all y and z
+    ni = 6 if form == 1 else 3                # buffers through the inverse passes
+    nf = {0: 6, 1: 3, 2: 5}[form]             # buffers through the forward passes
     table = {
         "fft_x": 2 * s64, "fft_y": 2 * s64, "fft_z": 2 * s64,
-        "step_inv_z": 3 * cube + 3 * zy,
-        "step_inv_y": 3 * zy + 3 * full,
-        "step_x": 3 * full + 6 * full,
-        "step_fwd_y": 6 * full + 6 * zy,
-        "step_fwd_z": 6 * zy + 6 * cube + 6 * cube_ro,   # products in; fields, rhs_old in/out
+        "step_inv_z": 3 * cube + ni * zy,
+        "step_inv_y": ni * zy + ni * full,
+        "step_x": ni * full + nf * full,
+        "step_fwd_y": nf * full + nf * zy,
+        "step_fwd_z": nf * zy + 6 * cube + 6 * cube_ro,   # products in; fields, rhs_old in/out
     }
     return table.get(cls)
 
 
-def accuracy_run(n, steps, every):
+def accuracy_run(n, steps, every, form=0):
     """Untimed: FP64 and mixed from the same field for `steps` steps, E(k)
     sampled every `every` steps and averaged; skewness, energy and
     dissipation at the end.  Returns the errors of the mixed form against
@@ -293,6 +305,8 @@ def accuracy_run(n, steps, every):
     res = {}
     for prec in (0, 1):
         r = Run(params(n), precision=prec)
+        if form:
+            r.h.set_nonlinear(form)
         r.begin_new()
         eks = []
         for i in range(steps):
@@ -311,7 +325,7 @@ def accuracy_run(n, steps, every):
             "skewness_rel_err_vs_fp64": abs(res[1][2] - res[0][2]) / abs(res[0][2])}
 
 
-def copy_bandwidth(n, reps=10):
+def copy_bandwidth(n, reps=20):
     """Two copies of one slot (wrk(2) = wrk(1), read + write of every byte):
     the runtime's copy kernel (hipMemcpyAsync device to device), which also
     scales the FETCH_SIZE / WRITE_SIZE counters, and the library's own
@@ -330,7 +344,7 @@ def copy_bandwidth(n, reps=10):
         h.synchronize()
         t = (time.perf_counter() - t0) / reps
         best = None
-        for blocks in (1024, 2048, 4096, 8192):
+        for blocks in (512, 1024, 2048, 4096, 8192, 16384, 65536):
             ms, byt = measure.stream_copy(h, reps, blocks)
             gbs = byt / (ms * 1e-3) / 1e9
             if best is None or gbs > best[0]:
@@ -409,8 +423,12 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--form", type=int, default=2, choices=[0, 1, 2],
+                    help="nonlinear term of the timed step: 0 divergence (the reference's "
+                         "products), 1 rotational, 2 divergence of the trace-free tensor "
+                         "(default; five forward transforms, the same fields to rounding)")
     ap.add_argument("--no-rotational", action="store_true",
-                    help="skip the timing of the rotational (u x omega) form")
+                    help="skip the timing of the rotational and deviatoric forms")
     ap.add_argument("--workload", default="c3", choices=["c3", "c4", "fft"] + sorted(WORKLOADS),
                     help="c3 (default) prints the benchmark line; c1, c2, c4, c5 time the other "
                          "configurations; fft is the transform microbenchmark")
@@ -432,11 +450,12 @@ def main():
         run_workload(a.workload, a.steps, a.warmup)
         return
     n = a.n
-    t64, tim64, st64, _ = run_form(n, 0, a.steps, a.warmup)
+    form = a.form
+    t64, tim64, st64, _ = run_form(n, 0, a.steps, a.warmup, form=form)
     mixed = None
     if not a.no_mixed:
         try:
-            t32, tim32, st32, _ = run_form(n, 1, a.steps, a.warmup)
+            t32, tim32, st32, _ = run_form(n, 1, a.steps, a.warmup, form=form)
             e64, e32 = st64.e_spec, st32.e_spec
             sig = e64 > 1e-12 * e64.max()
             mixed = {"s_per_step": t32, "speedup_vs_fp64": t64 / t32,
@@ -445,28 +464,34 @@ def main():
                      "eps_rel_err_vs_fp64": abs(st32.eps_v - st64.eps_v) / st64.eps_v,
                      "after_steps": a.steps + a.warmup}
             if a.accuracy_steps > 0:
-                mixed["accuracy"] = accuracy_run(n, a.accuracy_steps, 10)
+                mixed["accuracy"] = accuracy_run(n, a.accuracy_steps, 10, form)
         except Exception as e:  # reported, never silently replaced
             mixed = {"error": str(e)}
-    # the rotational (u x omega) form of the same step, FP64
-    rotational = None
+    # the other forms of the nonlinear term, FP64: rotational (u x omega)
+    # and the trace-free tensor's divergence; same fields to rounding
+    forms = {}
     if not a.no_rotational:
-        tr, timr, str_, _ = run_form(n, 0, a.steps, a.warmup, form=1)
-        rotational = {"s_per_step": tr, "vs_divergence_form": tr / t64,
-                      "energy_rel_diff_vs_divergence": abs(str_.energy - st64.energy) / st64.energy,
-                      "kernel_ms_per_step": {k: v[0] / a.steps for k, v in timr.items() if v[1]}}
+        for name, fo in (("divergence_fp64", 0), ("rotational_fp64", 1), ("deviatoric_fp64", 2)):
+            if fo == form:
+                continue
+            tr, timr, str_, _ = run_form(n, 0, a.steps, a.warmup, form=fo)
+            forms[name] = {"s_per_step": tr, "vs_headline_form": tr / t64,
+                           "energy_rel_diff_vs_headline_form":
+                               abs(str_.energy - st64.energy) / st64.energy,
+                           "kernel_ms_per_step": {k: v[0] / a.steps for k, v in timr.items()
+                                                  if v[1]}}
     copy = copy_bandwidth(n)
     copy_gbs = copy["stream_copy_GBs"]
     # dominant kernel class of the FP64 step
     best, best_ms = None, -1.0
     for cls, (ms, cnt) in tim64.items():
-        if cnt > 0 and ms > best_ms and kernel_bytes(n, cls, 0) is not None:
+        if cnt > 0 and ms > best_ms and kernel_bytes(n, cls, 0, form) is not None:
             best, best_ms = cls, ms
     roof = None
     if best is not None:
         ms, cnt = tim64[best]
         avg_s = ms / cnt / 1e3
-        byt = kernel_bytes(n, best, 0)
+        byt = kernel_bytes(n, best, 0, form)
         ach = byt / avg_s / 1e9
         roof = {"bound": "hbm", "kernel_class": best, "achieved": ach, "peak": PEAK_HBM_GBS,
                 "unit": "GB/s", "frac": ach / PEAK_HBM_GBS, "traffic": measured_traffic(n, best),
@@ -478,7 +503,7 @@ def main():
     # others are HBM-bound
     by_kernel = {}
     for cls, (ms, cnt) in tim64.items():
-        byt = kernel_bytes(n, cls, 0)
+        byt = kernel_bytes(n, cls, 0, form)
         if cnt > 0 and byt is not None:
             ach = byt / (ms / cnt / 1e3) / 1e9
             by_kernel[cls] = {"achieved_GBs": ach, "frac": ach / PEAK_HBM_GBS,
@@ -508,7 +533,7 @@ def main():
         "data": "synthetic: seeded RANDu field, spectrum k^4 exp(-2(k/3)^2), RN1=67748937",
         "config": {"workload": f"{n}^3 forced DNS, RE=800, 2/3-rule, KFMAX=2, FAMP=0.5, "
                                "variable DT, no scalars", "grid": n, "precision": "fp64",
-                   "parallelism": "single GPU"},
+                   "nonlinear_form": FORM_NAMES[form], "parallelism": "single GPU"},
         "ns_per_point_step": t64 / n ** 3 * 1e9,
         "step_alg_bytes": step_bytes,
         "step_hbm_roofline_frac": step_bytes / t64 / (PEAK_HBM_GBS * 1e9),
@@ -517,7 +542,7 @@ def main():
         "model_bytes_over_step_time_vs_peak": model_bytes / t64 / (PEAK_HBM_GBS * 1e9),
         "copy": copy,
         "mixed": mixed,
-        "rotational_fp64": rotational,
+        "nonlinear_forms": forms,
         "kernel_ms_per_step": {k: v[0] / a.steps for k, v in tim64.items() if v[1]},
         "roofline": roof,
         "roofline_by_kernel": by_kernel,

@@ -137,26 +137,18 @@ __global__ void k_advance(Geo g, double* f, const double* r, double* ro, double 
 __global__ void k_fill_zero(double* p, int64_t count) {
     GRID_STRIDE(i, count) p[i] = 0.0;
 }
-// Plain streaming copy of n16 16-byte words (nontemporal dwordx4 loads and
-// stores, four words in flight per thread): the HBM rate an ideal pass that
-// reads and writes each byte once reaches on this part, the yardstick the
-// step's kernels are set beside.
+// Plain streaming copy of n16 16-byte words (one nontemporal dwordx4 load
+// and store per thread and iteration, grid-stride): the HBM rate an ideal
+// pass that reads and writes each byte once reaches on this part, the
+// yardstick the step's kernels are set beside.  Unrolling four words per
+// thread measured slower (4.2-5.6 against 5.0-6.0 TB/s over 1024-65536
+// blocks on a 1.1 GB slot).
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 __global__ __launch_bounds__(256) void k_stream_copy(const u32x4* __restrict__ src,
                                                      u32x4* __restrict__ dst, int64_t n16) {
     const int64_t stride = (int64_t)gridDim.x * 256;
-    int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    for (; i + 3 * stride < n16; i += 4 * stride) {
-        const u32x4 a = __builtin_nontemporal_load(src + i);
-        const u32x4 b = __builtin_nontemporal_load(src + i + stride);
-        const u32x4 c = __builtin_nontemporal_load(src + i + 2 * stride);
-        const u32x4 d = __builtin_nontemporal_load(src + i + 3 * stride);
-        __builtin_nontemporal_store(a, dst + i);
-        __builtin_nontemporal_store(b, dst + i + stride);
-        __builtin_nontemporal_store(c, dst + i + 2 * stride);
-        __builtin_nontemporal_store(d, dst + i + 3 * stride);
-    }
-    for (; i < n16; i += stride) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n16; i += stride)
+        __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
 }
 
 __global__ void k_narrow(const double* __restrict__ src, float* __restrict__ dst, int64_t count) {

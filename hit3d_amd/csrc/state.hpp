@@ -72,7 +72,9 @@ struct Lib {
     std::vector<float*> W32;
 
     bool fuse = true;  // hit3d_step uses the fused pipeline where it applies
-    bool rot = false;  // rotational (u x omega) form of the fused step's nonlinear term
+    // form of the fused step's nonlinear term: 0 divergence (the reference's),
+    // 1 rotational (u x omega), 2 divergence of the trace-free tensor
+    int nlform = 0;
 
     // hit3d_step replayed from captured HIP graphs (graph mode).  A capture is
     // valid for the slot pointers it started from (the out-of-place passes

@@ -297,7 +297,11 @@ __device__ __forceinline__ cpx<T> herm_load(const cpx<T>* rowp, int t, int s) {
     return cpx<T>{re, im};
 }
 
-template <int N, typename T>
+// DEV (the deviatoric form): the products of the trace-free tensor
+// D_ij = u_i u_j - delta_ij |u|^2/3, five of them: (D11, D12), (D13, D22) and
+// D23 alone into buffers 0..4.  Its divergence differs from that of u_i u_j
+// by a gradient, which the projection removes.
+template <int N, typename T, bool DEV = false>
 __global__ __launch_bounds__((XP<N, T>::THREADS)) void k_xp(SG g, W6<T> Wb,
                                                           const cpx<double>* __restrict__ twd,
                                                           T norm, double* __restrict__ blockmax) {
@@ -314,19 +318,37 @@ __global__ __launch_bounds__((XP<N, T>::THREADS)) void k_xp(SG g, W6<T> Wb,
     __shared__ double redm[C::THREADS / 64 + 1];
     // With radix 8 throughout (N a power of 8) the five transforms of a row
     // share one set of twiddles, computed once per thread: FP32 x pass 1.02
-    // -> 0.85 ms, FP64 1.72 -> 1.68 ms at 512^3 (at 174 VGPRs, two
-    // wavefronts per SIMD, which beat three with spills); otherwise each
-    // transform derives its own
+    // -> 0.85 ms, FP64 1.72 -> 1.68 ms at 512^3; otherwise each transform
+    // derives its own (three wavefronts per SIMD forced by the register
+    // limit instead spill and double the time).
+    // FP64: the shared twiddles live in a per-block LDS table instead of
+    // registers (212 -> 158 VGPRs, three wavefronts per SIMD instead of two):
+    // 1.51 -> 1.48 ms at 512^3
     constexpr bool CACHE = E == 8 && N >= 64 && N == (1 << (3 * stage_index(N, 8)));
-    using TC = std::conditional_t<CACHE, TwCache<T, N, E>, void>;
-    TwCache<T, N, E> tcache;
-    const TC* tcp = nullptr;
+    constexpr bool LC = CACHE && sizeof(T) == 8;
+    using TC = std::conditional_t<CACHE, std::conditional_t<LC, TwLds<T, N, E>, TwCache<T, N, E>>,
+                                  void>;
+    __shared__ cpx<T> twl[LC ? TwLds<T, N, E>::SIZE : 1];
     const int t = threadIdx.x % TPF;
     const int rr = threadIdx.x / TPF;
     const int64_t row = (int64_t)blockIdx.x * C::ROWS + rr;
     const int partner = (TPF - t) & (TPF - 1);
     cpx<T>* L = lds[rr];
-    if constexpr (CACHE) {
+    TwCache<T, N, E> tcache;
+    TwLds<T, N, E> twv{twl, t};
+    const TC* tcp = nullptr;
+    if constexpr (LC) {
+        if (rr == 0) {
+            TwCache<T, N, E> tcv;
+            fill_twiddles<N, E>(tcv, t, twd + N);
+#pragma unroll
+            for (int si = 0; si < TwCache<T, N, E>::NS; ++si)
+#pragma unroll
+                for (int m = 1; m < E; ++m) twl[(si * E + m) * TPF + t] = tcv.w[si][m];
+        }
+        __syncthreads();
+        tcp = &twv;
+    } else if constexpr (CACHE) {
         fill_twiddles<N, E>(tcache, t, twd + N);
         tcp = &tcache;
     }
@@ -376,10 +398,20 @@ __global__ __launch_bounds__((XP<N, T>::THREADS)) void k_xp(SG g, W6<T> Wb,
         cpx<T> z[1][E];
 #pragma unroll
         for (int s = 0; s < E; ++s) {
-            if (pr == 0) z[0][s] = cpx<T>{u[s] * u[s], u[s] * v[s]};
-            if (pr == 1) z[0][s] = cpx<T>{u[s] * w[s], v[s] * v[s]};
-            if (pr == 2) z[0][s] = cpx<T>{v[s] * w[s], w[s] * w[s]};
+            if constexpr (DEV) {
+                const T t3 = ((u[s] * u[s] + v[s] * v[s]) + w[s] * w[s]) * (T)(1.0 / 3.0);
+                if (pr == 0) z[0][s] = cpx<T>{u[s] * u[s] - t3, u[s] * v[s]};
+                if (pr == 1) z[0][s] = cpx<T>{u[s] * w[s], v[s] * v[s] - t3};
+                if (pr == 2) z[0][s] = cpx<T>{v[s] * w[s], (T)0};
+            } else {
+                if (pr == 0) z[0][s] = cpx<T>{u[s] * u[s], u[s] * v[s]};
+                if (pr == 1) z[0][s] = cpx<T>{u[s] * w[s], v[s] * v[s]};
+                if (pr == 2) z[0][s] = cpx<T>{v[s] * w[s], w[s] * w[s]};
+            }
         }
+        // DEV: the last pair holds one real row, whose second output (zero)
+        // is not stored
+        const bool store_q = !(DEV && pr == 2);
         lds_sync<WAVE>();
         multi_fft<N, E, -1, T, 1, 1, FS, PS, WAVE, true, double, TC>(z, t, L, twd + N, tcp);
         cpx<T>* outp = reinterpret_cast<cpx<T>*>(Wb.p[2 * pr]) + row * g.PA;
@@ -395,7 +427,7 @@ __global__ __launch_bounds__((XP<N, T>::THREADS)) void k_xp(SG g, W6<T> Wb,
                     if (k < NK) {
                         const cpx<T> zk = z[0][s];
                         st_g<kStream<N>>(&outp[k], scale(zk + zm, (T)0.5));
-                        st_g<kStream<N>>(&outq[k], scale(mul_mi(zk - zm), (T)0.5));
+                        if (store_q) st_g<kStream<N>>(&outq[k], scale(mul_mi(zk - zm), (T)0.5));
                     }
                 }
             }
@@ -414,7 +446,7 @@ __global__ __launch_bounds__((XP<N, T>::THREADS)) void k_xp(SG g, W6<T> Wb,
                     const cpx<T> zm = conj(L[padi<PS>((N - k) & (N - 1))]);
                     const cpx<T> zk = z[0][s];
                     st_g<kStream<N>>(&outp[k], scale(zk + zm, (T)0.5));
-                    st_g<kStream<N>>(&outq[k], scale(mul_mi(zk - zm), (T)0.5));
+                    if (store_q) st_g<kStream<N>>(&outq[k], scale(mul_mi(zk - zm), (T)0.5));
                 }
             }
         }
@@ -455,16 +487,32 @@ __global__ __launch_bounds__((XP<N, T>::THREADS)) void k_xp_rot(SG g, W6<T> Wb,
     constexpr bool WAVE = TPF <= 64;
     __shared__ cpx<T> lds[C::ROWS][FS];
     __shared__ double redm[C::THREADS / 64 + 1];
+    // twiddles cached in registers, as in k_xp's FP32 form
     constexpr bool CACHE = E == 8 && N >= 64 && N == (1 << (3 * stage_index(N, 8)));
-    using TC = std::conditional_t<CACHE, TwCache<T, N, E>, void>;
-    TwCache<T, N, E> tcache;
-    const TC* tcp = nullptr;
+    constexpr bool LC = false;  // the LDS table made this pass slower (1.43 -> 1.66 ms)
+    using TC = std::conditional_t<CACHE, std::conditional_t<LC, TwLds<T, N, E>, TwCache<T, N, E>>,
+                                  void>;
+    __shared__ cpx<T> twl[LC ? TwLds<T, N, E>::SIZE : 1];
     const int t = threadIdx.x % TPF;
     const int rr = threadIdx.x / TPF;
     const int64_t row = (int64_t)blockIdx.x * C::ROWS + rr;
     const int partner = (TPF - t) & (TPF - 1);
     cpx<T>* L = lds[rr];
-    if constexpr (CACHE) {
+    TwCache<T, N, E> tcache;
+    TwLds<T, N, E> twv{twl, t};
+    const TC* tcp = nullptr;
+    if constexpr (LC) {
+        if (rr == 0) {
+            TwCache<T, N, E> tcv;
+            fill_twiddles<N, E>(tcv, t, twd + N);
+#pragma unroll
+            for (int si = 0; si < TwCache<T, N, E>::NS; ++si)
+#pragma unroll
+                for (int m = 1; m < E; ++m) twl[(si * E + m) * TPF + t] = tcv.w[si][m];
+        }
+        __syncthreads();
+        tcp = &twv;
+    } else if constexpr (CACHE) {
         fill_twiddles<N, E>(tcache, t, twd + N);
         tcp = &tcache;
     }
@@ -590,13 +638,15 @@ struct Upd {
     int kfmax, forced, first;
 };
 
-// ROT: three buffers hold (u x omega)_i, which is the nonlinear term itself
-// (r_i = (u x omega)_i^ - nu k^2 u_i); otherwise six hold the products
-// u_i u_j and r_i = -i k_j (u_i u_j)^ - nu k^2 u_i.
-template <int N, typename T, bool ROT = false>
+// FORM 0: six buffers hold the products u_i u_j, r_i = -i k_j (u_i u_j)^ -
+// nu k^2 u_i.  FORM 1 (rotational): three hold (u x omega)_i, the nonlinear
+// term itself, r_i = (u x omega)_i^ - nu k^2 u_i.  FORM 2 (deviatoric): five
+// hold D11, D12, D13, D22, D23 of the trace-free tensor, D33 = -(D11 + D22).
+template <int N, typename T, int FORM = 0>
 __global__ __launch_bounds__((CZ<N, T>::THREADS)) void k_fwd_z_update(
     SG g, W6<T> Q, Upd u, const cpx<double>* __restrict__ twd) {
-    constexpr int NP = ROT ? 3 : 6;
+    constexpr bool ROT = FORM == 1;
+    constexpr int NP = FORM == 1 ? 3 : FORM == 2 ? 5 : 6;
     constexpr int E = CZ<N, T>::E, B = CZ<N, T>::B, TPF = CZ<N, T>::TPF;
     constexpr int PS = 0;
     __shared__ cpx<T> lds[padded_size<PS>(N * B)];
@@ -677,6 +727,30 @@ __global__ __launch_bounds__((CZ<N, T>::THREADS)) void k_fwd_z_update(
                 continue;
             }
             const T kzT = (T)wavenum(t + s * TPF, N);
+            if constexpr (FORM == 2) {
+                // t_3 = kx D13 + ky D23 - kz (D11 + D22)
+                if (p == 0) {
+                    acc[0][s] = scale(v[s], kxT);
+                    acc[2][s] = scale(v[s], -kzT);
+                }
+                if (p == 1) {
+                    acc[0][s] = acc[0][s] + scale(v[s], kyT);
+                    acc[1][s] = scale(v[s], kxT);
+                }
+                if (p == 2) {
+                    acc[0][s] = acc[0][s] + scale(v[s], kzT);
+                    acc[2][s] = acc[2][s] + scale(v[s], kxT);
+                }
+                if (p == 3) {
+                    acc[1][s] = acc[1][s] + scale(v[s], kyT);
+                    acc[2][s] = acc[2][s] - scale(v[s], kzT);
+                }
+                if (p == 4) {
+                    acc[1][s] = acc[1][s] + scale(v[s], kzT);
+                    acc[2][s] = acc[2][s] + scale(v[s], kyT);
+                }
+                continue;
+            }
             // p: 0 uu, 1 uv, 2 uw, 3 vv, 4 vw, 5 ww
             if (p == 0) acc[0][s] = scale(v[s], kxT);
             if (p == 1) {
@@ -796,8 +870,10 @@ static void dispatch_step(int n, F&& f) {
     }
 }
 
-template <int N, typename T, bool ROT>
+template <int N, typename T, int FORM>
 static void run(Lib& L, const double* dtp, bool first, const cpx<T>* tw) {
+    constexpr bool ROT = FORM == 1;
+    constexpr int NFWD = FORM == 1 ? 3 : FORM == 2 ? 5 : 6;
     if (L.kmax != Kc<N>::K) throw std::runtime_error("the fused step expects kmax = n/3");
     SG g;
     g.K = L.kmax;
@@ -838,14 +914,14 @@ static void run(Lib& L, const double* dtp, bool first, const cpx<T>* tw) {
             hipLaunchKernelGGL((k_xp_rot<N, T>), dim3(nb), dim3(XP<N, T>::THREADS), 0, L.st, g,
                                Wb, L.tw64, norm, L.dmax);
         else
-            hipLaunchKernelGGL((k_xp<N, T>), dim3(nb), dim3(XP<N, T>::THREADS), 0, L.st, g, Wb,
-                               L.tw64, norm, L.dmax);
+            hipLaunchKernelGGL((k_xp<N, T, FORM == 2>), dim3(nb), dim3(XP<N, T>::THREADS), 0, L.st,
+                               g, Wb, L.tw64, norm, L.dmax);
         launch_max_fold(L.dmax, nb, L.dscal, L.st);
     }
     {
         Timed tm(TC_STEP_FWD_Y);
-        hipLaunchKernelGGL((k_y<N, -1, T>), dim3(ncb, N, ROT ? 3 : 6), dim3(CY<N, T>::THREADS), 0,
-                           L.st, g, Wb, tw);
+        hipLaunchKernelGGL((k_y<N, -1, T>), dim3(ncb, N, NFWD), dim3(CY<N, T>::THREADS), 0, L.st,
+                           g, Wb, tw);
     }
     {
         Timed tm(TC_STEP_FWD_Z);
@@ -863,8 +939,8 @@ static void run(Lib& L, const double* dtp, bool first, const cpx<T>* tw) {
         u.forced = forced ? 1 : 0;
         u.first = first ? 1 : 0;
         const int ncbz = (g.NK + CZ<N, T>::B - 1) / CZ<N, T>::B;
-        hipLaunchKernelGGL((k_fwd_z_update<N, T, ROT>), dim3(ncbz, nky), dim3(CZ<N, T>::THREADS),
-                           0, L.st, g, Wb, u, L.tw64);
+        hipLaunchKernelGGL((k_fwd_z_update<N, T, FORM>), dim3(ncbz, nky),
+                           dim3(CZ<N, T>::THREADS), 0, L.st, g, Wb, u, L.tw64);
     }
 }
 
@@ -879,13 +955,16 @@ void fused_step(const double* dtp, bool first, bool mixed) {
     Lib& L = lib();
     dispatch_step(L.n, [&](auto NC) {
         constexpr int N = decltype(NC)::value;
-        if (mixed) {
-            if (L.rot) run<N, float, true>(L, dtp, first, L.tw32);
-            else run<N, float, false>(L, dtp, first, L.tw32);
-        } else {
-            if (L.rot) run<N, double, true>(L, dtp, first, L.tw64);
-            else run<N, double, false>(L, dtp, first, L.tw64);
-        }
+        auto go = [&](auto TT) {
+            using T = decltype(TT);
+            const cpx<T>* tw = mixed ? reinterpret_cast<const cpx<T>*>(L.tw32)
+                                     : reinterpret_cast<const cpx<T>*>(L.tw64);
+            if (L.nlform == 1) run<N, T, 1>(L, dtp, first, tw);
+            else if (L.nlform == 2) run<N, T, 2>(L, dtp, first, tw);
+            else run<N, T, 0>(L, dtp, first, tw);
+        };
+        if (mixed) go(float{});
+        else go(double{});
     });
 }
 

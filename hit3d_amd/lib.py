@@ -278,8 +278,9 @@ class Hit3D:
         _check(self.lib.hit3d_set_les(_i(les_model), _d(c_smag)))
 
     def set_nonlinear(self, form: int):
-        """0: divergence form (the reference's); 1: rotational u x omega form
-        of the fused 2/3-rule step."""
+        """Nonlinear term of the fused 2/3-rule step: 0 the divergence form
+        (the reference's), 1 the rotational u x omega form, 2 the divergence
+        of the trace-free product tensor (five forward transforms)."""
         _check(self.lib.hit3d_set_nonlinear(_i(form)))
 
     def set_fusion(self, on: bool):

@@ -180,14 +180,16 @@ int hit3d_sgs_features(const double *delta, float *X);
 
 /* Form of the nonlinear term in hit3d_step's fused 2/3-rule pipeline:
  * 0 (default) the divergence form -i k_j (u_i u_j)^ of rhs_velocity.f90:56-102;
- * 1 the rotational form (u x omega)^, omega = i k x u-hat, an option with no
- * reference counterpart: it differs from the divergence form by the gradient
- * i k (|u|^2/2)^, which the projection after the update removes, so the
- * fields agree to rounding while rhs_old holds the rotational RHS.  Same
- * number of transforms per step (6 inverse pruned to the cube, 3 forward).
- * The phase shift rejects form 1; a step that cannot run fused (active
- * scalars, LES, fusion off) fails while form 1 is selected.  The unfused
- * entry points (hit3d_rhs_velocity) keep the divergence form. */
+ * 1 the rotational form (u x omega)^, omega = i k x u-hat (6 inverse and 3
+ * forward transforms); 2 the divergence form of the trace-free tensor
+ * u_i u_j - delta_ij |u|^2/3 (3 inverse and 5 forward transforms instead of
+ * 6).  Forms 1 and 2 have no reference counterpart: they differ from form 0
+ * by a gradient (i k (|u|^2/2)^, i k (|u|^2/3)^), which the projection
+ * after the update removes, so the fields agree to rounding while rhs_old
+ * holds the form's own RHS.  The phase shift rejects forms 1 and 2; a step
+ * that cannot run fused (active scalars, LES, fusion off) fails while one
+ * of them is selected.  The unfused entry points (hit3d_rhs_velocity) keep
+ * the divergence form. */
 int hit3d_set_nonlinear(const int64_t *form);
 
 /* Selects how hit3d_step runs: *on != 0 (default) the fused pipeline where

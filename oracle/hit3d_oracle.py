@@ -319,6 +319,31 @@ def _pad(p, n):
     return out
 
 
+def rhs_velocity_deviatoric(fields: np.ndarray, nu: float, n: int):
+    """The divergence form of rhs_velocity.f90:56-102 (2/3 rule) applied to
+    the trace-free product tensor D_ij = u_i u_j - delta_ij |u|^2/3, an
+    option of this build: five independent products instead of six, and
+    r-hat_i = -i k_j D_ij^ - nu k^2 u-hat_i on ialias == 0 (the mask of
+    :104-144), with D_33 = -(D_11 + D_22).  It differs from the divergence
+    form by the gradient i k (|u|^2/3)^, which the projection removes.
+    Returns (rhs[3] interleaved, physical velocity[3])."""
+    kx, ky, kz = _kgrid(n)
+    ia = ialias(n)[:, :, 0::2]
+    U = [to_complex(fields[c]) for c in range(3)]
+    phys = [fft_inverse(fields[c]) for c in range(3)]
+    u, v, w = (p[:, :, :n] for p in phys)
+    t3 = ((u * u + v * v) + w * w) / 3.0
+    D11, D12, D13, D22, D23 = (to_complex(fft_forward(_pad(d, n)))
+                               for d in (u * u - t3, u * v, u * w, v * v - t3, v * w))
+    D33 = -(D11 + D22)
+    lap = -nu * ((kx ** 2 + ky ** 2) + kz ** 2)
+    t = [-((kx * D11 + ky * D12) + kz * D13),
+         -((kx * D12 + ky * D22) + kz * D23),
+         -((kx * D13 + ky * D23) + kz * D33)]
+    rhs = [np.where(ia > 0, 0.0, 1j * t[c] + lap * U[c]) for c in range(3)]
+    return np.stack([to_interleaved(r) for r in rhs]), np.stack(phys)
+
+
 def rhs_velocity_rotational(fields: np.ndarray, nu: float, n: int):
     """The rotational form of the velocity RHS with the 2/3 rule, an option
     of this build replacing the divergence form of rhs_velocity.f90:56-102:
@@ -939,6 +964,7 @@ class Config:
     les_model: int = 0
     c_smag: float = 0.18
     rotational: bool = False   # u x omega form of the velocity RHS (2/3 rule only)
+    deviatoric: bool = False   # trace-free product tensor (2/3 rule only)
 
     @property
     def kmax(self):
@@ -977,10 +1003,11 @@ def step(cfg: Config, st: State) -> State:
         new, old = advance(fields[3:3 + ns], rs, rhs_old[3:3 + ns], st.dt, st.fos)
         fields[3:3 + ns] = new
         rhs_old[3:3 + ns] = old
-    if cfg.rotational:
+    if cfg.rotational or cfg.deviatoric:
         if cfg.dealias != 0:
-            raise ValueError("the rotational form is built for the 2/3 rule")
-        rhs, phys = rhs_velocity_rotational(fields, cfg.nu, n)
+            raise ValueError("the rotational and deviatoric forms are built for the 2/3 rule")
+        form = rhs_velocity_rotational if cfg.rotational else rhs_velocity_deviatoric
+        rhs, phys = form(fields, cfg.nu, n)
     else:
         rhs, phys = rhs_velocity(fields, cfg.nu, cfg.dealias, n)
     if tv is not None:

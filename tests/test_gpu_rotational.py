@@ -1,16 +1,17 @@
-"""The rotational form of the nonlinear term (u x omega, an option of the
-fused 2/3-rule step) through the C ABI against the oracle's rotational
-variant (oracle.rhs_velocity_rotational, pinned by a closed form and by the
-identity projected(u x omega) = projected(-d_j u_i u_j) for band-limited
-fields).
+"""Two options of the fused 2/3-rule step for the nonlinear term, through
+the C ABI against the oracle's variants: the rotational form u x omega
+(oracle.rhs_velocity_rotational) and the divergence of the trace-free
+product tensor (oracle.rhs_velocity_deviatoric), both pinned by closed
+forms and by the identity that their projection equals that of the
+divergence form for band-limited fields.
 
-* The first step's rhs_old holds the unprojected RHS, which tells the two
-  forms apart: equal to the oracle's rotational RHS plus forcing (1e-12),
-  and different from the divergence form's by a gradient.
-* A forced trajectory equals the oracle's rotational trajectory (1e-11),
-  and the divergence form's from the same field (the projection removes
-  the gradient part): fields within 1e-11, FP64; the mixed form within
-  1e-5 of the FP64 rotational step.
+* The first step's rhs_old holds the unprojected RHS, which tells the forms
+  apart: equal to the oracle's variant plus forcing (1e-12), and different
+  from the divergence form's by a gradient.
+* A forced trajectory equals the oracle's trajectory of the same form
+  (1e-11), and the divergence form's from the same field (the projection
+  removes the gradient part): fields within 1e-11 in FP64; each mixed form
+  within 1e-5 of its FP64 step.
 """
 import numpy as np
 import pytest
@@ -72,10 +73,43 @@ def test_rotational_rhs_and_trajectory_match_oracle(oracle, n):
         h.close()
 
 
+@pytest.mark.parametrize("n", [32, 64])
+def test_deviatoric_rhs_and_trajectory_match_oracle(oracle, n):
+    """Form 2 (the trace-free tensor): the first step's rhs_old equals the
+    oracle's deviatoric RHS plus forcing (1e-12), differs from the
+    divergence form's, and a forced trajectory equals the oracle's."""
+    note(f"deviatoric form n={n}")
+    cfg = oracle.Config(n=n, nu=1 / 400.0, dt=0.002, dealias=0, flow_type=1, kfmax=2,
+                        famp=0.5, deviatoric=True)
+    h = hit3d_amd.Hit3D(n)
+    try:
+        h.set_physics(cfg.nu, flow_type=1, kfmax=2, famp=0.5)
+        h.set_nonlinear(2)
+        f0 = seeded(h)
+        c, e, fov, _ = h.step(cfg.dt, True, True)
+        r, _ = oracle.rhs_velocity_deviatoric(f0, cfg.nu, n)
+        r, e_ref = oracle.force_velocity(f0, r, 2, 0.5, n)
+        got = np.stack([h.get(RHS_OLD, k) for k in (1, 2, 3)])
+        assert rel(got, r) < 1e-12
+        rd, _ = oracle.rhs_velocity(f0, cfg.nu, 0, n)
+        assert rel(got, oracle.force_velocity(f0, rd, 2, 0.5, n)[0]) > 1e-3
+        st = oracle.step(cfg, oracle.State(fields=f0, rhs_old=np.zeros_like(f0), dt=cfg.dt))
+        assert abs(c - st.courant) < 1e-12 * st.courant
+        for it in range(4):
+            c, e, fov, _ = h.step(cfg.dt, fov, True)
+            st = oracle.step(cfg, st)
+            assert abs(c - st.courant) < 1e-12 * st.courant, it
+            assert abs(e - st.energy) < 1e-12 * st.energy, it
+        got = np.stack([h.get(FIELDS, k) for k in (1, 2, 3)])
+        assert rel(got, st.fields) < 1e-11
+    finally:
+        h.close()
+
+
 def test_rotational_equals_divergence_and_mixed(oracle):
     """128^3: five fused steps of each form from one field (graph replay on)
-    give the same fields to 1e-11; the mixed rotational form is within 1e-5
-    of the FP64 rotational one."""
+    give the same fields to 1e-11; the mixed rotational and deviatoric
+    forms are within 1e-5 of their FP64 ones."""
     n = 128
     note("rotational vs divergence n=128")
     h = hit3d_amd.Hit3D(n, n_scalars=3)
@@ -86,7 +120,7 @@ def test_rotational_equals_divergence_and_mixed(oracle):
         for c in range(3):
             h.copy(FIELDS, 4 + c, FIELDS, 1 + c)   # keep the initial field on the device
         out = {}
-        for form, prec in ((0, 0), (1, 0), (1, 1)):
+        for form, prec in ((0, 0), (1, 0), (1, 1), (2, 0), (2, 1)):
             for c in range(3):
                 h.copy(FIELDS, 1 + c, FIELDS, 4 + c)
             h.set_nonlinear(form)
@@ -95,9 +129,11 @@ def test_rotational_equals_divergence_and_mixed(oracle):
                 _, _, fov, _ = h.step(1e-3, fov, True, precision=prec)
             out[(form, prec)] = np.stack([h.get(FIELDS, k) for k in (1, 2, 3)])
         assert rel(out[(1, 0)], out[(0, 0)]) < 1e-11
-        err = rel(out[(1, 1)], out[(1, 0)])
-        note(f"mixed rotational vs FP64 rotational after 5 steps: {err:.2e}")
-        assert err < 1e-5
+        assert rel(out[(2, 0)], out[(0, 0)]) < 1e-11
+        for form in (1, 2):
+            err = rel(out[(form, 1)], out[(form, 0)])
+            note(f"mixed form {form} vs FP64 after 5 steps: {err:.2e}")
+            assert err < 1e-5
     finally:
         h.close()
 
@@ -119,6 +155,6 @@ def test_rotational_form_limits():
         with pytest.raises(hit3d_amd.Hit3dError, match="rotational"):
             h.step(1e-3, True, True, int_scalars=True)
         with pytest.raises(hit3d_amd.Hit3dError, match="form"):
-            h.set_nonlinear(2)
+            h.set_nonlinear(3)
     finally:
         h.close()

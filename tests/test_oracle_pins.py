@@ -342,3 +342,51 @@ def test_rotational_step_equals_divergence_step(oracle):
     assert np.abs(b.fields - a.fields).max() < 1e-12 * np.abs(a.fields).max()
     assert abs(b.courant - a.courant) < 1e-12 * a.courant
     assert abs(b.energy - a.energy) < 1e-12 * a.energy
+
+
+# ------------------------------------------------------------ deviatoric form
+
+def test_deviatoric_rhs_shear_closed_form(oracle):
+    """u = (sin y, 0, 0): the divergence form's nonlinear term is zero, the
+    trace-free tensor's is grad(|u|^2/3) = (0, sin(2y)/3, 0).  Catches a
+    trace removed with the wrong factor or on the wrong components."""
+    n, nu = 16, 0.1
+    z = np.zeros((n, n, n + 2))
+    f = np.stack([oracle.fft_forward(_pad(_modes_field(n, [((0, 1, 0), 0.0, 1.0)]), n)), z, z])
+    rhs, _ = oracle.rhs_velocity_deviatoric(f, nu, n)
+    got = [oracle.fft_inverse(r)[:, :, :n] for r in rhs]
+    ex = [_modes_field(n, [((0, 1, 0), 0.0, -nu)]),
+          _modes_field(n, [((0, 2, 0), 0.0, 1.0 / 3.0)]),
+          np.zeros((n, n, n))]
+    for g, e in zip(got, ex):
+        assert np.abs(g - e).max() < 1e-13
+
+
+def test_deviatoric_rhs_projects_to_divergence_form(oracle):
+    """Band-limited fields: projected deviatoric RHS = projected divergence
+    RHS to rounding, the difference is a gradient, and driver steps with
+    the two forms give the same fields."""
+    n, nu = 32, 0.02
+    rng = np.random.default_rng(6)
+    modes = [_random_modes(rng, [0, 1, 2, 3, -2], [0, 1, -1, 3], [0, 2, -1, -3], 6)
+             for _ in range(3)]
+    U = np.stack([oracle.fft_forward(_pad(_modes_field(n, md), n)) for md in modes])
+    U = oracle.dealias_all(oracle.pressure(U, n), n // 3, n)
+    r_div, _ = oracle.rhs_velocity(U, nu, 0, n)
+    r_dev, _ = oracle.rhs_velocity_deviatoric(U, nu, n)
+    pd, pv = oracle.pressure(r_div, n), oracle.pressure(r_dev, n)
+    assert np.abs(pv - pd).max() < 1e-12 * np.abs(pd).max()
+    d = [oracle.to_complex(r_dev[c] - r_div[c]) for c in range(3)]
+    assert np.abs(d[0]).max() > 1e-3 * np.abs(pd).max()
+    kx, ky, kz = oracle._kgrid(n)
+    curl = [ky * d[2] - kz * d[1], kz * d[0] - kx * d[2], kx * d[1] - ky * d[0]]
+    assert max(np.abs(c).max() for c in curl) < 1e-11 * np.abs(pd).max()
+    out = {}
+    for dev in (False, True):
+        cfg = oracle.Config(n=n, nu=0.01, dt=0.002, dealias=0, flow_type=1, kfmax=2, famp=0.5,
+                            deviatoric=dev)
+        st = oracle.State(fields=U.copy(), rhs_old=np.zeros_like(U), dt=cfg.dt)
+        for _ in range(3):
+            st = oracle.step(cfg, st)
+        out[dev] = st.fields
+    assert np.abs(out[True] - out[False]).max() < 1e-12 * np.abs(out[False]).max()
