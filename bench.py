@@ -360,20 +360,21 @@ def copy_bandwidth(n, reps=20):
             "stream_copy_bytes": best[3]}
 
 
-def measured_traffic(n, cls):
+def measured_traffic(n, cls, form=2):
     """HBM bytes per launch of a kernel class from the committed PMC summary
     (profiles/pmc_traffic.json: FETCH_SIZE + WRITE_SIZE per dispatch, scaled by
-    the same counters on a copy of known size), or None."""
+    the same counters on the streaming copy of known size), or None."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     try:
         with open(path) as f:
             d = json.load(f)
     except (OSError, ValueError):
         return None
-    if d.get("n") != n or cls not in d.get("classes", {}):
-        return None
-    c = d["classes"][cls]
-    return c["fetch_bytes"] + c["write_bytes"]
+    for run in d.get("runs", []):
+        if run.get("n") == n and run.get("form") == form and cls in run.get("classes", {}):
+            c = run["classes"][cls]
+            return c["fetch_bytes"] + c["write_bytes"]
+    return None
 
 
 def host_cpu():
@@ -494,7 +495,7 @@ def main():
         byt = kernel_bytes(n, best, 0, form)
         ach = byt / avg_s / 1e9
         roof = {"bound": "hbm", "kernel_class": best, "achieved": ach, "peak": PEAK_HBM_GBS,
-                "unit": "GB/s", "frac": ach / PEAK_HBM_GBS, "traffic": measured_traffic(n, best),
+                "unit": "GB/s", "frac": ach / PEAK_HBM_GBS, "traffic": measured_traffic(n, best, form),
                 "alg_bytes_per_launch": byt, "avg_launch_ms": avg_s * 1e3,
                 "launches_per_step": cnt / a.steps, "stream_copy_GBs": copy_gbs,
                 "frac_of_stream_copy": ach / copy_gbs}

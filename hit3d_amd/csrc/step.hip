@@ -93,6 +93,34 @@ struct Kc {
 };
 
 __device__ __forceinline__ bool kept(int j, int K, int n) { return j <= K || j >= n - K; }
+
+// Column blocks narrower than a 128-byte row segment split each segment
+// between two blocks.  Blocks are dispatched round-robin over the 8 XCDs,
+// each with its own L2, so two neighbouring blocks would each fetch the
+// whole segment from HBM (the 1024^3 z/update kernel moved 1.6x its
+// algorithmic bytes).  With PAIR the two halves go to blocks 8 apart, which
+// run on one XCD at about the same time, and the second finds the segment
+// in that L2.  The grid must have an even number of column blocks; returns
+// this block's logical (column block, row).
+template <typename T, int B>
+inline constexpr bool kPairHalves = B * (int)sizeof(cpx<T>) < 128;
+template <bool PAIR>
+__device__ __forceinline__ void block_coords(int& bx, int& by) {
+    if constexpr (!PAIR) {
+        bx = blockIdx.x;
+        by = blockIdx.y;
+    } else {
+        const int gx = gridDim.x, total = gx * gridDim.y;
+        const int l = blockIdx.x + gx * blockIdx.y;
+        int logical = l;
+        if (l < total - total % 16) {
+            const int grp = l / 16, r = l % 16;
+            logical = (grp * 8 + (r & 7)) * 2 + (r >> 3);
+        }
+        bx = logical % gx;
+        by = logical / gx;
+    }
+}
 // Whether any element t + s*TPF (0 <= t < TPF) of register slot s is kept;
 // constant once the slot loop is unrolled, so dead slots take no registers.
 template <int N, int TPF>
@@ -175,10 +203,12 @@ __global__ __launch_bounds__((CR<N, T>::THREADS)) void k_inv_z_rot(SG g, P3 F, W
     constexpr int E = CR<N, T>::E, B = CR<N, T>::B, TPF = CR<N, T>::TPF;
     constexpr int PS = 0;
     __shared__ cpx<T> lds[padded_size<PS>(N * B)];
+    int bx, by;
+    block_coords<kPairHalves<T, B>>(bx, by);
     const int c = threadIdx.x % B, t = threadIdx.x / B;
-    const int kx = blockIdx.x * B + c;
+    const int kx = bx * B + c;
     const bool valid = kx < Kc<N>::NK;
-    const int jy = kept_row(blockIdx.y, Kc<N>::K, N);
+    const int jy = kept_row(by, Kc<N>::K, N);
     const int kyv = wavenum(jy, N);
     // the velocity in the work precision (the mixed form's vorticity is
     // formed from the FP32-rounded velocity, in FP64, then rounded)
@@ -650,10 +680,12 @@ __global__ __launch_bounds__((CZ<N, T>::THREADS)) void k_fwd_z_update(
     constexpr int E = CZ<N, T>::E, B = CZ<N, T>::B, TPF = CZ<N, T>::TPF;
     constexpr int PS = 0;
     __shared__ cpx<T> lds[padded_size<PS>(N * B)];
+    int bx, by;
+    block_coords<kPairHalves<T, B>>(bx, by);
     const int c = threadIdx.x % B, t = threadIdx.x / B;
-    const int kx = blockIdx.x * B + c;
+    const int kx = bx * B + c;
     const bool valid = kx < Kc<N>::NK;
-    const int jy = kept_row(blockIdx.y, Kc<N>::K, N);
+    const int jy = kept_row(by, Kc<N>::K, N);
     const int ky = wavenum(jy, N);
     const T kxT = (T)kx, kyT = (T)ky;
     // FP64 with radix 8 throughout: the six transforms share the twiddles of
@@ -870,6 +902,14 @@ static void dispatch_step(int n, F&& f) {
     }
 }
 
+// column blocks of B columns covering nk, rounded up to an even count where
+// the kernel pairs half-segment blocks (block_coords)
+template <typename T, int B>
+static int col_blocks(int nk) {
+    const int c = (nk + B - 1) / B;
+    return kPairHalves<T, B> ? c + (c & 1) : c;
+}
+
 template <int N, typename T, int FORM>
 static void run(Lib& L, const double* dtp, bool first, const cpx<T>* tw) {
     constexpr bool ROT = FORM == 1;
@@ -894,7 +934,8 @@ static void run(Lib& L, const double* dtp, bool first, const cpx<T>* tw) {
     {
         Timed tm(TC_STEP_INV_Z);
         if constexpr (ROT)
-            hipLaunchKernelGGL((k_inv_z_rot<N, T>), dim3((g.NK + CR<N, T>::B - 1) / CR<N, T>::B, nky),
+            hipLaunchKernelGGL((k_inv_z_rot<N, T>),
+                               dim3(col_blocks<T, CR<N, T>::B>(g.NK), nky),
                                dim3(CR<N, T>::THREADS), 0, L.st, g, F, Wb, tw);
         else
             hipLaunchKernelGGL((k_inv_z<N, T>), dim3(ncb, nky, 3), dim3(CE<N, T>::THREADS), 0,
@@ -938,7 +979,7 @@ static void run(Lib& L, const double* dtp, bool first, const cpx<T>* tw) {
         u.kfmax = L.kfmax;
         u.forced = forced ? 1 : 0;
         u.first = first ? 1 : 0;
-        const int ncbz = (g.NK + CZ<N, T>::B - 1) / CZ<N, T>::B;
+        const int ncbz = col_blocks<T, CZ<N, T>::B>(g.NK);
         hipLaunchKernelGGL((k_fwd_z_update<N, T, FORM>), dim3(ncbz, nky),
                            dim3(CZ<N, T>::THREADS), 0, L.st, g, Wb, u, L.tw64);
     }

@@ -128,3 +128,27 @@ def test_mixed_single_rhs_1024(h):
     err = dmax / smax
     note(f"mixed single-RHS rel err at 1024^3: {err:.3e}")
     assert 1e-12 < err < 5e-5
+
+
+def test_deviatoric_step_1024(h):
+    """The deviatoric form at 1024^3 (its z/update kernel in paired
+    half-width column blocks) against the divergence form after two steps:
+    the same fields to 1e-12 (compared through the device statistics: E(k),
+    enstrophy and divergence)."""
+    note("deviatoric vs divergence step n=1024")
+    res = {}
+    try:
+        for form in (0, 2):
+            restore(h)
+            h.set_nonlinear(form)
+            fov = True
+            for _ in range(2):
+                cr, en, fov, _ = h.step(2.5e-4, fov, True)
+            e, ens = h.stats()
+            res[form] = (cr, en, e, ens, h.divergence())
+    finally:
+        h.set_nonlinear(0)
+    (c0, n0, e0, z0, _), (c2, n2, e2, z2, d2) = res[0], res[2]
+    assert abs(c2 - c0) <= 1e-13 * c0 and abs(n2 - n0) <= 1e-13 * n0
+    assert rel(e2, e0) < 1e-12 and abs(z2 - z0) <= 1e-12 * z0
+    assert max(abs(d2[0]), abs(d2[1])) < 1e-9

@@ -147,3 +147,32 @@ def test_mixed_single_rhs_512(h):
     err = rel(rhs[1], rhs[0])
     note(f"mixed single-RHS rel err at 512^3: {err:.3e}")
     assert 1e-12 < err < 1e-5
+
+
+def test_nonlinear_forms_512(h):
+    """The rotational and deviatoric forms of the fused step at 512^3 (their
+    kernels with the 512^3 block shapes, the FP32 vorticity pass's paired
+    half-width column blocks included) give the divergence form's fields
+    after two steps: FP64 within 1e-12, mixed within 1e-5 of FP64."""
+    note("nonlinear forms n=512")
+    h.set_physics(1 / 800.0, flow_type=1, kfmax=2, famp=0.5)
+    h.init_velocity(RN1, 2, 4, 3.0)
+    h.dealias_all()
+    f0 = np.stack([h.get(FIELDS, c) for c in (1, 2, 3)])
+    out = {}
+    try:
+        for form, prec in ((0, 0), (1, 0), (2, 0), (1, 1), (2, 1)):
+            for c in range(3):
+                h.put(FIELDS, c + 1, f0[c])
+            h.set_nonlinear(form)
+            fov = True
+            for _ in range(2):
+                _, _, fov, _ = h.step(5e-4, fov, True, precision=prec)
+            out[(form, prec)] = np.stack([h.get(FIELDS, c) for c in (1, 2, 3)])
+    finally:
+        h.set_nonlinear(0)
+    for form in (1, 2):
+        e64 = rel(out[(form, 0)], out[(0, 0)])
+        e32 = rel(out[(form, 1)], out[(form, 0)])
+        note(f"form {form}: FP64 vs divergence {e64:.2e}, mixed vs FP64 {e32:.2e}")
+        assert e64 < 1e-12 and e32 < 1e-5
