@@ -281,7 +281,7 @@ int hit3d_step(const double* dt, int64_t* fov, int64_t* fos, const int64_t* int_
         // scratch the step allocates on first use, before the pointer snapshot
         if (L.les_model == 11) mlp_alloc_scratch();
         if (!fused) batch_spares();
-        if (scal && L.dealias == 1) keep_slots();
+        if ((scal || (L.fuse && fused_ps_supported())) && L.dealias == 1) keep_slots();
         // the mixed step keeps rhs_old(1:3) in FP32, every other step in FP64
         rhs_old_format(mixed);
         const std::vector<void*> s0 = slot_state();

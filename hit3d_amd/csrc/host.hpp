@@ -86,6 +86,10 @@ void sums_to_host(const double* a, const double* b, const double* c, double out[
 void do_stat_scalars(double* diss, double* mean, double* var, double* mn, double* mx, double* es);
 void do_divergence(double* dmin, double* dmax);
 
+// fused velocity RHS + update of the phase-shift step (step_ps.hip)
+bool fused_ps_supported();
+void fused_ps_velocity(double* const shifted[3], bool first, const double* dtp);
+
 // one step (host_step.hip)
 void step_body(bool fused, bool mixed, bool scal, bool first_v, bool first_s);
 std::vector<void*> slot_state();

@@ -25,6 +25,16 @@ void step_body(bool fused, bool mixed, bool scal, bool first_v, bool first_s) {
     const bool keep = scal && L.dealias == 1 && L.keep.size() == 3;
     do_rhs_scalars(scal, keep);
     if (scal) do_advance(2, first_s, 0.0, dtp);
+    if (L.fuse && fused_ps_supported() && L.keep.size() == 3) {
+        // the velocity RHS, forcing, update and projection in the fused
+        // passes of step_ps.hip; they need the shifted physical velocity
+        // before the fields change
+        if (!keep)
+            for (int c = 0; c < 3; ++c) fft_inverse_from(L.F[1 + c], L.keep[c], +1);
+        double* sh[3] = {L.keep[0], L.keep[1], L.keep[2]};
+        fused_ps_velocity(sh, first_v, dtp);
+        return;
+    }
     do_rhs_velocity(keep);
     if (L.flow_type == 1) force_enqueue();
     {  // main.f90:156-164 and pressure.f90:13-47 in one pass

@@ -1,0 +1,391 @@
+// Fused velocity right-hand side of the phase-shift step (rhs_velocity.f90:
+// 150-410 with force_velocity, m_force.f90:101-182, the Euler/AB2 update,
+// main.f90:156-164, and pressure, pressure.f90:13-47), FP64.
+//
+// The reference forms the RHS in two halves: the products of the physical
+// velocity, and the products of the velocity shifted by half a cell
+// (exp(i theta), theta = (kx+ky+kz) dx/2), shifted back.  Here each half is
+// three passes over HBM:
+//   x  k_xprod6: physical u, v, w rows -> the six products' half spectra
+//      (real-to-complex through an N/2-point transform each; the unshifted
+//      half also takes the Courant max);
+//   y  the forward column pass of each product (in place);
+//   z  k_zps: the forward z transform of all six products of a column at
+//      once and the assembly -1/2 i k_j P_ij into the RHS:
+//      HALF (unshifted): r = -1/2 i k_j P_ij - nu k^2 u, the band forcing,
+//        zero where ialias >= 2; the fields take the AB2 term of the old RHS
+//        (u' = u - 1/2 dt rhs_old) and rhs_old takes r;
+//      FINAL (shifted): the products shifted back, r = rhs_old - 1/2 i k_j
+//        P_ij, zero where ialias >= 2, then u = project(u' + c dt r) with
+//        c = 1 (Euler) or 3/2 (AB2), and rhs_old = r.
+// Splitting the update this way needs no slot beyond the six products:
+// the half RHS waits in rhs_old, whose old value is already folded into
+// the fields.  The shifted physical velocity must exist before HALF changes
+// the fields (the caller keeps it in three slots).
+#include <stdexcept>
+
+#include "fft.hpp"
+#include "host.hpp"
+#include "passes.hpp"
+
+namespace hit3d {
+
+namespace {
+
+struct Q6 {
+    double* p[6];
+};
+
+// ------------------------------------------------------------------ x pass
+template <int N>
+struct XR {
+    static constexpr int M = N / 2;
+    static constexpr int E = (M < 8) ? M : 8;
+    static constexpr int TPF = M / E;
+    static constexpr int ROWS = (256 / TPF < N * N) ? 256 / TPF : N * N;
+    static constexpr int THREADS = ROWS * TPF;
+};
+
+template <int N, bool COURANT>
+__global__ __launch_bounds__(XR<N>::THREADS) void k_xprod6(const double* u, const double* v,
+                                                           const double* w, Q6 out, int64_t P,
+                                                           const cpx<double>* __restrict__ tw,
+                                                           double* __restrict__ blockmax) {
+    using C = XR<N>;
+    constexpr int M = C::M, E = C::E, TPF = C::TPF;
+    __shared__ cpx<double> lds[C::ROWS][M + 1];
+    __shared__ double redm[C::THREADS / 64 + 1];
+    const int t = threadIdx.x % TPF;
+    const int r = threadIdx.x / TPF;
+    const int64_t row = (int64_t)blockIdx.x * C::ROWS + r;
+    // a row of N reals as M complex pairs (x_2j, x_2j+1)
+    cpx<double> a[3][E];
+    const double* src[3] = {u, v, w};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const cpx<double>* rp = reinterpret_cast<const cpx<double>*>(src[c]) + row * P;
+#pragma unroll
+        for (int s = 0; s < E; ++s) a[c][s] = ld_g<kStream<N>>(&rp[t + s * TPF]);
+    }
+    double m = 0.0;
+    if constexpr (COURANT) {  // rhs_velocity.f90:31-32
+#pragma unroll
+        for (int s = 0; s < E; ++s) {
+            const double p0 = (fabs(a[0][s].x) + fabs(a[1][s].x)) + fabs(a[2][s].x);
+            const double p1 = (fabs(a[0][s].y) + fabs(a[1][s].y)) + fabs(a[2][s].y);
+            m = p0 > m ? p0 : m;
+            m = p1 > m ? p1 : m;
+        }
+    }
+    const cpx<double>* twN = tw + N;
+    // products uu, uv, uw, vv, vw, ww (rhs_velocity.f90:160-170)
+#pragma unroll
+    for (int p = 0; p < 6; ++p) {
+        const int i = p < 3 ? 0 : p < 5 ? 1 : 2;
+        const int j = p < 3 ? p : p < 5 ? p - 2 : 2;
+        cpx<double> z[E];
+#pragma unroll
+        for (int s = 0; s < E; ++s) z[s] = cpx<double>{a[i][s].x * a[j][s].x, a[i][s].y * a[j][s].y};
+        block_fft<M, E, -1, double, 1, kSwizzle>(z, t, lds[r], tw + M);
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < E; ++s) lds[r][t + s * TPF] = z[s];
+        __syncthreads();
+        double* op = out.p[0];
+        if (p == 1) op = out.p[1];
+        if (p == 2) op = out.p[2];
+        if (p == 3) op = out.p[3];
+        if (p == 4) op = out.p[4];
+        if (p == 5) op = out.p[5];
+        cpx<double>* rowp = reinterpret_cast<cpx<double>*>(op) + row * P;
+#pragma unroll
+        for (int s = 0; s < E; ++s) {
+            const int k = t + s * TPF;
+            const cpx<double> zk = lds[r][k];
+            const cpx<double> zm = conj(lds[r][(M - k) & (M - 1)]);
+            const cpx<double> e = scale(zk + zm, 0.5);
+            const cpx<double> o = scale(mul_mi(zk - zm), 0.5);
+            st_g<kStream<N>>(&rowp[k], e + cmul(twN[k], o));
+        }
+        if (t == 0) {
+            const cpx<double> z0 = lds[r][0];
+            st_g<kStream<N>>(&rowp[M], cpx<double>{z0.x - z0.y, 0.0});
+        }
+        __syncthreads();  // the next product's transform rewrites lds
+    }
+    if constexpr (COURANT) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const double q = __shfl_down(m, o, 64);
+            m = q > m ? q : m;
+        }
+        if ((threadIdx.x & 63) == 0) redm[threadIdx.x >> 6] = m;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double mm = 0.0;
+            for (int i = 0; i < (C::THREADS + 63) / 64; ++i) mm = redm[i] > mm ? redm[i] : mm;
+            blockmax[blockIdx.x] = mm;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ z pass
+// Columns (kx = 0..N/2, ky) of the six products, B columns per block (4 at
+// N = 1024, whose two halves of a 128-byte segment run on one XCD).
+template <int N>
+struct ZC {
+    static constexpr int E = 8;
+    static constexpr int TPF = N / E;
+    static constexpr int B = N >= 1024 ? 4 : 8;
+    static constexpr int THREADS = B * TPF;
+};
+
+template <int N>
+__device__ __forceinline__ void zc_coords(int& bx, int& by) {
+    if constexpr (ZC<N>::B * 16 >= 128) {
+        bx = blockIdx.x;
+        by = blockIdx.y;
+    } else {
+        const int gx = gridDim.x, total = gx * gridDim.y;
+        const int l = blockIdx.x + gx * blockIdx.y;
+        int logical = l;
+        if (l < total - total % 16) {
+            const int grp = l / 16, r = l % 16;
+            logical = (grp * 8 + (r & 7)) * 2 + (r >> 3);
+        }
+        bx = logical % gx;
+        by = logical / gx;
+    }
+}
+
+struct PsUpd {
+    double* F[3];
+    double* R[3];
+    const double* eshell;  // band energy per shell 1..kfmax (device)
+    double* energy_out;
+    const double* dtp;
+    const double* tab;     // cos, sin of (kx+ky+kz) dx/2 at 2(kx+ky+kz+N)
+    double nu, famp;
+    int kfmax, forced, first;
+};
+
+template <int N, bool FINAL>
+__global__ __launch_bounds__(ZC<N>::THREADS) void k_zps(Q6 Q, int64_t P, PsUpd u,
+                                                        const cpx<double>* __restrict__ tw) {
+    using C = ZC<N>;
+    constexpr int E = C::E, B = C::B, TPF = C::TPF;
+    constexpr int NC = N / 2 + 1;
+    __shared__ cpx<double> lds[N * B];
+    int bx, by;
+    zc_coords<N>(bx, by);
+    const int c = threadIdx.x % B, t = threadIdx.x / B;
+    const int kx = bx * B + c;
+    const bool valid = kx < NC;
+    const int jy = by;
+    const int ky = wavenum(jy, N);
+    const double kxd = kx, kyd = ky;
+    // t_i = k_j P_ij, rhs_velocity.f90:190-196 (shifted back first in FINAL,
+    // :282-398)
+    cpx<double> acc[3][E];
+#pragma unroll
+    for (int s = 0; s < E; ++s) acc[0][s] = acc[1][s] = acc[2][s] = cpx<double>{0.0, 0.0};
+#pragma unroll 1
+    for (int p = 0; p < 6; ++p) {
+        const double* qp = Q.p[0];
+        if (p == 1) qp = Q.p[1];
+        if (p == 2) qp = Q.p[2];
+        if (p == 3) qp = Q.p[3];
+        if (p == 4) qp = Q.p[4];
+        if (p == 5) qp = Q.p[5];
+        // layout after the x and y passes: (z, ky, kx)
+        const cpx<double>* src = reinterpret_cast<const cpx<double>*>(qp) + (int64_t)jy * P + kx;
+        cpx<double> v[E];
+#pragma unroll
+        for (int s = 0; s < E; ++s)
+            v[s] = valid ? ld_g<kStream<N>>(&src[(int64_t)(t + s * TPF) * N * P])
+                         : cpx<double>{0.0, 0.0};
+        const cpx<double>* twp = tw + N;
+        asm volatile("" : "+s"(twp));
+        block_fft<N, E, -1, double, B, 0, true, double>(v, t, lds + c, twp);
+#pragma unroll
+        for (int s = 0; s < E; ++s) {
+            const int e = t + s * TPF;
+            cpx<double> x = v[s];
+            if constexpr (FINAL) {  // shift back by exp(-i theta)
+                const int ti = kx + ky + wavenum(e, N) + N;
+                const double cs = u.tab[2 * ti], sn = u.tab[2 * ti + 1];
+                x = cpx<double>{x.x * cs + x.y * sn, x.y * cs - x.x * sn};
+            }
+            const double kzd = wavenum(e, N);
+            // p: 0 uu, 1 uv, 2 uw, 3 vv, 4 vw, 5 ww
+            if (p == 0) acc[0][s] = scale(x, kxd);
+            if (p == 1) {
+                acc[0][s] = acc[0][s] + scale(x, kyd);
+                acc[1][s] = scale(x, kxd);
+            }
+            if (p == 2) {
+                acc[0][s] = acc[0][s] + scale(x, kzd);
+                acc[2][s] = scale(x, kxd);
+            }
+            if (p == 3) acc[1][s] = acc[1][s] + scale(x, kyd);
+            if (p == 4) {
+                acc[1][s] = acc[1][s] + scale(x, kzd);
+                acc[2][s] = acc[2][s] + scale(x, kyd);
+            }
+            if (p == 5) acc[2][s] = acc[2][s] + scale(x, kzd);
+        }
+    }
+    double fac = 0.0;
+    if (!FINAL && u.forced) {  // m_force.f90:153-161
+        double en = 0.0;
+        for (int i = 0; i < u.kfmax; ++i) en += u.eshell[i];
+        fac = u.famp / en;
+        if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *u.energy_out = en;
+    }
+    if (!valid) return;
+    const double dt = *u.dtp;
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+        const int e = t + s * TPF;
+        const int kz = wavenum(e, N);
+        const double kzd = kz;
+        const int64_t off = ((int64_t)jy * N + e) * P + kx;
+        const bool zero = alias_count(kx, ky, kz, N) >= 2;  // rhs_velocity.f90:215,387
+        cpx<double> f[3], ro[3];
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) {
+            f[cc] = ld_g<kStream<N>>(&reinterpret_cast<const cpx<double>*>(u.F[cc])[off]);
+            ro[cc] = ld_g<kStream<N>>(&reinterpret_cast<const cpx<double>*>(u.R[cc])[off]);
+        }
+        if constexpr (!FINAL) {
+            const double lap = -u.nu * (kxd * kxd + kyd * kyd + kzd * kzd);
+            const bool fnode = u.forced && kx < N / 2 && [&] {
+                const int sh = shell_of(kx * kx + ky * ky + kz * kz);
+                return sh > 0 && sh <= u.kfmax;
+            }();
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) {
+                // r = 1/2 i (-t) + lap u  (rhs_velocity.f90:219-227)
+                cpx<double> r = {0.5 * acc[cc][s].y + lap * f[cc].x,
+                                 -0.5 * acc[cc][s].x + lap * f[cc].y};
+                if (zero) r = {0.0, 0.0};
+                if (fnode) r = {r.x + fac * f[cc].x, r.y + fac * f[cc].y};
+                const cpx<double> fo =
+                    u.first ? f[cc] : cpx<double>{f[cc].x - 0.5 * dt * ro[cc].x,
+                                                  f[cc].y - 0.5 * dt * ro[cc].y};
+                st_g<kStream<N>>(&reinterpret_cast<cpx<double>*>(u.F[cc])[off], fo);
+                st_g<kStream<N>>(&reinterpret_cast<cpx<double>*>(u.R[cc])[off], r);
+            }
+        } else {
+            const double cdt = (u.first ? 1.0 : 1.5) * dt;
+            cpx<double> fn[3];
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) {
+                // r += -1/2 i t  (rhs_velocity.f90:282-398), masked (:400-404)
+                cpx<double> r = {ro[cc].x + 0.5 * acc[cc][s].y, ro[cc].y - 0.5 * acc[cc][s].x};
+                if (zero) r = {0.0, 0.0};
+                fn[cc] = {f[cc].x + cdt * r.x, f[cc].y + cdt * r.y};
+                st_g<kStream<N>>(&reinterpret_cast<cpx<double>*>(u.R[cc])[off], r);
+            }
+            // projection, pressure.f90:21-43
+            const double k2 = kxd * kxd + kyd * kyd + kzd * kzd;
+            const double dre = kxd * fn[0].x + kyd * fn[1].x + kzd * fn[2].x;
+            const double dim = kxd * fn[0].y + kyd * fn[1].y + kzd * fn[2].y;
+            const double lapl = k2 == 0.0 ? 9e20 : k2;
+            const double pr = dre / lapl, pi = dim / lapl;
+            st_g<kStream<N>>(&reinterpret_cast<cpx<double>*>(u.F[0])[off],
+                             cpx<double>{fn[0].x - pr * kxd, fn[0].y - pi * kxd});
+            st_g<kStream<N>>(&reinterpret_cast<cpx<double>*>(u.F[1])[off],
+                             cpx<double>{fn[1].x - pr * kyd, fn[1].y - pi * kyd});
+            st_g<kStream<N>>(&reinterpret_cast<cpx<double>*>(u.F[2])[off],
+                             cpx<double>{fn[2].x - pr * kzd, fn[2].y - pi * kzd});
+        }
+    }
+}
+
+template <typename F>
+void dispatch_ps(int n, F&& f) {
+    switch (n) {
+        case 32: f(std::integral_constant<int, 32>{}); break;
+        case 64: f(std::integral_constant<int, 64>{}); break;
+        case 128: f(std::integral_constant<int, 128>{}); break;
+        case 256: f(std::integral_constant<int, 256>{}); break;
+        case 512: f(std::integral_constant<int, 512>{}); break;
+        case 1024: f(std::integral_constant<int, 1024>{}); break;
+        default: throw std::runtime_error("fused phase-shift step: unsupported grid size");
+    }
+}
+
+template <int N>
+void half(Lib& L, const double* const uvw[3], bool final_half, bool first, const double* dtp) {
+    using X = XR<N>;
+    using Z = ZC<N>;
+    const int64_t P = L.g.P;
+    Q6 q;
+    for (int i = 0; i < 6; ++i) q.p[i] = L.W[i + 1];
+    {
+        Timed tm(TC_FFT_X);
+        const int64_t nb = (int64_t)N * N / X::ROWS;  // <= n*n, the size of dmax
+        if (!final_half)
+            hipLaunchKernelGGL((k_xprod6<N, true>), dim3(nb), dim3(X::THREADS), 0, L.st, uvw[0],
+                               uvw[1], uvw[2], q, P, L.tw64, L.dmax);
+        else
+            hipLaunchKernelGGL((k_xprod6<N, false>), dim3(nb), dim3(X::THREADS), 0, L.st, uvw[0],
+                               uvw[1], uvw[2], q, P, L.tw64, L.dmax);
+        if (!final_half) launch_max_fold(L.dmax, nb, L.dscal, L.st);
+    }
+    for (int i = 1; i <= 6; ++i) pass_y(L.W[i], -1);
+    PsUpd u;
+    for (int i = 0; i < 3; ++i) {
+        u.F[i] = L.F[i + 1];
+        u.R[i] = L.R[i + 1];
+    }
+    u.eshell = L.dshell;
+    u.energy_out = L.dscal + 1;
+    u.dtp = dtp;
+    u.tab = L.ps_tab;
+    u.nu = L.nu;
+    u.famp = L.famp;
+    u.kfmax = L.kfmax;
+    u.forced = L.flow_type == 1 ? 1 : 0;
+    u.first = first ? 1 : 0;
+    int ncb = (N / 2 + 1 + Z::B - 1) / Z::B;
+    if (Z::B * 16 < 128) ncb += ncb & 1;
+    Timed tm(TC_FFT_Z);
+    if (!final_half)
+        hipLaunchKernelGGL((k_zps<N, false>), dim3(ncb, N), dim3(Z::THREADS), 0, L.st, q, P, u,
+                           L.tw64);
+    else
+        hipLaunchKernelGGL((k_zps<N, true>), dim3(ncb, N), dim3(Z::THREADS), 0, L.st, q, P, u,
+                           L.tw64);
+}
+
+}  // namespace
+
+bool fused_ps_supported() {
+    Lib& L = lib();
+    return L.dealias == 1 && L.nles == 0 && L.les_model == 0 && L.n >= 32;
+}
+
+// The velocity part of a phase-shift step after rhs_scalars: wrk(1:3) holds
+// the physical velocity and `shifted` the physical velocity shifted by
+// exp(i theta) (three slots, not wrk(1:6)).  Leaves fields(1:3) updated and
+// projected, rhs_old(1:3) the RHS with forcing, the Courant max in
+// dscal[0] and the band energy in dscal[1]; wrk(1:6) is scratch.
+void fused_ps_velocity(double* const shifted[3], bool first, const double* dtp) {
+    Lib& L = lib();
+    if (L.flow_type == 1) {
+        if (L.kfmax < 1 || L.kfmax > L.kmax) throw std::runtime_error("force_velocity: kfmax range");
+        Timed t(TC_REDUCE);
+        launch_shell_sums(L.g, L.F[1], L.F[2], L.F[3], 1, L.kfmax, L.dshell, L.st);
+    }
+    dispatch_ps(L.n, [&](auto NC) {
+        constexpr int N = decltype(NC)::value;
+        const double* un[3] = {L.W[1], L.W[2], L.W[3]};
+        half<N>(L, un, false, first, dtp);
+        const double* sh[3] = {shifted[0], shifted[1], shifted[2]};
+        half<N>(L, sh, true, first, dtp);
+    });
+}
+
+}  // namespace hit3d
