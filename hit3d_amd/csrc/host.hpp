@@ -89,6 +89,8 @@ void do_divergence(double* dmin, double* dmax);
 // fused velocity RHS + update of the phase-shift step (step_ps.hip)
 bool fused_ps_supported();
 void fused_ps_velocity(double* const shifted[3], bool first, const double* dtp);
+bool fused_ps_scalars_supported();
+void fused_ps_scalars(bool first, const double* dtp);
 
 // one step (host_step.hip)
 void step_body(bool fused, bool mixed, bool scal, bool first_v, bool first_s);

@@ -20,6 +20,14 @@ void step_body(bool fused, bool mixed, bool scal, bool first_v, bool first_s) {
         fused_step(dtp, first_v, mixed);
         return;
     }
+    if (L.fuse && scal && L.keep.size() == 3 && fused_ps_scalars_supported()) {
+        // phase shift: scalar RHS, scalar update, velocity RHS, forcing,
+        // update and projection in the fused passes of step_ps.hip
+        fused_ps_scalars(first_s, dtp);
+        double* sh[3] = {L.keep[0], L.keep[1], L.keep[2]};
+        fused_ps_velocity(sh, first_v, dtp);
+        return;
+    }
     // with the phase shift and scalars, the shifted velocity rhs_scalars
     // transforms is kept for rhs_velocity (the fields do not change between)
     const bool keep = scal && L.dealias == 1 && L.keep.size() == 3;

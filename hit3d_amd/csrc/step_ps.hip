@@ -22,6 +22,12 @@
 // the half RHS waits in rhs_old, whose old value is already folded into
 // the fields.  The shifted physical velocity must exist before HALF changes
 // the fields (the caller keeps it in three slots).
+//
+// The scalars (rhs_scalars.f90:144-271, 387-389, with their Euler/AB2
+// update) go the same way, three products phi u_j per half: k_xprod3, the
+// column passes, and k_zsc, whose shifted half waits in wrk(3+s) and whose
+// unshifted half adds it, the diffusion, the mean-gradient source and the
+// update.
 #include <stdexcept>
 
 #include "fft.hpp"
@@ -303,6 +309,164 @@ __global__ __launch_bounds__(ZC<N>::THREADS) void k_zps(Q6 Q, int64_t P, PsUpd u
     }
 }
 
+// ------------------------------------------------------------ scalars
+// x pass: a physical scalar row phi and the velocity rows -> the half
+// spectra of phi u, phi v, phi w (out may alias phi: each row is read
+// before it is written).
+template <int N>
+__global__ __launch_bounds__(XR<N>::THREADS) void k_xprod3(const double* phi, const double* u,
+                                                           const double* v, const double* w,
+                                                           Q6 out, int64_t P,
+                                                           const cpx<double>* __restrict__ tw) {
+    using C = XR<N>;
+    constexpr int M = C::M, E = C::E, TPF = C::TPF;
+    __shared__ cpx<double> lds[C::ROWS][M + 1];
+    const int t = threadIdx.x % TPF;
+    const int r = threadIdx.x / TPF;
+    const int64_t row = (int64_t)blockIdx.x * C::ROWS + r;
+    cpx<double> ph[E];
+    {
+        const cpx<double>* rp = reinterpret_cast<const cpx<double>*>(phi) + row * P;
+#pragma unroll
+        for (int s = 0; s < E; ++s) ph[s] = ld_g<kStream<N>>(&rp[t + s * TPF]);
+    }
+    const double* vel[3] = {u, v, w};
+    const cpx<double>* twN = tw + N;
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        const cpx<double>* vp = reinterpret_cast<const cpx<double>*>(vel[p]) + row * P;
+        cpx<double> z[E];
+#pragma unroll
+        for (int s = 0; s < E; ++s) {
+            const cpx<double> b = ld_g<kStream<N>>(&vp[t + s * TPF]);
+            z[s] = cpx<double>{ph[s].x * b.x, ph[s].y * b.y};
+        }
+        block_fft<M, E, -1, double, 1, kSwizzle>(z, t, lds[r], tw + M);
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < E; ++s) lds[r][t + s * TPF] = z[s];
+        __syncthreads();
+        double* op = out.p[0];
+        if (p == 1) op = out.p[1];
+        if (p == 2) op = out.p[2];
+        cpx<double>* rowp = reinterpret_cast<cpx<double>*>(op) + row * P;
+#pragma unroll
+        for (int s = 0; s < E; ++s) {
+            const int k = t + s * TPF;
+            const cpx<double> zk = lds[r][k];
+            const cpx<double> zm = conj(lds[r][(M - k) & (M - 1)]);
+            const cpx<double> e = scale(zk + zm, 0.5);
+            const cpx<double> o = scale(mul_mi(zk - zm), 0.5);
+            st_g<kStream<N>>(&rowp[k], e + cmul(twN[k], o));
+        }
+        if (t == 0) {
+            const cpx<double> z0 = lds[r][0];
+            st_g<kStream<N>>(&rowp[M], cpx<double>{z0.x - z0.y, 0.0});
+        }
+        __syncthreads();
+    }
+}
+
+struct ScUpd {
+    double* F;          // the scalar field
+    double* R;          // its rhs_old
+    double* H;          // the RHS of the shifted half (HALF writes, FINAL reads)
+    const double* U1;   // the mean-gradient source -u (type 0) or null
+    const double* tab;
+    const double* dtp;
+    double pe;
+    int first;
+};
+
+// z pass of the scalar products phi u_j (rhs_scalars.f90:144-262):
+// HALF (shifted): the products shifted back, H = -1/2 i k_j Q_j, zero where
+// ialias >= 2 (:200); FINAL (unshifted): r = H - 1/2 i k_j Q_j - pe k^2 phi
+// where ialias < 2 (:240-262), minus u for a type-0 scalar (:387-389), then
+// the Euler/AB2 update of the scalar (main.f90:137-146): phi += dt r (or
+// dt (3/2 r - 1/2 rhs_old)), rhs_old = r.
+template <int N, bool FINAL>
+__global__ __launch_bounds__(ZC<N>::THREADS) void k_zsc(Q6 Q, int64_t P, ScUpd u,
+                                                        const cpx<double>* __restrict__ tw) {
+    using C = ZC<N>;
+    constexpr int E = C::E, B = C::B, TPF = C::TPF;
+    constexpr int NC = N / 2 + 1;
+    __shared__ cpx<double> lds[N * B];
+    int bx, by;
+    zc_coords<N>(bx, by);
+    const int c = threadIdx.x % B, t = threadIdx.x / B;
+    const int kx = bx * B + c;
+    const bool valid = kx < NC;
+    const int jy = by;
+    const int ky = wavenum(jy, N);
+    const double kxd = kx, kyd = ky;
+    cpx<double> acc[E];
+#pragma unroll
+    for (int s = 0; s < E; ++s) acc[s] = cpx<double>{0.0, 0.0};
+#pragma unroll 1
+    for (int p = 0; p < 3; ++p) {
+        const double* qp = Q.p[0];
+        if (p == 1) qp = Q.p[1];
+        if (p == 2) qp = Q.p[2];
+        const cpx<double>* src = reinterpret_cast<const cpx<double>*>(qp) + (int64_t)jy * P + kx;
+        cpx<double> v[E];
+#pragma unroll
+        for (int s = 0; s < E; ++s)
+            v[s] = valid ? ld_g<kStream<N>>(&src[(int64_t)(t + s * TPF) * N * P])
+                         : cpx<double>{0.0, 0.0};
+        const cpx<double>* twp = tw + N;
+        asm volatile("" : "+s"(twp));
+        block_fft<N, E, -1, double, B, 0, true, double>(v, t, lds + c, twp);
+#pragma unroll
+        for (int s = 0; s < E; ++s) {
+            const int e = t + s * TPF;
+            cpx<double> x = v[s];
+            if constexpr (!FINAL) {  // shift back by exp(-i theta)
+                const int ti = kx + ky + wavenum(e, N) + N;
+                const double cs = u.tab[2 * ti], sn = u.tab[2 * ti + 1];
+                x = cpx<double>{x.x * cs + x.y * sn, x.y * cs - x.x * sn};
+            }
+            const double k = p == 0 ? kxd : p == 1 ? kyd : (double)wavenum(e, N);
+            acc[s] = acc[s] + scale(x, k);
+        }
+    }
+    if (!valid) return;
+    const double dt = *u.dtp;
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+        const int e = t + s * TPF;
+        const int kz = wavenum(e, N);
+        const double kzd = kz;
+        const int64_t off = ((int64_t)jy * N + e) * P + kx;
+        const bool zero = alias_count(kx, ky, kz, N) >= 2;
+        // -1/2 i t
+        const cpx<double> half = {0.5 * acc[s].y, -0.5 * acc[s].x};
+        if constexpr (!FINAL) {
+            st_g<kStream<N>>(&reinterpret_cast<cpx<double>*>(u.H)[off],
+                             zero ? cpx<double>{0.0, 0.0} : half);
+        } else {
+            const cpx<double> f = ld_g<kStream<N>>(&reinterpret_cast<const cpx<double>*>(u.F)[off]);
+            const cpx<double> h = ld_g<kStream<N>>(&reinterpret_cast<const cpx<double>*>(u.H)[off]);
+            const cpx<double> ro =
+                u.first ? cpx<double>{0.0, 0.0}
+                        : ld_g<kStream<N>>(&reinterpret_cast<const cpx<double>*>(u.R)[off]);
+            const double k2 = kxd * kxd + kyd * kyd + kzd * kzd;
+            cpx<double> r = h;
+            if (!zero) r = {(h.x + half.x) - u.pe * k2 * f.x, (h.y + half.y) - u.pe * k2 * f.y};
+            if (u.U1) {
+                const cpx<double> s1 =
+                    ld_g<kStream<N>>(&reinterpret_cast<const cpx<double>*>(u.U1)[off]);
+                r = {r.x - s1.x, r.y - s1.y};
+            }
+            const cpx<double> fn =
+                u.first ? cpx<double>{f.x + dt * r.x, f.y + dt * r.y}
+                        : cpx<double>{f.x + dt * (1.5 * r.x - 0.5 * ro.x),
+                                      f.y + dt * (1.5 * r.y - 0.5 * ro.y)};
+            st_g<kStream<N>>(&reinterpret_cast<cpx<double>*>(u.R)[off], r);
+            st_g<kStream<N>>(&reinterpret_cast<cpx<double>*>(u.F)[off], fn);
+        }
+    }
+}
+
 template <typename F>
 void dispatch_ps(int n, F&& f) {
     switch (n) {
@@ -360,7 +524,88 @@ void half(Lib& L, const double* const uvw[3], bool final_half, bool first, const
                            L.tw64);
 }
 
+template <int N>
+void scalar_half(Lib& L, const double* phi, const double* const vel[3], double* const q[3],
+                 bool final_half, const ScUpd& u) {
+    using X = XR<N>;
+    using Z = ZC<N>;
+    const int64_t P = L.g.P;
+    Q6 qq{};
+    for (int i = 0; i < 3; ++i) qq.p[i] = q[i];
+    {
+        Timed tm(TC_FFT_X);
+        const int64_t nb = (int64_t)N * N / X::ROWS;
+        hipLaunchKernelGGL((k_xprod3<N>), dim3(nb), dim3(X::THREADS), 0, L.st, phi, vel[0], vel[1],
+                           vel[2], qq, P, L.tw64);
+    }
+    for (int i = 0; i < 3; ++i) pass_y(q[i], -1);
+    int ncb = (N / 2 + 1 + Z::B - 1) / Z::B;
+    if (Z::B * 16 < 128) ncb += ncb & 1;
+    Timed tm(TC_FFT_Z);
+    if (!final_half)
+        hipLaunchKernelGGL((k_zsc<N, false>), dim3(ncb, N), dim3(Z::THREADS), 0, L.st, qq, P, u,
+                           L.tw64);
+    else
+        hipLaunchKernelGGL((k_zsc<N, true>), dim3(ncb, N), dim3(Z::THREADS), 0, L.st, qq, P, u,
+                           L.tw64);
+}
+
 }  // namespace
+
+bool fused_ps_scalars_supported() {
+    Lib& L = lib();
+    if (!fused_ps_supported() || L.ns < 1 || L.nw < 3 + L.ns + 2) return false;
+    for (int s = 0; s < L.ns; ++s)
+        if (L.stype[s] > 300) return false;  // the type-311 reaction takes the unfused path
+    return true;
+}
+
+// rhs_scalars (phase shift, rhs_scalars.f90:144-271, 387-389) and the
+// scalars' Euler/AB2 update in fused passes.  Leaves the shifted physical
+// velocity in lib().keep (three slots) and the physical velocity in
+// wrk(1:3), as rhs_scalars does (:229-232), for the velocity RHS.
+void fused_ps_scalars(bool first, const double* dtp) {
+    Lib& L = lib();
+    auto& F = L.F;
+    auto& W = L.W;
+    if (L.keep.size() != 3) throw std::runtime_error("internal: shifted-velocity slots missing");
+    for (int c = 0; c < 3; ++c) fft_inverse_from(F[1 + c], L.keep[c], +1);
+    dispatch_ps(L.n, [&](auto NC) {
+        constexpr int N = decltype(NC)::value;
+        const double* vs[3] = {L.keep[0], L.keep[1], L.keep[2]};
+        // shifted halves into wrk(3+s), products in wrk(1:3)
+        for (int s = 1; s <= L.ns; ++s) {
+            fft_inverse_from(F[3 + s], W[0], +1);
+            ScUpd u{};
+            u.H = W[3 + s];
+            u.tab = L.ps_tab;
+            u.dtp = dtp;
+            double* q[3] = {W[1], W[2], W[3]};
+            scalar_half<N>(L, W[0], vs, q, false, u);
+        }
+        // the physical velocity in wrk(1:3), then the unshifted halves,
+        // products in wrk(n1), wrk(n2) and wrk(0)
+        const double* src[3] = {F[1], F[2], F[3]};
+        double** dst[3] = {&W[1], &W[2], &W[3]};
+        for (int c = 0; c < 3; ++c) fft_inverse_from(src[c], *dst[c], 0);
+        const double* vu[3] = {W[1], W[2], W[3]};
+        const int n1 = L.ns + 4, n2 = L.ns + 5;
+        for (int s = 1; s <= L.ns; ++s) {
+            fft_inverse_from(F[3 + s], W[0], 0);
+            ScUpd u{};
+            u.F = F[3 + s];
+            u.R = L.R[3 + s];
+            u.H = W[3 + s];
+            u.U1 = L.stype[s - 1] == 0 ? F[1] : nullptr;
+            u.tab = L.ps_tab;
+            u.dtp = dtp;
+            u.pe = L.pe[s - 1];
+            u.first = first ? 1 : 0;
+            double* q[3] = {W[n1], W[n2], W[0]};
+            scalar_half<N>(L, W[0], vu, q, true, u);
+        }
+    });
+}
 
 bool fused_ps_supported() {
     Lib& L = lib();
