@@ -53,7 +53,7 @@ struct XR {
 };
 
 template <int N, bool COURANT>
-__global__ __launch_bounds__(XR<N>::THREADS) void k_xprod6(const double* u, const double* v,
+__global__ __launch_bounds__(XR<N>::THREADS, 2) void k_xprod6(const double* u, const double* v,
                                                            const double* w, Q6 out, int64_t P,
                                                            const cpx<double>* __restrict__ tw,
                                                            double* __restrict__ blockmax) {
@@ -250,61 +250,77 @@ __global__ __launch_bounds__(ZC<N>::THREADS) void k_zps(Q6 Q, int64_t P, PsUpd u
     }
     if (!valid) return;
     const double dt = *u.dtp;
+    // In chunks of H slots, every fields / rhs_old load of the chunk is
+    // issued before its first store: the compiler cannot tell the six
+    // arrays apart, and a store between loads would cost one memory round
+    // trip per load.  (FINAL keeps more values live: smaller chunks.)
+    constexpr int H = FINAL ? 2 : 4;
 #pragma unroll
-    for (int s = 0; s < E; ++s) {
-        const int e = t + s * TPF;
-        const int kz = wavenum(e, N);
-        const double kzd = kz;
-        const int64_t off = ((int64_t)jy * N + e) * P + kx;
-        const bool zero = alias_count(kx, ky, kz, N) >= 2;  // rhs_velocity.f90:215,387
-        cpx<double> f[3], ro[3];
+    for (int h = 0; h < E / H; ++h) {
+        cpx<double> f[3][H], ro[3][H];
 #pragma unroll
-        for (int cc = 0; cc < 3; ++cc) {
-            f[cc] = ld_g<kStream<N>>(&reinterpret_cast<const cpx<double>*>(u.F[cc])[off]);
-            ro[cc] = ld_g<kStream<N>>(&reinterpret_cast<const cpx<double>*>(u.R[cc])[off]);
+        for (int q = 0; q < H; ++q) {
+            const int e = t + (h * H + q) * TPF;
+            const int64_t off = ((int64_t)jy * N + e) * P + kx;
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) {
+                f[cc][q] = ld_g<kStream<N>>(&reinterpret_cast<const cpx<double>*>(u.F[cc])[off]);
+                ro[cc][q] = ld_g<kStream<N>>(&reinterpret_cast<const cpx<double>*>(u.R[cc])[off]);
+            }
         }
-        if constexpr (!FINAL) {
-            const double lap = -u.nu * (kxd * kxd + kyd * kyd + kzd * kzd);
-            const bool fnode = u.forced && kx < N / 2 && [&] {
-                const int sh = shell_of(kx * kx + ky * ky + kz * kz);
-                return sh > 0 && sh <= u.kfmax;
-            }();
 #pragma unroll
-            for (int cc = 0; cc < 3; ++cc) {
-                // r = 1/2 i (-t) + lap u  (rhs_velocity.f90:219-227)
-                cpx<double> r = {0.5 * acc[cc][s].y + lap * f[cc].x,
-                                 -0.5 * acc[cc][s].x + lap * f[cc].y};
-                if (zero) r = {0.0, 0.0};
-                if (fnode) r = {r.x + fac * f[cc].x, r.y + fac * f[cc].y};
-                const cpx<double> fo =
-                    u.first ? f[cc] : cpx<double>{f[cc].x - 0.5 * dt * ro[cc].x,
-                                                  f[cc].y - 0.5 * dt * ro[cc].y};
-                st_g<kStream<N>>(&reinterpret_cast<cpx<double>*>(u.F[cc])[off], fo);
-                st_g<kStream<N>>(&reinterpret_cast<cpx<double>*>(u.R[cc])[off], r);
-            }
-        } else {
-            const double cdt = (u.first ? 1.0 : 1.5) * dt;
-            cpx<double> fn[3];
+        for (int q = 0; q < H; ++q) {
+            const int s = h * H + q;
+            const int e = t + s * TPF;
+            const int kz = wavenum(e, N);
+            const double kzd = kz;
+            const int64_t off = ((int64_t)jy * N + e) * P + kx;
+            const bool zero = alias_count(kx, ky, kz, N) >= 2;  // rhs_velocity.f90:215,387
+            if constexpr (!FINAL) {
+                const double lap = -u.nu * (kxd * kxd + kyd * kyd + kzd * kzd);
+                const bool fnode = u.forced && kx < N / 2 && [&] {
+                    const int sh = shell_of(kx * kx + ky * ky + kz * kz);
+                    return sh > 0 && sh <= u.kfmax;
+                }();
 #pragma unroll
-            for (int cc = 0; cc < 3; ++cc) {
-                // r += -1/2 i t  (rhs_velocity.f90:282-398), masked (:400-404)
-                cpx<double> r = {ro[cc].x + 0.5 * acc[cc][s].y, ro[cc].y - 0.5 * acc[cc][s].x};
-                if (zero) r = {0.0, 0.0};
-                fn[cc] = {f[cc].x + cdt * r.x, f[cc].y + cdt * r.y};
-                st_g<kStream<N>>(&reinterpret_cast<cpx<double>*>(u.R[cc])[off], r);
+                for (int cc = 0; cc < 3; ++cc) {
+                    const cpx<double> fc = f[cc][q];
+                    // r = 1/2 i (-t) + lap u  (rhs_velocity.f90:219-227)
+                    cpx<double> r = {0.5 * acc[cc][s].y + lap * fc.x,
+                                     -0.5 * acc[cc][s].x + lap * fc.y};
+                    if (zero) r = {0.0, 0.0};
+                    if (fnode) r = {r.x + fac * fc.x, r.y + fac * fc.y};
+                    const cpx<double> fo =
+                        u.first ? fc : cpx<double>{fc.x - 0.5 * dt * ro[cc][q].x,
+                                                   fc.y - 0.5 * dt * ro[cc][q].y};
+                    st_g<kStream<N>>(&reinterpret_cast<cpx<double>*>(u.F[cc])[off], fo);
+                    st_g<kStream<N>>(&reinterpret_cast<cpx<double>*>(u.R[cc])[off], r);
+                }
+            } else {
+                const double cdt = (u.first ? 1.0 : 1.5) * dt;
+                cpx<double> fn[3];
+#pragma unroll
+                for (int cc = 0; cc < 3; ++cc) {
+                    // r += -1/2 i t  (rhs_velocity.f90:282-398), masked (:400-404)
+                    cpx<double> r = {ro[cc][q].x + 0.5 * acc[cc][s].y,
+                                     ro[cc][q].y - 0.5 * acc[cc][s].x};
+                    if (zero) r = {0.0, 0.0};
+                    fn[cc] = {f[cc][q].x + cdt * r.x, f[cc][q].y + cdt * r.y};
+                    st_g<kStream<N>>(&reinterpret_cast<cpx<double>*>(u.R[cc])[off], r);
+                }
+                // projection, pressure.f90:21-43
+                const double k2 = kxd * kxd + kyd * kyd + kzd * kzd;
+                const double dre = kxd * fn[0].x + kyd * fn[1].x + kzd * fn[2].x;
+                const double dim = kxd * fn[0].y + kyd * fn[1].y + kzd * fn[2].y;
+                const double lapl = k2 == 0.0 ? 9e20 : k2;
+                const double pr = dre / lapl, pi = dim / lapl;
+                st_g<kStream<N>>(&reinterpret_cast<cpx<double>*>(u.F[0])[off],
+                                 cpx<double>{fn[0].x - pr * kxd, fn[0].y - pi * kxd});
+                st_g<kStream<N>>(&reinterpret_cast<cpx<double>*>(u.F[1])[off],
+                                 cpx<double>{fn[1].x - pr * kyd, fn[1].y - pi * kyd});
+                st_g<kStream<N>>(&reinterpret_cast<cpx<double>*>(u.F[2])[off],
+                                 cpx<double>{fn[2].x - pr * kzd, fn[2].y - pi * kzd});
             }
-            // projection, pressure.f90:21-43
-            const double k2 = kxd * kxd + kyd * kyd + kzd * kzd;
-            const double dre = kxd * fn[0].x + kyd * fn[1].x + kzd * fn[2].x;
-            const double dim = kxd * fn[0].y + kyd * fn[1].y + kzd * fn[2].y;
-            const double lapl = k2 == 0.0 ? 9e20 : k2;
-            const double pr = dre / lapl, pi = dim / lapl;
-            st_g<kStream<N>>(&reinterpret_cast<cpx<double>*>(u.F[0])[off],
-                             cpx<double>{fn[0].x - pr * kxd, fn[0].y - pi * kxd});
-            st_g<kStream<N>>(&reinterpret_cast<cpx<double>*>(u.F[1])[off],
-                             cpx<double>{fn[1].x - pr * kyd, fn[1].y - pi * kyd});
-            st_g<kStream<N>>(&reinterpret_cast<cpx<double>*>(u.F[2])[off],
-                             cpx<double>{fn[2].x - pr * kzd, fn[2].y - pi * kzd});
         }
     }
 }
@@ -431,38 +447,50 @@ __global__ __launch_bounds__(ZC<N>::THREADS) void k_zsc(Q6 Q, int64_t P, ScUpd u
     }
     if (!valid) return;
     const double dt = *u.dtp;
+    constexpr int H = 4;  // loads of a chunk before its stores, as in k_zps
 #pragma unroll
-    for (int s = 0; s < E; ++s) {
-        const int e = t + s * TPF;
-        const int kz = wavenum(e, N);
-        const double kzd = kz;
-        const int64_t off = ((int64_t)jy * N + e) * P + kx;
-        const bool zero = alias_count(kx, ky, kz, N) >= 2;
-        // -1/2 i t
-        const cpx<double> half = {0.5 * acc[s].y, -0.5 * acc[s].x};
-        if constexpr (!FINAL) {
-            st_g<kStream<N>>(&reinterpret_cast<cpx<double>*>(u.H)[off],
-                             zero ? cpx<double>{0.0, 0.0} : half);
-        } else {
-            const cpx<double> f = ld_g<kStream<N>>(&reinterpret_cast<const cpx<double>*>(u.F)[off]);
-            const cpx<double> h = ld_g<kStream<N>>(&reinterpret_cast<const cpx<double>*>(u.H)[off]);
-            const cpx<double> ro =
-                u.first ? cpx<double>{0.0, 0.0}
-                        : ld_g<kStream<N>>(&reinterpret_cast<const cpx<double>*>(u.R)[off]);
-            const double k2 = kxd * kxd + kyd * kyd + kzd * kzd;
-            cpx<double> r = h;
-            if (!zero) r = {(h.x + half.x) - u.pe * k2 * f.x, (h.y + half.y) - u.pe * k2 * f.y};
-            if (u.U1) {
-                const cpx<double> s1 =
-                    ld_g<kStream<N>>(&reinterpret_cast<const cpx<double>*>(u.U1)[off]);
-                r = {r.x - s1.x, r.y - s1.y};
+    for (int h = 0; h < E / H; ++h) {
+        cpx<double> f[H], hh[H], ro[H], s1[H];
+        if constexpr (FINAL) {
+#pragma unroll
+            for (int q = 0; q < H; ++q) {
+                const int e = t + (h * H + q) * TPF;
+                const int64_t off = ((int64_t)jy * N + e) * P + kx;
+                f[q] = ld_g<kStream<N>>(&reinterpret_cast<const cpx<double>*>(u.F)[off]);
+                hh[q] = ld_g<kStream<N>>(&reinterpret_cast<const cpx<double>*>(u.H)[off]);
+                ro[q] = u.first ? cpx<double>{0.0, 0.0}
+                                : ld_g<kStream<N>>(&reinterpret_cast<const cpx<double>*>(u.R)[off]);
+                s1[q] = u.U1 ? ld_g<kStream<N>>(&reinterpret_cast<const cpx<double>*>(u.U1)[off])
+                             : cpx<double>{0.0, 0.0};
             }
-            const cpx<double> fn =
-                u.first ? cpx<double>{f.x + dt * r.x, f.y + dt * r.y}
-                        : cpx<double>{f.x + dt * (1.5 * r.x - 0.5 * ro.x),
-                                      f.y + dt * (1.5 * r.y - 0.5 * ro.y)};
-            st_g<kStream<N>>(&reinterpret_cast<cpx<double>*>(u.R)[off], r);
-            st_g<kStream<N>>(&reinterpret_cast<cpx<double>*>(u.F)[off], fn);
+        }
+#pragma unroll
+        for (int q = 0; q < H; ++q) {
+            const int s = h * H + q;
+            const int e = t + s * TPF;
+            const int kz = wavenum(e, N);
+            const double kzd = kz;
+            const int64_t off = ((int64_t)jy * N + e) * P + kx;
+            const bool zero = alias_count(kx, ky, kz, N) >= 2;
+            // -1/2 i t
+            const cpx<double> half = {0.5 * acc[s].y, -0.5 * acc[s].x};
+            if constexpr (!FINAL) {
+                st_g<kStream<N>>(&reinterpret_cast<cpx<double>*>(u.H)[off],
+                                 zero ? cpx<double>{0.0, 0.0} : half);
+            } else {
+                const double k2 = kxd * kxd + kyd * kyd + kzd * kzd;
+                cpx<double> r = hh[q];
+                if (!zero)
+                    r = {(hh[q].x + half.x) - u.pe * k2 * f[q].x,
+                         (hh[q].y + half.y) - u.pe * k2 * f[q].y};
+                r = {r.x - s1[q].x, r.y - s1[q].y};
+                const cpx<double> fn =
+                    u.first ? cpx<double>{f[q].x + dt * r.x, f[q].y + dt * r.y}
+                            : cpx<double>{f[q].x + dt * (1.5 * r.x - 0.5 * ro[q].x),
+                                          f[q].y + dt * (1.5 * r.y - 0.5 * ro[q].y)};
+                st_g<kStream<N>>(&reinterpret_cast<cpx<double>*>(u.R)[off], r);
+                st_g<kStream<N>>(&reinterpret_cast<cpx<double>*>(u.F)[off], fn);
+            }
         }
     }
 }
