@@ -668,6 +668,35 @@ struct Upd {
     int kfmax, forced, first;
 };
 
+// The column kx = ky = 0 of a product holds the product's plane sums, whose
+// common part (N^2 times the product's mean over the box) dwarfs every
+// other input of the z transforms.  In FP32 the butterflies that cancel it
+// leave errors of order eps N^3 |mean| on the high kz of that column, which
+// k_j P_ij multiplies by kz: the largest error of the mixed RHS, growing with
+// N.  Taking the column's mean out first changes only its kz = 0 output,
+// which the RHS multiplies by k = 0.  Called by the whole block (barriers);
+// `mine` marks the threads holding the column, t its thread position; lds
+// is the block's LDS tile of LDSN entries.
+template <int N, int E, int TPF, int LDSN, typename T>
+__device__ void remove_column_mean(cpx<T> (&v)[E], int t, bool mine, cpx<T>* lds) {
+    static_assert(TPF * sizeof(cpx<double>) <= LDSN * sizeof(cpx<T>), "partials exceed the LDS tile");
+    cpx<double>* part = reinterpret_cast<cpx<double>*>(lds);
+    cpx<double> s{0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < E; ++i) s = {s.x + (double)v[i].x, s.y + (double)v[i].y};
+    __syncthreads();  // the previous transform's last LDS reads are done
+    if (mine) part[t] = s;
+    __syncthreads();
+    if (mine) {
+        cpx<double> tot{0.0, 0.0};
+        for (int i = 0; i < TPF; ++i) tot = {tot.x + part[i].x, tot.y + part[i].y};
+        const cpx<T> m{(T)(tot.x / N), (T)(tot.y / N)};
+#pragma unroll
+        for (int i = 0; i < E; ++i) v[i] = {v[i].x - m.x, v[i].y - m.y};
+    }
+    __syncthreads();  // before the transform writes LDS
+}
+
 // FORM 0: six buffers hold the products u_i u_j, r_i = -i k_j (u_i u_j)^ -
 // nu k^2 u_i.  FORM 1 (rotational): three hold (u x omega)_i, the nonlinear
 // term itself, r_i = (u x omega)_i^ - nu k^2 u_i.  FORM 2 (deviatoric): five
@@ -748,6 +777,9 @@ __global__ __launch_bounds__((CZ<N, T>::THREADS)) void k_fwd_z_update(
         // across the loop.
         const cpx<double>* twp = twd + N;
         asm volatile("" : "+s"(twp));
+        if constexpr (sizeof(T) == 4)
+            if (bx == 0 && jy == 0)
+                remove_column_mean<N, E, TPF, padded_size<PS>(N * B)>(v, t, c == 0, lds);
         block_fft<N, E, -1, T, B, PS, true, double, TL>(v, t, lds + c, twp, tlp);
 #pragma unroll
         for (int s = 0; s < E; ++s) {

@@ -12,6 +12,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -173,10 +174,22 @@ def velocity_rng_state(n: int, rn1: float) -> int:
     return iy.value
 
 
+_open_instance = None   # weak reference to the open Hit3D, if any
+
+
 class Hit3D:
-    """The device-resident state of one run (one per process)."""
+    """The device-resident state of one run.  The library holds one grid
+    per process (hit3d_init replaces whatever state was there), so a second
+    instance while one is open is refused rather than left to drive the
+    first one's state."""
 
     def __init__(self, n: int, n_scalars: int = 0, n_les: int = 0, dealias: int = 0):
+        global _open_instance
+        other = _open_instance() if _open_instance is not None else None
+        if other is not None and not other._closed:
+            raise Hit3dError("the library holds one grid per process: close the open Hit3D "
+                             f"(n={other.n}) before opening another")
+        self._closed = True
         self.lib = load()
         self.n = n
         self.n_scalars = n_scalars
@@ -185,6 +198,8 @@ class Hit3D:
         self.nf = 3 + n_scalars + n_les
         self.nw = max(6, 3 + n_scalars + n_les + 2)
         _check(self.lib.hit3d_init(_i(n), _i(n), _i(n), _i(n_scalars), _i(n_les), _i(dealias)))
+        self._closed = False
+        _open_instance = weakref.ref(self)
 
     @property
     def kmax(self) -> int:
@@ -193,6 +208,7 @@ class Hit3D:
         return int(np.floor(float(self.n) / 3.0 * np.sqrt(2.0)))
 
     def close(self):
+        self._closed = True
         _check(self.lib.hit3d_finalize())
 
     # -- parameters and staging ---------------------------------------------

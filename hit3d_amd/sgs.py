@@ -215,6 +215,24 @@ def train(ds: Dataset, steps: int = 2000, batch: int = 65536, lr: float = 1e-3, 
     return Trained(p, norm, losses, val_loss, smag_val)
 
 
+def a_priori(tr: Trained, ds: Dataset, cs: float = 0.17) -> tuple:
+    """Standardised MSE of the trained closure and of Smagorinsky on a whole
+    dataset (for data the closure was not trained on), with the training
+    set's scaling."""
+    xm, xs, ym, ys = tr.norm
+    Xn = ((ds.X - xm) / xs).astype(np.float32)
+    Tn = ((ds.T - ym) / ys).astype(np.float32)
+    h = Hit3D(16)
+    try:
+        h.mlp_set_params(tr.params)
+        h.mlp_set_norm(xm, xs, ym, ys)
+        Y = h.mlp_forward(Xn)
+    finally:
+        h.close()
+    sm = (smagorinsky_tau(ds.X, ds.delta, cs) - ym) / ys
+    return float(np.mean((Y - Tn) ** 2)), float(np.mean((sm - Tn) ** 2))
+
+
 def les_run(ubar0: np.ndarray, model: str, steps: int, trained: Trained | None = None,
             cs: float = 0.17, re: float = 800.0, sample_every: int = 50, dt0: float = 2e-3):
     """Forced coarse LES from the filtered field ubar0 (physical, [3, m, m, m])
@@ -268,6 +286,50 @@ def spectrum_error(ek_les: np.ndarray, ek_ref: np.ndarray, kmin: int = 1) -> flo
     return float(np.mean(np.abs(a - b) / b))
 
 
+def judge_windows(a, snaps: list, m: int, delta: float, t_dns: float) -> dict:
+    """The closure trained on the first window's snapshots, then for every
+    window: a-priori errors on that window's data (held out in time for the
+    later windows) and 64^3 LES runs with the closure and with Smagorinsky
+    from the window's last filtered snapshot, their time-averaged E(k)
+    against the window's time-averaged filtered DNS."""
+    kmax = m // 3
+    wins = [snaps[w * a.snaps:(w + 1) * a.snaps] for w in range(a.windows)]
+    sets = [features(ws, m, delta) for ws in wins]
+    t0 = time.time()
+    tr = train(sets[0], a.train_steps, a.batch, a.lr)
+    log(f"[sgs] trained on window 1: validation {tr.val_loss:.4f} vs Smagorinsky "
+        f"{tr.smag_val_loss:.4f} ({time.time() - t0:.1f}s)")
+    out = []
+    for w, (ws, ds) in enumerate(zip(wins, sets)):
+        ek_ref = np.mean(ds.ek_filtered, axis=0)
+        if w == 0:
+            ap_mlp, ap_smag = tr.val_loss, tr.smag_val_loss   # the held-out 10 %
+        else:
+            ap_mlp, ap_smag = a_priori(tr, ds, a.cs)
+        row = {"window": w + 1, "t_first": ws[0].time, "t_last": ws[-1].time,
+               "a_priori_mlp": ap_mlp, "a_priori_smagorinsky": ap_smag}
+        for model in ("smag", "mlp"):
+            try:
+                ek, energy, fb = les_run(ws[-1].ubar, model, a.les_steps, tr, a.cs, a.re)
+                row[f"ek_err_{model}"] = spectrum_error(ek[:kmax], ek_ref[:kmax])
+                row[f"fallback_steps_{model}"] = int(fb)
+            except FloatingPointError as e:
+                row[f"ek_err_{model}"] = None
+                row[f"error_{model}"] = str(e)
+        log(f"[sgs] window {w + 1} (t {row['t_first']:.2f}-{row['t_last']:.2f}): a priori "
+            f"{ap_mlp:.3f} vs {ap_smag:.3f}; a posteriori {row.get('ek_err_mlp')} vs "
+            f"{row.get('ek_err_smag')}")
+        out.append(row)
+    wins_post = sum(1 for r in out if r.get("ek_err_mlp") is not None and r.get("ek_err_smag")
+                    is not None and r["ek_err_mlp"] < r["ek_err_smag"])
+    wins_prio = sum(1 for r in out if r["a_priori_mlp"] < r["a_priori_smagorinsky"])
+    return {"dns": a.dns, "ratio": a.ratio, "les_grid": m, "windows": out,
+            "closure_better_a_priori": wins_prio, "closure_better_a_posteriori": wins_post,
+            "train_steps": a.train_steps, "batch": a.batch, "snaps_per_window": a.snaps,
+            "every": a.every, "spinup": a.spinup, "les_steps": a.les_steps,
+            "dns_seconds": t_dns}
+
+
 def main(argv=None):
     """Train the closure on box-filtered DNS and compare the coarse LES with
     it and with Smagorinsky against the filtered-DNS spectrum; prints one
@@ -287,12 +349,19 @@ def main(argv=None):
     ap.add_argument("--les-steps", type=int, default=5000)
     ap.add_argument("--cs", type=float, default=0.17)
     ap.add_argument("--out", default="")
+    ap.add_argument("--windows", type=int, default=1,
+                    help="consecutive DNS windows of --snaps snapshots each: the closure is "
+                         "trained on the first, then judged a priori and a posteriori against "
+                         "every window's own filtered DNS")
     a = ap.parse_args(argv)
     m = a.dns // a.ratio
     delta = 2.0 * math.pi / m
     t0 = time.time()
-    snaps = run_dns(a.dns, a.ratio, a.spinup, a.snaps, a.every, a.re)
+    snaps = run_dns(a.dns, a.ratio, a.spinup, a.snaps * a.windows, a.every, a.re)
     t_dns = time.time() - t0
+    if a.windows > 1:
+        print(json.dumps(judge_windows(a, snaps, m, delta, t_dns)), flush=True)
+        return
     ds = features(snaps, m, delta)
     ek_ref = np.mean(ds.ek_filtered, axis=0)
     log(f"[sgs] dataset {ds.X.shape[0]} samples on {m}^3 ({t_dns:.0f}s of DNS)")

@@ -134,6 +134,23 @@ def test_step_rejects_bad_precision():
     h.close()
 
 
+def test_second_open_instance_refused():
+    """One grid per process: a second Hit3D while one is open is refused
+    (it would otherwise re-initialise the state the first one drives), and
+    allowed once the first is closed."""
+    h = hit3d_amd.Hit3D(16)
+    try:
+        with pytest.raises(hit3d_amd.Hit3dError, match="one grid per process"):
+            hit3d_amd.Hit3D(32)
+        assert h.n == 16
+        h.set_physics(0.01)
+        h.step(0.001, True, True)
+    finally:
+        h.close()
+    h2 = hit3d_amd.Hit3D(32)
+    h2.close()
+
+
 def test_mixed_rhs_old_storage_and_transitions():
     """The mixed step keeps rhs_old(1:3) in FP32 between steps.  Read back
     after a mixed step it holds FP32-representable values and zeros outside
