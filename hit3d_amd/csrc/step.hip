@@ -497,6 +497,179 @@ __global__ __launch_bounds__((XP<N, T>::THREADS)) void k_xp(SG g, W6<T> Wb,
     }
 }
 
+// ---------------------------------------------------------------- C (deviatoric, paired rows)
+// The deviatoric x pass over two rows A and B per thread group.  Two real
+// rows share one complex transform, and k_xp leaves two such slots half
+// used per row: w's inverse and the forward transform of D23 = v w, the
+// fifth product.  Here w_A + i w_B is one inverse transform and
+// D23_A + i D23_B one forward transform, so the two rows take 8 transforms
+// instead of 10 (each row: u + i v inverse, (D11, D12) and (D13, D22)
+// forward).  w_B waits in registers while row A is processed, D23_A while
+// row B is.  Rows B are ROWS after rows A; a block covers 2 ROWS rows.
+// The two rows in one kernel cost registers beyond those two arrays (FP64
+// at 512^3: 244 VGPRs, two wavefronts per SIMD, against 162 and three for
+// k_xp; forced to three it spills), so the pairing pays only where k_xp's
+// extra wavefront is worth less than a fifth of the transforms.
+template <int N, typename T>
+struct XP2 {
+    static constexpr int E = XP<N, T>::E;
+    static constexpr int TPF = XP<N, T>::TPF;
+    static constexpr int ROWS = (256 / TPF < N * N / 2) ? 256 / TPF : N * N / 2;
+    static constexpr int THREADS = ROWS * TPF;
+};
+
+template <int N, typename T>
+__global__ __launch_bounds__((XP2<N, T>::THREADS)) void k_xp_pair(SG g, W6<T> Wb,
+                                                                const cpx<double>* __restrict__ twd,
+                                                                T norm, double* __restrict__ blockmax) {
+    using C = XP2<N, T>;
+    constexpr int E = C::E, TPF = C::TPF;
+    constexpr int PS = kSwizzle;
+    constexpr int FS = padded_size<PS>(N);
+    constexpr int NK = Kc<N>::NK;
+    static_assert(64 % TPF == 0 || TPF % 64 == 0, "a row is part of a wavefront or whole ones");
+    constexpr bool WAVE = TPF <= 64;
+    __shared__ cpx<T> lds[C::ROWS][FS];
+    __shared__ double redm[C::THREADS / 64 + 1];
+    constexpr bool CACHE = E == 8 && N >= 64 && N == (1 << (3 * stage_index(N, 8)));
+    constexpr bool LC = CACHE && sizeof(T) == 8;
+    using TC = std::conditional_t<CACHE, std::conditional_t<LC, TwLds<T, N, E>, TwCache<T, N, E>>,
+                                  void>;
+    __shared__ cpx<T> twl[LC ? TwLds<T, N, E>::SIZE : 1];
+    const int t = threadIdx.x % TPF;
+    const int rr = threadIdx.x / TPF;
+    const int64_t rowA = (int64_t)blockIdx.x * (2 * C::ROWS) + rr;
+    const int64_t rowB = rowA + C::ROWS;
+    const int partner = (TPF - t) & (TPF - 1);
+    cpx<T>* L = lds[rr];
+    TwCache<T, N, E> tcache;
+    TwLds<T, N, E> twv{twl, t};
+    const TC* tcp = nullptr;
+    if constexpr (LC) {
+        if (rr == 0) {
+            TwCache<T, N, E> tcv;
+            fill_twiddles<N, E>(tcv, t, twd + N);
+#pragma unroll
+            for (int si = 0; si < TwCache<T, N, E>::NS; ++si)
+#pragma unroll
+                for (int m = 1; m < E; ++m) twl[(si * E + m) * TPF + t] = tcv.w[si][m];
+        }
+        __syncthreads();
+        tcp = &twv;
+    } else if constexpr (CACHE) {
+        fill_twiddles<N, E>(tcache, t, twd + N);
+        tcp = &tcache;
+    }
+    // inverse transform of two Hermitian half spectra a, b as one complex
+    // row a + i b; the real results come back as (re, im)
+    auto inverse_pair = [&](const T* pa, const T* pb, int64_t ra, int64_t rb, T(&ra_out)[E],
+                            T(&rb_out)[E]) {
+        const cpx<T>* A = reinterpret_cast<const cpx<T>*>(pa) + ra * g.PA;
+        const cpx<T>* B = reinterpret_cast<const cpx<T>*>(pb) + rb * g.PA;
+        cpx<T> z[1][E];
+#pragma unroll
+        for (int s = 0; s < E; ++s) {
+            const cpx<T> a = herm_load<N, TPF, NK>(A, t, s), b = herm_load<N, TPF, NK>(B, t, s);
+            z[0][s] = cpx<T>{a.x - b.y, a.y + b.x};
+        }
+        lds_sync<WAVE>();  // the previous transform's readers are done
+        multi_fft<N, E, 1, T, 1, 1, FS, PS, WAVE, true, double, TC>(z, t, L, twd + N, tcp);
+#pragma unroll
+        for (int s = 0; s < E; ++s) {
+            ra_out[s] = z[0][s].x * norm;
+            rb_out[s] = z[0][s].y * norm;
+        }
+    };
+    // forward transform of two real rows p + i q, split by the Hermitian
+    // symmetry into the half spectra stored at outp and outq
+    auto forward_pair = [&](cpx<T>(&z)[1][E], cpx<T>* outp, cpx<T>* outq) {
+        lds_sync<WAVE>();
+        multi_fft<N, E, -1, T, 1, 1, FS, PS, WAVE, true, double, TC>(z, t, L, twd + N, tcp);
+        if constexpr (WAVE) {
+#pragma unroll
+            for (int s = 0; s < E; ++s) {
+                if (s * TPF < NK) {  // the same for every lane
+                    const cpx<T> mine = (t == 0) ? z[0][(E - s) & (E - 1)] : z[0][E - 1 - s];
+                    const cpx<T> zm = conj(cpx<T>{__shfl(mine.x, partner, TPF),
+                                                  __shfl(mine.y, partner, TPF)});
+                    const int k = t + s * TPF;
+                    if (k < NK) {
+                        const cpx<T> zk = z[0][s];
+                        st_g<kStream<N>>(&outp[k], scale(zk + zm, (T)0.5));
+                        st_g<kStream<N>>(&outq[k], scale(mul_mi(zk - zm), (T)0.5));
+                    }
+                }
+            }
+        } else {
+            lds_sync<false>();
+#pragma unroll
+            for (int s = 0; s < E; ++s) L[padi<PS>(t + s * TPF)] = z[0][s];
+            lds_sync<false>();
+#pragma unroll
+            for (int s = 0; s < E; ++s) {
+                const int k = t + s * TPF;
+                if (s * TPF < NK && k < NK) {
+                    const cpx<T> zm = conj(L[padi<PS>((N - k) & (N - 1))]);
+                    const cpx<T> zk = z[0][s];
+                    st_g<kStream<N>>(&outp[k], scale(zk + zm, (T)0.5));
+                    st_g<kStream<N>>(&outq[k], scale(mul_mi(zk - zm), (T)0.5));
+                }
+            }
+        }
+    };
+    double m = 0.0;
+    // one row: u + i v, the Courant max, (D11, D12) and (D13, D22); D23 = v w
+    // is left in d23
+    auto one_row = [&](int64_t row, const T(&w)[E], T(&d23)[E]) {
+        T u[E], v[E];
+        inverse_pair(Wb.p[0], Wb.p[1], row, row, u, v);
+#pragma unroll
+        for (int s = 0; s < E; ++s) {
+            const double a = ((double)fabs(u[s]) + (double)fabs(v[s])) + (double)fabs(w[s]);
+            m = a > m ? a : m;
+        }
+#pragma unroll
+        for (int pr = 0; pr < 2; ++pr) {
+            cpx<T> z[1][E];
+#pragma unroll
+            for (int s = 0; s < E; ++s) {
+                const T t3 = ((u[s] * u[s] + v[s] * v[s]) + w[s] * w[s]) * (T)(1.0 / 3.0);
+                if (pr == 0) z[0][s] = cpx<T>{u[s] * u[s] - t3, u[s] * v[s]};
+                if (pr == 1) {
+                    z[0][s] = cpx<T>{u[s] * w[s], v[s] * v[s] - t3};
+                    d23[s] = v[s] * w[s];  // u, v, w die here
+                }
+            }
+            forward_pair(z, reinterpret_cast<cpx<T>*>(Wb.p[2 * pr]) + row * g.PA,
+                         reinterpret_cast<cpx<T>*>(Wb.p[2 * pr + 1]) + row * g.PA);
+        }
+    };
+    T wa[E], wb[E], da[E], db[E];
+    inverse_pair(Wb.p[2], Wb.p[2], rowA, rowB, wa, wb);
+    one_row(rowA, wa, da);
+    one_row(rowB, wb, db);
+    {
+        cpx<T> z[1][E];
+#pragma unroll
+        for (int s = 0; s < E; ++s) z[0][s] = cpx<T>{da[s], db[s]};
+        cpx<T>* out = reinterpret_cast<cpx<T>*>(Wb.p[4]);
+        forward_pair(z, out + rowA * g.PA, out + rowB * g.PA);
+    }
+    // block max -> one partial per block
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double q = __shfl_down(m, o, 64);
+        m = q > m ? q : m;
+    }
+    if ((threadIdx.x & 63) == 0) redm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double mm = 0.0;
+        for (int i = 0; i < (C::THREADS + 63) / 64; ++i) mm = redm[i] > mm ? redm[i] : mm;
+        blockmax[blockIdx.x] = mm;
+    }
+}
+
 // ---------------------------------------------------------------- C (rotational)
 // The x pass of the rotational form: three inverse transforms of row pairs
 // (u + i v, w + i omega_1, omega_2 + i omega_3), the Courant max, the cross
@@ -981,14 +1154,24 @@ static void run(Lib& L, const double* dtp, bool first, const cpx<T>* tw) {
     {
         Timed tm(TC_STEP_X);
         const T norm = (T)(1.0 / ((double)N * (double)N * (double)N));
-        const int64_t nb = (int64_t)N * N / XP<N, T>::ROWS;  // <= n*n, the size of dmax
+        // the deviatoric form's paired rows: FP64 up to 512^3 (1.45 -> 1.38 ms
+        // there at two wavefronts per SIMD, where k_xp runs three); at 1024^3
+        // the paired kernel needs all 256 registers (one wavefront per SIMD,
+        // 17.2 -> 22.3 ms), and in FP32 it gained nothing
+        constexpr bool PAIR = FORM == 2 && sizeof(T) == 8 && N <= 512;
+        // blocks <= n*n, the size of dmax
+        const int64_t nb = PAIR ? (int64_t)N * N / (2 * XP2<N, T>::ROWS)
+                                : (int64_t)N * N / XP<N, T>::ROWS;
         // FP64 twiddles in both forms (the FP32 form's recurrence in FP64)
         if constexpr (ROT)
             hipLaunchKernelGGL((k_xp_rot<N, T>), dim3(nb), dim3(XP<N, T>::THREADS), 0, L.st, g,
                                Wb, L.tw64, norm, L.dmax);
+        else if constexpr (PAIR)
+            hipLaunchKernelGGL((k_xp_pair<N, T>), dim3(nb), dim3(XP2<N, T>::THREADS), 0, L.st, g,
+                               Wb, L.tw64, norm, L.dmax);
         else
-            hipLaunchKernelGGL((k_xp<N, T, FORM == 2>), dim3(nb), dim3(XP<N, T>::THREADS), 0, L.st,
-                               g, Wb, L.tw64, norm, L.dmax);
+            hipLaunchKernelGGL((k_xp<N, T, FORM == 2>), dim3(nb), dim3(XP<N, T>::THREADS), 0,
+                               L.st, g, Wb, L.tw64, norm, L.dmax);
         launch_max_fold(L.dmax, nb, L.dscal, L.st);
     }
     {

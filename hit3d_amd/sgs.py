@@ -279,6 +279,48 @@ def les_run(ubar0: np.ndarray, model: str, steps: int, trained: Trained | None =
     return np.mean(eks[half:], axis=0), np.array(energy), fb
 
 
+def les_matched(ubar0: np.ndarray, model: str, times: list, trained: Trained | None = None,
+                cs: float = 0.17, re: float = 800.0, dt0: float = 2e-3) -> list:
+    """Forced coarse LES from the filtered field ubar0 for a given span of
+    time: E(k) (shells 1..kmax) each time the LES clock first reaches one of
+    `times` (offsets from the start, increasing), to set beside the filtered
+    DNS at the same times."""
+    m = ubar0.shape[1]
+    p = dns_params(m, re)
+    h = Hit3D(m)
+    out = []
+    try:
+        h.set_physics(p.nu, flow_type=1, kfmax=p.kfmax, famp=p.famp)
+        for c in range(3):
+            a = np.zeros((m, m, m + 2))
+            a[:, :, :m] = ubar0[c]
+            h.put(WRK, 1, a)
+            h.xFFT3d(1, 1)
+            h.copy(FIELDS, c + 1, WRK, 1)
+        h.dealias_all()
+        h.pressure()
+        if model == "mlp":
+            h.mlp_set_params(trained.params)
+            h.mlp_set_norm(*trained.norm)
+            h.set_les(11, cs)
+        else:
+            h.set_les(1, cs)
+        dt, fov, fos, t, i = dt0, True, True, 0.0, 0
+        for target in times:
+            while t < target:
+                courant, _, fov, fos = h.step(dt, fov, fos)
+                t += dt
+                dt = my_dt(dt, courant)
+                i += 1
+                if not np.isfinite(courant) or courant > 10.0:
+                    raise FloatingPointError(f"LES ({model}) diverged at step {i}")
+            e, _ = h.stats()
+            out.append(e)
+    finally:
+        h.close()
+    return out
+
+
 def spectrum_error(ek_les: np.ndarray, ek_ref: np.ndarray, kmin: int = 1) -> float:
     """Mean relative E(k) error over shells kmin..kmax of the LES."""
     k = min(len(ek_les), len(ek_ref))
@@ -308,6 +350,7 @@ def judge_windows(a, snaps: list, m: int, delta: float, t_dns: float) -> dict:
             ap_mlp, ap_smag = a_priori(tr, ds, a.cs)
         row = {"window": w + 1, "t_first": ws[0].time, "t_last": ws[-1].time,
                "a_priori_mlp": ap_mlp, "a_priori_smagorinsky": ap_smag}
+        offsets = [sn.time - ws[0].time for sn in ws[1:]]
         for model in ("smag", "mlp"):
             try:
                 ek, energy, fb = les_run(ws[-1].ubar, model, a.les_steps, tr, a.cs, a.re)
@@ -316,15 +359,29 @@ def judge_windows(a, snaps: list, m: int, delta: float, t_dns: float) -> dict:
             except FloatingPointError as e:
                 row[f"ek_err_{model}"] = None
                 row[f"error_{model}"] = str(e)
+            # time-matched: from the window's first filtered snapshot, E(k) at
+            # the times of the others against theirs
+            try:
+                eks = les_matched(ws[0].ubar, model, offsets, tr, a.cs, a.re)
+                row[f"ek_err_matched_{model}"] = float(np.mean(
+                    [spectrum_error(e[:kmax], r[:kmax]) for e, r in zip(eks, ds.ek_filtered[1:])]))
+            except FloatingPointError as e:
+                row[f"ek_err_matched_{model}"] = None
+                row[f"error_matched_{model}"] = str(e)
         log(f"[sgs] window {w + 1} (t {row['t_first']:.2f}-{row['t_last']:.2f}): a priori "
             f"{ap_mlp:.3f} vs {ap_smag:.3f}; a posteriori {row.get('ek_err_mlp')} vs "
-            f"{row.get('ek_err_smag')}")
+            f"{row.get('ek_err_smag')}; time-matched {row.get('ek_err_matched_mlp')} vs "
+            f"{row.get('ek_err_matched_smag')}")
         out.append(row)
     wins_post = sum(1 for r in out if r.get("ek_err_mlp") is not None and r.get("ek_err_smag")
                     is not None and r["ek_err_mlp"] < r["ek_err_smag"])
     wins_prio = sum(1 for r in out if r["a_priori_mlp"] < r["a_priori_smagorinsky"])
+    wins_matched = sum(1 for r in out if r.get("ek_err_matched_mlp") is not None
+                       and r.get("ek_err_matched_smag") is not None
+                       and r["ek_err_matched_mlp"] < r["ek_err_matched_smag"])
     return {"dns": a.dns, "ratio": a.ratio, "les_grid": m, "windows": out,
             "closure_better_a_priori": wins_prio, "closure_better_a_posteriori": wins_post,
+            "closure_better_time_matched": wins_matched,
             "train_steps": a.train_steps, "batch": a.batch, "snaps_per_window": a.snaps,
             "every": a.every, "spinup": a.spinup, "les_steps": a.les_steps,
             "dns_seconds": t_dns}
