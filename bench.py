@@ -492,7 +492,8 @@ def main():
                      "ek_max_rel_err_vs_fp64": float(np.max(np.abs(e32[sig] - e64[sig]) / e64[sig])),
                      "energy_rel_err_vs_fp64": abs(st32.energy - st64.energy) / st64.energy,
                      "eps_rel_err_vs_fp64": abs(st32.eps_v - st64.eps_v) / st64.eps_v,
-                     "after_steps": a.steps + a.warmup}
+                     "after_steps": a.steps + a.warmup,
+                     "kernel_ms_per_step": {k: v[0] / a.steps for k, v in tim32.items() if v[1]}}
             if a.accuracy_steps > 0:
                 mixed["accuracy"] = accuracy_run(n, a.accuracy_steps, 10, form)
         except Exception as e:  # reported, never silently replaced

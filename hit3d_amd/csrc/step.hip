@@ -670,6 +670,225 @@ __global__ __launch_bounds__((XP2<N, T>::THREADS)) void k_xp_pair(SG g, W6<T> Wb
     }
 }
 
+// ---------------------------------------------------------------- C (half-length rows)
+// The x pass with one real row per complex transform of half the length:
+// row x_0..x_{N-1} travels as z_m = x_{2m} + i x_{2m+1}, m < M = N/2.  At
+// N = 1024 this puts a row's transforms (M = 512, radix 8) inside one
+// wavefront, synchronised per wavefront, with the shared twiddle table of
+// the 512-point transforms; k_xp's 1024-point row transforms span two
+// wavefronts with block barriers at every exchange and ran at 0.33 of the
+// HBM bandwidth.  Per row: 3 inverse and NOUT forward M-point transforms.
+//   inverse (c2r of X_k, k <= K): Z_k = (X_k + conj X_{M-k})
+//            + i e^{+2 pi i k/N} (X_k - conj X_{M-k}), z = M-point inverse of Z;
+//   forward (r2c): Z = M-point transform of z,
+//            X_k = (Z_k + conj Z_{M-k})/2 - i e^{-2 pi i k/N} (Z_k - conj Z_{M-k})/2.
+// Entry k of a transform sits in lane t, slot s (k = t + s*TPF) and M-k in
+// lane (TPF-t) mod TPF, slot E-1-s (slot (E-s) mod E for t = 0), so the
+// forward split is a lane shuffle; the inverse loads X_{M-k} directly.
+// Products in buffer order: divergence form uu, uv, uw, vv, vw, ww; DEV (the
+// trace-free tensor) D11, D12, D13, D22, D23.  The e^{-2 pi i k/N} factors
+// are one table entry per thread times constants per slot, in FP64.
+template <int N, typename T>
+struct XH {
+    static constexpr int M = N / 2;
+    static constexpr int E = 8;
+    static constexpr int TPF = M / E;
+    static constexpr int ROWS = 4;
+    static constexpr int THREADS = ROWS * TPF;
+};
+
+template <int N, typename T, bool DEV>
+__global__ __launch_bounds__((XH<N, T>::THREADS)) void k_xh(SG g, W6<T> Wb,
+                                                          const cpx<double>* __restrict__ twd,
+                                                          T norm, double* __restrict__ blockmax) {
+    using C = XH<N, T>;
+    constexpr int M = C::M, E = C::E, TPF = C::TPF;
+    static_assert(N == 1024 && TPF == 64, "one row per wavefront; slot factors e^{-2 pi i s/16}");
+    constexpr int PS = kSwizzle;
+    constexpr int FS = padded_size<PS>(M);
+    constexpr int K = Kc<N>::K;
+    static_assert(K < M, "the Nyquist entry is outside the kept range");
+    // the shared twiddles in a per-block LDS table, in both precisions
+    // (FP32 in registers: 160 VGPRs, three wavefronts per SIMD; in LDS: 128,
+    // four)
+    using TC = TwLds<T, M, E>;
+    __shared__ cpx<T> lds[C::ROWS][FS];
+    __shared__ cpx<T> twl[TwLds<T, M, E>::SIZE];
+    __shared__ double redm[C::ROWS];
+    const int t = threadIdx.x % TPF;
+    const int rr = threadIdx.x / TPF;
+    const int64_t row = (int64_t)blockIdx.x * C::ROWS + rr;
+    const int partner = (TPF - t) & (TPF - 1);
+    cpx<T>* L = lds[rr];
+    TwLds<T, M, E> twv{twl, t};
+    const TC* tcp = &twv;
+    if (rr == 0) {
+        TwCache<T, M, E> tcv;
+        fill_twiddles<M, E>(tcv, t, twd + M);
+#pragma unroll
+        for (int si = 0; si < TwCache<T, M, E>::NS; ++si)
+#pragma unroll
+            for (int m = 1; m < E; ++m) twl[(si * E + m) * TPF + t] = tcv.w[si][m];
+    }
+    __syncthreads();
+    // e^{-2 pi i k/N} for k = t + s*TPF: a = e^{-2 pi i t/N} times the
+    // constant e^{-2 pi i s/16} (TPF/N = 1/16)
+    const cpx<double> at = twd[N + t];
+    auto wk = [&](int s) -> cpx<double> {
+        constexpr double c[8] = {1.0,
+                                 0.92387953251128675612818318939678828682,
+                                 0.70710678118654752440084436210484903928,
+                                 0.38268343236508977172845998403039886676,
+                                 0.0,
+                                 -0.38268343236508977172845998403039886676,
+                                 -0.70710678118654752440084436210484903928,
+                                 -0.92387953251128675612818318939678828682};
+        // e^{-2 pi i s/16} = cos(pi s/8) - i sin(pi s/8); sin(pi s/8) = cos(pi (4-s)/8)
+        constexpr double sn[8] = {0.0,
+                                  0.38268343236508977172845998403039886676,
+                                  0.70710678118654752440084436210484903928,
+                                  0.92387953251128675612818318939678828682,
+                                  1.0,
+                                  0.92387953251128675612818318939678828682,
+                                  0.70710678118654752440084436210484903928,
+                                  0.38268343236508977172845998403039886676};
+        // recomputed at every use: kept across the transforms, the eight
+        // factors would hold 32 registers for the whole kernel
+        cpx<double> a = at;
+        asm volatile("" : "+v"(a.x), "+v"(a.y));
+        const int s4 = s & 3;
+        const cpx<double> r = s4 == 0 ? a : cmul(a, cpx<double>{c[s4], -sn[s4]});
+        return s >= 4 ? mul_mi(r) : r;  // e^{-2 pi i (s+4)/16} = -i e^{-2 pi i s/16}
+    };
+    // inverse: half spectrum of buffer b -> x_{2m} + i x_{2m+1} (times norm)
+    auto inverse = [&](int b, cpx<T>(&z)[1][E]) {
+        const cpx<T>* X = reinterpret_cast<const cpx<T>*>(Wb.p[b]) + row * g.PA;
+#pragma unroll
+        for (int s = 0; s < E; ++s) {
+            const int k = t + s * TPF;
+            // A = X_k (k <= K; the imaginary part of X_0 is ignored, as by
+            // the reference's c2r), B = conj X_{M-k} (M-k <= K); branch-free
+            // loads from in-bounds addresses, slots with no live lane load
+            // nothing (constant once unrolled)
+            cpx<T> A{(T)0, (T)0}, B{(T)0, (T)0};
+            if (s * TPF <= K) {
+                const bool live = k <= K;
+                const cpx<T> a = X[live ? k : 0];
+                A = cpx<T>{live ? a.x : (T)0, (live && k != 0) ? a.y : (T)0};
+            }
+            if ((s + 1) * TPF - 1 >= M - K) {
+                const int km = M - k;
+                const bool live = km <= K;
+                const cpx<T> a = X[live ? km : 0];
+                B = cpx<T>{live ? a.x : (T)0, live ? -a.y : (T)0};
+            }
+            const cpx<double> w = wk(s);
+            const cpx<T> d = A - B;
+            // i conj(w) d
+            const cpx<T> wc{(T)w.x, (T)-w.y};
+            const cpx<T> r = mul_i(cmul(wc, d));
+            z[0][s] = (A + B) + r;
+        }
+        multi_fft<M, E, 1, T, 1, 1, FS, PS, true, true, double, TC>(z, t, L, twd + M, tcp);
+#pragma unroll
+        for (int s = 0; s < E; ++s) z[0][s] = scale(z[0][s], norm);
+    };
+    // forward: z (x_{2m} + i x_{2m+1}) -> half spectrum k <= K into buffer b
+    auto forward = [&](cpx<T>(&z)[1][E], int b) {
+        lds_sync<true>();  // the previous transform's readers are done
+        multi_fft<M, E, -1, T, 1, 1, FS, PS, true, true, double, TC>(z, t, L, twd + M, tcp);
+        cpx<T>* out = reinterpret_cast<cpx<T>*>(Wb.p[b]) + row * g.PA;
+#pragma unroll
+        for (int s = 0; s < E; ++s) {
+            if (s * TPF <= K) {  // the same for every lane
+                const cpx<T> mine = (t == 0) ? z[0][(E - s) & (E - 1)] : z[0][E - 1 - s];
+                const cpx<T> zm = conj(cpx<T>{__shfl(mine.x, partner, TPF),
+                                              __shfl(mine.y, partner, TPF)});
+                const int k = t + s * TPF;
+                if (k <= K) {
+                    const cpx<T> zk = z[0][s];
+                    const cpx<double> w = wk(s);
+                    const cpx<T> wt{(T)w.x, (T)w.y};
+                    // (zk + zm)/2 - i w (zk - zm)/2
+                    const cpx<T> o = mul_mi(cmul(wt, zk - zm));
+                    st_g<kStream<N>>(&out[k], scale((zk + zm) + o, (T)0.5));
+                }
+            }
+        }
+    };
+    cpx<T> u[1][E], v[1][E], w[1][E];
+    inverse(0, u);
+    lds_sync<true>();
+    inverse(1, v);
+    lds_sync<true>();
+    inverse(2, w);
+    // Courant number, rhs_velocity.f90:31-32 (FP64 max)
+    double m = 0.0;
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+        const double a0 = ((double)fabs(u[0][s].x) + (double)fabs(v[0][s].x)) + (double)fabs(w[0][s].x);
+        const double a1 = ((double)fabs(u[0][s].y) + (double)fabs(v[0][s].y)) + (double)fabs(w[0][s].y);
+        m = a0 > m ? a0 : m;
+        m = a1 > m ? a1 : m;
+    }
+    // products of the two real values each entry holds (rhs_velocity.f90:56-72)
+    auto prod = [&](int which, cpx<T>(&z)[1][E]) {
+#pragma unroll
+        for (int s = 0; s < E; ++s) {
+            const cpx<T> a = u[0][s], b = v[0][s], c = w[0][s];
+            cpx<T> r;
+            if constexpr (DEV) {
+                // D11 = u u - |u|^2/3, D22 = v v - |u|^2/3 (same grouping as k_xp)
+                if (which == 0 || which == 3) {
+                    // recomputed for D22: kept from D11, the trace held
+                    // registers across a transform (FP32: 179 -> 128 VGPRs)
+                    T ax = a.x, ay = a.y, bx = b.x, by = b.y, cx = c.x, cy = c.y;
+                    asm volatile("" : "+v"(ax), "+v"(ay), "+v"(bx), "+v"(by), "+v"(cx), "+v"(cy));
+                    const T t3x = ((ax * ax + bx * bx) + cx * cx) * (T)(1.0 / 3.0);
+                    const T t3y = ((ay * ay + by * by) + cy * cy) * (T)(1.0 / 3.0);
+                    r = which == 0 ? cpx<T>{a.x * a.x - t3x, a.y * a.y - t3y}
+                                   : cpx<T>{b.x * b.x - t3x, b.y * b.y - t3y};
+                }
+                if (which == 1) r = cpx<T>{a.x * b.x, a.y * b.y};
+                if (which == 2) r = cpx<T>{a.x * c.x, a.y * c.y};
+                if (which == 4) r = cpx<T>{b.x * c.x, b.y * c.y};
+            } else {
+                if (which == 0) r = cpx<T>{a.x * a.x, a.y * a.y};
+                if (which == 1) r = cpx<T>{a.x * b.x, a.y * b.y};
+                if (which == 2) r = cpx<T>{a.x * c.x, a.y * c.y};
+                if (which == 3) r = cpx<T>{b.x * b.x, b.y * b.y};
+                if (which == 4) r = cpx<T>{b.x * c.x, b.y * c.y};
+                if (which == 5) r = cpx<T>{c.x * c.x, c.y * c.y};
+            }
+            z[0][s] = r;
+        }
+    };
+    // order: the products of u first, so that u dies early
+    constexpr int NOUT = DEV ? 5 : 6;
+    constexpr int order_dev[5] = {0, 3, 1, 2, 4};
+    constexpr int order_div[6] = {0, 1, 2, 3, 4, 5};
+#pragma unroll
+    for (int i = 0; i < NOUT; ++i) {
+        const int which = DEV ? order_dev[i] : order_div[i];
+        cpx<T> z[1][E];
+        prod(which, z);
+        forward(z, which);
+    }
+    // block max -> one partial per block (folded by the next launch)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double q = __shfl_down(m, o, 64);
+        m = q > m ? q : m;
+    }
+    if ((threadIdx.x & 63) == 0) redm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double mm = 0.0;
+        for (int i = 0; i < C::ROWS; ++i) mm = redm[i] > mm ? redm[i] : mm;
+        blockmax[blockIdx.x] = mm;
+    }
+}
+
 // ---------------------------------------------------------------- C (rotational)
 // The x pass of the rotational form: three inverse transforms of row pairs
 // (u + i v, w + i omega_1, omega_2 + i omega_3), the Courant max, the cross
@@ -1159,11 +1378,17 @@ static void run(Lib& L, const double* dtp, bool first, const cpx<T>* tw) {
         // the paired kernel needs all 256 registers (one wavefront per SIMD,
         // 17.2 -> 22.3 ms), and in FP32 it gained nothing
         constexpr bool PAIR = FORM == 2 && sizeof(T) == 8 && N <= 512;
+        // 1024^3: half-length row transforms inside one wavefront
+        constexpr bool HALF = N == 1024 && !ROT;
         // blocks <= n*n, the size of dmax
-        const int64_t nb = PAIR ? (int64_t)N * N / (2 * XP2<N, T>::ROWS)
-                                : (int64_t)N * N / XP<N, T>::ROWS;
+        const int64_t nb = PAIR   ? (int64_t)N * N / (2 * XP2<N, T>::ROWS)
+                           : HALF ? (int64_t)N * N / XH<N, T>::ROWS
+                                  : (int64_t)N * N / XP<N, T>::ROWS;
         // FP64 twiddles in both forms (the FP32 form's recurrence in FP64)
-        if constexpr (ROT)
+        if constexpr (HALF)
+            hipLaunchKernelGGL((k_xh<N, T, FORM == 2>), dim3(nb), dim3(XH<N, T>::THREADS), 0,
+                               L.st, g, Wb, L.tw64, norm, L.dmax);
+        else if constexpr (ROT)
             hipLaunchKernelGGL((k_xp_rot<N, T>), dim3(nb), dim3(XP<N, T>::THREADS), 0, L.st, g,
                                Wb, L.tw64, norm, L.dmax);
         else if constexpr (PAIR)

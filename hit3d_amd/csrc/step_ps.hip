@@ -43,14 +43,92 @@ struct Q6 {
 };
 
 // ------------------------------------------------------------------ x pass
+// One row of N reals per M = N/2 point complex transform (z_m = x_2m +
+// i x_2m+1), TPF threads per row, every row inside one wavefront: the
+// exchanges synchronise per wavefront and the real-to-complex split pairs
+// entry k (lane t, slot s) with M-k (lane (TPF-t) mod TPF, slot E-1-s, or
+// (E-s) mod E for t = 0) by a lane shuffle.  Twiddles: a per-block LDS
+// table of each thread's radix-8 twiddles where M is a power of 8 (M = 64,
+// 512), else one table load per butterfly and its powers (TWREC).  As block
+// barriers with twiddle loads per butterfly and the split through LDS, the
+// 1024^3 passes ran one barrier per exchange across four rows.
 template <int N>
 struct XR {
     static constexpr int M = N / 2;
-    static constexpr int E = (M < 8) ? M : 8;
+    static constexpr int E = 8;
     static constexpr int TPF = M / E;
     static constexpr int ROWS = (256 / TPF < N * N) ? 256 / TPF : N * N;
     static constexpr int THREADS = ROWS * TPF;
+    static constexpr bool CACHE = M >= 64 && M == (1 << (3 * stage_index(M, 8)));
+    using TC = std::conditional_t<CACHE, TwLds<double, M, E>, void>;
+    static constexpr int TWSIZE = CACHE ? TwLds<double, M, E>::SIZE : 1;
+    static_assert(TPF <= 64 && 64 % TPF == 0, "a row inside one wavefront");
 };
+
+// The per-block twiddle table (first row's threads fill it; every thread
+// of the block must call this: it synchronises the block).
+template <int N>
+__device__ __forceinline__ void fill_row_twiddles(cpx<double>* twl, int t, int r,
+                                                  const cpx<double>* __restrict__ tw) {
+    using C = XR<N>;
+    if constexpr (C::CACHE) {
+        if (r == 0) {
+            TwCache<double, C::M, C::E> tcv;
+            fill_twiddles<C::M, C::E>(tcv, t, tw + C::M);
+#pragma unroll
+            for (int si = 0; si < TwCache<double, C::M, C::E>::NS; ++si)
+#pragma unroll
+                for (int m = 1; m < C::E; ++m) twl[(si * C::E + m) * C::TPF + t] = tcv.w[si][m];
+        }
+        __syncthreads();
+    }
+}
+
+// e^{-2 pi i k/N} for k = t + s*TPF (TPF = N/16): e^{-2 pi i t/N} times the
+// constant e^{-2 pi i s/16}, recomputed at every use (kept, the eight
+// factors would hold 32 registers through the row)
+__device__ __forceinline__ cpx<double> slot_twiddle(cpx<double> at, int s) {
+    constexpr double c[4] = {1.0, 0.92387953251128675612818318939678828682,
+                             0.70710678118654752440084436210484903928,
+                             0.38268343236508977172845998403039886676};
+    constexpr double sn[4] = {0.0, 0.38268343236508977172845998403039886676,
+                              0.70710678118654752440084436210484903928,
+                              0.92387953251128675612818318939678828682};
+    cpx<double> a = at;
+    asm volatile("" : "+v"(a.x), "+v"(a.y));
+    const int s4 = s & 3;
+    const cpx<double> r = s4 == 0 ? a : cmul(a, cpx<double>{c[s4], -sn[s4]});
+    return s >= 4 ? mul_mi(r) : r;  // e^{-2 pi i (s+4)/16} = -i e^{-2 pi i s/16}
+}
+
+// Forward transform of the row z (x_2m + i x_2m+1) and its half spectrum
+// X_k = (Z_k + conj Z_{M-k})/2 - i e^{-2 pi i k/N} (Z_k - conj Z_{M-k})/2,
+// k = 0..M, stored at rowp.
+template <int N>
+__device__ __forceinline__ void row_r2c_store(cpx<double> (&z)[1][XR<N>::E], int t,
+                                              cpx<double>* L, const cpx<double>* __restrict__ tw,
+                                              const typename XR<N>::TC* tcp, cpx<double> at,
+                                              cpx<double>* rowp) {
+    using C = XR<N>;
+    constexpr int M = C::M, E = C::E, TPF = C::TPF;
+    lds_sync<true>();  // the previous transform's readers are done
+    multi_fft<M, E, -1, double, 1, 1, padded_size<kSwizzle>(M), kSwizzle, true, true, double,
+              typename C::TC>(z, t, L, tw + M, tcp);
+    const int partner = (TPF - t) & (TPF - 1);
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+        // lane 0 pairs with itself (slot (E-s) mod E); selecting between the
+        // shuffled value and its own keeps z in registers (a select between
+        // two entries of z became a dynamically indexed array in scratch)
+        const cpx<double> sh{__shfl(z[0][E - 1 - s].x, partner, TPF),
+                             __shfl(z[0][E - 1 - s].y, partner, TPF)};
+        const cpx<double> zm = conj(t == 0 ? z[0][(E - s) & (E - 1)] : sh);
+        const cpx<double> zk = z[0][s];
+        const cpx<double> o = mul_mi(cmul(slot_twiddle(at, s), zk - zm));
+        st_g<kStream<N>>(&rowp[t + s * TPF], scale((zk + zm) + o, 0.5));
+    }
+    if (t == 0) st_g<kStream<N>>(&rowp[M], cpx<double>{z[0][0].x - z[0][0].y, 0.0});
+}
 
 template <int N, bool COURANT>
 __global__ __launch_bounds__(XR<N>::THREADS, 2) void k_xprod6(const double* u, const double* v,
@@ -59,7 +137,8 @@ __global__ __launch_bounds__(XR<N>::THREADS, 2) void k_xprod6(const double* u, c
                                                            double* __restrict__ blockmax) {
     using C = XR<N>;
     constexpr int M = C::M, E = C::E, TPF = C::TPF;
-    __shared__ cpx<double> lds[C::ROWS][M + 1];
+    __shared__ cpx<double> lds[C::ROWS][padded_size<kSwizzle>(M)];
+    __shared__ cpx<double> twl[C::TWSIZE];
     __shared__ double redm[C::THREADS / 64 + 1];
     const int t = threadIdx.x % TPF;
     const int r = threadIdx.x / TPF;
@@ -73,6 +152,11 @@ __global__ __launch_bounds__(XR<N>::THREADS, 2) void k_xprod6(const double* u, c
 #pragma unroll
         for (int s = 0; s < E; ++s) a[c][s] = ld_g<kStream<N>>(&rp[t + s * TPF]);
     }
+    fill_row_twiddles<N>(twl, t, r, tw);
+    TwLds<double, M, E> twv{twl, t};
+    const typename C::TC* tcp = nullptr;
+    if constexpr (C::CACHE) tcp = &twv;
+    const cpx<double> at = tw[N + t];
     double m = 0.0;
     if constexpr (COURANT) {  // rhs_velocity.f90:31-32
 #pragma unroll
@@ -83,41 +167,22 @@ __global__ __launch_bounds__(XR<N>::THREADS, 2) void k_xprod6(const double* u, c
             m = p1 > m ? p1 : m;
         }
     }
-    const cpx<double>* twN = tw + N;
     // products uu, uv, uw, vv, vw, ww (rhs_velocity.f90:160-170)
 #pragma unroll
     for (int p = 0; p < 6; ++p) {
         const int i = p < 3 ? 0 : p < 5 ? 1 : 2;
         const int j = p < 3 ? p : p < 5 ? p - 2 : 2;
-        cpx<double> z[E];
+        cpx<double> z[1][E];
 #pragma unroll
-        for (int s = 0; s < E; ++s) z[s] = cpx<double>{a[i][s].x * a[j][s].x, a[i][s].y * a[j][s].y};
-        block_fft<M, E, -1, double, 1, kSwizzle>(z, t, lds[r], tw + M);
-        __syncthreads();
-#pragma unroll
-        for (int s = 0; s < E; ++s) lds[r][t + s * TPF] = z[s];
-        __syncthreads();
+        for (int s = 0; s < E; ++s)
+            z[0][s] = cpx<double>{a[i][s].x * a[j][s].x, a[i][s].y * a[j][s].y};
         double* op = out.p[0];
         if (p == 1) op = out.p[1];
         if (p == 2) op = out.p[2];
         if (p == 3) op = out.p[3];
         if (p == 4) op = out.p[4];
         if (p == 5) op = out.p[5];
-        cpx<double>* rowp = reinterpret_cast<cpx<double>*>(op) + row * P;
-#pragma unroll
-        for (int s = 0; s < E; ++s) {
-            const int k = t + s * TPF;
-            const cpx<double> zk = lds[r][k];
-            const cpx<double> zm = conj(lds[r][(M - k) & (M - 1)]);
-            const cpx<double> e = scale(zk + zm, 0.5);
-            const cpx<double> o = scale(mul_mi(zk - zm), 0.5);
-            st_g<kStream<N>>(&rowp[k], e + cmul(twN[k], o));
-        }
-        if (t == 0) {
-            const cpx<double> z0 = lds[r][0];
-            st_g<kStream<N>>(&rowp[M], cpx<double>{z0.x - z0.y, 0.0});
-        }
-        __syncthreads();  // the next product's transform rewrites lds
+        row_r2c_store<N>(z, t, lds[r], tw, tcp, at, reinterpret_cast<cpx<double>*>(op) + row * P);
     }
     if constexpr (COURANT) {
 #pragma unroll
@@ -336,7 +401,8 @@ __global__ __launch_bounds__(XR<N>::THREADS) void k_xprod3(const double* phi, co
                                                            const cpx<double>* __restrict__ tw) {
     using C = XR<N>;
     constexpr int M = C::M, E = C::E, TPF = C::TPF;
-    __shared__ cpx<double> lds[C::ROWS][M + 1];
+    __shared__ cpx<double> lds[C::ROWS][padded_size<kSwizzle>(M)];
+    __shared__ cpx<double> twl[C::TWSIZE];
     const int t = threadIdx.x % TPF;
     const int r = threadIdx.x / TPF;
     const int64_t row = (int64_t)blockIdx.x * C::ROWS + r;
@@ -346,40 +412,25 @@ __global__ __launch_bounds__(XR<N>::THREADS) void k_xprod3(const double* phi, co
 #pragma unroll
         for (int s = 0; s < E; ++s) ph[s] = ld_g<kStream<N>>(&rp[t + s * TPF]);
     }
+    fill_row_twiddles<N>(twl, t, r, tw);
+    TwLds<double, M, E> twv{twl, t};
+    const typename C::TC* tcp = nullptr;
+    if constexpr (C::CACHE) tcp = &twv;
+    const cpx<double> at = tw[N + t];
     const double* vel[3] = {u, v, w};
-    const cpx<double>* twN = tw + N;
 #pragma unroll
     for (int p = 0; p < 3; ++p) {
         const cpx<double>* vp = reinterpret_cast<const cpx<double>*>(vel[p]) + row * P;
-        cpx<double> z[E];
+        cpx<double> z[1][E];
 #pragma unroll
         for (int s = 0; s < E; ++s) {
             const cpx<double> b = ld_g<kStream<N>>(&vp[t + s * TPF]);
-            z[s] = cpx<double>{ph[s].x * b.x, ph[s].y * b.y};
+            z[0][s] = cpx<double>{ph[s].x * b.x, ph[s].y * b.y};
         }
-        block_fft<M, E, -1, double, 1, kSwizzle>(z, t, lds[r], tw + M);
-        __syncthreads();
-#pragma unroll
-        for (int s = 0; s < E; ++s) lds[r][t + s * TPF] = z[s];
-        __syncthreads();
         double* op = out.p[0];
         if (p == 1) op = out.p[1];
         if (p == 2) op = out.p[2];
-        cpx<double>* rowp = reinterpret_cast<cpx<double>*>(op) + row * P;
-#pragma unroll
-        for (int s = 0; s < E; ++s) {
-            const int k = t + s * TPF;
-            const cpx<double> zk = lds[r][k];
-            const cpx<double> zm = conj(lds[r][(M - k) & (M - 1)]);
-            const cpx<double> e = scale(zk + zm, 0.5);
-            const cpx<double> o = scale(mul_mi(zk - zm), 0.5);
-            st_g<kStream<N>>(&rowp[k], e + cmul(twN[k], o));
-        }
-        if (t == 0) {
-            const cpx<double> z0 = lds[r][0];
-            st_g<kStream<N>>(&rowp[M], cpx<double>{z0.x - z0.y, 0.0});
-        }
-        __syncthreads();
+        row_r2c_store<N>(z, t, lds[r], tw, tcp, at, reinterpret_cast<cpx<double>*>(op) + row * P);
     }
 }
 
