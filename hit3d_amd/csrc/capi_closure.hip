@@ -313,4 +313,13 @@ int hit3d_mlp_fallbacks(int64_t* count) {
     return guarded([&] { *count = lib().mlp.fallbacks; }, false);
 }
 
+int hit3d_mlp_set_mix(const double* weight) {
+    return guarded([&] {
+        if (!(*weight >= 0.0 && *weight <= 1.0))
+            throw std::runtime_error("mlp: the Smagorinsky weight must lie in [0, 1]");
+        clear_graphs();
+        lib().mlp.mix = *weight;
+    }, false);
+}
+
 }  // extern "C"

@@ -131,7 +131,8 @@ void mlp_velocity() {
     check(hipMemsetAsync(M.bad, 0, sizeof(unsigned int), L.st), "hipMemsetAsync");
     {
         Timed t(TC_POINTWISE);
-        launch_mlp_les(L.g, gr, M.delta, M.norm, M.wb, M.p, tau, M.bad, L.st);
+        const double cd = L.c_smag * L.dx;
+        launch_mlp_les(L.g, gr, M.delta, M.norm, M.wb, M.p, tau, M.bad, M.mix, cd * cd, L.st);
     }
     unsigned int bad = 0;
     check(hipMemcpyAsync(&bad, M.bad, sizeof(bad), hipMemcpyDeviceToHost, L.st), "hipMemcpyAsync");

@@ -391,9 +391,12 @@ struct Ptr6 {
 // backscatter: with S the strain rate of the resolved velocity, a point where
 // the subgrid dissipation -tau:S is negative gets tau + ((-tau:S)/(S:S)) S,
 // whose dissipation is zero (the part of tau orthogonal to S is kept).
+// mix > 0: tau = (1 - mix) tau_network + mix tau_Smagorinsky before the
+// backscatter clearing, tau_Smagorinsky = -2 cd2 |S| S with cd2 = (c_s dx)^2
+// and |S| = sqrt(2 S:S) (les_model 1's stress, m_les.f90:1142-1215).
 __global__ __launch_bounds__(NT) void k_les(Geo g, Ptr9 gr, float delta, MlpNorm nrm,
                                             const uint16_t* __restrict__ wb, const float* __restrict__ p,
-                                            Ptr6 tau, unsigned int* bad) {
+                                            Ptr6 tau, unsigned int* bad, float mix, float cd2) {
     __shared__ FwdLds s;
     __shared__ int64_t offs[TS];
     __shared__ float graw[TS][9];
@@ -458,9 +461,15 @@ __global__ __launch_bounds__(NT) void k_les(Geo g, Ptr9 gr, float delta, MlpNorm
                 for (int c = 0; c < NOUT; ++c) {
                     t[c] = tl[i][c];
                     fin = fin && isfinite(t[c]);
-                    ts += w[c] * t[c] * S[c];
                     ss += w[c] * S[c] * S[c];
                 }
+                if (mix > 0.f) {
+                    const float nut2 = -2.f * cd2 * sqrtf(2.f * ss);
+#pragma unroll
+                    for (int c = 0; c < NOUT; ++c) t[c] = (1.f - mix) * t[c] + mix * (nut2 * S[c]);
+                }
+#pragma unroll
+                for (int c = 0; c < NOUT; ++c) ts += w[c] * t[c] * S[c];
                 if (!fin) atomicAdd(bad, 1u);
                 if (ts > 0.f && ss > 0.f) {  // -tau:S < 0: backscatter
                     const float a = -ts / ss;
@@ -570,7 +579,7 @@ void launch_mlp_forward(const float* X, int64_t count, const uint16_t* wb, const
 }
 void launch_mlp_les(const Geo& g, const double* const grad[9], double delta, const MlpNorm& nrm,
                     const uint16_t* wb, const float* p, double* const tau[6], unsigned int* bad,
-                    hipStream_t s) {
+                    double mix, double cd2, hipStream_t s) {
     Ptr9 gr;
     for (int i = 0; i < 9; ++i) gr.p[i] = grad[i];
     Ptr6 tp;
@@ -579,7 +588,7 @@ void launch_mlp_les(const Geo& g, const double* const grad[9], double delta, con
     if (total > ((int64_t)1 << 31)) throw std::runtime_error("closure inference: grid above 1024^3");
     const int64_t per = (int64_t)TS * FWD_TILES;
     hipLaunchKernelGGL(k_les, dim3((unsigned)((total + per - 1) / per)), dim3(NT), 0, s, g, gr,
-                       (float)delta, nrm, wb, p, tp, bad);
+                       (float)delta, nrm, wb, p, tp, bad, (float)mix, (float)cd2);
 }
 void launch_sgs_features(const Geo& g, const double* const grad[9], double delta, float* X,
                          hipStream_t s) {

@@ -49,10 +49,11 @@ void launch_mlp_forward(const float* X, int64_t count, const uint16_t* wb, const
 // Inference at every point of the grid from the nine physical gradient
 // slots du_i/dx_j (order 11,12,13,21,...,33) and the filter width: writes
 // the deviatoric stress tau_11, tau_12, tau_13, tau_22, tau_23, tau_33 into
-// six physical slots; *bad counts non-finite outputs.
+// six physical slots; *bad counts non-finite outputs.  mix > 0 blends in
+// the Smagorinsky stress -2 cd2 |S| S with that weight.
 void launch_mlp_les(const Geo& g, const double* const grad[9], double delta, const MlpNorm& nrm,
                     const uint16_t* wb, const float* p, double* const tau[6], unsigned int* bad,
-                    hipStream_t s);
+                    double mix, double cd2, hipStream_t s);
 // Features of every grid point: X[p][10] = (du_i/dx_j, delta), raw.
 void launch_sgs_features(const Geo& g, const double* const grad[9], double delta, float* X,
                          hipStream_t s);

@@ -116,6 +116,7 @@ struct Lib {
         unsigned int* bad = nullptr;
         int64_t fallbacks = 0;     // steps that used Smagorinsky instead
         double delta = 0.0;
+        double mix = 0.0;          // weight of the Smagorinsky stress in the LES stress
     } mlp;
 
     bool timing = false;

@@ -102,8 +102,14 @@ __device__ __forceinline__ bool kept(int j, int K, int n) { return j <= K || j >
 // run on one XCD at about the same time, and the second finds the segment
 // in that L2.  The grid must have an even number of column blocks; returns
 // this block's logical (column block, row).
+// The paired blocks load through the cache (not streaming): a streaming
+// load's line is gone from L2 before the partner block reads its other half,
+// and the 1024^3 z/update kernel fetched 1.30x its algorithmic bytes that
+// way (cached: 0.98x, 12.3-14.5 -> 11.4 ms).
 template <typename T, int B>
 inline constexpr bool kPairHalves = B * (int)sizeof(cpx<T>) < 128;
+template <int N, typename T, int B>
+inline constexpr bool kStreamLoad = kStream<N> && !kPairHalves<T, B>;
 template <bool PAIR>
 __device__ __forceinline__ void block_coords(int& bx, int& by) {
     if constexpr (!PAIR) {
@@ -225,7 +231,7 @@ __global__ __launch_bounds__((CR<N, T>::THREADS)) void k_inv_z_rot(SG g, P3 F, W
             }
             const int e = t + s * TPF;
             const bool ok = valid && kept(e, Kc<N>::K, N);
-            const cpx<double> x = ld_g<kStream<N>>(ok ? &src[(int64_t)e * g.P] : base);
+            const cpx<double> x = ld_g<kStreamLoad<N, T, B>>(ok ? &src[(int64_t)e * g.P] : base);
             uv[cc][s] = ok ? to<T>(x) : cpx<T>{(T)0, (T)0};
         }
     }
@@ -1148,7 +1154,7 @@ __global__ __launch_bounds__((CZ<N, T>::THREADS)) void k_fwd_z_update(
         const cpx<T>* src = reinterpret_cast<const cpx<T>*>(qp) + (int64_t)jy * g.PA + kx;
 #pragma unroll
         for (int s = 0; s < E; ++s)
-            dst[s] = valid ? ld_g<kStream<N>>(&src[(int64_t)(t + s * TPF) * N * g.PA]) : cpx<T>{(T)0, (T)0};
+            dst[s] = valid ? ld_g<kStreamLoad<N, T, B>>(&src[(int64_t)(t + s * TPF) * N * g.PA]) : cpx<T>{(T)0, (T)0};
     };
     cpx<T> nxt[E];
     if (PREFETCH) load(0, nxt);
@@ -1252,12 +1258,12 @@ __global__ __launch_bounds__((CZ<N, T>::THREADS)) void k_fwd_z_update(
             const int64_t off = ((int64_t)jy * N + e) * g.P + kx;
 #pragma unroll
             for (int cc = 0; cc < 3; ++cc) {
-                f[cc][q] = ld_g<kStream<N>>(&reinterpret_cast<const cpx<double>*>(u.F[cc])[off]);
+                f[cc][q] = ld_g<kStreamLoad<N, T, B>>(&reinterpret_cast<const cpx<double>*>(u.F[cc])[off]);
                 if (!u.first) {
                     if constexpr (sizeof(T) == 8) {
-                        o[cc][q] = ld_g<kStream<N>>(&reinterpret_cast<const cpx<double>*>(u.R[cc])[off]);
+                        o[cc][q] = ld_g<kStreamLoad<N, T, B>>(&reinterpret_cast<const cpx<double>*>(u.R[cc])[off]);
                     } else {
-                        const cpx<float> r32 = ld_g<kStream<N>>(&reinterpret_cast<const cpx<float>*>(u.R[cc])[off]);
+                        const cpx<float> r32 = ld_g<kStreamLoad<N, T, B>>(&reinterpret_cast<const cpx<float>*>(u.R[cc])[off]);
                         o[cc][q] = cpx<double>{(double)r32.x, (double)r32.y};
                     }
                 }

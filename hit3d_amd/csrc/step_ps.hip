@@ -211,6 +211,13 @@ struct ZC {
     static constexpr int THREADS = B * TPF;
 };
 
+// Paired half-segment blocks (B = 4 at 1024^3) load through the cache, so
+// that the partner block finds the other half of each line in L2 (streaming
+// loads fetched the lines again: the 2/3-rule z/update kernel moved 1.30x its
+// bytes that way).
+template <int N>
+inline constexpr bool kZLoadStream = kStream<N> && ZC<N>::B * 16 >= 128;
+
 template <int N>
 __device__ __forceinline__ void zc_coords(int& bx, int& by) {
     if constexpr (ZC<N>::B * 16 >= 128) {
@@ -273,7 +280,7 @@ __global__ __launch_bounds__(ZC<N>::THREADS) void k_zps(Q6 Q, int64_t P, PsUpd u
         cpx<double> v[E];
 #pragma unroll
         for (int s = 0; s < E; ++s)
-            v[s] = valid ? ld_g<kStream<N>>(&src[(int64_t)(t + s * TPF) * N * P])
+            v[s] = valid ? ld_g<kZLoadStream<N>>(&src[(int64_t)(t + s * TPF) * N * P])
                          : cpx<double>{0.0, 0.0};
         const cpx<double>* twp = tw + N;
         asm volatile("" : "+s"(twp));
@@ -329,8 +336,8 @@ __global__ __launch_bounds__(ZC<N>::THREADS) void k_zps(Q6 Q, int64_t P, PsUpd u
             const int64_t off = ((int64_t)jy * N + e) * P + kx;
 #pragma unroll
             for (int cc = 0; cc < 3; ++cc) {
-                f[cc][q] = ld_g<kStream<N>>(&reinterpret_cast<const cpx<double>*>(u.F[cc])[off]);
-                ro[cc][q] = ld_g<kStream<N>>(&reinterpret_cast<const cpx<double>*>(u.R[cc])[off]);
+                f[cc][q] = ld_g<kZLoadStream<N>>(&reinterpret_cast<const cpx<double>*>(u.F[cc])[off]);
+                ro[cc][q] = ld_g<kZLoadStream<N>>(&reinterpret_cast<const cpx<double>*>(u.R[cc])[off]);
             }
         }
 #pragma unroll
@@ -478,7 +485,7 @@ __global__ __launch_bounds__(ZC<N>::THREADS) void k_zsc(Q6 Q, int64_t P, ScUpd u
         cpx<double> v[E];
 #pragma unroll
         for (int s = 0; s < E; ++s)
-            v[s] = valid ? ld_g<kStream<N>>(&src[(int64_t)(t + s * TPF) * N * P])
+            v[s] = valid ? ld_g<kZLoadStream<N>>(&src[(int64_t)(t + s * TPF) * N * P])
                          : cpx<double>{0.0, 0.0};
         const cpx<double>* twp = tw + N;
         asm volatile("" : "+s"(twp));
@@ -507,11 +514,11 @@ __global__ __launch_bounds__(ZC<N>::THREADS) void k_zsc(Q6 Q, int64_t P, ScUpd u
             for (int q = 0; q < H; ++q) {
                 const int e = t + (h * H + q) * TPF;
                 const int64_t off = ((int64_t)jy * N + e) * P + kx;
-                f[q] = ld_g<kStream<N>>(&reinterpret_cast<const cpx<double>*>(u.F)[off]);
-                hh[q] = ld_g<kStream<N>>(&reinterpret_cast<const cpx<double>*>(u.H)[off]);
+                f[q] = ld_g<kZLoadStream<N>>(&reinterpret_cast<const cpx<double>*>(u.F)[off]);
+                hh[q] = ld_g<kZLoadStream<N>>(&reinterpret_cast<const cpx<double>*>(u.H)[off]);
                 ro[q] = u.first ? cpx<double>{0.0, 0.0}
-                                : ld_g<kStream<N>>(&reinterpret_cast<const cpx<double>*>(u.R)[off]);
-                s1[q] = u.U1 ? ld_g<kStream<N>>(&reinterpret_cast<const cpx<double>*>(u.U1)[off])
+                                : ld_g<kZLoadStream<N>>(&reinterpret_cast<const cpx<double>*>(u.R)[off]);
+                s1[q] = u.U1 ? ld_g<kZLoadStream<N>>(&reinterpret_cast<const cpx<double>*>(u.U1)[off])
                              : cpx<double>{0.0, 0.0};
             }
         }

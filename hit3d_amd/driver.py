@@ -376,25 +376,28 @@ def read_particles_binary(path: str, n: int):
     return rec["ipart"].astype(np.int32), _to_grid_units(rec["xyz"].astype(np.float64), n), t
 
 
-def write_closure_file(path: str, params: np.ndarray, norm):
+def write_closure_file(path: str, params: np.ndarray, norm, mix: float = 0.0):
     """The learned closure for les_model 11 as the Fortran driver reads it
     (<run_name>.sgs): 6288 float32 parameters, then float64 xmean(10),
-    xstd(10), ymean(6), ystd(6)."""
+    xstd(10), ymean(6), ystd(6) and the weight of the Smagorinsky stress in
+    the LES stress (hit3d_mlp_set_mix)."""
     xm, xs, ym, ys = (np.asarray(v, "<f8") for v in norm)
     with open(path, "wb") as f:
         f.write(np.asarray(params, "<f4").tobytes())
-        for v in (xm, xs, ym, ys):
+        for v in (xm, xs, ym, ys, np.array([mix], "<f8")):
             f.write(v.tobytes())
 
 
 def read_closure_file(path: str):
-    """Returns (params[6288] float32, (xmean, xstd, ymean, ystd))."""
+    """Returns (params[6288] float32, (xmean, xstd, ymean, ystd), mix); a
+    file without the trailing weight (32 doubles) has mix 0."""
     raw = open(path, "rb").read()
-    if len(raw) != 6288 * 4 + 32 * 8:
+    if len(raw) not in (6288 * 4 + 32 * 8, 6288 * 4 + 33 * 8):
         raise ValueError(f"{path}: not a closure file ({len(raw)} bytes)")
     params = np.frombuffer(raw[:6288 * 4], "<f4").copy()
     d = np.frombuffer(raw[6288 * 4:], "<f8")
-    return params, (d[0:10].copy(), d[10:20].copy(), d[20:26].copy(), d[26:32].copy())
+    mix = float(d[32]) if len(d) > 32 else 0.0
+    return params, (d[0:10].copy(), d[10:20].copy(), d[20:26].copy(), d[26:32].copy()), mix
 
 
 class Run:
@@ -420,9 +423,10 @@ class Run:
         if p.les_model == 1:
             self.h.set_les(1, 0.18)   # m_les.f90:1148
         if p.les_model == 11:
-            params, norm = read_closure_file(closure)
+            params, norm, mix = read_closure_file(closure)
             self.h.mlp_set_params(params)
             self.h.mlp_set_norm(*norm)
+            self.h.mlp_set_mix(mix)
             self.h.set_les(11, 0.18)
         self.ipart = None
         self.int_particles = False

@@ -234,15 +234,19 @@ contains
 
   subroutine closure_in                      ! les_model 11: <run_name>.sgs
     real(c_float) :: params(6288)
-    real*8 :: xm(10), xs(10), ym(6), ys(6)
+    real*8 :: xm(10), xs(10), ym(6), ys(6), mix
     integer :: u
-    inquire(file=run_name//'.sgs', exist=there)
+    integer(8) :: fsize
+    inquire(file=run_name//'.sgs', exist=there, size=fsize)
     if (.not. there) stop 'les_model 11 needs the closure file <run_name>.sgs'
     open(newunit=u, file=run_name//'.sgs', access='stream', form='unformatted', status='old')
     read(u) params, xm, xs, ym, ys
+    mix = 0.d0                               ! files without the blend weight
+    if (fsize >= 6288*4 + 33*8) read(u) mix
     close(u)
     call h3d_check(c_mlp_set_params(params), 'mlp_set_params')
     call h3d_check(c_mlp_set_norm(xm, xs, ym, ys), 'mlp_set_norm')
+    call h3d_check(c_mlp_set_mix(mix), 'mlp_set_mix')
     call h3d_check(c_set_les(11_c_int64_t, 0.18d0), 'set_les')
   end subroutine closure_in
 

@@ -259,6 +259,10 @@ module hit3d_hip
        import :: c_int, c_int64_t
        integer(c_int64_t), intent(out) :: count
      end function c_mlp_fallbacks
+     integer(c_int) function c_mlp_set_mix(weight) bind(C, name="hit3d_mlp_set_mix")
+       import :: c_int, c_double
+       real(c_double), intent(in) :: weight
+     end function c_mlp_set_mix
      integer(c_int) function c_box_filter(ratio, ubar, tau) bind(C, name="hit3d_box_filter")
        import :: c_int, c_int64_t, c_double
        integer(c_int64_t), intent(in) :: ratio

@@ -59,6 +59,7 @@ _SIGNATURES = {
                                ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
                                _F64P],
     "hit3d_mlp_fallbacks": [_I64P],
+    "hit3d_mlp_set_mix": [_F64P],
     "hit3d_box_filter": [_I64P, _F64P, _F64P],
     "hit3d_sgs_features": [_F64P, ctypes.POINTER(ctypes.c_float)],
     "hit3d_set_les": [_I64P, _F64P],
@@ -452,6 +453,11 @@ class Hit3D:
         Y = np.zeros((X.shape[0], 6), np.float32)
         _check(self.lib.hit3d_mlp_forward(_i(X.shape[0]), self._fp(X), self._fp(Y)))
         return Y
+
+    def mlp_set_mix(self, weight: float):
+        """LES stress of les_model 11: (1 - weight) of the network's plus
+        weight times the Smagorinsky stress of the set_les constant."""
+        _check(self.lib.hit3d_mlp_set_mix(_d(weight)))
 
     def mlp_fallbacks(self) -> int:
         c = ctypes.c_int64(0)

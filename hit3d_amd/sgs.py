@@ -233,10 +233,42 @@ def a_priori(tr: Trained, ds: Dataset, cs: float = 0.17) -> tuple:
     return float(np.mean((Y - Tn) ** 2)), float(np.mean((sm - Tn) ** 2))
 
 
+def dissipation_scale(tr: Trained, ds: Dataset) -> float:
+    """Ratio of the data's mean subgrid dissipation <-tau:S> to that of the
+    closure's stress as the LES applies it (after the backscatter clipping of
+    k_les).  A regression fit shrinks its predictions toward the mean, and
+    with them the energy it drains; scaling the stress by this ratio (a
+    degree-one change: the clipping commutes with it) matches the mean
+    drain of the data it was trained on."""
+    xm, xs, ym, ys = tr.norm
+    Xn = ((ds.X - xm) / xs).astype(np.float32)
+    h = Hit3D(16)
+    try:
+        h.mlp_set_params(tr.params)
+        h.mlp_set_norm(xm, xs, ym, ys)
+        Y = h.mlp_forward(Xn)
+    finally:
+        h.close()
+    tau = Y.astype(np.float64) * ys + ym
+    g = ds.X[:, :9].astype(np.float64)
+    S = np.stack([g[:, 0], 0.5 * (g[:, 1] + g[:, 3]), 0.5 * (g[:, 2] + g[:, 6]), g[:, 4],
+                  0.5 * (g[:, 5] + g[:, 7]), g[:, 8]], axis=1)
+    w = np.array([1.0, 2.0, 2.0, 1.0, 2.0, 1.0])
+    ts = (w * tau * S).sum(1)
+    ss = (w * S * S).sum(1)
+    back = (ts > 0) & (ss > 0)
+    tau[back] += (-ts[back] / ss[back])[:, None] * S[back]
+    d_pred = -(w * tau * S).sum(1).mean()
+    d_true = -(w * ds.T.astype(np.float64) * S).sum(1).mean()
+    return float(d_true / d_pred)
+
+
 def les_run(ubar0: np.ndarray, model: str, steps: int, trained: Trained | None = None,
-            cs: float = 0.17, re: float = 800.0, sample_every: int = 50, dt0: float = 2e-3):
+            cs: float = 0.17, re: float = 800.0, sample_every: int = 50, dt0: float = 2e-3,
+            mix: float = 0.0):
     """Forced coarse LES from the filtered field ubar0 (physical, [3, m, m, m])
-    with the learned closure (model 'mlp') or Smagorinsky ('smag').  Returns
+    with the learned closure (model 'mlp', its stress blended with weight mix
+    of the Smagorinsky stress) or Smagorinsky ('smag').  Returns
     the time-averaged E(k) (shells 1..kmax), the energy history and the count
     of fallback steps."""
     m = ubar0.shape[1]
@@ -256,6 +288,7 @@ def les_run(ubar0: np.ndarray, model: str, steps: int, trained: Trained | None =
         if model == "mlp":
             h.mlp_set_params(trained.params)
             h.mlp_set_norm(*trained.norm)
+            h.mlp_set_mix(mix)
             h.set_les(11, cs)
         elif model == "smag":
             h.set_les(1, cs)
@@ -280,7 +313,7 @@ def les_run(ubar0: np.ndarray, model: str, steps: int, trained: Trained | None =
 
 
 def les_matched(ubar0: np.ndarray, model: str, times: list, trained: Trained | None = None,
-                cs: float = 0.17, re: float = 800.0, dt0: float = 2e-3) -> list:
+                cs: float = 0.17, re: float = 800.0, dt0: float = 2e-3, mix: float = 0.0) -> list:
     """Forced coarse LES from the filtered field ubar0 for a given span of
     time: E(k) (shells 1..kmax) each time the LES clock first reaches one of
     `times` (offsets from the start, increasing), to set beside the filtered
@@ -302,6 +335,7 @@ def les_matched(ubar0: np.ndarray, model: str, times: list, trained: Trained | N
         if model == "mlp":
             h.mlp_set_params(trained.params)
             h.mlp_set_norm(*trained.norm)
+            h.mlp_set_mix(mix)
             h.set_les(11, cs)
         else:
             h.set_les(1, cs)
@@ -341,6 +375,15 @@ def judge_windows(a, snaps: list, m: int, delta: float, t_dns: float) -> dict:
     tr = train(sets[0], a.train_steps, a.batch, a.lr)
     log(f"[sgs] trained on window 1: validation {tr.val_loss:.4f} vs Smagorinsky "
         f"{tr.smag_val_loss:.4f} ({time.time() - t0:.1f}s)")
+    # the data's mean subgrid dissipation over the closure's (after the
+    # backscatter clearing)
+    cal = dissipation_scale(tr, sets[0])
+    log(f"[sgs] mean dissipation, data over closure: {cal:.3f}")
+    # the network alone, and blended half and half with Smagorinsky's stress;
+    # the control smag_half is the blend's Smagorinsky part alone (constant
+    # cs / sqrt 2: half the stress)
+    models = {"smag": ("smag", None, 0.0, a.cs), "mlp": ("mlp", tr, 0.0, a.cs),
+              "mlp_mix": ("mlp", tr, 0.5, a.cs), "smag_half": ("smag", None, 0.0, a.cs / math.sqrt(2))}
     out = []
     for w, (ws, ds) in enumerate(zip(wins, sets)):
         ek_ref = np.mean(ds.ek_filtered, axis=0)
@@ -351,9 +394,9 @@ def judge_windows(a, snaps: list, m: int, delta: float, t_dns: float) -> dict:
         row = {"window": w + 1, "t_first": ws[0].time, "t_last": ws[-1].time,
                "a_priori_mlp": ap_mlp, "a_priori_smagorinsky": ap_smag}
         offsets = [sn.time - ws[0].time for sn in ws[1:]]
-        for model in ("smag", "mlp"):
+        for model, (kind, trm, mix, cs) in models.items():
             try:
-                ek, energy, fb = les_run(ws[-1].ubar, model, a.les_steps, tr, a.cs, a.re)
+                ek, energy, fb = les_run(ws[-1].ubar, kind, a.les_steps, trm, cs, a.re, mix=mix)
                 row[f"ek_err_{model}"] = spectrum_error(ek[:kmax], ek_ref[:kmax])
                 row[f"fallback_steps_{model}"] = int(fb)
             except FloatingPointError as e:
@@ -362,16 +405,21 @@ def judge_windows(a, snaps: list, m: int, delta: float, t_dns: float) -> dict:
             # time-matched: from the window's first filtered snapshot, E(k) at
             # the times of the others against theirs
             try:
-                eks = les_matched(ws[0].ubar, model, offsets, tr, a.cs, a.re)
+                eks = les_matched(ws[0].ubar, kind, offsets, trm, cs, a.re, mix=mix)
                 row[f"ek_err_matched_{model}"] = float(np.mean(
                     [spectrum_error(e[:kmax], r[:kmax]) for e, r in zip(eks, ds.ek_filtered[1:])]))
+                # E_LES / E_DNS per shell, averaged over the matched times
+                row[f"ek_ratio_matched_{model}"] = [round(float(x), 3) for x in np.mean(
+                    [e[:kmax] / r[:kmax] for e, r in zip(eks, ds.ek_filtered[1:])], axis=0)]
             except FloatingPointError as e:
                 row[f"ek_err_matched_{model}"] = None
                 row[f"error_matched_{model}"] = str(e)
         log(f"[sgs] window {w + 1} (t {row['t_first']:.2f}-{row['t_last']:.2f}): a priori "
             f"{ap_mlp:.3f} vs {ap_smag:.3f}; a posteriori {row.get('ek_err_mlp')} vs "
-            f"{row.get('ek_err_smag')}; time-matched {row.get('ek_err_matched_mlp')} vs "
-            f"{row.get('ek_err_matched_smag')}")
+            f"{row.get('ek_err_smag')} (blend {row.get('ek_err_mlp_mix')}); time-matched "
+            f"{row.get('ek_err_matched_mlp')} vs {row.get('ek_err_matched_smag')} (blend "
+            f"{row.get('ek_err_matched_mlp_mix')}; Smagorinsky at cs/sqrt 2: "
+            f"{row.get('ek_err_smag_half')}, {row.get('ek_err_matched_smag_half')})")
         out.append(row)
     wins_post = sum(1 for r in out if r.get("ek_err_mlp") is not None and r.get("ek_err_smag")
                     is not None and r["ek_err_mlp"] < r["ek_err_smag"])
@@ -379,7 +427,20 @@ def judge_windows(a, snaps: list, m: int, delta: float, t_dns: float) -> dict:
     wins_matched = sum(1 for r in out if r.get("ek_err_matched_mlp") is not None
                        and r.get("ek_err_matched_smag") is not None
                        and r["ek_err_matched_mlp"] < r["ek_err_matched_smag"])
+    wins_mix = sum(1 for r in out if r.get("ek_err_mlp_mix") is not None and r.get("ek_err_smag")
+                   is not None and r["ek_err_mlp_mix"] < r["ek_err_smag"])
+    wins_mix_matched = sum(1 for r in out if r.get("ek_err_matched_mlp_mix") is not None
+                           and r.get("ek_err_matched_smag") is not None
+                           and r["ek_err_matched_mlp_mix"] < r["ek_err_matched_smag"])
+    wins_mix_half = sum(1 for r in out if r.get("ek_err_mlp_mix") is not None
+                        and r.get("ek_err_smag_half") is not None
+                        and r["ek_err_mlp_mix"] < r["ek_err_smag_half"]
+                        and r["ek_err_matched_mlp_mix"] < r["ek_err_matched_smag_half"])
     return {"dns": a.dns, "ratio": a.ratio, "les_grid": m, "windows": out,
+            "blend_better_than_its_smagorinsky_part_both_measures": wins_mix_half,
+            "dissipation_data_over_closure": cal,
+            "blend_better_a_posteriori": wins_mix,
+            "blend_better_time_matched": wins_mix_matched,
             "closure_better_a_priori": wins_prio, "closure_better_a_posteriori": wins_post,
             "closure_better_time_matched": wins_matched,
             "train_steps": a.train_steps, "batch": a.batch, "snaps_per_window": a.snaps,
@@ -405,7 +466,12 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--les-steps", type=int, default=5000)
     ap.add_argument("--cs", type=float, default=0.17)
-    ap.add_argument("--out", default="")
+    ap.add_argument("--out", default="",
+                    help="write the trained closure as a closure file (<run_name>.sgs) for "
+                         "les_model 11, with the blend weight --mix")
+    ap.add_argument("--mix", type=float, default=0.5,
+                    help="weight of the Smagorinsky stress in the blended LES stress (the "
+                         "model 'mlp_mix' of the comparison and the weight written by --out)")
     ap.add_argument("--windows", type=int, default=1,
                     help="consecutive DNS windows of --snaps snapshots each: the closure is "
                          "trained on the first, then judged a priori and a posteriori against "
@@ -434,10 +500,10 @@ def main(argv=None):
            "train_seconds": t_train, "dns_seconds": t_dns}
     u0 = snaps[-1].ubar
     kmax = m // 3
-    for model in ("smag", "mlp"):
+    for model, kind, mix in (("smag", "smag", 0.0), ("mlp", "mlp", 0.0), ("mlp_mix", "mlp", a.mix)):
         t0 = time.time()
         try:
-            ek, energy, fb = les_run(u0, model, a.les_steps, tr, a.cs, a.re)
+            ek, energy, fb = les_run(u0, kind, a.les_steps, tr, a.cs, a.re, mix=mix)
             res[f"ek_err_{model}"] = spectrum_error(ek[:kmax], ek_ref[:kmax])
             res[f"energy_{model}"] = float(energy[len(energy) // 2:].mean())
             res[f"fallback_steps_{model}"] = int(fb)
@@ -450,8 +516,8 @@ def main(argv=None):
     res["ek_filtered_dns"] = [float(v) for v in ek_ref[:kmax]]
     res["energy_filtered_dns"] = float(np.mean([e.sum() for e in ds.ek_filtered]))
     if a.out:
-        np.savez(a.out, params=tr.params, xmean=tr.norm[0], xstd=tr.norm[1], ymean=tr.norm[2],
-                 ystd=tr.norm[3], losses=tr.losses)
+        from .driver import write_closure_file
+        write_closure_file(a.out, tr.params, tr.norm, mix=a.mix)
     print(json.dumps(res), flush=True)
 
 

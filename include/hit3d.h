@@ -155,6 +155,12 @@ int hit3d_mlp_grads(const int64_t *count, const float *X, const float *T, float 
                     double *loss);
 int hit3d_mlp_forward(const int64_t *count, const float *X, float *Y);
 int hit3d_mlp_fallbacks(int64_t *count);
+/* The LES stress of les_model 11 as a blend (no counterpart in the
+ * reference): tau = (1 - weight) tau_network + weight tau_Smagorinsky, the
+ * second -2 (c_smag dx)^2 |S| S_ij with the constant of hit3d_set_les,
+ * before the backscatter clearing; weight in [0, 1], 0 (the network alone)
+ * by default. */
+int hit3d_mlp_set_mix(const double *weight);
 
 /* hit3d_mlp_grads without any device allocation: no counterpart in the
  * reference (the closure has none); for callers that cannot allocate --

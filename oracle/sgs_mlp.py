@@ -116,6 +116,20 @@ def box_filter(u: np.ndarray, ratio: int):
     return a.mean(axis=(1, 3, 5))
 
 
+def blend_smagorinsky(tau: np.ndarray, grads: np.ndarray, mix: float, cd2: float) -> np.ndarray:
+    """(1 - mix) tau + mix tau_S with the Smagorinsky stress tau_S = -2 cd2
+    |S| S, |S| = sqrt(2 S:S), cd2 = (c_s dx)^2 (les_model 1's stress,
+    m_les.f90:1142-1215 with rhs_velocity's -2 nu_t S_ij); the build's blend of
+    the learned closure, which has no reference counterpart.  tau[p, 6] in the
+    order 11, 12, 13, 22, 23, 33."""
+    g = grads.reshape(-1, 3, 3)
+    S = np.stack([g[:, 0, 0], 0.5 * (g[:, 0, 1] + g[:, 1, 0]), 0.5 * (g[:, 0, 2] + g[:, 2, 0]),
+                  g[:, 1, 1], 0.5 * (g[:, 1, 2] + g[:, 2, 1]), g[:, 2, 2]], axis=1)
+    w = np.array([1.0, 2.0, 2.0, 1.0, 2.0, 1.0])
+    smag = np.sqrt(2.0 * (w * S * S).sum(1))
+    return (1.0 - mix) * tau + mix * (-2.0 * cd2 * smag)[:, None] * S
+
+
 def clip_backscatter(tau: np.ndarray, grads: np.ndarray) -> np.ndarray:
     """Where the subgrid dissipation -tau:S is negative (S the strain rate
     from grads[p, 3i+j] = du_i/dx_j), add (-tau:S)/(S:S) S: the dissipation

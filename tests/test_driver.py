@@ -105,3 +105,21 @@ def test_tmp4_layout(tmp_path):
     raw = path.read_bytes()
     assert list(np.frombuffer(raw[:12], "<i4")) == [4, 3, 2]
     assert np.array_equal(np.frombuffer(raw[12:], "<f4"), a.ravel())
+
+
+def test_closure_file_round_trip(tmp_path):
+    """<run_name>.sgs: parameters, scaling and the blend weight come back
+    unchanged; a file without the weight (32 doubles) reads as weight 0."""
+    from hit3d_amd.driver import read_closure_file, write_closure_file
+    rng = np.random.default_rng(1)
+    params = rng.standard_normal(6288).astype(np.float32)
+    norm = (rng.standard_normal(10), 1 + rng.random(10), rng.standard_normal(6), 1 + rng.random(6))
+    path = tmp_path / "a.sgs"
+    write_closure_file(str(path), params, norm, mix=0.5)
+    assert path.stat().st_size == 6288 * 4 + 33 * 8
+    p2, n2, mix = read_closure_file(str(path))
+    assert np.array_equal(p2, params) and mix == 0.5
+    assert all(np.array_equal(a, b) for a, b in zip(n2, norm))
+    old = tmp_path / "b.sgs"
+    old.write_bytes(path.read_bytes()[:-8])
+    assert read_closure_file(str(old))[2] == 0.0

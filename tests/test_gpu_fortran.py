@@ -204,15 +204,16 @@ def test_fortran_les_particles_snapshots(tmp_path):
 
 @pytest.mark.skipif(not os.path.exists(EXE), reason="Fortran driver not built")
 def test_fortran_learned_closure(tmp_path):
-    """les_model 11 with the closure read from <run_name>.sgs: the Fortran
-    and Python drivers give the same restart file."""
+    """les_model 11 with the closure read from <run_name>.sgs (its stress
+    blended half and half with Smagorinsky's): the Fortran and Python
+    drivers give the same restart file."""
     from hit3d_amd.driver import write_closure_file
     print("fortran driver: learned closure", flush=True)
     rng = np.random.default_rng(3)
     params = (rng.standard_normal(6288) * 0.05).astype(np.float32)
     norm = (np.zeros(10), np.ones(10), np.zeros(6), np.full(6, 1e-3))
     path = tmp_path / "closure.sgs"
-    write_closure_file(str(path), params, norm)
+    write_closure_file(str(path), params, norm, mix=0.5)
     run = "c4mlp00032"
     text = _input(32, 0, 4, 2, 1000, 1000, 1, 800.0, -0.002, 0, les_model=11)
     f, p = _run_both(tmp_path, run, text, closure=path.read_bytes())

@@ -197,6 +197,59 @@ def test_les_rhs_with_closure_matches_oracle(oracle):
         h.close()
 
 
+def test_les_rhs_blend_with_smagorinsky(oracle):
+    """les_model 11 with the stress blended with Smagorinsky's
+    (hit3d_mlp_set_mix): half and half against the oracle's blend of the
+    same network, and the weight 1 against les_model 1 itself, whose stress
+    it then is (to the closure kernel's FP32 rounding; the Smagorinsky stress
+    dissipates everywhere, so the backscatter clearing leaves it alone)."""
+    note("les closure blended with smagorinsky")
+    n = 32
+    cs = 0.17
+    rng = np.random.default_rng(9)
+    p = params(10)
+    xmean, xstd = rng.standard_normal(10) * 0.1, 1.0 + rng.random(10)
+    ymean, ystd = rng.standard_normal(6) * 0.01, 0.05 + 0.1 * rng.random(6)
+    h = hit3d_amd.Hit3D(n)
+    try:
+        h.set_physics(0.005)
+        phys = np.zeros((3, n, n, n + 2))
+        phys[:, :, :, :n] = rng.standard_normal((3, n, n, n))
+        spec = np.stack([oracle.fft_forward(q) for q in phys])
+        spec = oracle.dealias_all(oracle.pressure(spec, n), n // 3, n)
+        for c in range(3):
+            h.put(FIELDS, c + 1, spec[c])
+        h.mlp_set_params(flat(p))
+        h.mlp_set_norm(xmean, xstd, ymean, ystd)
+        h.set_les(11, cs)
+        base = oracle.rhs_velocity(spec, 0.005, 0, n)[0]
+        grads = oracle.velocity_gradients(spec, n)
+        delta = 2 * np.pi / n
+        raw = np.stack([g.ravel() for g in grads] + [np.full(n ** 3, delta)], axis=1)
+        X = (raw.astype(np.float32) - xmean.astype(np.float32)) / xstd.astype(np.float32)
+        Y, _ = om.forward(p, X.astype(np.float64), exact=False)
+        t6 = np.stack([Y[:, c] * ystd[c] + ymean[c] for c in range(6)], axis=1)
+        cd2 = (cs * delta) ** 2
+        t6 = om.clip_backscatter(om.blend_smagorinsky(t6, raw[:, :9], 0.5, cd2), raw[:, :9])
+        want = oracle.stress_divergence(base, [t6[:, c].reshape(n, n, n) for c in range(6)], n)
+        h.mlp_set_mix(0.5)
+        h.rhs_scalars(False)
+        h.rhs_velocity(1e-3)
+        got = np.stack([h.get(WRK, c) for c in (1, 2, 3)])
+        assert h.mlp_fallbacks() == 0
+        assert relnorm(got - base, want - base) < 5e-3
+        h.mlp_set_mix(1.0)
+        h.rhs_scalars(False)
+        h.rhs_velocity(1e-3)
+        got1 = np.stack([h.get(WRK, c) for c in (1, 2, 3)])
+        want1 = oracle.les_rhs_velocity(spec, base, oracle.turb_visc_smag(spec, n, cs), n)
+        assert relnorm(got1 - base, want1 - base) < 1e-5
+        with pytest.raises(hit3d_amd.Hit3dError):
+            h.mlp_set_mix(1.5)
+    finally:
+        h.close()
+
+
 def test_closure_gain_over_smagorinsky():
     """What the trained network gains over the method without it: on a small
     forced case (128^3 DNS, filter ratio 4, 32^3 LES) the closure trained by
@@ -213,7 +266,10 @@ def test_closure_gain_over_smagorinsky():
     ref = np.mean(ds.ek_filtered, axis=0)[:10]
     ek_s, _, _ = sgs.les_run(snaps[-1].ubar, "smag", 4000, tr)
     ek_m, _, fb = sgs.les_run(snaps[-1].ubar, "mlp", 4000, tr)
+    ek_b, _, fb_b = sgs.les_run(snaps[-1].ubar, "mlp", 4000, tr, mix=0.5)
     err_s, err_m = sgs.spectrum_error(ek_s[:10], ref), sgs.spectrum_error(ek_m[:10], ref)
-    note(f"E(k) error: closure {err_m:.3f}, Smagorinsky {err_s:.3f}, fallback steps {fb}")
-    assert fb == 0
+    err_b = sgs.spectrum_error(ek_b[:10], ref)
+    note(f"E(k) error: closure {err_m:.3f}, blended with Smagorinsky {err_b:.3f}, "
+         f"Smagorinsky {err_s:.3f}, fallback steps {fb}, {fb_b}")
+    assert fb == 0 and fb_b == 0
     assert err_m < err_s

@@ -95,3 +95,30 @@ def test_backscatter_clip_zeroes_negative_dissipation():
     assert np.array_equal(out[d0 >= 0], tau[d0 >= 0])
     assert np.abs(d1[d0 < 0]).max() < 1e-12
     assert (d0 < 0).sum() > 100
+
+
+def test_blend_smagorinsky_limits():
+    """The blend with weight 1 is les_model 1's stress -2 nu_t S_ij with the
+    reference path's eddy viscosity (turb_visc_smag, m_les.f90:1142-1215),
+    and with weight 0 the network's stress itself; catches a missing factor
+    2 in |S| or in the symmetric double sum."""
+    from oracle import hit3d_oracle as o
+    n, cs = 8, 0.17
+    rng = np.random.default_rng(5)
+    phys = np.zeros((3, n, n, n + 2))
+    phys[:, :, :, :n] = rng.standard_normal((3, n, n, n))
+    spec = np.stack([o.fft_forward(q) for q in phys])
+    spec = o.dealias_all(o.pressure(spec, n), n // 3, n)
+    grads = o.velocity_gradients(spec, n)
+    g = np.stack([a.ravel() for a in grads], axis=1)
+    tv = o.turb_visc_smag(spec, n, cs).ravel()
+    tau = rng.standard_normal((n ** 3, 6))
+    cd2 = (cs * 2 * np.pi / n) ** 2
+    full = m.blend_smagorinsky(tau, g, 1.0, cd2)
+    G = g.reshape(-1, 3, 3)
+    for c, (i, j) in enumerate([(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]):
+        S = 0.5 * (G[:, i, j] + G[:, j, i])
+        assert np.allclose(full[:, c], -2.0 * tv * S, rtol=1e-12, atol=1e-14)
+    assert np.array_equal(m.blend_smagorinsky(tau, g, 0.0, cd2), tau)
+    half = m.blend_smagorinsky(tau, g, 0.5, cd2)
+    assert np.allclose(half, 0.5 * tau + 0.5 * full, rtol=1e-14, atol=1e-15)
