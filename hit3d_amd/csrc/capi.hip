@@ -42,6 +42,12 @@ int hit3d_init(const int64_t* nx, const int64_t* ny, const int64_t* nz_all,
         const double pi = 4.0 * std::atan(1.0);
         L.dx = 2.0 * pi / (double)L.n;
         check(hipStreamCreateWithFlags(&L.st, hipStreamNonBlocking), "hipStreamCreate");
+        {
+            int dev = 0;
+            check(hipGetDevice(&dev), "hipGetDevice");
+            check(hipDeviceGetAttribute(&L.ncu, hipDeviceAttributeMultiprocessorCount, dev),
+                  "hipDeviceGetAttribute");
+        }
         static_scratch_setup();
         const size_t sb = slot_bytes();
         L.F.assign(L.nf + 1, nullptr);

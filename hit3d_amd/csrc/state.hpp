@@ -67,6 +67,7 @@ struct Lib {
                                      // [4..5] physical sums, [8..9] min/max keys
     double* hpin = nullptr;          // pinned host mirror of dscal
     hipStream_t st = nullptr;
+    int ncu = 0;  // compute units of the device (persistent launches)
 
     // mixed-precision work buffers (allocated on first use): nine FP32 slots
     std::vector<float*> W32;
