@@ -1,6 +1,7 @@
 // Axis passes of the 3-D real FFT (forward: x R2C, y C2C, z C2C with the
 // (y,z) -> (kz,ky) transpose; inverse in reverse order), replacing the FFTW
 // plane/stick loops and the slab transpose of xFFT3d (x_fftw.f90:852-1085).
+#include <algorithm>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -129,6 +130,92 @@ __global__ __launch_bounds__(B* N / E) void k_cols(Slots3 io, int64_t a_in, int6
     }
 }
 
+// compute units of the current device (queried once)
+static int device_cus() {
+    static int n = 0;
+    if (n == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            throw std::runtime_error("column pass: device query failed");
+        n = std::max(1, n);
+    }
+    return n;
+}
+
+// The plain and phase-shifted column passes as a persistent grid at
+// 1024^3: a block's 128 KB of LDS leaves one block per compute unit, which
+// otherwise holds no loads in flight while it transforms.  Each block loops
+// over column groups (column block, group a, slot) with the next group's
+// loads issued before the current group's transform.
+template <int N, int E, int B, int DIR, typename T, int PH>
+__global__ __launch_bounds__(B* N / E) void k_colsp(Slots3 io, int64_t a_in, int64_t a_out,
+                                                    int64_t s_in, int64_t s_out, int ncols,
+                                                    int ncb, int na, int ngroups,
+                                                    const cpx<T>* __restrict__ tw,
+                                                    const double* __restrict__ tab) {
+    constexpr int TPF = N / E;
+    constexpr int PS = 0;  // unpadded, as the fused step's column passes (registers)
+    __shared__ cpx<T> lds[padded_size<PS>(N * B)];
+    const int c = threadIdx.x % B;
+    const int t = threadIdx.x / B;
+    auto at = [&](int q, int& col, int& a, int& z) {
+        col = (q % ncb) * B + c;
+        const int rest = q / ncb;
+        a = rest % na;
+        z = rest / na;
+    };
+    // entry indices inside a slot fit 32 bits (N^2 P < 2^31 up to 1024^3)
+    const uint32_t si = (uint32_t)s_in, so = (uint32_t)s_out;
+    auto load = [&](int q, cpx<T>(&v)[E]) {
+        int col, a, z;
+        at(q, col, a, z);
+        const void* ip = io.in[0];
+        if (z == 1) ip = io.in[1];
+        if (z == 2) ip = io.in[2];
+        const cpx<T>* src = static_cast<const cpx<T>*>(ip);
+        const uint32_t idx = (uint32_t)a * (uint32_t)a_in + (uint32_t)col;
+        const bool valid = col < ncols;
+#pragma unroll
+        for (int s = 0; s < E; ++s)
+            v[s] = valid ? ld_g<kStream<N>>(&src[idx + (uint32_t)(t + s * TPF) * si])
+                         : cpx<T>{(T)0, (T)0};
+    };
+    int q = blockIdx.x;
+    if (q >= ngroups) return;
+    cpx<T> nxt[E];
+    load(q, nxt);
+    for (; q < ngroups; q += gridDim.x) {
+        cpx<T> v[E];
+#pragma unroll
+        for (int s = 0; s < E; ++s) v[s] = nxt[s];
+        if (q + (int)gridDim.x < ngroups) load(q + gridDim.x, nxt);
+        int col, a, z;
+        at(q, col, a, z);
+        const bool valid = col < ncols;
+        if constexpr (PH > 0)
+            if (valid)
+#pragma unroll
+                for (int s = 0; s < E; ++s) v[s] = phase<N, 1>(v[s], col, a, t + s * TPF, tab);
+        // the twiddle pointer hidden from the optimiser: hoisted out of the
+        // loop, the twiddles of all stages would stay in registers
+        const cpx<T>* twp = tw + N;
+        asm volatile("" : "+s"(twp));
+        block_fft<N, E, DIR, T, B, PS, sizeof(T) == 8>(v, t, lds + c, twp);
+        if (!valid) continue;
+        if constexpr (PH < 0)
+#pragma unroll
+            for (int s = 0; s < E; ++s) v[s] = phase<N, -1>(v[s], col, a, t + s * TPF, tab);
+        void* op = io.out[0];
+        if (z == 1) op = io.out[1];
+        if (z == 2) op = io.out[2];
+        cpx<T>* dst = static_cast<cpx<T>*>(op);
+        const uint32_t idx = (uint32_t)a * (uint32_t)a_out + (uint32_t)col;
+#pragma unroll
+        for (int s = 0; s < E; ++s) st_g<kStream<N>>(&dst[idx + (uint32_t)(t + s * TPF) * so], v[s]);
+    }
+}
+
 template <int N, typename T>
 static void cols_n(int dir, const cpx<T>* const* in, cpx<T>* const* out, int nb, int64_t a_in,
                    int64_t a_out, int64_t s_in, int64_t s_out, const cpx<T>* tw, hipStream_t s,
@@ -146,6 +233,30 @@ static void cols_n(int dir, const cpx<T>* const* in, cpx<T>* const* out, int nb,
     dim3 grid((ncols + B - 1) / B, N, nb);
     dim3 block(B * N / E);
     if (ph != 0 && !tab) throw std::runtime_error("column pass: phase table missing");
+    if constexpr (N >= 1024 && std::is_same_v<T, double>) {
+        if (!acc.r[0]) {
+            const int ncb = (int)grid.x, ngroups = ncb * N * nb;
+            auto go = [&](auto kern) {
+                static int per = 0;
+                if (per == 0) {
+                    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, (int)block.x, 0) !=
+                        hipSuccess)
+                        throw std::runtime_error("column pass: occupancy query failed");
+                    per = std::max(1, per);
+                }
+                const int nblk = std::min(per * device_cus(), ngroups);
+                hipLaunchKernelGGL(kern, dim3(nblk), block, 0, s, io, a_in, a_out, s_in, s_out,
+                                   ncols, ncb, N, ngroups, tw, tab);
+            };
+            if constexpr (std::is_same_v<T, double>) {
+                if (dir > 0 && ph > 0) return go(k_colsp<N, E, B, 1, T, 1>);
+                if (dir < 0 && ph < 0) return go(k_colsp<N, E, B, -1, T, -1>);
+            }
+            if (ph != 0) throw std::runtime_error("column pass: unsupported option");
+            if (dir < 0) return go(k_colsp<N, E, B, -1, T, 0>);
+            return go(k_colsp<N, E, B, 1, T, 0>);
+        }
+    }
     if constexpr (std::is_same_v<T, double>) {
         if (dir < 0 && ph < 0 && acc.r[0]) {
             hipLaunchKernelGGL((k_cols<N, E, B, -1, T, -1, true>), grid, block, 0, s, io, a_in,
