@@ -35,6 +35,9 @@ FORM_NAMES = {0: "divergence -i k_j (u_i u_j)^ (the reference's)",
               1: "rotational (u x omega)^",
               2: "divergence of the trace-free tensor u_i u_j - delta_ij |u|^2/3 (differs "
                  "from the reference's by a gradient the projection removes)"}
+PS_FIVE_PRODUCTS = ("divergence of u_i u_j - delta_ij w w in both phase-shift halves, five "
+                    "products (differs from the reference's by a gradient the projection "
+                    "removes)")
 METRIC = "s/step at 512^3 FP64 vs mixed (E(k) err stated) + % of HBM roofline"
 
 
@@ -181,10 +184,11 @@ def run_fft_micro(sizes=(256, 512, 1024), reps=20, warmup=3):
     print(json.dumps(out), flush=True)
 
 
-def run_workload(name, steps, warmup):
-    """One of the other configurations: FP64 driver steps (unfused where the
-    phase shift or scalars apply), with particles moved every step when the
-    configuration has them.  Prints its own JSON line."""
+def run_workload(name, steps, warmup, form=0):
+    """One of the other configurations: FP64 driver steps, with particles
+    moved every step when the configuration has them; form 2 runs the
+    phase-shift configurations with five products instead of six (the same
+    fields to rounding).  Prints its own JSON line."""
     from hit3d_amd.driver import Params, Run
     w = WORKLOADS[name]
     n = w["n"]
@@ -193,6 +197,8 @@ def run_workload(name, steps, warmup):
     scheme = kw.pop("particles_tracking_scheme", 0)
     p = Params(nx=n, ny=n, nz_all=n, isp_type=2, ir_exp=4, peak_wavenum=3.0, **kw)
     r = Run(p)
+    if form and p.dealias == 1:
+        r.h.set_nonlinear(form)
     t0 = time.time()
     r.begin_new()
     if nptot:
@@ -224,6 +230,7 @@ def run_workload(name, steps, warmup):
     s = slot_bytes(n)
     b_alg = 6 * w["n_fft"] * s + 4 * (3 + w["ns"]) * s
     out = {"workload": name, "config": w["desc"], "s_per_step": t, "ns_per_point_step": t / n ** 3 * 1e9,
+           "nonlinear_form": (PS_FIVE_PRODUCTS if (form == 2 and p.dealias == 1) else FORM_NAMES[0]),
            "plan_alg_bytes": b_alg,
            "plan_bytes_over_step_time_vs_peak": b_alg / t / (PEAK_HBM_GBS * 1e9),
            "particles": nptot, "steps": steps, "warmup": warmup,
@@ -476,7 +483,7 @@ def main():
         run_fft_micro()
         return
     if a.workload != "c3":
-        run_workload(a.workload, a.steps, a.warmup)
+        run_workload(a.workload, a.steps, a.warmup, a.form)
         return
     if a.accuracy_only:
         t0 = time.perf_counter()

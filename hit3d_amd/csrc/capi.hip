@@ -268,10 +268,13 @@ int hit3d_step(const double* dt, int64_t* fov, int64_t* fos, const int64_t* int_
         const bool scal = (*int_scalars != 0) && L.ns > 0;
         const bool fused = !scal && L.fuse && L.les_model == 0 && fused_step_supported(mixed);
         if (!fused && mixed) throw std::runtime_error("mixed precision needs the fused path");
-        if (!fused && L.nlform != 0)
+        // the deviatoric form also runs in the fused phase-shift velocity step
+        const bool ps_dev = L.nlform == 2 && L.dealias == 1 && L.fuse && fused_ps_supported();
+        if (!fused && L.nlform != 0 && !ps_dev)
             throw std::runtime_error(
-                "the rotational and deviatoric forms run in the fused 2/3-rule step only (no "
-                "active scalars, no LES, fusion on)");
+                "the rotational and deviatoric forms run in the fused steps only (2/3 rule: no "
+                "active scalars, no LES; the deviatoric form also with the phase shift; fusion "
+                "on)");
         const bool first_v = *fov != 0, first_s = *fos != 0;
         // the step size goes to the device (dscal[12]) ahead of the step's
         // work, so a captured step replays with the current dt
@@ -693,9 +696,8 @@ int hit3d_set_nonlinear(const int64_t* form) {
             throw std::runtime_error(
                 "nonlinear form must be 0 (divergence, the reference's), 1 (rotational) or 2 "
                 "(deviatoric)");
-        if (*form != 0 && L.dealias != 0)
-            throw std::runtime_error(
-                "the rotational and deviatoric forms are built for the 2/3 rule (dealias 0)");
+        if (*form == 1 && L.dealias != 0)
+            throw std::runtime_error("the rotational form is built for the 2/3 rule (dealias 0)");
         clear_graphs();  // captured steps hold the form in their kernels
         L.nlform = (int)*form;
     }, false);

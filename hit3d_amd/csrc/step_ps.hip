@@ -130,7 +130,12 @@ __device__ __forceinline__ void row_r2c_store(cpx<double> (&z)[1][XR<N>::E], int
     if (t == 0) st_g<kStream<N>>(&rowp[M], cpx<double>{z[0][0].x - z[0][0].y, 0.0});
 }
 
-template <int N, bool COURANT>
+// DEV: the five products of D_ij = u_i u_j - delta_ij w w (uu - ww, uv, uw,
+// vv - ww, vw; D33 = 0) instead of the six u_i u_j: the divergence differs
+// by the gradient of ww, which the final projection removes (the
+// phase-shift mask keeps it a gradient: it zeroes whole wavevectors), and
+// k_zps assembles k_j D_ij from them exactly as from the first five u_i u_j.
+template <int N, bool COURANT, bool DEV = false>
 __global__ __launch_bounds__(XR<N>::THREADS, 2) void k_xprod6(const double* u, const double* v,
                                                            const double* w, Q6 out, int64_t P,
                                                            const cpx<double>* __restrict__ tw,
@@ -167,15 +172,25 @@ __global__ __launch_bounds__(XR<N>::THREADS, 2) void k_xprod6(const double* u, c
             m = p1 > m ? p1 : m;
         }
     }
-    // products uu, uv, uw, vv, vw, ww (rhs_velocity.f90:160-170)
+    // products uu, uv, uw, vv, vw, ww (rhs_velocity.f90:160-170); DEV: uu - ww,
+    // uv, uw, vv - ww, vw
+    constexpr int NP = DEV ? 5 : 6;
 #pragma unroll
-    for (int p = 0; p < 6; ++p) {
+    for (int p = 0; p < NP; ++p) {
         const int i = p < 3 ? 0 : p < 5 ? 1 : 2;
         const int j = p < 3 ? p : p < 5 ? p - 2 : 2;
         cpx<double> z[1][E];
 #pragma unroll
-        for (int s = 0; s < E; ++s)
+        for (int s = 0; s < E; ++s) {
             z[0][s] = cpx<double>{a[i][s].x * a[j][s].x, a[i][s].y * a[j][s].y};
+            if (DEV && (p == 0 || p == 3)) {
+                // ww recomputed for vv - ww (kept from uu - ww, it held
+                // registers across three transforms and spilled)
+                cpx<double> c = a[2][s];
+                if (p == 3) asm volatile("" : "+v"(c.x), "+v"(c.y));
+                z[0][s] = cpx<double>{z[0][s].x - c.x * c.x, z[0][s].y - c.y * c.y};
+            }
+        }
         double* op = out.p[0];
         if (p == 1) op = out.p[1];
         if (p == 2) op = out.p[2];
@@ -247,7 +262,7 @@ struct PsUpd {
     int kfmax, forced, first;
 };
 
-template <int N, bool FINAL>
+template <int N, bool FINAL, bool DEV = false>
 __global__ __launch_bounds__(ZC<N>::THREADS) void k_zps(Q6 Q, int64_t P, PsUpd u,
                                                         const cpx<double>* __restrict__ tw) {
     using C = ZC<N>;
@@ -267,8 +282,9 @@ __global__ __launch_bounds__(ZC<N>::THREADS) void k_zps(Q6 Q, int64_t P, PsUpd u
     cpx<double> acc[3][E];
 #pragma unroll
     for (int s = 0; s < E; ++s) acc[0][s] = acc[1][s] = acc[2][s] = cpx<double>{0.0, 0.0};
+    constexpr int NP = DEV ? 5 : 6;
 #pragma unroll 1
-    for (int p = 0; p < 6; ++p) {
+    for (int p = 0; p < NP; ++p) {
         const double* qp = Q.p[0];
         if (p == 1) qp = Q.p[1];
         if (p == 2) qp = Q.p[2];
@@ -295,7 +311,8 @@ __global__ __launch_bounds__(ZC<N>::THREADS) void k_zps(Q6 Q, int64_t P, PsUpd u
                 x = cpx<double>{x.x * cs + x.y * sn, x.y * cs - x.x * sn};
             }
             const double kzd = wavenum(e, N);
-            // p: 0 uu, 1 uv, 2 uw, 3 vv, 4 vw, 5 ww
+            // p: 0 uu, 1 uv, 2 uw, 3 vv, 4 vw, 5 ww (DEV: 0 uu - ww, 3 vv - ww,
+            // no 5)
             if (p == 0) acc[0][s] = scale(x, kxd);
             if (p == 1) {
                 acc[0][s] = acc[0][s] + scale(x, kyd);
@@ -566,7 +583,7 @@ void dispatch_ps(int n, F&& f) {
     }
 }
 
-template <int N>
+template <int N, bool DEV>
 void half(Lib& L, const double* const uvw[3], bool final_half, bool first, const double* dtp) {
     using X = XR<N>;
     using Z = ZC<N>;
@@ -577,14 +594,14 @@ void half(Lib& L, const double* const uvw[3], bool final_half, bool first, const
         Timed tm(TC_FFT_X);
         const int64_t nb = (int64_t)N * N / X::ROWS;  // <= n*n, the size of dmax
         if (!final_half)
-            hipLaunchKernelGGL((k_xprod6<N, true>), dim3(nb), dim3(X::THREADS), 0, L.st, uvw[0],
-                               uvw[1], uvw[2], q, P, L.tw64, L.dmax);
+            hipLaunchKernelGGL((k_xprod6<N, true, DEV>), dim3(nb), dim3(X::THREADS), 0, L.st,
+                               uvw[0], uvw[1], uvw[2], q, P, L.tw64, L.dmax);
         else
-            hipLaunchKernelGGL((k_xprod6<N, false>), dim3(nb), dim3(X::THREADS), 0, L.st, uvw[0],
-                               uvw[1], uvw[2], q, P, L.tw64, L.dmax);
+            hipLaunchKernelGGL((k_xprod6<N, false, DEV>), dim3(nb), dim3(X::THREADS), 0, L.st,
+                               uvw[0], uvw[1], uvw[2], q, P, L.tw64, L.dmax);
         if (!final_half) launch_max_fold(L.dmax, nb, L.dscal, L.st);
     }
-    for (int i = 1; i <= 6; ++i) pass_y(L.W[i], -1);
+    for (int i = 1; i <= (DEV ? 5 : 6); ++i) pass_y(L.W[i], -1);
     PsUpd u;
     for (int i = 0; i < 3; ++i) {
         u.F[i] = L.F[i + 1];
@@ -603,11 +620,11 @@ void half(Lib& L, const double* const uvw[3], bool final_half, bool first, const
     if (Z::B * 16 < 128) ncb += ncb & 1;
     Timed tm(TC_FFT_Z);
     if (!final_half)
-        hipLaunchKernelGGL((k_zps<N, false>), dim3(ncb, N), dim3(Z::THREADS), 0, L.st, q, P, u,
-                           L.tw64);
+        hipLaunchKernelGGL((k_zps<N, false, DEV>), dim3(ncb, N), dim3(Z::THREADS), 0, L.st, q, P,
+                           u, L.tw64);
     else
-        hipLaunchKernelGGL((k_zps<N, true>), dim3(ncb, N), dim3(Z::THREADS), 0, L.st, q, P, u,
-                           L.tw64);
+        hipLaunchKernelGGL((k_zps<N, true, DEV>), dim3(ncb, N), dim3(Z::THREADS), 0, L.st, q, P,
+                           u, L.tw64);
 }
 
 template <int N>
@@ -713,9 +730,14 @@ void fused_ps_velocity(double* const shifted[3], bool first, const double* dtp) 
     dispatch_ps(L.n, [&](auto NC) {
         constexpr int N = decltype(NC)::value;
         const double* un[3] = {L.W[1], L.W[2], L.W[3]};
-        half<N>(L, un, false, first, dtp);
         const double* sh[3] = {shifted[0], shifted[1], shifted[2]};
-        half<N>(L, sh, true, first, dtp);
+        if (L.nlform == 2) {
+            half<N, true>(L, un, false, first, dtp);
+            half<N, true>(L, sh, true, first, dtp);
+        } else {
+            half<N, false>(L, un, false, first, dtp);
+            half<N, false>(L, sh, true, first, dtp);
+        }
     });
 }
 

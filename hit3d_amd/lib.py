@@ -295,9 +295,10 @@ class Hit3D:
         _check(self.lib.hit3d_set_les(_i(les_model), _d(c_smag)))
 
     def set_nonlinear(self, form: int):
-        """Nonlinear term of the fused 2/3-rule step: 0 the divergence form
-        (the reference's), 1 the rotational u x omega form, 2 the divergence
-        of the trace-free product tensor (five forward transforms)."""
+        """Nonlinear term of the fused steps: 0 the divergence form (the
+        reference's), 1 the rotational u x omega form (2/3 rule), 2 five
+        products instead of six: the trace-free tensor u_i u_j - delta_ij
+        |u|^2/3 (2/3 rule) or u_i u_j - delta_ij w w (phase shift)."""
         _check(self.lib.hit3d_set_nonlinear(_i(form)))
 
     def set_fusion(self, on: bool):

@@ -184,18 +184,20 @@ int hit3d_mlp_grads_static(const int64_t *count, const float *X, const float *T,
 int hit3d_box_filter(const int64_t *ratio, double *ubar, double *tau);
 int hit3d_sgs_features(const double *delta, float *X);
 
-/* Form of the nonlinear term in hit3d_step's fused 2/3-rule pipeline:
+/* Form of the nonlinear term in hit3d_step's fused pipelines:
  * 0 (default) the divergence form -i k_j (u_i u_j)^ of rhs_velocity.f90:56-102;
  * 1 the rotational form (u x omega)^, omega = i k x u-hat (6 inverse and 3
- * forward transforms); 2 the divergence form of the trace-free tensor
- * u_i u_j - delta_ij |u|^2/3 (3 inverse and 5 forward transforms instead of
- * 6).  Forms 1 and 2 have no reference counterpart: they differ from form 0
- * by a gradient (i k (|u|^2/2)^, i k (|u|^2/3)^), which the projection
- * after the update removes, so the fields agree to rounding while rhs_old
- * holds the form's own RHS.  The phase shift rejects forms 1 and 2; a step
- * that cannot run fused (active scalars, LES, fusion off) fails while one
- * of them is selected.  The unfused entry points (hit3d_rhs_velocity) keep
- * the divergence form. */
+ * forward transforms; 2/3 rule only); 2 five products instead of six: with
+ * the 2/3 rule the divergence of the trace-free tensor u_i u_j - delta_ij
+ * |u|^2/3 (3 inverse and 5 forward transforms), with the phase shift (the
+ * fused velocity step of rhs_velocity.f90:150-410) that of u_i u_j - delta_ij
+ * w w in both halves.  Forms 1 and 2 have no reference counterpart: they
+ * differ from form 0 by a gradient (i k (|u|^2/2)^, i k (|u|^2/3)^,
+ * i k (w w)^, masked like the rest of the RHS), which the projection after
+ * the update removes, so the fields agree to rounding while rhs_old holds
+ * the form's own RHS.  A step that cannot run fused (2/3 rule with active
+ * scalars or LES, fusion off) fails while form 1 or 2 is selected.  The
+ * unfused entry points (hit3d_rhs_velocity) keep the divergence form. */
 int hit3d_set_nonlinear(const int64_t *form);
 
 /* Selects how hit3d_step runs: *on != 0 (default) the fused pipeline where

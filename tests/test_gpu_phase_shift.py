@@ -98,3 +98,48 @@ def test_fused_phase_shift_matches_oracle(oracle, n):
             assert np.all(got[c][ia >= 2] == 0.0)
     finally:
         h.close()
+
+
+@pytest.mark.parametrize("n,ns", [(32, 2), (64, 0)])
+def test_phase_shift_five_product_form(n, ns):
+    """hit3d_set_nonlinear(2) in the fused phase-shift step: the five
+    products uu - ww, uv, uw, vv - ww, vw instead of the six u_i u_j.  Their
+    divergence differs by the gradient of ww, masked where ialias >= 2 like
+    the rest of the RHS, which the projection removes: after 3 driver
+    steps the fields (scalars included) equal the six-product form's to
+    rounding, the Courant numbers and band energies too, and the difference
+    of the two rhs_old is parallel to k at every wavevector (k x dr = 0)
+    and not zero."""
+    note(f"phase shift five-product form n={n} scalars={ns}")
+    h = hit3d_amd.Hit3D(n, n_scalars=ns, dealias=1)
+    try:
+        nu = 1 / 400.0
+        h.set_physics(nu, flow_type=1, kfmax=2, famp=0.5, pe=tuple(nu / s for s in (0.7, 1.0)[:ns]),
+                      scalar_type=(0,) * ns)
+        h.init_velocity(RN1, 2, 4, 3.0)
+        h.dealias_all()
+        nf = 3 + ns
+        f0 = np.stack([h.get(FIELDS, c + 1) for c in range(nf)])
+        a = _run(h, n, nf, 3, True, ns > 0, f0)
+        h.set_nonlinear(2)
+        try:
+            b = _run(h, n, nf, 3, True, ns > 0, f0)
+        finally:
+            h.set_nonlinear(0)
+        for (c1, e1), (c2, e2) in zip(a[0], b[0]):
+            assert abs(c1 - c2) <= 1e-13 * c2 and abs(e1 - e2) <= 1e-13 * e2
+        note(f"fields {rel(b[1], a[1]):.2e}")
+        assert rel(b[1], a[1]) < 1e-12
+        # rhs_old difference: a gradient i k phi in the (ky, kz, kx re/im) layout
+        d = b[2][:3] - a[2][:3]
+        kx = np.arange(n // 2 + 1)[None, None, :]
+        kk = np.array([j if 2 * j <= n else j - n for j in range(n)], float)  # x_fftw.f90:280-296
+        ky, kz = kk[:, None, None], kk[None, :, None]
+        dc = d[:, :, :, 0::2] + 1j * d[:, :, :, 1::2]
+        cross = [ky * dc[2] - kz * dc[1], kz * dc[0] - kx * dc[2], kx * dc[1] - ky * dc[0]]
+        kmax = np.sqrt(3.0) * n / 2
+        scale = np.abs(dc).max()
+        assert scale > 1e-8 * np.abs(a[2][:3]).max()
+        assert max(np.abs(c).max() for c in cross) <= 1e-11 * kmax * scale
+    finally:
+        h.close()
