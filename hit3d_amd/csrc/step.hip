@@ -15,6 +15,9 @@
 //   E  forward z transform of the products + RHS assembly -i k_j (u_i u_j)^
 //      - nu k^2 u + band forcing + Euler/AB2 update + projection, writing
 //      fields and rhs_old.
+// At 1024^3, C runs as k_xh (each real row one half-length transform inside
+// a wavefront) and A, B, D as persistent grids (k_inv_zp, k_yp) that load
+// their next column group while transforming the current one.
 // The work buffers (wrk slots) use a pruned row pitch PA >= kmax+1 and T =
 // double (FP64 form) or float (mixed form: FP32 transforms and nonlinear
 // term, FP64 state, update, forcing energy and Courant reduction; rhs_old
