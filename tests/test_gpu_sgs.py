@@ -255,7 +255,8 @@ def test_closure_gain_over_smagorinsky():
     forced case (128^3 DNS, filter ratio 4, 32^3 LES) the closure trained by
     the library's own kernels has a lower validation error than the
     Smagorinsky stress (Cs = 0.17) a priori, and its LES spectrum is closer to
-    the filtered DNS than the Smagorinsky LES's a posteriori.  Every kernel is
+    the filtered DNS than the Smagorinsky LES's a posteriori, alone and
+    blended half and half with the Smagorinsky stress.  Every kernel is
     deterministic, so the comparison repeats exactly."""
     from hit3d_amd import sgs
     note("closure gain (128^3 -> 32^3)")
@@ -272,4 +273,5 @@ def test_closure_gain_over_smagorinsky():
     note(f"E(k) error: closure {err_m:.3f}, blended with Smagorinsky {err_b:.3f}, "
          f"Smagorinsky {err_s:.3f}, fallback steps {fb}, {fb_b}")
     assert fb == 0 and fb_b == 0
-    assert err_m < err_s
+    # measured 0.370 (network), 0.452 (blend), 0.521 (Smagorinsky)
+    assert err_m < err_s and err_b < err_s
