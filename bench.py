@@ -542,9 +542,9 @@ def main():
                 "alg_bytes_per_launch": byt, "avg_launch_ms": avg_s * 1e3,
                 "launches_per_step": cnt / a.steps, "stream_copy_GBs": copy_gbs,
                 "frac_of_stream_copy": ach / copy_gbs}
-    # every kernel class of the FP64 step against the same peak: the dominant
-    # one above is the x pass, bound by arithmetic and LDS exchanges; the
-    # others are HBM-bound
+    # every kernel class of the FP64 step against the same peak (the dominant
+    # one above is the x pass or the z/update kernel, which take about the
+    # same time at 512^3 and trade places from one GPU to the next)
     by_kernel = {}
     for cls, (ms, cnt) in tim64.items():
         byt = kernel_bytes(n, cls, 0, form)
