@@ -390,3 +390,60 @@ def test_deviatoric_rhs_projects_to_divergence_form(oracle):
             st = oracle.step(cfg, st)
         out[dev] = st.fields
     assert np.abs(out[True] - out[False]).max() < 1e-12 * np.abs(out[False]).max()
+
+
+# ------------------------------------------------------------ multi-step closed form
+
+def _abc_field(oracle, n, a, b, c):
+    """The ABC flow u = (a sin z + c cos y, b sin x + a cos z, c sin y + b cos x)
+    as spectral fields.  It is a Beltrami field (curl u = u), so u x omega = 0
+    and the nonlinear term -(u . grad) u = -grad(|u|^2/2) is a gradient that
+    the projection removes: the flow is an exact Navier-Stokes solution whose
+    modes (all |k| = 1) decay together as du/dt = -nu u."""
+    comps = [[((0, 0, 1), 0.0, a), ((0, 1, 0), c, 0.0)],
+             [((1, 0, 0), 0.0, b), ((0, 0, 1), a, 0.0)],
+             [((0, 1, 0), 0.0, c), ((1, 0, 0), b, 0.0)]]
+    return np.stack([oracle.fft_forward(_pad(_modes_field(n, md), n)) for md in comps])
+
+
+def abc_amplitudes(steps, dt, rate):
+    """Amplitude y_m of the field after m driver steps (y_0 = 1): an Euler
+    first step, then AB2 (main.f90:124-176), for dy/dt = rate(y)."""
+    ys, r_prev = [1.0], None
+    for _ in range(steps):
+        r = rate(ys[-1])
+        ys.append(ys[-1] + dt * (r if r_prev is None else 1.5 * r - 0.5 * r_prev))
+        r_prev = r
+    return ys
+
+
+@pytest.mark.parametrize("dealias,form,forced", [(0, "divergence", False), (1, "divergence", False),
+                                                 (0, "rotational", False), (0, "deviatoric", False),
+                                                 (0, "divergence", True)])
+def test_step_abc_flow_closed_form(oracle, dealias, form, forced):
+    """Twenty driver steps of the ABC flow, whose nonlinear term is nonzero
+    (|grad(|u|^2/2)| ~ |u|^2) but a pure gradient: the fields stay f0 y_m,
+    with y_m from the Euler/AB2 recurrence of dy/dt = -nu y (forced, KFMAX = 2:
+    (FAMP/E - nu) y with the band energy E = E0 y^2, all of it in shell 1).
+    A multi-step trajectory with a closed form: catches a nonlinear term left
+    unprojected or projected before the update, an AB2 history that keeps
+    the gradient part, and a forcing that drifts over steps."""
+    n, nu, dt, steps, famp = 16, 0.05, 0.01, 20, 0.5
+    a, b, c = 1.0, 0.8, 0.6
+    f0 = _abc_field(oracle, n, a, b, c)
+    e0 = (a * a + b * b + c * c) / 2.0
+    cfg = oracle.Config(n=n, nu=nu, dt=dt, dealias=dealias, flow_type=1 if forced else 0,
+                        kfmax=2, famp=famp, rotational=form == "rotational",
+                        deviatoric=form == "deviatoric")
+    rate = (lambda y: (famp / (e0 * y * y) - nu) * y) if forced else (lambda y: -nu * y)
+    ys = abc_amplitudes(steps, dt, rate)
+    nl, _ = oracle.rhs_velocity(f0, 0.0, 0, n)
+    assert np.abs(nl).max() > 0.1 * np.abs(f0).max()   # the nonlinear term is not zero
+    st = oracle.State(fields=f0.copy(), rhs_old=np.zeros_like(f0), dt=dt)
+    scale = np.abs(f0).max()
+    for m in range(1, steps + 1):
+        st = oracle.step(cfg, st)
+        assert np.abs(st.fields - ys[m] * f0).max() < 1e-12 * scale, m
+        if forced:
+            assert abs(st.energy - e0 * ys[m - 1] ** 2) < 1e-12 * e0
+    assert abs(ys[-1] - 1.0) > 1e-3   # the amplitude has moved
