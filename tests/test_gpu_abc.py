@@ -12,12 +12,12 @@ E = E0 y^2, all of it in shell 1).  The Courant number of step m is
 y_{m-1} max(|u0| + |v0| + |w0|) dt/dx (rhs_velocity.f90:31-38).
 
 The 2/3-rule step in its three nonlinear forms and the phase-shift step
-(six and five products) at 64^3 over 30 steps, and the benchmark's
-configuration (512^3, forced, deviatoric form) in FP64 and in the mixed form.
-Tolerances: fields 1e-12 of max |f0| in FP64, 1e-6 in the mixed form (its
-single-RHS bound is 1e-5 of the RHS, and a step moves the field by dt times
-the RHS); Courant number and band energy 1e-12 relative (FP64); E(k) outside
-shell 1 below 1e-20 of E0 (FP64).
+(six and five products) at 64^3 over 30 steps, the benchmark's
+configuration (512^3, forced, deviatoric form) and 1024^3 in FP64 and in
+the mixed form.  Tolerances: fields 1e-12 of max |f0| in FP64, 1e-6 in the
+mixed form (its single-RHS bound is 1e-5 of the RHS, and a step moves the
+field by dt times the RHS); Courant number and band energy 1e-12 relative
+(FP64); E(k) outside shell 1 below 1e-20 of E0 (FP64), 1e-12 (mixed).
 """
 import numpy as np
 import pytest
@@ -62,6 +62,32 @@ def load_abc(h, n):
         h.xFFT3d(1, 1)
         h.copy(FIELDS, comp + 1, WRK, 1)
     return umax
+
+
+def abc_spectral(n):
+    """The ABC fields' half spectra [ky, kz, kx re/im] written directly (the
+    forward transform: sign -1, unnormalised): cos(k.x) has n^3/2 at k and at
+    -k, sin(k.x) -i n^3/2 at k and +i n^3/2 at -k; a mode with kx = 0 stores
+    both entries, one with kx = 1 only the first.  Used where the physical
+    slots would not be worth their host memory (1024^3)."""
+    h3 = float(n) ** 3 / 2.0
+    out = [np.zeros((n, n, n + 2)) for _ in range(3)]
+
+    def mode(a, k, re, im):
+        kx, ky, kz = k
+        a[ky % n, kz % n, 2 * kx] += re
+        a[ky % n, kz % n, 2 * kx + 1] += im
+        if kx == 0:
+            a[-ky % n, -kz % n, 0] += re
+            a[-ky % n, -kz % n, 1] -= im
+
+    mode(out[0], (0, 0, 1), 0.0, -A * h3)   # a sin z
+    mode(out[0], (0, 1, 0), C * h3, 0.0)    # c cos y
+    mode(out[1], (1, 0, 0), 0.0, -B * h3)   # b sin x
+    mode(out[1], (0, 0, 1), A * h3, 0.0)    # a cos z
+    mode(out[2], (0, 1, 0), 0.0, -C * h3)   # c sin y
+    mode(out[2], (1, 0, 0), B * h3, 0.0)    # b cos x
+    return out
 
 
 def amplitudes(steps, dt, nu, famp, forced):
@@ -111,6 +137,8 @@ def test_abc_flow_64(dealias, form, forced):
         h.set_nonlinear(form)
         umax = load_abc(h, n)
         f0 = np.stack([h.get(FIELDS, c) for c in (1, 2, 3)])
+        # the directly written spectra (used at 1024^3) are the transforms
+        assert np.abs(np.stack(abc_spectral(n)) - f0).max() < 1e-12 * np.abs(f0).max()
         ys = run_abc(h, n, umax, steps, dt, nu, famp, forced, 0)
         f = np.stack([h.get(FIELDS, c) for c in (1, 2, 3)])
         err = np.abs(f - ys[-1] * f0).max() / np.abs(f0).max()
@@ -154,6 +182,41 @@ def test_abc_flow_512_benchmark_configuration():
             note(f"  y_{steps} = {ys[-1]:.6f}, w field error {err:.2e}")
             assert err < tol
             del w, w0
+    finally:
+        h.set_nonlinear(0)
+        h.close()
+
+
+def test_abc_flow_1024():
+    """1024^3 (its own kernels: half-length x pass, persistent column passes,
+    paired half-width z/update blocks), forced, deviatoric form: four steps
+    in FP64 and in the mixed form.  The spectra are written directly, and
+    the check goes through the device statistics, so no slot crosses PCIe
+    back: band energy per step E0 y^2, E(k) in shell 1 E0 y^2 and nothing
+    elsewhere (an entry off by d relative would put d^2 E0 there)."""
+    n, steps, dt, nu, famp = 1024, 4, 1e-3, 1 / 1600.0, 0.5
+    h = hit3d_amd.Hit3D(n, dealias=0)
+    try:
+        h.set_physics(nu, flow_type=1, kfmax=2, famp=famp)
+        h.set_nonlinear(2)
+        for precision in (0, 1):
+            note(f"abc flow n=1024 precision={precision}")
+            for comp, a in enumerate(abc_spectral(n)):
+                h.put(FIELDS, comp + 1, a)
+                del a
+            ys = amplitudes(steps, dt, nu, famp, True)
+            tol = 1e-12 if precision == 0 else 1e-6
+            fov = True
+            for m in range(1, steps + 1):
+                _, en, fov, _ = h.step(dt, fov, True, precision=precision)
+                assert abs(en - E0 * ys[m - 1] ** 2) <= tol * E0, (m, en)
+            e, _ = h.stats()
+            note(f"  y_{steps} = {ys[-1]:.6f}, shell 1 {e[0] / (E0 * ys[-1] ** 2) - 1:.2e}, "
+                 f"other shells {e[1:].sum() / E0:.2e}")
+            assert abs(e[0] - E0 * ys[-1] ** 2) < tol * E0
+            assert e[1:].sum() < (1e-20 if precision == 0 else 1e-12) * E0
+            d = h.divergence()
+            assert max(abs(d[0]), abs(d[1])) < 1e-9
     finally:
         h.set_nonlinear(0)
         h.close()
