@@ -2,6 +2,7 @@
 // (y,z) -> (kz,ky) transpose; inverse in reverse order), replacing the FFTW
 // plane/stick loops and the slab transpose of xFFT3d (x_fftw.f90:852-1085).
 #include <algorithm>
+#include <vector>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -130,6 +131,22 @@ __global__ __launch_bounds__(B* N / E) void k_cols(Slots3 io, int64_t a_in, int6
     }
 }
 
+// blocks of kernel k resident per compute unit at the given block size,
+// queried once per kernel (the persistent column kernels share one function
+// type, so the cache is keyed by the kernel's address)
+template <typename K>
+static int resident_per_cu(K k, int threads) {
+    static std::vector<std::pair<const void*, int>> cache;
+    for (const auto& e : cache)
+        if (e.first == reinterpret_cast<const void*>(k)) return e.second;
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, threads, 0) != hipSuccess)
+        throw std::runtime_error("column pass: occupancy query failed");
+    per = std::max(1, per);
+    cache.emplace_back(reinterpret_cast<const void*>(k), per);
+    return per;
+}
+
 // compute units of the current device (queried once)
 static int device_cus() {
     static int n = 0;
@@ -237,14 +254,7 @@ static void cols_n(int dir, const cpx<T>* const* in, cpx<T>* const* out, int nb,
         if (!acc.r[0]) {
             const int ncb = (int)grid.x, ngroups = ncb * N * nb;
             auto go = [&](auto kern) {
-                static int per = 0;
-                if (per == 0) {
-                    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, (int)block.x, 0) !=
-                        hipSuccess)
-                        throw std::runtime_error("column pass: occupancy query failed");
-                    per = std::max(1, per);
-                }
-                const int nblk = std::min(per * device_cus(), ngroups);
+                const int nblk = std::min(resident_per_cu(kern, (int)block.x) * device_cus(), ngroups);
                 hipLaunchKernelGGL(kern, dim3(nblk), block, 0, s, io, a_in, a_out, s_in, s_out,
                                    ncols, ncb, N, ngroups, tw, tab);
             };
