@@ -129,7 +129,9 @@ CASES_64 = [
 
 @pytest.mark.parametrize("dealias,form,forced", CASES_64)
 def test_abc_flow_64(dealias, form, forced):
-    n, steps, dt, nu, famp = 64, 30, 0.01, 0.05, 0.5
+    # dt keeps every kept mode inside AB2's stability region (nu k^2 dt <=
+    # 0.48 for the phase shift's |k| <= 43.7), so roundoff does not grow
+    n, steps, dt, nu, famp = 64, 30, 0.005, 0.05, 0.5
     note(f"abc flow n={n} dealias={dealias} form={form} forced={forced}")
     h = hit3d_amd.Hit3D(n, dealias=dealias)
     try:
