@@ -1,6 +1,6 @@
 """The learned subgrid-stress closure's workflow on the GPU library: training
 data from box-filtered DNS, standardisation, training, and coarse LES runs
-with the closure or with Smagorinsky, judged by the filtered-DNS spectrum.
+with the closure or with Smagorinsky, measured against the filtered-DNS spectrum.
 
 No counterpart in the reference: the closure stands in for the eddy
 viscosity of m_les.f90:1142-1215 (kept as the fallback).  Every field
@@ -362,7 +362,7 @@ def spectrum_error(ek_les: np.ndarray, ek_ref: np.ndarray, kmin: int = 1) -> flo
     return float(np.mean(np.abs(a - b) / b))
 
 
-def judge_windows(a, snaps: list, m: int, delta: float, t_dns: float) -> dict:
+def compare_windows(a, snaps: list, m: int, delta: float, t_dns: float) -> dict:
     """The closure trained on the first window's snapshots, then for every
     window: a-priori errors on that window's data (held out in time for the
     later windows) and 64^3 LES runs with the closure and with Smagorinsky
@@ -474,7 +474,7 @@ def main(argv=None):
                          "model 'mlp_mix' of the comparison and the weight written by --out)")
     ap.add_argument("--windows", type=int, default=1,
                     help="consecutive DNS windows of --snaps snapshots each: the closure is "
-                         "trained on the first, then judged a priori and a posteriori against "
+                         "trained on the first, then compared a priori and a posteriori against "
                          "every window's own filtered DNS")
     a = ap.parse_args(argv)
     m = a.dns // a.ratio
@@ -483,7 +483,7 @@ def main(argv=None):
     snaps = run_dns(a.dns, a.ratio, a.spinup, a.snaps * a.windows, a.every, a.re)
     t_dns = time.time() - t0
     if a.windows > 1:
-        print(json.dumps(judge_windows(a, snaps, m, delta, t_dns)), flush=True)
+        print(json.dumps(compare_windows(a, snaps, m, delta, t_dns)), flush=True)
         return
     ds = features(snaps, m, delta)
     ek_ref = np.mean(ds.ek_filtered, axis=0)
