@@ -508,3 +508,14 @@ def test_step_uniform_flow_scalars_closed_form(oracle, dealias):
         assert np.abs(st.fields[:3] - f0[:3]).max() < 1e-12 * np.abs(f0[:3]).max()
     # the scalars moved: advected (phase) and diffused (amplitude)
     assert np.abs(st.fields[3:] - f0[3:]).max() > 1e-2 * scale
+
+
+def test_abc_spectra_written_directly_are_the_transforms(oracle):
+    """The 1024^3 GPU check writes the ABC flow's half spectra entry by
+    entry (`abc_spectral`); they must equal the forward transform of the
+    physical fields, including the conjugate entries of the kx = 0 modes."""
+    from test_gpu_abc import abc_physical, abc_spectral
+    n = 16
+    for p, s in zip(abc_physical(n), abc_spectral(n)):
+        f = oracle.fft_forward(p)
+        assert np.abs(f - s).max() < 1e-12 * np.abs(f).max()
